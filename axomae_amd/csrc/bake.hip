@@ -1,0 +1,232 @@
+// Batched environment-map bakes: one launch covers every output texel of every map.
+//  * diffuse irradiance: TextureOperations<float>::computeDiffuseIrradiance
+//    (TextureProcessing.cpp:9-64, TextureProcessing.h:253-301) - pgc3d-hashed cosine
+//    samples per texel, looked up with the nearest-texel rule uvSample degenerates to;
+//  * GGX prefiltered specular: envmap_prefilter.frag:1-116 on equirect, level 0.
+// Each output texel is one lane; samples are summed in the reference's order.
+#include <algorithm>
+#include <vector>
+
+#include "device_math.h"
+#include "nova_internal.h"
+
+namespace nova_hip {
+
+struct BakeMap {
+  const float *env;
+  float *out;
+  int32_t w, h, c, ow, oh;
+  float roughness;
+  uint64_t first;  // first global texel index of this map
+};
+
+__device__ __forceinline__ unsigned uv_to_pixel(float coord, unsigned dim) { return (unsigned)(long long)(coord * (float)dim); }
+
+// sphericalToCartesian<float> (math_spherical.h:33-39)
+__device__ __forceinline__ f3 sph_to_cart(float phi, float theta) {
+  const float z = cos_f(theta);
+  const float x = sin_f(theta) * cos_f(phi);
+  const float y = sin_f(theta) * sin_f(phi);
+  return mk3(x, y, z);
+}
+
+// discreteSample + uvSample (TextureProcessing.h:173-210)
+__device__ __forceinline__ f3 discrete(const BakeMap &m, int x, int y) {
+  int xc, yc;
+  x = x < 0 ? 0 : x;  // coordinates come from unsigned texel indices; a NaN direction must stay in bounds
+  y = y < 0 ? 0 : y;
+  if (x >= m.w) xc = x % m.w;
+  else if (x < 0) xc = m.w + (x % m.w);
+  else xc = x;
+  if (y >= m.h) yc = y % m.h;
+  else if (y < 0) yc = m.h + (y % m.h);
+  else yc = y;
+  const float *p = m.env + ((size_t)yc * m.w + xc) * m.c;
+  return mk3(p[0], p[1], p[2]);
+}
+
+__device__ __forceinline__ f3 uv_sample(const BakeMap &m, float u, float v) {
+  const float ui = floorf(u), vi = floorf(v);
+  const float up = (u > 1. || u < 0.) ? u - ui : u;
+  const float vp = (v > 1. || v < 0.) ? v - vi : v;
+  const float px = (float)min(uv_to_pixel(up, (unsigned)m.w), (unsigned)m.w), py = (float)min(uv_to_pixel(vp, (unsigned)m.h), (unsigned)m.h);
+  const float tlx = floorf(px), tly = floorf(py);
+  const float fu = (px - tlx) / ((tlx + 1) - tlx), fv = (py - tly) / ((tly + 1) - tly);
+  const f3 tl = discrete(m, (int)tlx, (int)tly), tr = discrete(m, (int)(tlx + 1), (int)tly);
+  const f3 bl = discrete(m, (int)tlx, (int)(tly + 1)), br = discrete(m, (int)(tlx + 1), (int)(tly + 1));
+  const float hd = 1 - fu;
+  const f3 top = hd * tl + fu * tr;
+  const f3 bot = hd * bl + fu * br;
+  return (1 - fv) * top + fv * bot;
+}
+
+__device__ __forceinline__ f3 dir_sample(const BakeMap &m, const float *lut, f3 d) {
+  const float phi = atan2_approx(d.y, d.x), theta = acos_lt(lut, d.z);
+  const double inv_2pi = 1.f / (2.f * 3.14159265358979323846), inv_pi = 1.f / 3.14159265358979323846;
+  return uv_sample(m, (float)(phi * inv_2pi), (float)(theta * inv_pi));
+}
+
+// math::importance_sampling::pgc3d (math_importance_sampling.h:7-21)
+__device__ __forceinline__ f3 pgc3d(unsigned x, unsigned y, unsigned z) {
+  x = x * 1664525u + 1013904223u;
+  y = y * 1664525u + 1013904223u;
+  z = z * 1664525u + 1013904223u;
+  x += y * z;
+  y += z * x;
+  z += x * y;
+  x ^= x >> 16u;
+  y ^= y >> 16u;
+  z ^= z >> 16u;
+  x += y * z;
+  y += z * x;
+  z += x * y;
+  const float s = 1.f / 4294967296.f;
+  return mk3((float)x * s, (float)y * s, (float)z * s);
+}
+
+__device__ __forceinline__ int find_map(const BakeMap *maps, int count, uint64_t g) {
+  int lo = 0, hi = count - 1;
+  while (lo < hi) {
+    int mid = (lo + hi + 1) >> 1;
+    if (maps[mid].first <= g)
+      lo = mid;
+    else
+      hi = mid - 1;
+  }
+  return lo;
+}
+
+__global__ void irradiance_kernel(const BakeMap *__restrict__ maps, int count, uint64_t total, const float *__restrict__ lut, int samples) {
+  for (uint64_t g = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; g < total; g += (uint64_t)gridDim.x * blockDim.x) {
+    const BakeMap m = maps[find_map(maps, count, g)];
+    const uint64_t local = g - m.first;
+    const unsigned i = (unsigned)(local % (unsigned)m.ow), j = (unsigned)(local / (unsigned)m.ow);
+    const float u = (float)((double)i / (double)m.ow), v = (float)((double)j / (double)m.oh);
+    const f3 nrm = sph_to_cart((float)(2 * 3.14159265358979323846 * u), (float)(3.14159265358979323846 * v));
+    const float aphi = atan2_approx(nrm.y, nrm.x), atheta = acos_lt(lut, nrm.z);
+    const double inv_2pi = 1.f / (2.f * 3.14159265358979323846), inv_pi = 1.f / 3.14159265358979323846;
+    const unsigned px = uv_to_pixel((float)(aphi * inv_2pi), (unsigned)m.ow), py = uv_to_pixel((float)(atheta * inv_pi), (unsigned)m.oh);
+    const f3 right = mk3(1.f, 0.f, 0.f);
+    const float dd = dot(right, nrm);
+    f3 tangent = mk3(0.f, 1.f, 0.f);
+    if (1.0 - fabs((double)dd) > 1e-6) tangent = normalize(cross(right, nrm));
+    const f3 bitangent = cross(nrm, tangent);
+    f3 irr = mk3(0.f, 0.f, 0.f);
+    for (int s = 0; s < samples; s++) {
+      const f3 r = pgc3d(px, py, (unsigned)s);
+      const float sphi = (float)(2 * 3.14159265358979323846 * r.x);
+      const float stheta = asin_f(sqrtf(r.y));
+      f3 c = sph_to_cart(sphi, stheta);
+      c = c.x * tangent + c.y * bitangent + c.z * nrm;
+      irr = irr + dir_sample(m, lut, c);
+    }
+    irr = irr / (float)samples;
+    float *o = m.out + ((size_t)j * m.ow + i) * m.c;
+    o[0] = irr.x;
+    o[1] = irr.y;
+    o[2] = irr.z;
+    for (int k = 3; k < m.c; k++) o[k] = 0.f;
+  }
+}
+
+__device__ __forceinline__ float radical_inverse(unsigned bits) {
+  bits = (bits << 16u) | (bits >> 16u);
+  bits = ((bits & 0x55555555u) << 1u) | ((bits & 0xAAAAAAAAu) >> 1u);
+  bits = ((bits & 0x33333333u) << 2u) | ((bits & 0xCCCCCCCCu) >> 2u);
+  bits = ((bits & 0x0F0F0F0Fu) << 4u) | ((bits & 0xF0F0F0F0u) >> 4u);
+  bits = ((bits & 0x00FF00FFu) << 8u) | ((bits & 0xFF00FF00u) >> 8u);
+  return (float)bits * 2.3283064365386963e-10f;
+}
+
+__global__ void prefilter_kernel(const BakeMap *__restrict__ maps, int count, uint64_t total, const float *__restrict__ lut, int samples) {
+  for (uint64_t g = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; g < total; g += (uint64_t)gridDim.x * blockDim.x) {
+    const BakeMap m = maps[find_map(maps, count, g)];
+    const uint64_t local = g - m.first;
+    const unsigned i = (unsigned)(local % (unsigned)m.ow), j = (unsigned)(local / (unsigned)m.ow);
+    const float u = (float)((double)i / (double)m.ow), v = (float)((double)j / (double)m.oh);
+    const f3 N = normalize(sph_to_cart((float)(2 * 3.14159265358979323846 * u), (float)(3.14159265358979323846 * v)));
+    f3 col = mk3(0.f, 0.f, 0.f);
+    if (m.roughness == 0.f) {
+      col = dir_sample(m, lut, N);
+    } else {
+      float total_w = 0.f;
+      const float a = m.roughness * m.roughness;
+      const f3 up = fabsf(N.z) < 0.999f ? mk3(0.f, 0.f, 1.f) : mk3(1.f, 0.f, 0.f);
+      const f3 t = normalize(cross(up, N));
+      const f3 b = cross(N, t);
+      for (int s = 0; s < samples; s++) {
+        const float xi0 = (float)s / (float)samples, xi1 = radical_inverse((unsigned)s);
+        const float phi = 2.0f * 3.14159265f * xi0;
+        const float ct = sqrtf((1.0f - xi1) / (1.0f + (a * a - 1.0f) * xi1));
+        const float st = sqrtf(1.0f - ct * ct);
+        const f3 H = normalize(t * (cos_f(phi) * st) + b * (sin_f(phi) * st) + N * ct);
+        const f3 L = reflect(-N, H);
+        const float ndl = fmaxf(dot(N, L), 0.f);
+        if (ndl > 0.f) {
+          col = col + dir_sample(m, lut, L) * ndl;
+          total_w += ndl;
+        }
+      }
+      col = col / total_w;
+    }
+    float *o = m.out + ((size_t)j * m.ow + i) * m.c;
+    o[0] = col.x;
+    o[1] = col.y;
+    o[2] = col.z;
+    for (int k = 3; k < m.c; k++) o[k] = 0.f;
+  }
+}
+
+static int launch_bake(bool prefilter, const float *const *env, const int32_t *w, const int32_t *h, const int32_t *c, uint32_t count, float *const *out,
+                       const int32_t *ow, const int32_t *oh, const float *rough, int32_t samples, hipStream_t s) {
+  if (count == 0) return NOVA_HIP_SUCCESS;
+  if (!env || !w || !h || !c || !out || !ow || !oh || samples <= 0) return NOVA_HIP_INVALID_ARGUMENT;
+  std::vector<BakeMap> maps(count);
+  uint64_t total = 0;
+  for (uint32_t i = 0; i < count; i++) {
+    if (!env[i] || !out[i] || w[i] < 1 || h[i] < 1 || c[i] < 3 || c[i] > 4 || ow[i] < 1 || oh[i] < 1) return NOVA_HIP_INVALID_ARGUMENT;
+    maps[i] = BakeMap{env[i], out[i], w[i], h[i], c[i], ow[i], oh[i], rough ? rough[i] : 0.f, total};
+    total += (uint64_t)ow[i] * oh[i];
+  }
+  // acos table and map descriptors live in one device allocation per call
+  std::vector<float> lut(kAcosLut);
+  for (int i = 0; i < kAcosLut; i++) lut[i] = (float)std::acos(((double)i / (double)kAcosLut) * 2.0 - 1.0);
+  static float *d_lut = nullptr;
+  static int lut_dev = -1;
+  int dev = 0;
+  if (hipGetDevice(&dev) != hipSuccess) return NOVA_HIP_DEVICE_ERROR;
+  if (!d_lut || lut_dev != dev) {
+    if (hipMalloc(&d_lut, kAcosLut * sizeof(float)) != hipSuccess) return NOVA_HIP_OUT_OF_MEMORY;
+    if (hipMemcpy(d_lut, lut.data(), kAcosLut * sizeof(float), hipMemcpyHostToDevice) != hipSuccess) return NOVA_HIP_DEVICE_ERROR;
+    lut_dev = dev;
+  }
+  BakeMap *d_maps = nullptr;
+  if (hipMalloc(&d_maps, count * sizeof(BakeMap)) != hipSuccess) return NOVA_HIP_OUT_OF_MEMORY;
+  if (hipMemcpyAsync(d_maps, maps.data(), count * sizeof(BakeMap), hipMemcpyHostToDevice, s) != hipSuccess) {
+    hipFree(d_maps);
+    return NOVA_HIP_DEVICE_ERROR;
+  }
+  const unsigned block = 64;
+  const unsigned grid = (unsigned)std::min<uint64_t>((total + block - 1) / block, 1u << 20);
+  if (prefilter)
+    hipLaunchKernelGGL(prefilter_kernel, dim3(grid), dim3(block), 0, s, d_maps, (int)count, total, d_lut, samples);
+  else
+    hipLaunchKernelGGL(irradiance_kernel, dim3(grid), dim3(block), 0, s, d_maps, (int)count, total, d_lut, samples);
+  hipError_t e = hipGetLastError();
+  if (e == hipSuccess) e = hipStreamSynchronize(s);
+  hipFree(d_maps);
+  return e == hipSuccess ? NOVA_HIP_SUCCESS : NOVA_HIP_DEVICE_ERROR;
+}
+
+}  // namespace nova_hip
+
+extern "C" int nova_hip_bake_irradiance(const float *const *env_dev, const int32_t *w, const int32_t *h, const int32_t *c, uint32_t count,
+                                        float *const *out_dev, const int32_t *out_w, const int32_t *out_h, int32_t samples, void *stream) {
+  return nova_hip::launch_bake(false, env_dev, w, h, c, count, out_dev, out_w, out_h, nullptr, samples, (hipStream_t)stream);
+}
+
+extern "C" int nova_hip_bake_prefilter(const float *const *env_dev, const int32_t *w, const int32_t *h, const int32_t *c, uint32_t count,
+                                       float *const *out_dev, const int32_t *out_w, const int32_t *out_h, const float *roughness, int32_t samples,
+                                       void *stream) {
+  return nova_hip::launch_bake(true, env_dev, w, h, c, count, out_dev, out_w, out_h, roughness, samples, (hipStream_t)stream);
+}

@@ -1,0 +1,643 @@
+// Wavefront path integrator and 4-wide BVH closest-hit traversal.
+//
+// Per batch of frames (1 spp each, as the reference's Engine renders them):
+//   generate  : camera rays for every (frame, pixel) path slot, Sobol dims 0-1
+//   per bounce: extend (closest hit of the active queue)
+//               shade  (hit record, Principled BSDF sample, Russian roulette,
+//                       spawn; survivors are compacted into the next queue with a
+//                       wave ballot/mbcnt prefix and one atomic per wave)
+//   accumulate: accum += partial(frame); final = accum / (frame + 1), in frame order.
+//
+// The reference evaluates Li recursively (Integrator.cpp:59-98):
+//   V_k = DENAN(Le_k + c_k * V_{k+1}),  c_k = f * |cos| / pdf,
+// with V = envmap(dir) on a miss or once the depth budget is spent, and V_k = 0 when
+// the BSDF sample fails. The shade stage evaluates the same sum front to back,
+// L += T * Le_k; T *= c_k, where DENAN at level k is equivalent to ending the path with
+// L unchanged as soon as c_k has a non-finite component (V_{k+1} is always finite).
+// Alpha follows the same recursion (1 per scattering level plus the terminal alpha).
+#include <algorithm>
+#include <vector>
+
+#include "device_math.h"
+#include "device_shading.h"
+#include "nova_internal.h"
+
+namespace nova_hip {
+
+constexpr int kTraceBlock = 256;
+constexpr int kLdsStack = 24;
+
+struct TraceCtx {
+  const Node4 *nodes;
+  const TriMT *tris;
+  uint32_t *gstack;     // overflow entries: [level - kLdsStack][global thread]
+  uint32_t gstride;     // number of threads in the grid
+};
+
+// Moller-Trumbore exactly as Triangle::hit orders it (Triangle.h:158-175); accepts
+// tnear <= t <= tbest, so a later triangle at an equal t replaces the earlier one.
+__device__ __forceinline__ bool mt_test(const TriMT &tr, f3 o, f3 d, float tnear, float tbest, float &t, float &u, float &v) {
+  const f3 v0 = mk3(tr.a.x, tr.a.y, tr.a.z), e1 = mk3(tr.b.x, tr.b.y, tr.b.z), e2 = mk3(tr.c.x, tr.c.y, tr.c.z);
+  const f3 P = cross(d, e2);
+  const float det = dot(P, e1);
+  if (det == 0.f) return false;
+  const float inv_det = 1.f / det;
+  const f3 T = o - v0;
+  const float uu = dot(P, T) * inv_det;
+  if (uu < 0 || uu > 1.f) return false;
+  const f3 Q = cross(T, e1);
+  const float vv = dot(Q, d) * inv_det;
+  if (vv < 0.f || (uu + vv) > 1.f) return false;
+  const float tt = dot(Q, e2) * inv_det;
+  if (tt < tnear || tt > tbest) return false;
+  t = tt;
+  u = uu;
+  v = vv;
+  return true;
+}
+
+// Closest hit over the 4-wide BVH. Boxes are tested with a slightly widened slab so no
+// box holding a hit the reference's division-form test accepts is culled; triangle
+// tests are exact, so the hit is the reference's except at equal-t ties.
+__device__ __forceinline__ int trace_closest(const TraceCtx &tc, uint32_t *lds, uint32_t gtid, f3 o, f3 d, float tnear, float &t_out, float &u_out,
+                                             float &v_out) {
+  const float kUp = 1.000001f;
+  const f3 inv = mk3(1.f / d.x, 1.f / d.y, 1.f / d.z);
+  float tbest = 3.402823466e+38f;
+  int best = -1;
+  float bu = 0.f, bv = 0.f;
+  int sp = 0;
+  int32_t node = 0;
+  const int lane_off = threadIdx.x;
+  // A tree walk visits each node once; the cap only guards against a malformed tree.
+  for (uint32_t visits = 0; visits < (1u << 24); visits++) {
+    const float4 *np = reinterpret_cast<const float4 *>(tc.nodes + node);
+    const float4 mnx = np[0], mxx = np[1], mny = np[2], mxy = np[3], mnz = np[4], mxz = np[5];
+    const int4 ch = *reinterpret_cast<const int4 *>(np + 6);
+    const int4 cn = *reinterpret_cast<const int4 *>(np + 7);
+    const float bmnx[4] = {mnx.x, mnx.y, mnx.z, mnx.w}, bmxx[4] = {mxx.x, mxx.y, mxx.z, mxx.w};
+    const float bmny[4] = {mny.x, mny.y, mny.z, mny.w}, bmxy[4] = {mxy.x, mxy.y, mxy.z, mxy.w};
+    const float bmnz[4] = {mnz.x, mnz.y, mnz.z, mnz.w}, bmxz[4] = {mxz.x, mxz.y, mxz.z, mxz.w};
+    const int chv[4] = {ch.x, ch.y, ch.z, ch.w}, cnv[4] = {cn.x, cn.y, cn.z, cn.w};
+    float tent[4];
+    int ord[4];
+    int nhit = 0;
+    for (int k = 0; k < 4; k++) {
+      float tx1 = (bmnx[k] - o.x) * inv.x, tx2 = (bmxx[k] - o.x) * inv.x;
+      float ty1 = (bmny[k] - o.y) * inv.y, ty2 = (bmxy[k] - o.y) * inv.y;
+      float tz1 = (bmnz[k] - o.z) * inv.z, tz2 = (bmxz[k] - o.z) * inv.z;
+      float t0 = fmaxf(fmaxf(fminf(tx1, tx2), fminf(ty1, ty2)), fminf(tz1, tz2));
+      float t1 = fminf(fminf(fmaxf(tx1, tx2), fmaxf(ty1, ty2)), fmaxf(tz1, tz2)) * kUp;
+      bool hit = chv[k] >= 0 && t0 <= t1 && t1 >= 0.f && t0 <= tbest * kUp;
+      if (hit) {
+        tent[nhit] = t0;
+        ord[nhit] = k;
+        nhit++;
+      }
+    }
+    // near-to-far order of the hit children
+    for (int a = 1; a < nhit; a++)
+      for (int b = a; b > 0 && tent[b] < tent[b - 1]; b--) {
+        float tf = tent[b];
+        tent[b] = tent[b - 1];
+        tent[b - 1] = tf;
+        int ti = ord[b];
+        ord[b] = ord[b - 1];
+        ord[b - 1] = ti;
+      }
+    int32_t next = -1;
+    for (int a = nhit - 1; a >= 0; a--) {  // far to near: push inner nodes, keep the nearest
+      const int k = ord[a];
+      if (cnv[k] > 0) continue;
+      if (next >= 0) {
+        if (sp < kLdsStack)
+          lds[sp * kTraceBlock + lane_off] = (uint32_t)next;
+        else
+          tc.gstack[(size_t)(sp - kLdsStack) * tc.gstride + gtid] = (uint32_t)next;
+        sp++;
+      }
+      next = chv[k];
+    }
+    for (int a = 0; a < nhit; a++) {  // leaves now, near to far
+      const int k = ord[a];
+      if (cnv[k] == 0) continue;
+      if (tent[a] > tbest * kUp) continue;
+      const int32_t start = chv[k], cnt = cnv[k];
+      for (int32_t i = start; i < start + cnt; i++) {
+        const TriMT tr = tc.tris[i];
+        float tt, uu, vv;
+        if (mt_test(tr, o, d, tnear, tbest, tt, uu, vv)) {
+          tbest = tt;
+          best = __float_as_int(tr.a.w);
+          bu = uu;
+          bv = vv;
+        }
+      }
+    }
+    if (next >= 0) {
+      node = next;
+      continue;
+    }
+    if (sp == 0) break;
+    sp--;
+    node = (int32_t)(sp < kLdsStack ? lds[sp * kTraceBlock + lane_off] : tc.gstack[(size_t)(sp - kLdsStack) * tc.gstride + gtid]);
+  }
+  t_out = tbest;
+  u_out = bu;
+  v_out = bv;
+  return best;
+}
+
+// ------------------------------------------------------------------ kernels
+
+struct Queues {
+  float *ray[2];      // 6 planes of cap floats each: ox oy oz dx dy dz
+  int32_t *slot[2];
+  uint32_t *count;    // [0], [1]: queue sizes; [2]: segments traced
+  int32_t *hit_prim;
+  float *hit_t, *hit_u, *hit_v;
+  float *T, *L, *A;   // path state by slot: T,L 3 planes of cap each; A alpha
+  float *partial;     // frames x W x H x 4
+  uint32_t cap;
+};
+
+struct FrameParams {
+  int32_t width, height, frame0, frames;
+  uint32_t offset;
+  int32_t vinvert;
+  float inv_P[16], inv_V[16];
+};
+
+__device__ __forceinline__ uint32_t pixel_offset(const FrameParams &fp, uint32_t p) {
+  const uint32_t x = p % (uint32_t)fp.width, y = p / (uint32_t)fp.width;
+  return fp.vinvert ? (((uint32_t)fp.height - 1 - y) * (uint32_t)fp.width + x) * 4u : (y * (uint32_t)fp.width + x) * 4u;
+}
+
+__device__ __forceinline__ uint32_t sample_index(const FrameParams &fp, uint32_t slot) {
+  const uint32_t npix = (uint32_t)fp.width * (uint32_t)fp.height;
+  const uint32_t f = (uint32_t)fp.frame0 + slot / npix;
+  return pixel_offset(fp, slot % npix) * f + fp.offset;  // Integrator.h:80, u32 wraparound
+}
+
+// AbstractIntegrator::render per pixel (Integrator.h:53-95) + ray_inv_mat (math_camera.h:46-58)
+__global__ __launch_bounds__(256) void generate_kernel(FrameParams fp, const uint32_t *__restrict__ sobol, Queues q, uint32_t active) {
+  const uint32_t n = q.cap;  // plane stride
+  for (uint32_t slot = blockIdx.x * blockDim.x + threadIdx.x; slot < active; slot += gridDim.x * blockDim.x) {
+    const uint32_t npix = (uint32_t)fp.width * (uint32_t)fp.height;
+    const uint32_t p = slot % npix;
+    const int x = (int)(p % (uint32_t)fp.width), y = (int)(p / (uint32_t)fp.width);
+    const uint32_t idx = sample_index(fp, slot);
+    const uint32_t g = idx ^ (idx >> 1);
+    const float s0 = u32_to_unit(owen_scramble(sobol_xor(sobol + 0 * 32, g)));
+    const float s1 = u32_to_unit(owen_scramble(sobol_xor(sobol + 1 * 32, g)));
+    const float dx = s0 * 0.0005f, dy = s1 * 0.0005f;
+    const int sy = fp.height - y;
+    const float nx = (float)(2 * x - fp.width) / (float)fp.width + dx;
+    const float ny = (float)(fp.height - 2 * sy) / (float)fp.height + dy;
+    const float *P = fp.inv_P, *V = fp.inv_V;
+    float ox = mat4_row(P, 0, nx, ny, -1.f, 1.f), oy = mat4_row(P, 1, nx, ny, -1.f, 1.f), oz = mat4_row(P, 2, nx, ny, -1.f, 1.f),
+          ow = mat4_row(P, 3, nx, ny, -1.f, 1.f);
+    ox = ox / ow;
+    oy = oy / ow;
+    oz = oz / ow;
+    ow = ow / ow;
+    const float wx = mat4_row(V, 0, ox, oy, oz, 1.f), wy = mat4_row(V, 1, ox, oy, oz, 1.f), wz = mat4_row(V, 2, ox, oy, oz, 1.f);
+    const float ddx = mat4_row(P, 0, nx, ny, -1.f, 0.f), ddy = mat4_row(P, 1, nx, ny, -1.f, 0.f);
+    float vx = mat4_row(V, 0, ddx, ddy, -1.f, 0.f), vy = mat4_row(V, 1, ddx, ddy, -1.f, 0.f), vz = mat4_row(V, 2, ddx, ddy, -1.f, 0.f),
+          vw = mat4_row(V, 3, ddx, ddy, -1.f, 0.f);
+    const float xx = vx * vx, yy = vy * vy, zz = vz * vz, ww = vw * vw;
+    const float inv_len = 1.f / sqrtf((xx + yy) + (zz + ww));  // glm::normalize(vec4)
+    vx = vx * inv_len;
+    vy = vy * inv_len;
+    vz = vz * inv_len;
+    float *r = q.ray[0];
+    r[0 * (size_t)n + slot] = wx;
+    r[1 * (size_t)n + slot] = wy;
+    r[2 * (size_t)n + slot] = wz;
+    r[3 * (size_t)n + slot] = vx;
+    r[4 * (size_t)n + slot] = vy;
+    r[5 * (size_t)n + slot] = vz;
+    q.slot[0][slot] = (int32_t)slot;
+    q.T[slot] = 1.f;
+    q.T[(size_t)n + slot] = 1.f;
+    q.T[2 * (size_t)n + slot] = 1.f;
+    q.L[slot] = 0.f;
+    q.L[(size_t)n + slot] = 0.f;
+    q.L[2 * (size_t)n + slot] = 0.f;
+    q.A[slot] = 0.f;
+  }
+  if (blockIdx.x == 0 && threadIdx.x == 0) {
+    q.count[0] = active;
+    q.count[1] = 0;
+  }
+}
+
+__global__ __launch_bounds__(kTraceBlock) void extend_kernel(TraceCtx tc, Queues q, int which) {
+  __shared__ uint32_t lds[kLdsStack * kTraceBlock];
+  const uint32_t n = q.count[which];
+  const float *r = q.ray[which];
+  const size_t cap = q.cap;
+  const uint32_t gtid = blockIdx.x * blockDim.x + threadIdx.x;
+  if (gtid == 0) q.count[2] += n;
+  for (uint32_t i = gtid; i < n; i += gridDim.x * blockDim.x) {
+    const f3 o = mk3(r[i], r[cap + i], r[2 * cap + i]);
+    const f3 d = mk3(r[3 * cap + i], r[4 * cap + i], r[5 * cap + i]);
+    float t, u, v;
+    const int prim = trace_closest(tc, lds, gtid, o, d, 0.000001f, t, u, v);
+    q.hit_prim[i] = prim;
+    q.hit_t[i] = t;
+    q.hit_u[i] = u;
+    q.hit_v[i] = v;
+  }
+}
+
+struct ShadeParams {
+  FrameParams fp;
+  int32_t bounce, last, rr;  // rr: this bounce rolls Russian roulette
+  uint32_t dim0;             // first sampler dimension of this scatter
+};
+
+__device__ __forceinline__ void write_partial(const Queues &q, const FrameParams &fp, uint32_t slot, float lx, float ly, float lz, float a) {
+  const uint32_t npix = (uint32_t)fp.width * (uint32_t)fp.height;
+  const uint32_t f = slot / npix;
+  float4 *dst = reinterpret_cast<float4 *>(q.partial + (size_t)f * npix * 4 + pixel_offset(fp, slot % npix));
+  *dst = make_float4(lx, ly, lz, a);
+}
+
+__global__ __launch_bounds__(256) void shade_kernel(ShadeParams sp, ShadeCtx sc, const uint32_t *__restrict__ sobol, Queues q, int which) {
+  const uint32_t n = q.count[which];
+  const size_t cap = q.cap;
+  const float *rin = q.ray[which];
+  float *rout = q.ray[which ^ 1];
+  int32_t *sout = q.slot[which ^ 1];
+  const uint32_t *dv0 = sobol + (size_t)sp.dim0 * 32;
+  const uint32_t stride = gridDim.x * blockDim.x;
+  // every lane of a wave runs the same number of iterations so the ballot below is uniform
+  const uint32_t iters = (n + stride - 1) / stride;
+  for (uint32_t it = 0; it < iters; it++) {
+    const uint32_t i = it * stride + blockIdx.x * blockDim.x + threadIdx.x;
+    bool cont = false;
+    f3 no = mk3(0, 0, 0), nd = mk3(0, 0, 0);
+    int32_t slot = -1;
+    if (i < n) {
+      slot = q.slot[which][i];
+      const f3 o = mk3(rin[i], rin[cap + i], rin[2 * cap + i]);
+      const f3 d = mk3(rin[3 * cap + i], rin[4 * cap + i], rin[5 * cap + i]);
+      f3 T = mk3(q.T[slot], q.T[cap + slot], q.T[2 * cap + slot]);
+      f3 L = mk3(q.L[slot], q.L[cap + slot], q.L[2 * cap + slot]);
+      float A = q.A[slot];
+      const int prim = q.hit_prim[i];
+      bool done = true;
+      if (prim < 0) {
+        const float4 env = sample_envmap(sc, d);
+        L = L + T * mk3(env.x, env.y, env.z);
+        A = A + env.w;
+      } else {
+        const ShadeTri tr = sc.shade[prim];
+        const HitRec h = make_hit(sc, tr, o, d, q.hit_t[i], q.hit_u[i], q.hit_v[i]);
+        const MatRec mat = sc.mats[tr.material];
+        const float ONE_255 = 1.f / 255.f;
+        // PrincipledMaterial::scatter (NovaMaterials.h:89-137)
+        f3 nrm = mk3(0.f, 0.f, 1.f);
+        if (mat.normal >= 0) {
+          float4 t4 = sample_u32(sc, mat.normal, h.u, h.v);
+          nrm = mk3(t4.x * ONE_255, t4.y * ONE_255, t4.z * ONE_255);
+        }
+        const f3 ns = normalize(mul(h.frame, nrm * 2.f - 1.f));
+        const f3 wo = normalize(-d);
+        BsdfIn bi;
+        bi.ns = ns;
+        bi.ng = h.ng;
+        bi.dpdu = h.dpdu;
+        bi.dpdv = h.dpdv;
+        bi.wo_dot_ng = dot(wo, h.ng);
+        if (mat.albedo_f32 >= 0) {
+          float4 a4 = sample_f32_bilinear(sc, mat.albedo_f32, h.u, h.v);
+          bi.albedo = mk3(a4.x, a4.y, a4.z);
+        } else if (mat.albedo >= 0) {
+          float4 a4 = sample_u32(sc, mat.albedo, h.u, h.v);
+          bi.albedo = mk3(a4.x * ONE_255, a4.y * ONE_255, a4.z * ONE_255);
+        } else {
+          bi.albedo = mk3(1.f, 1.f, 1.f);
+        }
+        f3 met = mk3(0.f, 0.f, 0.f);
+        if (mat.metallic >= 0) {
+          float4 m4 = sample_u32(sc, mat.metallic, h.u, h.v);
+          met = mk3(m4.x * ONE_255, m4.y * ONE_255, m4.z * ONE_255);
+        }
+        bi.roughness = met.y;
+        if (mat.roughness_f32 >= 0) bi.roughness = sample_f32_bilinear(sc, mat.roughness_f32, h.u, h.v).x;
+        bi.metal = met.z;
+        const bool any_eta = mat.eta[0] != 0.f || mat.eta[1] != 0.f || mat.eta[2] != 0.f;
+        const bool any_k = mat.k[0] != 0.f || mat.k[1] != 0.f || mat.k[2] != 0.f;
+        bi.transmission = (any_eta && !any_k) ? 1.f : 0.f;
+        bi.anisotropy = mat.anisotropy;
+        bi.eta = mk3(mat.eta[0], mat.eta[1], mat.eta[2]);
+
+        // sampleBSDF (NovaMaterials.h:63-79): dims dim0 (lobe), dim0+1..2 (u0), dim0+3 (uc)
+        const uint32_t idx = sample_index(sp.fp, (uint32_t)slot);
+        const uint32_t g = idx ^ (idx >> 1);
+        const float uc = u32_to_unit(owen_scramble(sobol_xor(dv0, g)));
+        const float u0 = u32_to_unit(owen_scramble(sobol_xor(dv0 + 32, g)));
+        const float u1 = u32_to_unit(owen_scramble(sobol_xor(dv0 + 64, g)));
+        Principled b;
+        b.configure(bi);
+        b.construct(uc);
+        const VNDF ggx(b.anisotropy, b.roughness);
+        if (b.flags(ggx) & (BX_GLOSSY | BX_REFLECTION)) {
+          if (bi.wo_dot_ng >= 0) {
+            const f3 nsc = normalize(correct_specular_shading_normal(bi.ng, wo, bi.ns));
+            bi.ns = nsc;
+            const int lobe = b.lobe;
+            const float lw = b.lobe_weight, lp = b.lobe_prob;
+            b.configure(bi);
+            b.lobe = lobe;
+            b.lobe_weight = lw;
+            b.lobe_prob = lp;
+          }
+        }
+        const float uc2 = u32_to_unit(owen_scramble(sobol_xor(dv0 + 96, g)));
+        // PrincipledBSDF::sample_f (BSDF.h:107-130)
+        BsdfSample smp;
+        bool ok = false;
+        if (b.lobe != LOBE_NONE) {
+          const f3 lwo = normalize(mul_t(b.frame, wo));
+          if ((b.flags(ggx) & (BX_REFLECTION | BX_TRANSMISSION)) && lwo.z != 0.f) {
+            if (b.lobe == LOBE_DIFFUSE)
+              ok = sample_disney(b, lwo, u0, u1, smp);
+            else if (b.lobe == LOBE_SPECULAR)
+              ok = sample_specular(b, ggx, lwo, u0, u1, smp);
+            else
+              ok = sample_dielectric(b, ggx, lwo, uc2, u0, u1, smp);
+            if (ok && ((smp.f.x == 0.f && smp.f.y == 0.f && smp.f.z == 0.f) || smp.pdf == 0.f || smp.wi.z == 0.f)) ok = false;
+            if (ok) smp.wi = normalize(mul(b.frame, smp.wi));
+          }
+        }
+        if (ok) {
+          f3 c = smp.f * smp.costheta / smp.pdf;
+          if (isfinite(c.x) && isfinite(c.y) && isfinite(c.z)) {
+            f3 Le = mk3(0.f, 0.f, 0.f);
+            if (mat.emissive >= 0) {
+              float4 e4 = sample_u32(sc, mat.emissive, h.u, h.v);
+              Le = mk3(e4.x * ONE_255, e4.y * ONE_255, e4.z * ONE_255) * mat.emission_scale;
+            }
+            bool survive = true;
+            if (sp.rr) {  // Russian-roulette extension: one more dimension after the scatter's four
+              const f3 tp = T * c;
+              const float qv = fminf(1.f, fmaxf(tp.x, fmaxf(tp.y, tp.z)));
+              const float ur = u32_to_unit(owen_scramble(sobol_xor(dv0 + 128, g)));
+              if (!(ur < qv))
+                survive = false;
+              else
+                c = c / qv;
+            }
+            L = L + T * Le;
+            A = A + 1.f;
+            if (survive) {
+              T = T * c;
+              // Ray::spawn (Ray.h:22-27)
+              const f3 wi = smp.wi;
+              nd = normalize(wi);
+              no = h.pos + h.ng * (1e-3f + 1e-3f * gsign(dot(h.ng, wi)));
+              if (sp.last) {  // depth budget spent: V = envmap(direction) without traversal
+                const float4 env = sample_envmap(sc, nd);
+                L = L + T * mk3(env.x, env.y, env.z);
+                A = A + env.w;
+              } else {
+                done = false;
+                cont = true;
+              }
+            }
+          }
+        }
+      }
+      q.T[slot] = T.x;
+      q.T[cap + slot] = T.y;
+      q.T[2 * cap + slot] = T.z;
+      if (done) {
+        write_partial(q, sp.fp, (uint32_t)slot, L.x, L.y, L.z, A);
+      } else {
+        q.L[slot] = L.x;
+        q.L[cap + slot] = L.y;
+        q.L[2 * cap + slot] = L.z;
+        q.A[slot] = A;
+      }
+    }
+    // compact survivors into the next queue
+    const uint64_t m = __ballot(cont);
+    const uint32_t lane_rank = __builtin_amdgcn_mbcnt_hi((uint32_t)(m >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)m, 0));
+    uint32_t base = 0;
+    const int first = __builtin_ffsll((long long)m) - 1;
+    if (m != 0) {
+      if ((int)(threadIdx.x & 63) == first) base = atomicAdd(&q.count[which ^ 1], (uint32_t)__popcll(m));
+      base = __shfl(base, first, 64);
+    }
+    if (cont) {
+      const uint32_t j = base + lane_rank;
+      rout[j] = no.x;
+      rout[cap + j] = no.y;
+      rout[2 * cap + j] = no.z;
+      rout[3 * cap + j] = nd.x;
+      rout[4 * cap + j] = nd.y;
+      rout[5 * cap + j] = nd.z;
+      sout[j] = slot;
+    }
+  }
+}
+
+__global__ void reset_count_kernel(uint32_t *count, int which) { count[which] = 0; }
+
+// Engine::writeFramebuffer per frame, in frame order (engine.cpp:351-363)
+__global__ void accumulate_kernel(const float *__restrict__ partial, int frames, int frame0, size_t npix4, float *__restrict__ accum,
+                                  float *__restrict__ final_out, float *__restrict__ partial_out) {
+  for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < npix4; i += (size_t)gridDim.x * blockDim.x) {
+    float a = accum[i];
+    float fin = 0.f;
+    for (int f = 0; f < frames; f++) {
+      a += partial[(size_t)f * npix4 + i];
+      fin = a / float(frame0 + f + 1);
+    }
+    accum[i] = a;
+    final_out[i] = fin;
+    if (partial_out) partial_out[i] = partial[(size_t)(frames - 1) * npix4 + i];
+  }
+}
+
+// closest-hit entry point: rays n x 6 (AoS)
+__global__ __launch_bounds__(kTraceBlock) void closest_hit_kernel(TraceCtx tc, const float *__restrict__ rays, uint64_t n, int32_t *__restrict__ prim,
+                                                                  float *__restrict__ tuv) {
+  __shared__ uint32_t lds[kLdsStack * kTraceBlock];
+  const uint32_t gtid = blockIdx.x * blockDim.x + threadIdx.x;
+  for (uint64_t i = gtid; i < n; i += (uint64_t)gridDim.x * blockDim.x) {
+    const f3 o = mk3(rays[i * 6], rays[i * 6 + 1], rays[i * 6 + 2]);
+    const f3 d = mk3(rays[i * 6 + 3], rays[i * 6 + 4], rays[i * 6 + 5]);
+    float t, u, v;
+    const int p = trace_closest(tc, lds, gtid, o, d, 0.000001f, t, u, v);
+    prim[i] = p;
+    tuv[i * 3] = p >= 0 ? t : 0.f;
+    tuv[i * 3 + 1] = u;
+    tuv[i * 3 + 2] = v;
+  }
+}
+
+static int device_cus(nova_hip_ctx *ctx) {
+  int cus = 0;
+  if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, ctx->device) != hipSuccess || cus <= 0) cus = 256;
+  return cus;
+}
+
+static int ensure_gstack(nova_hip_ctx *ctx, uint32_t threads) {
+  const uint32_t bound = traversal_stack_bound(ctx);
+  const size_t need = bound > (uint32_t)kLdsStack ? (size_t)(bound - kLdsStack) * threads : 1;
+  if (ctx->gstack && ctx->gstack_entries >= need) return NOVA_HIP_SUCCESS;
+  if (ctx->gstack) hipFree(ctx->gstack);
+  ctx->gstack = nullptr;
+  ctx->gstack_entries = 0;
+  NOVA_HIP_CHECK(ctx, hipMalloc(&ctx->gstack, need * sizeof(uint32_t)));
+  ctx->gstack_entries = need;
+  return NOVA_HIP_SUCCESS;
+}
+
+}  // namespace nova_hip
+
+using namespace nova_hip;
+
+extern "C" int nova_hip_closest_hit(nova_hip_ctx *ctx, const float *rays_dev, uint64_t n, int32_t *prim_dev, float *tuv_dev, void *stream) {
+  if (!ctx) return NOVA_HIP_INVALID_ARGUMENT;
+  if (!ctx->bvh_ready) return set_error(ctx, NOVA_HIP_SCENE_NOT_PROCESSED, "BVH not built");
+  if (n == 0) return NOVA_HIP_SUCCESS;
+  if (!rays_dev || !prim_dev || !tuv_dev) return set_error(ctx, NOVA_HIP_INVALID_ARGUMENT, "null buffer");
+  NOVA_HIP_CHECK(ctx, hipSetDevice(ctx->device));
+  const uint32_t blocks = (uint32_t)std::min<uint64_t>((n + kTraceBlock - 1) / kTraceBlock, (uint64_t)device_cus(ctx) * 8);
+  const uint32_t threads = blocks * kTraceBlock;
+  int st = ensure_gstack(ctx, threads);
+  if (st) return st;
+  TraceCtx tc{ctx->nodes4, ctx->tri_leaf, ctx->gstack, threads};
+  hipLaunchKernelGGL(closest_hit_kernel, dim3(blocks), dim3(kTraceBlock), 0, (hipStream_t)stream, tc, rays_dev, n, prim_dev, tuv_dev);
+  NOVA_HIP_CHECK(ctx, hipGetLastError());
+  return NOVA_HIP_SUCCESS;
+}
+
+extern "C" int nova_hip_render_frames(nova_hip_ctx *ctx, const nova_hip_render_params *p, float *accum_dev, float *final_dev, float *partial_dev,
+                                      void *stream) {
+  if (!ctx || !p) return NOVA_HIP_INVALID_ARGUMENT;
+  if (!ctx->bvh_ready) return set_error(ctx, NOVA_HIP_SCENE_NOT_PROCESSED, "scene/BVH not built");
+  if (p->width <= 0 || p->height <= 0 || p->width > 65536 || p->height > 65536)
+    return set_error(ctx, NOVA_HIP_INVALID_RENDERBUFFER_DIM, "Wrong buffer width / height dimension.");
+  if (p->frame_count <= 0 || p->frame_start < 0 || p->max_depth < 0) return set_error(ctx, NOVA_HIP_INVALID_ARGUMENT, "bad frame range or depth");
+  if (!accum_dev || !final_dev) return set_error(ctx, NOVA_HIP_INVALID_BUFFER_STATE, "null render buffer");
+  // sampler dimensions: 2 + sum over scatters of (4 + RR roll) must stay inside the table
+  {
+    uint32_t dims = 2;
+    for (int k = 0; k <= p->max_depth; k++) dims += 4 + ((p->rr_start >= 0 && k >= p->rr_start) ? 1 : 0);
+    if (dims > 256) return set_error(ctx, NOVA_HIP_INVALID_ARGUMENT, "max_depth needs more than 256 sampler dimensions");
+  }
+  hipStream_t s = (hipStream_t)stream;
+  NOVA_HIP_CHECK(ctx, hipSetDevice(ctx->device));
+  const size_t npix = (size_t)p->width * p->height;
+  // frames per batch: enough paths to fill the chip, bounded by memory
+  int fb = (int)std::max<size_t>(1, std::min<size_t>((size_t)p->frame_count, (4u << 20) / npix));
+  const size_t cap = npix * fb;
+  if (cap >= (1u << 31)) return set_error(ctx, NOVA_HIP_INVALID_ARGUMENT, "image too large");
+  hipError_t e;
+  Queues q;
+  q.cap = (uint32_t)cap;
+  float *rays = (float *)ensure_scratch(ctx, 0, cap * 12 * sizeof(float), &e);
+  if (!rays) return check_hip(ctx, e, "queue alloc");
+  int32_t *slots = (int32_t *)ensure_scratch(ctx, 1, cap * 2 * sizeof(int32_t), &e);
+  if (!slots) return check_hip(ctx, e, "queue alloc");
+  uint32_t *counts = (uint32_t *)ensure_scratch(ctx, 2, 64, &e);
+  if (!counts) return check_hip(ctx, e, "counter alloc");
+  float *hits = (float *)ensure_scratch(ctx, 3, cap * 4 * sizeof(float), &e);
+  if (!hits) return check_hip(ctx, e, "hit alloc");
+  float *state = (float *)ensure_scratch(ctx, 4, cap * 7 * sizeof(float), &e);
+  if (!state) return check_hip(ctx, e, "state alloc");
+  float *partial = (float *)ensure_scratch(ctx, 5, cap * 4 * sizeof(float), &e);
+  if (!partial) return check_hip(ctx, e, "partial alloc");
+  q.ray[0] = rays;
+  q.ray[1] = rays + cap * 6;
+  q.slot[0] = slots;
+  q.slot[1] = slots + cap;
+  q.count = counts;
+  q.hit_prim = (int32_t *)hits;
+  q.hit_t = hits + cap;
+  q.hit_u = hits + 2 * cap;
+  q.hit_v = hits + 3 * cap;
+  q.T = state;
+  q.L = state + 3 * cap;
+  q.A = state + 6 * cap;
+  q.partial = partial;
+
+  const int cus = device_cus(ctx);
+  const uint32_t trace_blocks = (uint32_t)std::min<size_t>((cap + kTraceBlock - 1) / kTraceBlock, (size_t)cus * 8);
+  const uint32_t trace_threads = trace_blocks * kTraceBlock;
+  int st = ensure_gstack(ctx, trace_threads);
+  if (st) return st;
+  TraceCtx tc{ctx->nodes4, ctx->tri_leaf, ctx->gstack, trace_threads};
+  ShadeCtx sc{ctx->textures, ctx->tex_u32, ctx->tex_f32, ctx->materials, ctx->mesh_xf, ctx->shade, ctx->acos_lut, ctx->envmap};
+  const uint32_t shade_blocks = (uint32_t)std::min<size_t>((cap + 255) / 256, (size_t)cus * 8);
+  NOVA_HIP_CHECK(ctx, hipMemsetAsync(counts, 0, 64, s));
+
+  hipEvent_t ev0, ev1;
+  NOVA_HIP_CHECK(ctx, hipEventCreate(&ev0));
+  NOVA_HIP_CHECK(ctx, hipEventCreate(&ev1));
+  NOVA_HIP_CHECK(ctx, hipEventRecord(ev0, s));
+  uint64_t launches = 0;
+  for (int f0 = p->frame_start; f0 < p->frame_start + p->frame_count; f0 += fb) {
+    const int frames = std::min(fb, p->frame_start + p->frame_count - f0);
+    FrameParams fp;
+    fp.width = p->width;
+    fp.height = p->height;
+    fp.frame0 = f0;
+    fp.frames = frames;
+    fp.offset = p->max_samples_offset;
+    fp.vinvert = p->vertical_invert;
+    for (int i = 0; i < 16; i++) {
+      fp.inv_P[i] = p->inv_P[i];
+      fp.inv_V[i] = p->inv_V[i];
+    }
+    const uint32_t active = (uint32_t)(npix * frames);
+    hipLaunchKernelGGL(generate_kernel, dim3(std::min<uint32_t>((active + 255) / 256, (uint32_t)cus * 8)), dim3(256), 0, s, fp, ctx->sobol_table, q,
+                       active);
+    NOVA_HIP_CHECK(ctx, hipGetLastError());
+    uint32_t dim = 2;
+    int which = 0;
+    for (int k = 0; k <= p->max_depth; k++) {
+      hipLaunchKernelGGL(extend_kernel, dim3(trace_blocks), dim3(kTraceBlock), 0, s, tc, q, which);
+      hipLaunchKernelGGL(reset_count_kernel, dim3(1), dim3(1), 0, s, counts, which ^ 1);
+      ShadeParams spp;
+      spp.fp = fp;
+      spp.bounce = k;
+      spp.last = (k == p->max_depth) ? 1 : 0;
+      spp.rr = (p->rr_start >= 0 && k >= p->rr_start) ? 1 : 0;
+      spp.dim0 = dim;
+      hipLaunchKernelGGL(shade_kernel, dim3(shade_blocks), dim3(256), 0, s, spp, sc, ctx->sobol_table, q, which);
+      NOVA_HIP_CHECK(ctx, hipGetLastError());
+      dim += 4 + (spp.rr ? 1 : 0);
+      which ^= 1;
+      launches++;
+    }
+    hipLaunchKernelGGL(accumulate_kernel, dim3(std::min<size_t>((npix * 4 + 255) / 256, (size_t)cus * 16)), dim3(256), 0, s, partial, frames, f0,
+                       npix * 4, accum_dev, final_dev,
+                       (f0 + frames == p->frame_start + p->frame_count) ? partial_dev : (float *)nullptr);
+    NOVA_HIP_CHECK(ctx, hipGetLastError());
+  }
+  NOVA_HIP_CHECK(ctx, hipEventRecord(ev1, s));
+  NOVA_HIP_CHECK(ctx, hipEventSynchronize(ev1));
+  float ms = 0.f;
+  hipEventElapsedTime(&ms, ev0, ev1);
+  hipEventDestroy(ev0);
+  hipEventDestroy(ev1);
+  uint32_t seg = 0;
+  NOVA_HIP_CHECK(ctx, hipMemcpy(&seg, counts + 2, sizeof(uint32_t), hipMemcpyDeviceToHost));
+  ctx->stats.total_ms = ms;
+  ctx->stats.segments = seg;
+  ctx->stats.extend_launches = launches;
+  return NOVA_HIP_SUCCESS;
+}
+
+extern "C" int nova_hip_render_stats_get(nova_hip_ctx *ctx, nova_hip_render_stats *out) {
+  if (!ctx || !out) return NOVA_HIP_INVALID_ARGUMENT;
+  *out = ctx->stats;
+  return NOVA_HIP_SUCCESS;
+}

@@ -1,0 +1,125 @@
+// Internal layouts of the HIP backend (host + device).
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include <string>
+#include <vector>
+
+#include "../../include/nova_hip.h"
+
+namespace nova_hip {
+
+// Traversal record, 48 B, in BVH leaf order: (v0, prim id) (e1, 0) (e2, 0).
+// e1 = v1 - v0, e2 = v2 - v0 are the edges Triangle::hit forms (Triangle.h:158-160); a
+// triangle with repeated indices gets zero edges so that det == 0 rejects it, as
+// Triangle::isDegenerate does.
+struct alignas(16) TriMT {
+  float4 a, b, c;
+};
+
+// Shading record, 192 B, in global primitive order: object-space corners and the
+// per-vertex attributes the Embree-path hit record interpolates (embree.cpp:287-390).
+struct alignas(16) ShadeTri {
+  float p0[3], p1[3], p2[3];
+  float uv0[2], uv1[2], uv2[2];
+  float n0[3], n1[3], n2[3];
+  float t0[3], t1[3], t2[3];
+  float b0[3], b1[3], b2[3];
+  int32_t material, mesh;
+  int32_t pad[2];
+};
+static_assert(sizeof(ShadeTri) == 192, "ShadeTri layout");
+
+// Per-mesh 3x3 blocks: mat3(transform.m) and the normal matrix transpose(inverse(mat3(m))),
+// column-major.
+struct MeshXf {
+  float m3[9];
+  float n3[9];
+};
+
+struct MatRec {
+  int32_t albedo, metallic, normal, emissive, albedo_f32, roughness_f32;
+  float eta[3], k[3];
+  float anisotropy, emission_scale;
+};
+
+struct TexRec {
+  int32_t width, height, channels, invert_u, invert_v, format;
+  uint64_t offset;  // element offset into the u32 or f32 texel pool
+};
+
+// Binary SAH node with the reference layout (Bvhnl, BvhBuilder.h:10-15).
+struct BNode {
+  float mn[3], mx[3];
+  int32_t left, count;
+};
+
+// 4-wide node, 128 B: child boxes SoA, then per-child reference. count[k] == 0: child[k] is
+// an inner node index; count[k] > 0: a leaf of triangles [child[k], child[k]+count[k]) in the
+// TriMT array; child[k] == -1: empty slot.
+struct alignas(16) Node4 {
+  float minx[4], maxx[4], miny[4], maxy[4], minz[4], maxz[4];
+  int32_t child[4];
+  int32_t count[4];
+};
+static_assert(sizeof(Node4) == 128, "Node4 layout");
+
+struct DeviceBuffer {
+  void *ptr = nullptr;
+  size_t bytes = 0;
+};
+
+}  // namespace nova_hip
+
+struct nova_hip_ctx {
+  int device = 0;
+  std::string last_error;
+
+  // scene
+  uint32_t prim_count = 0;
+  uint32_t mesh_count = 0;
+  nova_hip::TriMT *tri_world = nullptr;      // world-space MT records, primitive order
+  nova_hip::ShadeTri *shade = nullptr;       // shading records, primitive order
+  float *prim_aabb = nullptr;                // 6 floats per primitive (min3, max3)
+  float *prim_centroid = nullptr;            // 3 floats per primitive
+  nova_hip::MeshXf *mesh_xf = nullptr;
+  nova_hip::MatRec *materials = nullptr;
+  nova_hip::TexRec *textures = nullptr;
+  uint32_t *tex_u32 = nullptr;
+  float *tex_f32 = nullptr;
+  int32_t envmap = -1;
+  float *acos_lut = nullptr;
+  uint32_t *sobol_table = nullptr;
+  std::vector<nova_hip::TexRec> host_textures;
+  bool scene_ready = false;
+
+  // BVH
+  int32_t *prim_idx = nullptr;
+  nova_hip::BNode *bnodes = nullptr;
+  uint32_t bnode_count = 0;
+  nova_hip::Node4 *nodes4 = nullptr;
+  nova_hip::TriMT *tri_leaf = nullptr;       // MT records in leaf order (prim id in a.w)
+  nova_hip_bvh_info bvh_info{};
+  bool bvh_ready = false;
+
+  // integrator scratch (grown on demand)
+  nova_hip::DeviceBuffer scratch[16];
+  uint32_t *gstack = nullptr;                // overflow traversal stacks
+  size_t gstack_entries = 0;
+  nova_hip_render_stats stats{};
+};
+
+namespace nova_hip {
+// error helpers shared by the translation units
+int set_error(nova_hip_ctx *ctx, int code, const std::string &msg);
+int check_hip(nova_hip_ctx *ctx, hipError_t e, const char *what);
+void *ensure_scratch(nova_hip_ctx *ctx, int slot, size_t bytes, hipError_t *err);
+uint32_t traversal_stack_bound(const nova_hip_ctx *ctx);
+}  // namespace nova_hip
+
+#define NOVA_HIP_CHECK(ctx, expr)                                   \
+  do {                                                              \
+    hipError_t _e = (expr);                                         \
+    if (_e != hipSuccess) return nova_hip::check_hip(ctx, _e, #expr); \
+  } while (0)

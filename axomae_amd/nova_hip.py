@@ -1,0 +1,273 @@
+"""ctypes binding of libnova_hip.so (the C ABI declared in include/nova_hip.h).
+
+Host-side mirror of the reference's engine calls for the device path: a
+:class:`HipEngine` uploads a scene (Engine::buildScene), builds the acceleration
+structure (Engine::buildAcceleration) and renders frames into device RGBA32F
+buffers (Engine::renderGpu + writeFramebuffer). torch provides device memory and
+streams. The library must be built (`python -m axomae_amd.build`); nothing here
+falls back to a CPU path: a missing library or device raises.
+"""
+import ctypes as C
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libnova_hip.so")
+
+ERRORS = {0: "SUCCESS", 1: "INVALID_ENGINE_STATE", 2: "INVALID_BUFFER_STATE", 4: "INVALID_SCENE_TYPE", 5: "INVALID_TRANSFORM_TYPE",
+          9: "SCENE_NOT_PROCESSED", 11: "OUT_OF_MEMORY", 15: "INVALID_ARGUMENT", 16: "RENDERER_EXCEPTION", 100: "DEVICE_ERROR",
+          101: "INVALID_RENDERBUFFER_DIM"}
+
+EXPORTS = ["nova_hip_create", "nova_hip_destroy", "nova_hip_last_error", "nova_hip_scene_upload", "nova_hip_bvh_build", "nova_hip_bvh_read",
+           "nova_hip_closest_hit", "nova_hip_render_frames", "nova_hip_render_stats_get", "nova_hip_sobol_fill", "nova_hip_bake_irradiance",
+           "nova_hip_bake_prefilter"]
+
+
+class NovaHipError(RuntimeError):
+    def __init__(self, code, msg):
+        super().__init__(f"{ERRORS.get(code, code)}: {msg}")
+        self.code = code
+
+
+class MeshDesc(C.Structure):
+    _fields_ = [("vertices", C.c_void_p), ("normals", C.c_void_p), ("tangents", C.c_void_p), ("bitangents", C.c_void_p), ("uv", C.c_void_p),
+                ("indices", C.c_void_p), ("vertex_count", C.c_uint64), ("index_count", C.c_uint64), ("transform", C.c_float * 16),
+                ("material", C.c_int32)]
+
+
+class MaterialDesc(C.Structure):
+    _fields_ = [("albedo", C.c_int32), ("metallic", C.c_int32), ("normal", C.c_int32), ("emissive", C.c_int32), ("albedo_f32", C.c_int32),
+                ("roughness_f32", C.c_int32), ("eta", C.c_float * 3), ("k", C.c_float * 3), ("anisotropy", C.c_float),
+                ("emission_scale", C.c_float)]
+
+
+class TextureDesc(C.Structure):
+    _fields_ = [("width", C.c_int32), ("height", C.c_int32), ("channels", C.c_int32), ("invert_u", C.c_int32), ("invert_v", C.c_int32),
+                ("format", C.c_int32), ("data", C.c_void_p)]
+
+
+class SceneDesc(C.Structure):
+    _fields_ = [("meshes", C.POINTER(MeshDesc)), ("mesh_count", C.c_uint32), ("materials", C.POINTER(MaterialDesc)),
+                ("material_count", C.c_uint32), ("textures", C.POINTER(TextureDesc)), ("texture_count", C.c_uint32),
+                ("envmap_texture", C.c_int32)]
+
+
+class BvhInfo(C.Structure):
+    _fields_ = [("primitive_count", C.c_uint32), ("binary_nodes", C.c_uint32), ("binary_depth", C.c_uint32), ("wide_nodes", C.c_uint32),
+                ("wide_depth", C.c_uint32), ("leaves", C.c_uint32), ("build_ms", C.c_float)]
+
+
+class RenderParams(C.Structure):
+    _fields_ = [("width", C.c_int32), ("height", C.c_int32), ("max_depth", C.c_int32), ("frame_start", C.c_int32), ("frame_count", C.c_int32),
+                ("max_samples_offset", C.c_uint32), ("vertical_invert", C.c_int32), ("rr_start", C.c_int32), ("inv_P", C.c_float * 16),
+                ("inv_V", C.c_float * 16)]
+
+
+class RenderStats(C.Structure):
+    _fields_ = [("total_ms", C.c_float), ("extend_ms", C.c_float), ("shade_ms", C.c_float), ("other_ms", C.c_float), ("segments", C.c_uint64),
+                ("extend_launches", C.c_uint64)]
+
+
+BNODE_DTYPE = np.dtype([("min", np.float32, 3), ("max", np.float32, 3), ("left", np.int32), ("count", np.int32)])
+
+_LIB = None
+
+
+def load():
+    """Load libnova_hip.so; raises if it has not been built."""
+    global _LIB
+    if _LIB is not None:
+        return _LIB
+    if not os.path.exists(LIB_PATH):
+        raise ImportError(f"{LIB_PATH} is missing: build it with `python -m axomae_amd.build`")
+    L = C.CDLL(LIB_PATH)
+    P = C.c_void_p
+    L.nova_hip_create.argtypes = [C.c_int, C.POINTER(P)]
+    L.nova_hip_destroy.argtypes = [P]
+    L.nova_hip_last_error.argtypes = [P]
+    L.nova_hip_last_error.restype = C.c_char_p
+    L.nova_hip_scene_upload.argtypes = [P, C.POINTER(SceneDesc), P]
+    L.nova_hip_bvh_build.argtypes = [P, C.c_int, C.POINTER(BvhInfo), P]
+    L.nova_hip_bvh_read.argtypes = [P, P, P]
+    L.nova_hip_closest_hit.argtypes = [P, P, C.c_uint64, P, P, P]
+    L.nova_hip_render_frames.argtypes = [P, C.POINTER(RenderParams), P, P, P, P]
+    L.nova_hip_render_stats_get.argtypes = [P, C.POINTER(RenderStats)]
+    L.nova_hip_sobol_fill.argtypes = [P, P, C.c_uint64, P, P, P]
+    L.nova_hip_bake_irradiance.argtypes = [C.POINTER(P), P, P, P, C.c_uint32, C.POINTER(P), P, P, C.c_int32, P]
+    L.nova_hip_bake_prefilter.argtypes = [C.POINTER(P), P, P, P, C.c_uint32, C.POINTER(P), P, P, P, C.c_int32, P]
+    _LIB = L
+    return L
+
+
+def _torch():
+    import torch
+    if not torch.cuda.is_available():
+        raise RuntimeError("no HIP device visible: the Nova HIP path needs an MI355X")
+    return torch
+
+
+def _stream(torch):
+    return C.c_void_p(torch.cuda.current_stream().cuda_stream)
+
+
+def sobol_fill(idx, dims, return_bits=False):
+    """Device Owen-scrambled Sobol samples for (index, dim) pairs (torch uint32/int32 tensors on cuda)."""
+    torch = _torch()
+    L = load()
+    idx = idx.to(device="cuda", dtype=torch.int32).contiguous()
+    dims = dims.to(device="cuda", dtype=torch.int32).contiguous()
+    out = torch.empty(idx.shape, dtype=torch.float32, device="cuda")
+    bits = torch.empty(idx.shape, dtype=torch.int32, device="cuda") if return_bits else None
+    rc = L.nova_hip_sobol_fill(idx.data_ptr(), dims.data_ptr(), idx.numel(), out.data_ptr(), bits.data_ptr() if bits is not None else None,
+                               _stream(torch))
+    if rc:
+        raise NovaHipError(rc, "sobol_fill")
+    return (out, bits) if return_bits else out
+
+
+class HipEngine:
+    """One device context holding a scene, its BVH and render scratch."""
+
+    def __init__(self, device=0):
+        self.torch = _torch()
+        self.L = load()
+        self.ctx = C.c_void_p()
+        rc = self.L.nova_hip_create(device, C.byref(self.ctx))
+        if rc:
+            raise NovaHipError(rc, "nova_hip_create")
+        self.info = None
+        self.scene = None
+
+    def close(self):
+        if self.ctx:
+            self.L.nova_hip_destroy(self.ctx)
+            self.ctx = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def _check(self, rc, what):
+        if rc:
+            msg = self.L.nova_hip_last_error(self.ctx)
+            raise NovaHipError(rc, f"{what}: {msg.decode() if msg else ''}")
+
+    # ---------------------------------------------------------------- scene
+    def upload(self, scene):
+        keep = []
+
+        def ptr(a, dtype):
+            a = np.ascontiguousarray(a, dtype=dtype)
+            keep.append(a)
+            return a.ctypes.data
+
+        meshes = (MeshDesc * len(scene.meshes))()
+        for i, m in enumerate(scene.meshes):
+            d = meshes[i]
+            d.vertices = ptr(m.vertices, np.float32)
+            d.normals = ptr(m.normals, np.float32)
+            d.tangents = ptr(m.tangents, np.float32)
+            d.bitangents = ptr(m.bitangents, np.float32)
+            d.uv = ptr(m.uv, np.float32)
+            d.indices = ptr(m.indices, np.uint32)
+            d.vertex_count = len(m.vertices) // 3
+            d.index_count = len(m.indices)
+            d.transform[:] = [float(x) for x in np.asarray(m.transform, dtype=np.float32)]
+            d.material = m.material
+        mats = (MaterialDesc * max(1, len(scene.materials)))()
+        for i, m in enumerate(scene.materials):
+            d = mats[i]
+            d.albedo, d.metallic, d.normal, d.emissive = m.albedo, m.metallic, m.normal, m.emissive
+            d.albedo_f32, d.roughness_f32 = m.albedo_f32, m.roughness_f32
+            d.eta[:] = [float(x) for x in m.eta]
+            d.k[:] = [float(x) for x in m.k]
+            d.anisotropy = m.anisotropy
+            d.emission_scale = m.emission_scale
+        texs = (TextureDesc * max(1, len(scene.textures)))()
+        for i, t in enumerate(scene.textures):
+            d = texs[i]
+            d.width, d.height, d.channels = t.width, t.height, t.channels
+            d.invert_u, d.invert_v = int(t.invert_u), int(t.invert_v)
+            d.format = 1 if t.is_f32 else 0
+            d.data = ptr(t.data, np.float32 if t.is_f32 else np.uint32)
+        sd = SceneDesc(meshes, len(scene.meshes), mats, len(scene.materials), texs, len(scene.textures), scene.envmap)
+        self._check(self.L.nova_hip_scene_upload(self.ctx, C.byref(sd), _stream(self.torch)), "scene_upload")
+        self.scene = scene
+        self.prim_count = scene.triangle_count
+
+    def build_bvh(self):
+        info = BvhInfo()
+        self._check(self.L.nova_hip_bvh_build(self.ctx, 4, C.byref(info), _stream(self.torch)), "bvh_build")
+        self.info = info
+        return info
+
+    def bvh_arrays(self):
+        prim = np.empty(self.info.primitive_count, dtype=np.int32)
+        nodes = np.empty(self.info.binary_nodes, dtype=BNODE_DTYPE)
+        self._check(self.L.nova_hip_bvh_read(self.ctx, prim.ctypes.data, nodes.ctypes.data), "bvh_read")
+        return prim, nodes
+
+    def closest_hit(self, rays):
+        """rays: (n, 6) float32 array/tensor of origin+direction; returns (prim int32, tuv float32) tensors."""
+        torch = self.torch
+        r = torch.as_tensor(np.asarray(rays, dtype=np.float32)).reshape(-1, 6).to("cuda") if not torch.is_tensor(rays) else rays.reshape(-1, 6).to(
+            device="cuda", dtype=torch.float32).contiguous()
+        n = r.shape[0]
+        prim = torch.empty(n, dtype=torch.int32, device="cuda")
+        tuv = torch.empty((n, 3), dtype=torch.float32, device="cuda")
+        self._check(self.L.nova_hip_closest_hit(self.ctx, r.data_ptr(), n, prim.data_ptr(), tuv.data_ptr(), _stream(torch)), "closest_hit")
+        return prim, tuv
+
+    # --------------------------------------------------------------- render
+    def render(self, width, height, frames, max_depth, frame_start=0, rr_start=-1, max_samples_offset=0, vertical_invert=False, accum=None,
+               inv_P=None, inv_V=None):
+        """Renders frames [frame_start, frame_start+frames); returns (final, accum) device tensors (H, W, 4)."""
+        torch = self.torch
+        p = RenderParams()
+        p.width, p.height, p.max_depth = width, height, max_depth
+        p.frame_start, p.frame_count = frame_start, frames
+        p.max_samples_offset = max_samples_offset
+        p.vertical_invert = int(vertical_invert)
+        p.rr_start = rr_start
+        p.inv_P[:] = [float(x) for x in (self.scene.inv_P if inv_P is None else inv_P)]
+        p.inv_V[:] = [float(x) for x in (self.scene.inv_V if inv_V is None else inv_V)]
+        if accum is None:
+            accum = torch.zeros((height, width, 4), dtype=torch.float32, device="cuda")
+        final = torch.empty((height, width, 4), dtype=torch.float32, device="cuda")
+        self._check(self.L.nova_hip_render_frames(self.ctx, C.byref(p), accum.data_ptr(), final.data_ptr(), None, _stream(torch)), "render_frames")
+        return final, accum
+
+    def render_stats(self):
+        s = RenderStats()
+        self._check(self.L.nova_hip_render_stats_get(self.ctx, C.byref(s)), "render_stats")
+        return s
+
+
+def bake(kind, envmaps, out_sizes, samples, roughness=None):
+    """Batched irradiance ('irradiance') or prefilter ('prefilter') bake of FP32 envmaps; returns numpy arrays."""
+    torch = _torch()
+    L = load()
+    n = len(envmaps)
+    ins = [torch.as_tensor(np.ascontiguousarray(e.data, dtype=np.float32)).to("cuda") for e in envmaps]
+    outs = [torch.zeros(oh * ow * e.channels, dtype=torch.float32, device="cuda") for e, (ow, oh) in zip(envmaps, out_sizes)]
+    P = C.c_void_p
+    env_ptrs = (P * n)(*[t.data_ptr() for t in ins])
+    out_ptrs = (P * n)(*[t.data_ptr() for t in outs])
+    w = np.array([e.width for e in envmaps], dtype=np.int32)
+    h = np.array([e.height for e in envmaps], dtype=np.int32)
+    c = np.array([e.channels for e in envmaps], dtype=np.int32)
+    ow = np.array([s[0] for s in out_sizes], dtype=np.int32)
+    oh = np.array([s[1] for s in out_sizes], dtype=np.int32)
+    st = _stream(torch)
+    if kind == "irradiance":
+        rc = L.nova_hip_bake_irradiance(env_ptrs, w.ctypes.data, h.ctypes.data, c.ctypes.data, n, out_ptrs, ow.ctypes.data, oh.ctypes.data, samples, st)
+    else:
+        r = np.asarray(roughness, dtype=np.float32)
+        rc = L.nova_hip_bake_prefilter(env_ptrs, w.ctypes.data, h.ctypes.data, c.ctypes.data, n, out_ptrs, ow.ctypes.data, oh.ctypes.data,
+                                       r.ctypes.data, samples, st)
+    if rc:
+        raise NovaHipError(rc, f"bake {kind}")
+    return [o.cpu().numpy().reshape(s[1], s[0], e.channels) for o, s, e in zip(outs, out_sizes, envmaps)]

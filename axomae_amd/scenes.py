@@ -1,0 +1,361 @@
+"""Deterministic synthetic workloads for the Nova hot path.
+
+Each generator returns a :class:`Scene`: meshes with full vertex attributes
+(positions, normals, tangents, bitangents, UVs), Principled materials built from
+u32 texture packs (albedo, metallic = (_, roughness, metal), flat normal map,
+emission), an FP32 RGBA environment map and a camera given as inverse projection
+/ inverse view matrices (column-major, as the reference's camera holder stores
+them, engine.cpp:380-429).
+
+All arithmetic that defines geometry runs in float64 and is rounded once to
+float32, so the same seed gives the same scene on every machine. Transforms are
+identity: geometry is generated in world space.
+"""
+from dataclasses import dataclass, field
+import math
+import numpy as np
+
+FLAT_NORMAL = 0xFF8080FF  # (a, r, g, b) = (255, 128, 128, 255)
+
+
+def pack_rgba8(r, g, b, a=255):
+    """Pack channel bytes the way the reference's u32 textures are read:
+    (a << 24) | (r << 16) | (g << 8) | b (NovaTextureInterface.h:78-88, is_rgba = false)."""
+    r, g, b, a = (np.asarray(x, dtype=np.uint32) for x in (r, g, b, a))
+    return (a << 24) | (r << 16) | (g << 8) | b
+
+
+@dataclass
+class Texture:
+    width: int
+    height: int
+    channels: int
+    data: np.ndarray  # uint32 (width*height) or float32 (width*height*channels)
+    invert_u: bool = False
+    invert_v: bool = False
+
+    @property
+    def is_f32(self):
+        return self.data.dtype == np.float32
+
+
+@dataclass
+class Material:
+    albedo: int = -1
+    metallic: int = -1
+    normal: int = -1
+    emissive: int = -1
+    albedo_f32: int = -1
+    roughness_f32: int = -1
+    eta: tuple = (0.0, 0.0, 0.0)
+    k: tuple = (0.0, 0.0, 0.0)
+    anisotropy: float = 0.0
+    emission_scale: float = 1.0
+
+
+@dataclass
+class Mesh:
+    vertices: np.ndarray
+    normals: np.ndarray
+    tangents: np.ndarray
+    bitangents: np.ndarray
+    uv: np.ndarray
+    indices: np.ndarray
+    material: int
+    transform: np.ndarray = field(default_factory=lambda: np.eye(4, dtype=np.float32).T.reshape(-1).copy())
+
+    @property
+    def triangle_count(self):
+        return len(self.indices) // 3
+
+
+@dataclass
+class Scene:
+    meshes: list = field(default_factory=list)
+    materials: list = field(default_factory=list)
+    textures: list = field(default_factory=list)
+    envmap: int = -1
+    inv_P: np.ndarray = None
+    inv_V: np.ndarray = None
+    name: str = ""
+
+    @property
+    def triangle_count(self):
+        return sum(m.triangle_count for m in self.meshes)
+
+    def add_texture(self, tex):
+        self.textures.append(tex)
+        return len(self.textures) - 1
+
+    def add_material(self, mat):
+        self.materials.append(mat)
+        return len(self.materials) - 1
+
+    def constant_u32(self, value):
+        return self.add_texture(Texture(2, 2, 4, np.full(4, value, dtype=np.uint32)))
+
+    def principled(self, albedo_rgb, roughness=0.0, metal=0.0, emission=None, emission_scale=1.0, eta=None):
+        a = np.clip(np.round(np.asarray(albedo_rgb) * 255.0), 0, 255).astype(np.uint32)
+        alb = self.constant_u32(int(pack_rgba8(a[0], a[1], a[2])))
+        met = self.constant_u32(int(pack_rgba8(0, int(round(roughness * 255.0)), int(round(metal * 255.0)))))
+        nrm = self.constant_u32(FLAT_NORMAL)
+        mat = Material(albedo=alb, metallic=met, normal=nrm)
+        if emission is not None:
+            e = np.clip(np.round(np.asarray(emission) * 255.0), 0, 255).astype(np.uint32)
+            mat.emissive = self.constant_u32(int(pack_rgba8(e[0], e[1], e[2])))
+            mat.emission_scale = float(emission_scale)
+        if eta is not None:
+            mat.eta = (float(eta),) * 3
+        return self.add_material(mat)
+
+
+# ----------------------------------------------------------------- cameras
+
+def perspective(fovy_deg, aspect, near, far):
+    f = 1.0 / math.tan(math.radians(fovy_deg) / 2.0)
+    P = np.zeros((4, 4))
+    P[0, 0] = f / aspect
+    P[1, 1] = f
+    P[2, 2] = -(far + near) / (far - near)
+    P[2, 3] = -(2.0 * far * near) / (far - near)
+    P[3, 2] = -1.0
+    return P
+
+
+def look_at(eye, center, up):
+    eye, center, up = (np.asarray(v, dtype=np.float64) for v in (eye, center, up))
+    f = center - eye
+    f /= np.linalg.norm(f)
+    s = np.cross(f, up)
+    s /= np.linalg.norm(s)
+    u = np.cross(s, f)
+    V = np.eye(4)
+    V[0, :3], V[1, :3], V[2, :3] = s, u, -f
+    V[0, 3], V[1, 3], V[2, 3] = -s @ eye, -u @ eye, f @ eye
+    return V
+
+
+def colmajor(M):
+    """Row-major numpy 4x4 -> 16 floats in glm value_ptr (column-major) order."""
+    return np.ascontiguousarray(np.asarray(M, dtype=np.float64).T.reshape(-1).astype(np.float32))
+
+
+def set_camera(scene, eye, center, fovy, aspect, near=0.1, far=1000.0):
+    P = perspective(fovy, aspect, near, far)
+    V = look_at(eye, center, (0.0, 1.0, 0.0))
+    scene.inv_P = colmajor(np.linalg.inv(P))
+    scene.inv_V = colmajor(np.linalg.inv(V))
+
+
+# --------------------------------------------------------------- geometry
+
+def _f32(a):
+    return np.ascontiguousarray(np.asarray(a, dtype=np.float64).astype(np.float32))
+
+
+def _normalize(v):
+    return v / np.linalg.norm(v, axis=-1, keepdims=True)
+
+
+def quad_mesh(corners, material):
+    """Quad p0,p1,p2,p3 (counter-clockwise seen from the normal side), UV (0,0),(1,0),(1,1),(0,1)."""
+    p = np.asarray(corners, dtype=np.float64)
+    t = _normalize(p[1] - p[0])
+    b = _normalize(p[3] - p[0])
+    n = _normalize(np.cross(t, b))
+    uv = np.array([[0, 0], [1, 0], [1, 1], [0, 1]], dtype=np.float64)
+    return Mesh(_f32(p.reshape(-1)), _f32(np.tile(n, 4)), _f32(np.tile(t, 4)), _f32(np.tile(b, 4)), _f32(uv.reshape(-1)),
+                np.array([0, 1, 2, 0, 2, 3], dtype=np.uint32), material)
+
+
+def box_mesh(center, half, angle, material):
+    """Axis-aligned box rotated about +y by `angle`; 6 faces x 2 triangles, per-face vertices."""
+    c, s = math.cos(angle), math.sin(angle)
+    R = np.array([[c, 0, s], [0, 1, 0], [-s, 0, c]])
+    faces = []
+    for axis in range(3):
+        for sign in (-1.0, 1.0):
+            n = np.zeros(3)
+            n[axis] = sign
+            u = np.zeros(3)
+            u[(axis + 1) % 3] = 1.0
+            v = np.cross(n, u)
+            quad = [n * half[axis] + (-u * half[(axis + 1) % 3]) + (-v * half[(axis + 2) % 3]),
+                    n * half[axis] + (u * half[(axis + 1) % 3]) + (-v * half[(axis + 2) % 3]),
+                    n * half[axis] + (u * half[(axis + 1) % 3]) + (v * half[(axis + 2) % 3]),
+                    n * half[axis] + (-u * half[(axis + 1) % 3]) + (v * half[(axis + 2) % 3])]
+            faces.append(([R @ q + np.asarray(center) for q in quad], R @ n, R @ u, R @ v))
+    verts, nrm, tan, bit, uv, idx = [], [], [], [], [], []
+    for fi, (quad, n, u, v) in enumerate(faces):
+        verts += quad
+        nrm += [n] * 4
+        tan += [u] * 4
+        bit += [v] * 4
+        uv += [[0, 0], [1, 0], [1, 1], [0, 1]]
+        base = 4 * fi
+        idx += [base, base + 1, base + 2, base, base + 2, base + 3]
+    return Mesh(_f32(np.reshape(verts, -1)), _f32(np.reshape(nrm, -1)), _f32(np.reshape(tan, -1)), _f32(np.reshape(bit, -1)),
+                _f32(np.reshape(uv, -1)), np.asarray(idx, dtype=np.uint32), material)
+
+
+def _tangent_frame(n):
+    up = np.where(np.abs(n[:, 1:2]) < 0.999, np.array([[0.0, 1.0, 0.0]]), np.array([[1.0, 0.0, 0.0]]))
+    t = _normalize(np.cross(up, n))
+    b = np.cross(n, t)
+    return t, b
+
+
+def octasphere_mesh(center, radius, level, material):
+    """Octahedron subdivided `level` times (8 * 4**level triangles), projected to a sphere.
+    Vertices are per triangle (no sharing) so every triangle carries its own UV seam fix."""
+    v = np.array([[1, 0, 0], [-1, 0, 0], [0, 1, 0], [0, -1, 0], [0, 0, 1], [0, 0, -1]], dtype=np.float64)
+    faces = np.array([[4, 0, 2], [0, 5, 2], [5, 1, 2], [1, 4, 2], [0, 4, 3], [5, 0, 3], [1, 5, 3], [4, 1, 3]])
+    tris = v[faces]  # (T, 3, 3)
+    for _ in range(level):
+        a, b, c = tris[:, 0], tris[:, 1], tris[:, 2]
+        ab, bc, ca = _normalize(a + b), _normalize(b + c), _normalize(c + a)
+        tris = np.stack([np.stack([a, ab, ca], 1), np.stack([ab, b, bc], 1), np.stack([ca, bc, c], 1), np.stack([ab, bc, ca], 1)], 1)
+        tris = tris.reshape(-1, 3, 3)
+    tris = _normalize(tris)
+    n = tris.reshape(-1, 3)
+    u = (np.arctan2(n[:, 2], n[:, 0]) / (2 * math.pi)) % 1.0
+    vv = np.arccos(np.clip(n[:, 1], -1, 1)) / math.pi
+    u = u.reshape(-1, 3)
+    span = u.max(1) - u.min(1)
+    u = np.where((span > 0.5)[:, None] & (u < 0.5), u + 1.0, u).reshape(-1)
+    t, b = _tangent_frame(n)
+    pos = np.asarray(center)[None, :] + radius * n
+    idx = np.arange(len(pos), dtype=np.uint32)
+    return Mesh(_f32(pos.reshape(-1)), _f32(n.reshape(-1)), _f32(t.reshape(-1)), _f32(b.reshape(-1)),
+                _f32(np.stack([u, vv], 1).reshape(-1)), idx, material)
+
+
+def _fbm(rng, n, octaves=6):
+    """Value-noise fBm on an (n+1)^2 grid in [0,1)^2, float64."""
+    h = np.zeros((n + 1, n + 1))
+    amp, freq = 1.0, 4
+    xs = np.linspace(0.0, 1.0, n + 1)
+    for _ in range(octaves):
+        g = rng.standard_normal((freq + 1, freq + 1))
+        gx = xs * freq
+        i0 = np.minimum(gx.astype(int), freq - 1)
+        f = gx - i0
+        f = f * f * (3 - 2 * f)
+        a = g[i0][:, i0] * (1 - f)[None, :] + g[i0][:, i0 + 1] * f[None, :]
+        b = g[i0 + 1][:, i0] * (1 - f)[None, :] + g[i0 + 1][:, i0 + 1] * f[None, :]
+        h += amp * (a * (1 - f)[:, None] + b * f[:, None])
+        amp *= 0.5
+        freq *= 2
+    return h
+
+
+def terrain_mesh(rng, cells, extent, base_y, amplitude, material):
+    """cells x cells grid over [-extent, extent]^2 in x/z, fBm heights; 2*cells^2 triangles."""
+    h = base_y + amplitude * _fbm(rng, cells)
+    xs = np.linspace(-extent, extent, cells + 1)
+    X, Z = np.meshgrid(xs, xs)  # [row=z, col=x]
+    P = np.stack([X, h, Z], -1)
+    dx = np.gradient(h, xs, axis=1)
+    dz = np.gradient(h, xs, axis=0)
+    n = _normalize(np.stack([-dx, np.ones_like(h), -dz], -1)).reshape(-1, 3)
+    t = _normalize(np.stack([np.ones_like(h), dx, np.zeros_like(h)], -1)).reshape(-1, 3)
+    t = _normalize(t - (t * n).sum(1, keepdims=True) * n)
+    b = np.cross(n, t)
+    uvs = np.stack(np.meshgrid(np.linspace(0, 1, cells + 1), np.linspace(0, 1, cells + 1)), -1).reshape(-1, 2)
+    r = np.arange(cells)
+    i00 = (r[:, None] * (cells + 1) + r[None, :]).reshape(-1)
+    i10, i01, i11 = i00 + 1, i00 + (cells + 1), i00 + (cells + 2)
+    idx = np.stack([i00, i01, i11, i00, i11, i10], 1).reshape(-1).astype(np.uint32)
+    return Mesh(_f32(P.reshape(-1)), _f32(n.reshape(-1)), _f32(t.reshape(-1)), _f32(b.reshape(-1)), _f32(uvs.reshape(-1)), idx, material)
+
+
+def sky_envmap(width, height, sun_dir, sun_intensity=50.0, sun_size=0.02):
+    """FP32 RGBA equirect sky: zenith/horizon gradient plus an analytic sun disc.
+    Texel (i, j) covers the direction with phi = 2*pi*u, theta = pi*v, z-up,
+    matching the envmap lookup (math_spherical.h:16-52)."""
+    u = (np.arange(width) + 0.5) / width
+    v = (np.arange(height) + 0.5) / height
+    phi, theta = np.meshgrid(2 * math.pi * u, math.pi * v)
+    d = np.stack([np.sin(theta) * np.cos(phi), np.sin(theta) * np.sin(phi), np.cos(theta)], -1)
+    up = d[..., 1]  # scenes use +y as up
+    t = np.clip(up * 0.5 + 0.5, 0, 1)[..., None]
+    horizon = np.array([0.9, 0.85, 0.8])
+    zenith = np.array([0.25, 0.45, 0.9])
+    ground = np.array([0.2, 0.18, 0.16])
+    col = np.where(up[..., None] >= 0, horizon * (1 - t) + zenith * t, ground)
+    s = np.asarray(sun_dir, dtype=np.float64)
+    s /= np.linalg.norm(s)
+    cosang = np.clip((d * s).sum(-1), -1, 1)
+    col = col + sun_intensity * np.exp(-(1 - cosang) / sun_size)[..., None] * np.array([1.0, 0.95, 0.85])
+    rgba = np.concatenate([col, np.ones_like(col[..., :1])], -1)
+    return Texture(width, height, 4, _f32(rgba.reshape(-1)))
+
+
+# -------------------------------------------------------------- workloads
+
+def box_room(seed=0xA70AE001, width=256, height=256):
+    """Box room: 5 walls (open front), an emissive ceiling quad (radiance 15 via the
+    emission-scale extension), two boxes rotated by seeded angles, one of them a
+    rough dielectric (IOR 1.5, roughness 0.1). 36 triangles (5*2 + 2 + 2*12)."""
+    rng = np.random.default_rng(seed)
+    sc = Scene(name="box_room")
+    alb = rng.uniform(0.2, 0.8, size=(8, 3))
+    # "Lambertian" surfaces are Principled materials with metal 0 and a broad (roughness 0.8)
+    # specular lobe; roughness 0 would make that always-present lobe a near-delta.
+    wall = [sc.principled(alb[i], roughness=0.8) for i in range(5)]
+    light = sc.principled((1, 1, 1), roughness=0.8, emission=(1.0, 1.0, 1.0), emission_scale=15.0)
+    box_a = sc.principled(alb[5], roughness=0.8)
+    glass = sc.principled(alb[6] * 0 + 1.0, roughness=0.1, eta=1.5)
+    L = 1.0
+    sc.meshes += [
+        quad_mesh([(-L, -L, L), (L, -L, L), (L, -L, -L), (-L, -L, -L)], wall[0]),   # floor (+y)
+        quad_mesh([(-L, L, -L), (L, L, -L), (L, L, L), (-L, L, L)], wall[1]),       # ceiling (-y)
+        quad_mesh([(-L, -L, -L), (L, -L, -L), (L, L, -L), (-L, L, -L)], wall[2]),   # back (+z)
+        quad_mesh([(-L, -L, L), (-L, -L, -L), (-L, L, -L), (-L, L, L)], wall[3]),   # left (+x)
+        quad_mesh([(L, -L, -L), (L, -L, L), (L, L, L), (L, L, -L)], wall[4]),       # right (-x)
+        quad_mesh([(-0.3, 0.99, -0.3), (0.3, 0.99, -0.3), (0.3, 0.99, 0.3), (-0.3, 0.99, 0.3)], light),
+        box_mesh((-0.4, -0.55, -0.3), (0.3, 0.45, 0.3), rng.uniform(0, 2 * math.pi), box_a),
+        box_mesh((0.45, -0.7, 0.25), (0.25, 0.3, 0.25), rng.uniform(0, 2 * math.pi), glass),
+    ]
+    sc.envmap = sc.add_texture(Texture(2, 2, 4, np.zeros(16, dtype=np.float32)))
+    set_camera(sc, (0.0, 0.0, 3.6), (0.0, 0.0, 0.0), 40.0, width / height)
+    return sc
+
+
+def million_triangles(seed=0xA70AE002, width=1920, height=1080, terrain_cells=512, spheres=64, sphere_level=5,
+                      sky=(2048, 1024)):
+    """1,048,576 triangles: a 512x512-cell fBm terrain (524,288 triangles, Disney diffuse)
+    plus 64 octahedral spheres at subdivision 5 (8,192 triangles each, GGX conductors with
+    roughness U[0.05, 0.8]), centred in a 40^3 box around the origin; procedural sky."""
+    rng = np.random.default_rng(seed)
+    sc = Scene(name="million_triangles")
+    terrain_mat = sc.principled((0.55, 0.5, 0.42), roughness=0.7)
+    sc.meshes.append(terrain_mesh(rng, terrain_cells, 20.0, -6.0, 3.0, terrain_mat))
+    for _ in range(spheres):
+        c = rng.uniform(-20, 20, size=3)
+        c[1] = rng.uniform(-6.0, 14.0)
+        r = rng.uniform(0.1, 2.0)
+        m = sc.principled(rng.uniform(0.3, 0.95, size=3), roughness=rng.uniform(0.05, 0.8), metal=1.0)
+        sc.meshes.append(octasphere_mesh(c, r, sphere_level, m))
+    sc.envmap = sc.add_texture(sky_envmap(sky[0], sky[1], (0.4, 0.8, 0.3)))
+    set_camera(sc, (0.0, 10.0, 34.0), (0.0, -2.0, 0.0), 50.0, width / height)
+    return sc
+
+
+def scaled_million(scale_cells=64, spheres=8, sphere_level=3, width=128, height=72, seed=0xA70AE002):
+    """The million-triangle workload's generator at reduced size, for parity tests."""
+    return million_triangles(seed=seed, width=width, height=height, terrain_cells=scale_cells, spheres=spheres,
+                             sphere_level=sphere_level, sky=(256, 128))
+
+
+def bake_envmaps(seed=0xA70AE005, count=32):
+    """Equirect FP32 RGBA environment maps of uneven sizes with a seeded sun."""
+    rng = np.random.default_rng(seed)
+    sizes = [(512, 256), (1024, 512), (2048, 1024), (4096, 2048)]
+    maps = []
+    for _ in range(count):
+        w, h = sizes[rng.integers(0, len(sizes))]
+        d = rng.standard_normal(3)
+        inten = float(np.exp(rng.uniform(np.log(1e2), np.log(1e5))))
+        maps.append(sky_envmap(w, h, d, sun_intensity=inten * 1e-3, sun_size=0.002))
+    return maps

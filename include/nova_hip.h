@@ -1,0 +1,201 @@
+/*
+ * nova_hip.h - C ABI of the MI355X (gfx950) Nova hot path.
+ *
+ * This is the drop-in boundary for the slot the reference fills with its
+ * CUDA/OptiX backend: the device accelerator built by Engine::buildAcceleration
+ * and launched once per frame through DeviceIntersectorInterface::traverse.
+ * Every function returns an int status (NOVA_HIP_SUCCESS = 0); no C++ exception
+ * crosses the ABI and nova_hip_last_error() returns the message of the last
+ * failure on a context. A context is driven from one host thread; all device
+ * work is ordered on the stream passed to it (NULL = the default stream).
+ *
+ * Citations name files of the reference (HamilcarR/Axomae) relative to its root.
+ */
+#ifndef NOVA_HIP_H
+#define NOVA_HIP_H
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* Status codes: 0..16 keep nova::ERROR_STATE's numbering (corelib/nova/api_common.h:7-27);
+ * 100+ are device-layer failures the reference reports as RENDERER_EXCEPTION
+ * (corelib/nova/api/engine.cpp:478-486). */
+enum nova_hip_status {
+  NOVA_HIP_SUCCESS = 0,
+  NOVA_HIP_INVALID_ENGINE_STATE = 1,
+  NOVA_HIP_INVALID_BUFFER_STATE = 2,
+  NOVA_HIP_INVALID_SCENE_TYPE = 4,
+  NOVA_HIP_INVALID_TRANSFORM_TYPE = 5,
+  NOVA_HIP_SCENE_NOT_PROCESSED = 9,
+  NOVA_HIP_OUT_OF_MEMORY = 11,
+  NOVA_HIP_INVALID_ARGUMENT = 15,
+  NOVA_HIP_RENDERER_EXCEPTION = 16,
+  NOVA_HIP_DEVICE_ERROR = 100,
+  NOVA_HIP_INVALID_RENDERBUFFER_DIM = 101 /* nova::exception::INVALID_RENDERBUFFER_DIM (utils/DrawEngine.cpp:44-47) */
+};
+
+typedef struct nova_hip_ctx nova_hip_ctx;
+
+/* Context on one HIP device. Replaces DeviceAcceleratorInterface::make()
+ * (corelib/nova/aggregate/api/device_accelerator_factory.cpp:9-17) and the
+ * CUDA context the engine's user callback sets per frame (api_engine.h:31-48). */
+int nova_hip_create(int device, nova_hip_ctx **out);
+/* Releases every device buffer: DeviceAcceleratorInterface::cleanup() + destructor
+ * (corelib/nova/aggregate/api/optix/host.cpp:542-580). */
+int nova_hip_destroy(nova_hip_ctx *ctx);
+const char *nova_hip_last_error(const nova_hip_ctx *ctx);
+
+/* ------------------------------------------------------------------ scene */
+
+/* One triangle mesh: the Object3D attribute spans (corelib/internal/geometry/Object3D.h:387-394)
+ * plus its model matrix (column-major, glm::value_ptr order) and material index.
+ * normals/tangents/bitangents are vertex_count*3 floats, uv vertex_count*2. */
+typedef struct {
+  const float *vertices, *normals, *tangents, *bitangents, *uv;
+  const uint32_t *indices;
+  uint64_t vertex_count, index_count;
+  float transform[16];
+  int32_t material;
+} nova_hip_mesh_desc;
+
+/* PrincipledMaterial texture pack + refractive indices (corelib/nova/material/NovaMaterials.h:45-138,
+ * corelib/nova/api/material.cpp:212-226). Texture fields are indices into the scene's
+ * texture list (-1 = absent). albedo_f32 / roughness_f32 and emission_scale are the FP32
+ * material-texture and emission extensions (1.0 = reference behaviour). */
+typedef struct {
+  int32_t albedo, metallic, normal, emissive;
+  int32_t albedo_f32, roughness_f32;
+  float eta[3], k[3];
+  float anisotropy;
+  float emission_scale;
+} nova_hip_material_desc;
+
+enum nova_hip_texture_format { NOVA_HIP_TEX_U32 = 0, NOVA_HIP_TEX_F32 = 1 };
+
+/* TextureRawData (corelib/nova/texturing/nova_texturing.h:98-112): u32 texels packed
+ * (a<<24)|(r<<16)|(g<<8)|b, or `channels` floats per texel. */
+typedef struct {
+  int32_t width, height, channels;
+  int32_t invert_u, invert_v;
+  int32_t format;
+  const void *data;
+} nova_hip_texture_desc;
+
+/* primitive_aggregate_data_s (corelib/nova/aggregate/aggregate_datastructures.h:17-20) with
+ * the texture/material holders of NovaResourceManager. Global primitive order is
+ * mesh-major, triangle-minor (corelib/nova/api/engine.cpp:159-175). */
+typedef struct {
+  const nova_hip_mesh_desc *meshes;
+  uint32_t mesh_count;
+  const nova_hip_material_desc *materials;
+  uint32_t material_count;
+  const nova_hip_texture_desc *textures;
+  uint32_t texture_count;
+  int32_t envmap_texture; /* TextureResourcesHolder::getCurrentEnvmap (nova_texturing.h:130-133) */
+} nova_hip_scene_desc;
+
+/* Uploads and flattens the scene on the device (host buffers are copied; the caller may
+ * free them afterwards). Replaces the OptiX GAS input assembly of OptixAccelerator::build
+ * (corelib/nova/aggregate/api/optix/host.cpp:432-528) and the managed-memory views of
+ * NovaResourceManager (corelib/nova/manager/NovaResourceManager.h:28-82). */
+int nova_hip_scene_upload(nova_hip_ctx *ctx, const nova_hip_scene_desc *scene, void *stream);
+
+/* ------------------------------------------------------------------- BVH */
+
+typedef struct {
+  uint32_t primitive_count;
+  uint32_t binary_nodes;   /* nodes of the SAH tree, BvhtlBuilder numbering differs (pairs still adjacent) */
+  uint32_t binary_depth;
+  uint32_t wide_nodes;     /* 4-wide nodes after the collapse */
+  uint32_t wide_depth;
+  uint32_t leaves;
+  float build_ms;          /* device time of SAH build + collapse */
+} nova_hip_bvh_info;
+
+/* GPU SAH build with the reference builder's exact split decisions
+ * (BvhtlBuilder::buildTriangleBasedScene, PERFORMANCE = 4 candidates per axis, SAH:
+ * corelib/nova/aggregate/native/BvhBuilder.cpp:37-253), then a collapse to a 4-wide BVH.
+ * `build_candidates` must be 4 (the reference's default build type). The primitive
+ * order prim_idx equals the reference builder's bit for bit. */
+int nova_hip_bvh_build(nova_hip_ctx *ctx, int build_candidates, nova_hip_bvh_info *info, void *stream);
+
+/* Copies the builder outputs to host memory: prim_idx (primitive_count int32) and the
+ * binary nodes as Bvhnl records {float min[3], max[3]; int32 left, primitive_count}
+ * (BvhBuilder.h:10-15), binary_nodes entries. Either pointer may be NULL. */
+int nova_hip_bvh_read(nova_hip_ctx *ctx, int32_t *prim_idx, void *binary_nodes);
+
+/* Closest hit for a batch of rays in device memory: rays are n x {ox,oy,oz,dx,dy,dz}
+ * floats; outputs are the global primitive id (-1 on miss) and {t,u,v} per ray.
+ * The same kernel runs in the integrator's extend stage. Replaces
+ * GenericHostAccelerator::hit (corelib/nova/aggregate/acceleration_interface.h:13-32,
+ * native.cpp:56-114 + Triangle.h:146-213). tnear is 1e-6 as Ray's default (ray/Ray.h:15). */
+int nova_hip_closest_hit(nova_hip_ctx *ctx, const float *rays_dev, uint64_t n, int32_t *prim_dev, float *tuv_dev, void *stream);
+
+/* ---------------------------------------------------------------- render */
+
+typedef struct {
+  int32_t width, height;
+  int32_t max_depth;        /* RenderOptions::getMaxDepth: maxDepth+1 scatters (Integrator.cpp:62-66) */
+  int32_t frame_start;      /* first frame index (Engine frame_index) */
+  int32_t frame_count;      /* frames rendered, 1 spp each (engine.cpp:498-525) */
+  uint32_t max_samples_offset; /* renderer_max_samples added to the sample index (Integrator.h:80) */
+  int32_t vertical_invert;  /* EngineResourcesHolder::vertical_invert (Integrator.h:38-42) */
+  int32_t rr_start;         /* Russian-roulette extension: first scatter index that rolls, -1 = off */
+  float inv_P[16], inv_V[16]; /* CameraResourcesHolder inv_P / inv_V, column-major */
+} nova_hip_render_params;
+
+/* Renders frames [frame_start, frame_start+frame_count) of the path integrator
+ * (PathIntegrator::Li, corelib/nova/integrator/Integrator.cpp:59-98, with the CPU
+ * index policy idx = 4*(y*W+x)*frame + max_samples_offset) and accumulates them as
+ * Engine::writeFramebuffer does (engine.cpp:351-363): accum += partial,
+ * final = accum / (frame+1). accum_dev and final_dev are device RGBA32F buffers of
+ * width*height*4 floats; accum must hold the previous frames' sum (zero at frame 0).
+ * partial_dev (optional) receives the last frame's radiance (HdrBufferStruct::partial_buffer).
+ * Replaces gpu_draw -> device_start_integrator -> OptixIntersector::traverse
+ * (corelib/nova/utils/DrawEngine_GPU.cpp:14-20, integrator/cuda/job_dispatch.cpp:7-12,
+ * aggregate/api/optix/device.cpp:18-21). */
+int nova_hip_render_frames(nova_hip_ctx *ctx, const nova_hip_render_params *params, float *accum_dev, float *final_dev,
+                           float *partial_dev, void *stream);
+
+/* Per-kernel device time of the last nova_hip_render_frames call, in milliseconds, and
+ * the number of path segments traced (extend launches x rays). */
+typedef struct {
+  float total_ms, extend_ms, shade_ms, other_ms;
+  uint64_t segments;
+  uint64_t extend_launches;
+} nova_hip_render_stats;
+int nova_hip_render_stats_get(nova_hip_ctx *ctx, nova_hip_render_stats *out);
+
+/* --------------------------------------------------------------- sampler */
+
+/* Owen-scrambled Sobol samples: out[i] = SobolGenerator::generate(idx[i], dim[i]) with the
+ * reference's seed 0xDEADBEEF and 256 x 32 direction table
+ * (corelib/internal/common/math/math_random.h:29-97). Device pointers; dims must be < 256.
+ * out_bits (optional) receives the scrambled 32-bit integers before the float convert. */
+int nova_hip_sobol_fill(const uint32_t *idx_dev, const uint32_t *dim_dev, uint64_t n, float *out_dev, uint32_t *out_bits_dev,
+                        void *stream);
+
+/* ----------------------------------------------------------------- bakes */
+
+/* Diffuse irradiance of FP32 equirect maps, batched in one launch:
+ * TextureOperations<float>::computeDiffuseIrradiance (sources/processing/TextureProcessing.cpp:9-64,
+ * TextureProcessing.h:253-301). env_dev[i] is a device pointer to map i (w[i] x h[i] x c[i]
+ * floats); out_dev[i] receives out_w[i] x out_h[i] x c[i] floats. The pointer/size arrays
+ * are host arrays of length count. */
+int nova_hip_bake_irradiance(const float *const *env_dev, const int32_t *w, const int32_t *h, const int32_t *c, uint32_t count,
+                             float *const *out_dev, const int32_t *out_w, const int32_t *out_h, int32_t samples, void *stream);
+
+/* GGX-prefiltered specular of FP32 equirect maps at one roughness per output, batched:
+ * the GLSL baker envmap_prefilter.frag:1-116 driven by RenderPipeline::preFilterEnvmap
+ * (sources/renderer/RenderPipeline.cpp:94-150), on equirect with level-0 lookups. */
+int nova_hip_bake_prefilter(const float *const *env_dev, const int32_t *w, const int32_t *h, const int32_t *c, uint32_t count,
+                            float *const *out_dev, const int32_t *out_w, const int32_t *out_h, const float *roughness,
+                            int32_t samples, void *stream);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* NOVA_HIP_H */

@@ -1,0 +1,172 @@
+// Oracle (test infrastructure only): C entry points for ctypes. Loaded only by
+// tests/, the smoke check and bench.py's CPU-baseline leg.
+#include "oracle.hpp"
+#include <atomic>
+#include <cstring>
+#include <thread>
+
+using namespace oracle;
+
+extern "C" {
+
+void oracle_sobol_bits(const uint32_t *idx, const uint32_t *dims, size_t n, uint32_t *out) {
+  for (size_t i = 0; i < n; i++) out[i] = sobol_sample_bits(idx[i], dims[i]);
+}
+
+void oracle_sobol_float(const uint32_t *idx, const uint32_t *dims, size_t n, float *out) {
+  for (size_t i = 0; i < n; i++) out[i] = sobol_sample(idx[i], dims[i]);
+}
+
+void oracle_direction_table(uint32_t *out) { std::memcpy(out, kSobolTable, sizeof(kSobolTable)); }
+
+void oracle_acos_lut(float *out) {
+  const auto &l = acos_lut();
+  std::memcpy(out, l.data(), l.size() * sizeof(float));
+}
+
+void oracle_acos_lt(const float *a, size_t n, float *out) {
+  for (size_t i = 0; i < n; i++) out[i] = acos_lt(a[i]);
+}
+
+void oracle_atan2_approx(const float *y, const float *x, size_t n, float *out) {
+  for (size_t i = 0; i < n; i++) out[i] = atan2_approx(y[i], x[i]);
+}
+
+void *oracle_scene_new() { return new Scene(); }
+void oracle_scene_free(void *s) { delete static_cast<Scene *>(s); }
+
+int oracle_scene_add_texture_u32(void *sp, int w, int h, int channels, int inv_u, int inv_v, const uint32_t *data) {
+  Scene &s = *static_cast<Scene *>(sp);
+  Texture t;
+  t.width = w;
+  t.height = h;
+  t.channels = channels;
+  t.invert_u = inv_u != 0;
+  t.invert_v = inv_v != 0;
+  t.u32.assign(data, data + (size_t)w * h);
+  s.textures.push_back(std::move(t));
+  return (int)s.textures.size() - 1;
+}
+
+int oracle_scene_add_texture_f32(void *sp, int w, int h, int channels, int inv_u, int inv_v, const float *data) {
+  Scene &s = *static_cast<Scene *>(sp);
+  Texture t;
+  t.width = w;
+  t.height = h;
+  t.channels = channels;
+  t.invert_u = inv_u != 0;
+  t.invert_v = inv_v != 0;
+  t.f32.assign(data, data + (size_t)w * h * channels);
+  s.textures.push_back(std::move(t));
+  return (int)s.textures.size() - 1;
+}
+
+// tex: albedo, metallic, normal, emissive (u32 ids), albedo_f32, roughness_f32 (f32 ids); -1 = absent
+int oracle_scene_add_material(void *sp, const int32_t *tex, const float *eta, const float *k, float aniso, float emission_scale) {
+  Scene &s = *static_cast<Scene *>(sp);
+  Material m;
+  m.albedo = tex[0];
+  m.metallic = tex[1];
+  m.normal = tex[2];
+  m.emissive = tex[3];
+  m.albedo_f32 = tex[4];
+  m.roughness_f32 = tex[5];
+  for (int i = 0; i < 3; i++) {
+    m.eta[i] = eta[i];
+    m.k[i] = k[i];
+  }
+  m.anisotropy = aniso;
+  m.emission_scale = emission_scale;
+  s.materials.push_back(m);
+  return (int)s.materials.size() - 1;
+}
+
+int oracle_scene_add_mesh(void *sp, size_t nv, const float *vertices, const float *normals, const float *tangents, const float *bitangents,
+                          const float *uv, size_t ni, const uint32_t *indices, const float *transform, int material) {
+  Scene &s = *static_cast<Scene *>(sp);
+  Mesh m;
+  m.vertices.assign(vertices, vertices + nv * 3);
+  m.normals.assign(normals, normals + nv * 3);
+  m.tangents.assign(tangents, tangents + nv * 3);
+  m.bitangents.assign(bitangents, bitangents + nv * 3);
+  m.uv.assign(uv, uv + nv * 2);
+  m.indices.assign(indices, indices + ni);
+  if (transform)
+    for (int i = 0; i < 16; i++) m.transform.m[i] = transform[i];
+  m.material = material;
+  s.meshes.push_back(std::move(m));
+  return (int)s.meshes.size() - 1;
+}
+
+void oracle_scene_set_envmap(void *sp, int tex) { static_cast<Scene *>(sp)->envmap = tex; }
+void oracle_scene_finalize(void *sp) { static_cast<Scene *>(sp)->finalize(); }
+size_t oracle_scene_prim_count(void *sp) { return static_cast<Scene *>(sp)->prim_mesh.size(); }
+
+void *oracle_bvh_build(void *sp) {
+  Bvh *b = new Bvh();
+  bvh_build(*static_cast<Scene *>(sp), *b);
+  return b;
+}
+void oracle_bvh_free(void *b) { delete static_cast<Bvh *>(b); }
+int oracle_bvh_nodes_used(void *b) { return static_cast<Bvh *>(b)->nodes_used; }
+// nodes: nodes_used x 8 words (min3, max3 as float bits; left, count as int32)
+void oracle_bvh_get(void *bp, int32_t *prim_idx, void *nodes) {
+  Bvh &b = *static_cast<Bvh *>(bp);
+  std::memcpy(prim_idx, b.prim_idx.data(), b.prim_idx.size() * sizeof(int32_t));
+  std::memcpy(nodes, b.nodes.data(), (size_t)b.nodes_used * sizeof(BvhNode));
+}
+
+// rays: n x 6 floats (origin, direction); out prim (-1 = miss) and t,u,v
+void oracle_closest_hit(void *sp, void *bp, const float *rays, size_t n, int brute, int threads, int32_t *prim, float *tuv) {
+  Scene &s = *static_cast<Scene *>(sp);
+  std::atomic<size_t> next{0};
+  auto work = [&]() {
+    for (;;) {
+      size_t k0 = next.fetch_add(256);
+      if (k0 >= n) break;
+      for (size_t k = k0; k < k0 + 256 && k < n; k++) {
+        vec3 o(rays[k * 6], rays[k * 6 + 1], rays[k * 6 + 2]), d(rays[k * 6 + 3], rays[k * 6 + 4], rays[k * 6 + 5]);
+        float t = 0, u = 0, v = 0;
+        int p = brute ? closest_hit_brute(s, o, d, 0.000001f, t, u, v) : closest_hit(s, *static_cast<Bvh *>(bp), o, d, 0.000001f, t, u, v);
+        prim[k] = p;
+        tuv[k * 3] = t;
+        tuv[k * 3 + 1] = u;
+        tuv[k * 3 + 2] = v;
+      }
+    }
+  };
+  std::vector<std::thread> pool;
+  for (int i = 0; i < (threads > 1 ? threads : 1); i++) pool.emplace_back(work);
+  for (auto &t : pool) t.join();
+}
+
+// Progressive CPU render: frames [frame_start, frame_start + frames) at 1 spp each,
+// accumulated as Engine::writeFramebuffer does. accum/final: width*height*4 floats.
+void oracle_render(void *sp, void *bp, int width, int height, int frame_start, int frames, int max_depth, uint32_t max_samples_offset,
+                   int vertical_invert, int rr_start, const float *inv_P, const float *inv_V, int threads, float *accum, float *final_out) {
+  RenderParams p;
+  p.width = width;
+  p.height = height;
+  p.frame_start = frame_start;
+  p.frames = frames;
+  p.max_depth = max_depth;
+  p.max_samples_offset = max_samples_offset;
+  p.vertical_invert = vertical_invert != 0;
+  p.rr_start = rr_start;
+  for (int i = 0; i < 16; i++) {
+    p.inv_P.m[i] = inv_P[i];
+    p.inv_V.m[i] = inv_V[i];
+  }
+  p.threads = threads;
+  render(*static_cast<Scene *>(sp), *static_cast<Bvh *>(bp), p, accum, final_out);
+}
+
+void oracle_bake_irradiance(const float *env, int w, int h, int c, int ow, int oh, int samples, int threads, float *out) {
+  bake_irradiance(env, w, h, c, ow, oh, samples, out, threads);
+}
+
+void oracle_bake_prefilter(const float *env, int w, int h, int c, int ow, int oh, float roughness, int samples, int threads, float *out) {
+  bake_prefilter(env, w, h, c, ow, oh, roughness, samples, out, threads);
+}
+
+}  // extern "C"

@@ -1,0 +1,195 @@
+// Oracle (test infrastructure only): environment-map bakes.
+//  * diffuse irradiance: TextureOperations<float>::computeDiffuseIrradiance and
+//    helpers (TextureProcessing.cpp:9-64, TextureProcessing.h:173-301);
+//  * GGX prefiltered specular: the GLSL baker envmap_prefilter.frag:1-116 restated
+//    on an equirect source with level-0 sampling (no CPU version exists in the
+//    reference, so this one is pinned only by closed forms).
+#include "oracle.hpp"
+#include <atomic>
+#include <cmath>
+#include <thread>
+
+namespace oracle {
+
+// math::importance_sampling::pgc3d (math_importance_sampling.h:7-21)
+static vec3 pgc3d(unsigned x, unsigned y, unsigned z) {
+  x = x * 1664525u + 1013904223u;
+  y = y * 1664525u + 1013904223u;
+  z = z * 1664525u + 1013904223u;
+  x += y * z;
+  y += z * x;
+  z += x * y;
+  x ^= x >> 16u;
+  y ^= y >> 16u;
+  z ^= z >> 16u;
+  x += y * z;
+  y += z * x;
+  z += x * y;
+  const float s = 1.f / float(0xFFFFFFFFu);
+  return vec3((float)x * s, (float)y * s, (float)z * s);
+}
+
+static unsigned uv_to_pixel(float coord, unsigned dim) { return (unsigned)(int64_t)(coord * (float)dim); }
+
+// sphericalToCartesian<float> (math_spherical.h:33-39)
+static vec3 sph_to_cart(float phi, float theta) {
+  const float z = std::cos(theta);
+  const float x = std::sin(theta) * std::cos(phi);
+  const float y = std::sin(theta) * std::sin(phi);
+  return {x, y, z};
+}
+
+struct EnvView {
+  const float *data;
+  int w, h, c;
+  // discreteSample + wrapAroundPixelCoords (TextureProcessing.h:173-180, 234-251)
+  vec3 discrete(int x, int y) const {
+    int xc, yc;
+    if (x >= w) xc = x % w;
+    else if (x < 0) xc = w + (x % w);
+    else xc = x;
+    if (y >= h) yc = y % h;
+    else if (y < 0) yc = h + (y % h);
+    else yc = y;
+    int idx = (yc * w + xc) * c;
+    return {data[idx], data[idx + 1], data[idx + 2]};
+  }
+  // uvSample (TextureProcessing.h:199-210): integer pixel coords give the top-left texel.
+  vec3 uv_sample(float u, float v) const {
+    float ui = std::floor(u), vi = std::floor(v);
+    float up = (u > 1. || u < 0.) ? u - ui : u;
+    float vp = (v > 1. || v < 0.) ? v - vi : v;
+    float px = (float)uv_to_pixel(up, (unsigned)w), py = (float)uv_to_pixel(vp, (unsigned)h);
+    float tlx = std::floor(px), tly = std::floor(py);
+    float fu = (px - tlx) / ((tlx + 1) - tlx), fv = (py - tly) / ((tly + 1) - tly);
+    vec3 tl = discrete((int)tlx, (int)tly), tr = discrete((int)(tlx + 1), (int)tly);
+    vec3 bl = discrete((int)tlx, (int)(tly + 1)), br = discrete((int)(tlx + 1), (int)(tly + 1));
+    float hd = 1 - fu;
+    vec3 top = hd * tl + fu * tr;
+    vec3 bot = hd * bl + fu * br;
+    return (1 - fv) * top + fv * bot;
+  }
+  vec3 dir_sample(const vec3 &d) const {
+    float phi = atan2_approx(d.y, d.x), theta = acos_lt(d.z);
+    const double inv_2pi = 1.f / (2.f * M_PI), inv_pi = 1.f / M_PI;
+    float u = (float)(phi * inv_2pi), v = (float)(theta * inv_pi);
+    return uv_sample(u, v);
+  }
+};
+
+template <class F>
+static void parallel_rows(int n, int threads, F &&fn) {
+  std::atomic<int> next{0};
+  auto worker = [&]() {
+    for (;;) {
+      int k = next.fetch_add(1);
+      if (k >= n) break;
+      fn(k);
+    }
+  };
+  if (threads <= 1) {
+    worker();
+    return;
+  }
+  std::vector<std::thread> pool;
+  for (int i = 0; i < threads; i++) pool.emplace_back(worker);
+  for (auto &t : pool) t.join();
+}
+
+// texel (i, j) of an out_w x out_h map: TextureProcessing.h:253-301
+static vec3 irradiance_texel(const EnvView &env, unsigned i, unsigned j, unsigned ow, unsigned oh, unsigned samples) {
+  float u = (float)((double)i / (double)ow), v = (float)((double)j / (double)oh);
+  float phi = (float)(2 * M_PI * u), theta = (float)(M_PI * v);
+  vec3 nrm = sph_to_cart(phi, theta);
+  float aphi = atan2_approx(nrm.y, nrm.x), atheta = acos_lt(nrm.z);
+  const double inv_2pi = 1.f / (2.f * M_PI), inv_pi = 1.f / M_PI;
+  float uu = (float)(aphi * inv_2pi), vv = (float)(atheta * inv_pi);
+  const unsigned px = uv_to_pixel(uu, ow), py = uv_to_pixel(vv, oh);
+  vec3 right(1.f, 0.f, 0.f);
+  float dd = dot(right, nrm);
+  vec3 tangent(0.f, 1.f, 0.f);
+  if (1.0 - std::fabs(dd) > 1e-6) tangent = normalize(cross(right, nrm));
+  vec3 bitangent = cross(nrm, tangent);
+  vec3 irr(0.f);
+  for (unsigned s = 0; s < samples; s++) {
+    vec3 r = pgc3d(px, py, s);
+    float sphi = (float)(2 * M_PI * r.x);
+    float stheta = std::asin(std::sqrt(r.y));
+    vec3 c = sph_to_cart(sphi, stheta);
+    c = c.x * tangent + c.y * bitangent + c.z * nrm;
+    irr = irr + env.dir_sample(c);
+  }
+  return irr / (float)samples;
+}
+
+void bake_irradiance(const float *env, int env_w, int env_h, int env_c, int out_w, int out_h, int samples, float *out, int threads) {
+  EnvView ev{env, env_w, env_h, env_c};
+  parallel_rows(out_w, threads, [&](int i) {
+    for (int j = 0; j < out_h; j++) {
+      vec3 irr = irradiance_texel(ev, (unsigned)i, (unsigned)j, (unsigned)out_w, (unsigned)out_h, (unsigned)samples);
+      unsigned idx = ((unsigned)j * out_w + i) * env_c;
+      out[idx] = irr.x;
+      out[idx + 1] = irr.y;
+      out[idx + 2] = irr.z;
+      for (int k = 3; k < env_c; k++) out[idx + k] = 0.f;
+    }
+  });
+}
+
+// envmap_prefilter.frag:28-35 radicalInverse and Hammersley
+static float radical_inverse(unsigned bits) {
+  bits = (bits << 16u) | (bits >> 16u);
+  bits = ((bits & 0x55555555u) << 1u) | ((bits & 0xAAAAAAAAu) >> 1u);
+  bits = ((bits & 0x33333333u) << 2u) | ((bits & 0xCCCCCCCCu) >> 2u);
+  bits = ((bits & 0x0F0F0F0Fu) << 4u) | ((bits & 0xF0F0F0F0u) >> 4u);
+  bits = ((bits & 0x00FF00FFu) << 8u) | ((bits & 0xFF00FF00u) >> 8u);
+  return (float)bits * 2.3283064365386963e-10f;
+}
+
+// ImportanceSampleGGX (envmap_prefilter.frag:52-70)
+static vec3 importance_sample_ggx(float xi0, float xi1, const vec3 &N, float r) {
+  const float PI = 3.14159265f;
+  float a = r * r;
+  float phi = 2.0f * PI * xi0;
+  float ct = std::sqrt((1.0f - xi1) / (1.0f + (a * a - 1.0f) * xi1));
+  float st = std::sqrt(1.0f - ct * ct);
+  vec3 H(std::cos(phi) * st, std::sin(phi) * st, ct);
+  vec3 up = std::fabs(N.z) < 0.999f ? vec3(0.f, 0.f, 1.f) : vec3(1.f, 0.f, 0.f);
+  vec3 t = normalize(cross(up, N));
+  vec3 b = cross(N, t);
+  vec3 sv = t * H.x + b * H.y + N * H.z;
+  return normalize(sv);
+}
+
+void bake_prefilter(const float *env, int env_w, int env_h, int env_c, int out_w, int out_h, float roughness, int samples, float *out, int threads) {
+  EnvView ev{env, env_w, env_h, env_c};
+  parallel_rows(out_h, threads, [&](int j) {
+    for (int i = 0; i < out_w; i++) {
+      float u = (float)((double)i / (double)out_w), v = (float)((double)j / (double)out_h);
+      vec3 N = normalize(sph_to_cart((float)(2 * M_PI * u), (float)(M_PI * v)));
+      vec3 col(0.f);
+      if (roughness == 0.f) {
+        col = ev.dir_sample(N);
+      } else {
+        float total = 0.f;
+        for (int s = 0; s < samples; s++) {
+          vec3 H = importance_sample_ggx((float)s / (float)samples, radical_inverse((unsigned)s), N, roughness);
+          vec3 L = reflect(-N, H);
+          float ndl = std::fmax(dot(N, L), 0.f);
+          if (ndl > 0.f) {
+            col = col + ev.dir_sample(L) * ndl;
+            total += ndl;
+          }
+        }
+        col = col / total;
+      }
+      unsigned idx = ((unsigned)j * out_w + i) * env_c;
+      out[idx] = col.x;
+      out[idx + 1] = col.y;
+      out[idx + 2] = col.z;
+      for (int k = 3; k < env_c; k++) out[idx + k] = 0.f;
+    }
+  });
+}
+
+}  // namespace oracle
