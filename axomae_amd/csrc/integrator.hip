@@ -59,10 +59,14 @@ __device__ __forceinline__ bool mt_test(const TriMT &tr, f3 o, f3 d, float tnear
 // Closest hit over the 4-wide BVH. Boxes are tested with a slightly widened slab so no
 // box holding a hit the reference's division-form test accepts is culled; triangle
 // tests are exact, so the hit is the reference's except at equal-t ties.
+template <bool kCount>
 __device__ __forceinline__ int trace_closest(const TraceCtx &tc, uint32_t *lds, uint32_t gtid, f3 o, f3 d, float tnear, float &t_out, float &u_out,
-                                             float &v_out) {
+                                             float &v_out, uint32_t &nodes_visited, uint32_t &tri_tests) {
   const float kUp = 1.000001f;
-  const f3 inv = mk3(1.f / d.x, 1.f / d.y, 1.f / d.z);
+  // A zero direction component takes a huge finite reciprocal instead of infinity, so a ray
+  // running inside a box face plane gives 0 * big = 0 (inside the slab) rather than NaN.
+  const f3 inv = mk3(d.x != 0.f ? 1.f / d.x : copysignf(3.0e38f, d.x), d.y != 0.f ? 1.f / d.y : copysignf(3.0e38f, d.y),
+                     d.z != 0.f ? 1.f / d.z : copysignf(3.0e38f, d.z));
   float tbest = 3.402823466e+38f;
   int best = -1;
   float bu = 0.f, bv = 0.f;
@@ -71,6 +75,7 @@ __device__ __forceinline__ int trace_closest(const TraceCtx &tc, uint32_t *lds, 
   const int lane_off = threadIdx.x;
   // A tree walk visits each node once; the cap only guards against a malformed tree.
   for (uint32_t visits = 0; visits < (1u << 24); visits++) {
+    if (kCount) nodes_visited++;
     const float4 *np = reinterpret_cast<const float4 *>(tc.nodes + node);
     const float4 mnx = np[0], mxx = np[1], mny = np[2], mxy = np[3], mnz = np[4], mxz = np[5];
     const int4 ch = *reinterpret_cast<const int4 *>(np + 6);
@@ -125,6 +130,7 @@ __device__ __forceinline__ int trace_closest(const TraceCtx &tc, uint32_t *lds, 
       const int32_t start = chv[k], cnt = cnv[k];
       for (int32_t i = start; i < start + cnt; i++) {
         const TriMT tr = tc.tris[i];
+        if (kCount) tri_tests++;
         float tt, uu, vv;
         if (mt_test(tr, o, d, tnear, tbest, tt, uu, vv)) {
           tbest = tt;
@@ -154,6 +160,7 @@ struct Queues {
   float *ray[2];      // 6 planes of cap floats each: ox oy oz dx dy dz
   int32_t *slot[2];
   uint32_t *count;    // [0], [1]: queue sizes; [2]: segments traced
+  unsigned long long *stats64;  // [0] node visits, [1] triangle tests, [2] escaped segments
   int32_t *hit_prim;
   float *hit_t, *hit_u, *hit_v;
   float *T, *L, *A;   // path state by slot: T,L 3 planes of cap each; A alpha
@@ -232,8 +239,10 @@ __global__ __launch_bounds__(256) void generate_kernel(FrameParams fp, const uin
   }
 }
 
+template <bool kCount>
 __global__ __launch_bounds__(kTraceBlock) void extend_kernel(TraceCtx tc, Queues q, int which) {
   __shared__ uint32_t lds[kLdsStack * kTraceBlock];
+  uint32_t nv = 0, nt = 0;
   const uint32_t n = q.count[which];
   const float *r = q.ray[which];
   const size_t cap = q.cap;
@@ -243,11 +252,15 @@ __global__ __launch_bounds__(kTraceBlock) void extend_kernel(TraceCtx tc, Queues
     const f3 o = mk3(r[i], r[cap + i], r[2 * cap + i]);
     const f3 d = mk3(r[3 * cap + i], r[4 * cap + i], r[5 * cap + i]);
     float t, u, v;
-    const int prim = trace_closest(tc, lds, gtid, o, d, 0.000001f, t, u, v);
+    const int prim = trace_closest<kCount>(tc, lds, gtid, o, d, 0.000001f, t, u, v, nv, nt);
     q.hit_prim[i] = prim;
     q.hit_t[i] = t;
     q.hit_u[i] = u;
     q.hit_v[i] = v;
+  }
+  if (kCount) {
+    atomicAdd(&q.stats64[0], (unsigned long long)nv);
+    atomicAdd(&q.stats64[1], (unsigned long long)nt);
   }
 }
 
@@ -276,7 +289,7 @@ __global__ __launch_bounds__(256) void shade_kernel(ShadeParams sp, ShadeCtx sc,
   const uint32_t iters = (n + stride - 1) / stride;
   for (uint32_t it = 0; it < iters; it++) {
     const uint32_t i = it * stride + blockIdx.x * blockDim.x + threadIdx.x;
-    bool cont = false;
+    bool cont = false, escaped = false;
     f3 no = mk3(0, 0, 0), nd = mk3(0, 0, 0);
     int32_t slot = -1;
     if (i < n) {
@@ -289,6 +302,7 @@ __global__ __launch_bounds__(256) void shade_kernel(ShadeParams sp, ShadeCtx sc,
       const int prim = q.hit_prim[i];
       bool done = true;
       if (prim < 0) {
+        escaped = true;
         const float4 env = sample_envmap(sc, d);
         L = L + T * mk3(env.x, env.y, env.z);
         A = A + env.w;
@@ -423,6 +437,9 @@ __global__ __launch_bounds__(256) void shade_kernel(ShadeParams sp, ShadeCtx sc,
         q.A[slot] = A;
       }
     }
+    const uint64_t esc = __ballot(escaped);
+    if (esc != 0 && (int)(threadIdx.x & 63) == __builtin_ffsll((long long)esc) - 1)
+      atomicAdd(&q.stats64[2], (unsigned long long)__popcll(esc));
     // compact survivors into the next queue
     const uint64_t m = __ballot(cont);
     const uint32_t lane_rank = __builtin_amdgcn_mbcnt_hi((uint32_t)(m >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)m, 0));
@@ -472,7 +489,8 @@ __global__ __launch_bounds__(kTraceBlock) void closest_hit_kernel(TraceCtx tc, c
     const f3 o = mk3(rays[i * 6], rays[i * 6 + 1], rays[i * 6 + 2]);
     const f3 d = mk3(rays[i * 6 + 3], rays[i * 6 + 4], rays[i * 6 + 5]);
     float t, u, v;
-    const int p = trace_closest(tc, lds, gtid, o, d, 0.000001f, t, u, v);
+    uint32_t nv = 0, nt = 0;
+    const int p = trace_closest<false>(tc, lds, gtid, o, d, 0.000001f, t, u, v, nv, nt);
     prim[i] = p;
     tuv[i * 3] = p >= 0 ? t : 0.f;
     tuv[i * 3 + 1] = u;
@@ -548,6 +566,7 @@ extern "C" int nova_hip_render_frames(nova_hip_ctx *ctx, const nova_hip_render_p
   if (!slots) return check_hip(ctx, e, "queue alloc");
   uint32_t *counts = (uint32_t *)ensure_scratch(ctx, 2, 64, &e);
   if (!counts) return check_hip(ctx, e, "counter alloc");
+  unsigned long long *stats64 = (unsigned long long *)(counts + 8);
   float *hits = (float *)ensure_scratch(ctx, 3, cap * 4 * sizeof(float), &e);
   if (!hits) return check_hip(ctx, e, "hit alloc");
   float *state = (float *)ensure_scratch(ctx, 4, cap * 7 * sizeof(float), &e);
@@ -559,6 +578,7 @@ extern "C" int nova_hip_render_frames(nova_hip_ctx *ctx, const nova_hip_render_p
   q.slot[0] = slots;
   q.slot[1] = slots + cap;
   q.count = counts;
+  q.stats64 = stats64;
   q.hit_prim = (int32_t *)hits;
   q.hit_t = hits + cap;
   q.hit_u = hits + 2 * cap;
@@ -578,6 +598,16 @@ extern "C" int nova_hip_render_frames(nova_hip_ctx *ctx, const nova_hip_render_p
   const uint32_t shade_blocks = (uint32_t)std::min<size_t>((cap + 255) / 256, (size_t)cus * 8);
   NOVA_HIP_CHECK(ctx, hipMemsetAsync(counts, 0, 64, s));
 
+  const bool timing = (p->flags & NOVA_HIP_RENDER_TIMING) != 0;
+  const bool counting = (p->flags & NOVA_HIP_RENDER_COUNT_TRAVERSAL) != 0;
+  std::vector<hipEvent_t> kev;  // pairs around extend, then shade launches
+  auto stamp = [&](void) -> int {
+    hipEvent_t ev;
+    NOVA_HIP_CHECK(ctx, hipEventCreate(&ev));
+    kev.push_back(ev);
+    NOVA_HIP_CHECK(ctx, hipEventRecord(ev, s));
+    return NOVA_HIP_SUCCESS;
+  };
   hipEvent_t ev0, ev1;
   NOVA_HIP_CHECK(ctx, hipEventCreate(&ev0));
   NOVA_HIP_CHECK(ctx, hipEventCreate(&ev1));
@@ -603,7 +633,12 @@ extern "C" int nova_hip_render_frames(nova_hip_ctx *ctx, const nova_hip_render_p
     uint32_t dim = 2;
     int which = 0;
     for (int k = 0; k <= p->max_depth; k++) {
-      hipLaunchKernelGGL(extend_kernel, dim3(trace_blocks), dim3(kTraceBlock), 0, s, tc, q, which);
+      if (timing && (st = stamp())) return st;
+      if (counting)
+        hipLaunchKernelGGL(extend_kernel<true>, dim3(trace_blocks), dim3(kTraceBlock), 0, s, tc, q, which);
+      else
+        hipLaunchKernelGGL(extend_kernel<false>, dim3(trace_blocks), dim3(kTraceBlock), 0, s, tc, q, which);
+      if (timing && (st = stamp())) return st;
       hipLaunchKernelGGL(reset_count_kernel, dim3(1), dim3(1), 0, s, counts, which ^ 1);
       ShadeParams spp;
       spp.fp = fp;
@@ -611,7 +646,9 @@ extern "C" int nova_hip_render_frames(nova_hip_ctx *ctx, const nova_hip_render_p
       spp.last = (k == p->max_depth) ? 1 : 0;
       spp.rr = (p->rr_start >= 0 && k >= p->rr_start) ? 1 : 0;
       spp.dim0 = dim;
+      if (timing && (st = stamp())) return st;
       hipLaunchKernelGGL(shade_kernel, dim3(shade_blocks), dim3(256), 0, s, spp, sc, ctx->sobol_table, q, which);
+      if (timing && (st = stamp())) return st;
       NOVA_HIP_CHECK(ctx, hipGetLastError());
       dim += 4 + (spp.rr ? 1 : 0);
       which ^= 1;
@@ -629,10 +666,27 @@ extern "C" int nova_hip_render_frames(nova_hip_ctx *ctx, const nova_hip_render_p
   hipEventDestroy(ev0);
   hipEventDestroy(ev1);
   uint32_t seg = 0;
+  unsigned long long s64[3] = {0, 0, 0};
   NOVA_HIP_CHECK(ctx, hipMemcpy(&seg, counts + 2, sizeof(uint32_t), hipMemcpyDeviceToHost));
+  NOVA_HIP_CHECK(ctx, hipMemcpy(s64, stats64, sizeof(s64), hipMemcpyDeviceToHost));
+  float ext_ms = 0.f, shade_ms = 0.f;
+  for (size_t k = 0; k + 3 < kev.size(); k += 4) {
+    float a = 0.f, b = 0.f;
+    hipEventElapsedTime(&a, kev[k], kev[k + 1]);
+    hipEventElapsedTime(&b, kev[k + 2], kev[k + 3]);
+    ext_ms += a;
+    shade_ms += b;
+  }
+  for (hipEvent_t ev : kev) hipEventDestroy(ev);
   ctx->stats.total_ms = ms;
+  ctx->stats.extend_ms = ext_ms;
+  ctx->stats.shade_ms = shade_ms;
+  ctx->stats.other_ms = timing ? ms - ext_ms - shade_ms : 0.f;
   ctx->stats.segments = seg;
   ctx->stats.extend_launches = launches;
+  ctx->stats.node_visits = counting ? s64[0] : 0;
+  ctx->stats.triangle_tests = counting ? s64[1] : 0;
+  ctx->stats.escaped = s64[2];
   return NOVA_HIP_SUCCESS;
 }
 

@@ -61,12 +61,16 @@ class BvhInfo(C.Structure):
 class RenderParams(C.Structure):
     _fields_ = [("width", C.c_int32), ("height", C.c_int32), ("max_depth", C.c_int32), ("frame_start", C.c_int32), ("frame_count", C.c_int32),
                 ("max_samples_offset", C.c_uint32), ("vertical_invert", C.c_int32), ("rr_start", C.c_int32), ("inv_P", C.c_float * 16),
-                ("inv_V", C.c_float * 16)]
+                ("inv_V", C.c_float * 16), ("flags", C.c_int32)]
+
+
+RENDER_TIMING = 1
+RENDER_COUNT_TRAVERSAL = 2
 
 
 class RenderStats(C.Structure):
     _fields_ = [("total_ms", C.c_float), ("extend_ms", C.c_float), ("shade_ms", C.c_float), ("other_ms", C.c_float), ("segments", C.c_uint64),
-                ("extend_launches", C.c_uint64)]
+                ("extend_launches", C.c_uint64), ("node_visits", C.c_uint64), ("triangle_tests", C.c_uint64), ("escaped", C.c_uint64)]
 
 
 BNODE_DTYPE = np.dtype([("min", np.float32, 3), ("max", np.float32, 3), ("left", np.int32), ("count", np.int32)])
@@ -223,7 +227,7 @@ class HipEngine:
 
     # --------------------------------------------------------------- render
     def render(self, width, height, frames, max_depth, frame_start=0, rr_start=-1, max_samples_offset=0, vertical_invert=False, accum=None,
-               inv_P=None, inv_V=None):
+               inv_P=None, inv_V=None, flags=0):
         """Renders frames [frame_start, frame_start+frames); returns (final, accum) device tensors (H, W, 4)."""
         torch = self.torch
         p = RenderParams()
@@ -232,6 +236,7 @@ class HipEngine:
         p.max_samples_offset = max_samples_offset
         p.vertical_invert = int(vertical_invert)
         p.rr_start = rr_start
+        p.flags = flags
         p.inv_P[:] = [float(x) for x in (self.scene.inv_P if inv_P is None else inv_P)]
         p.inv_V[:] = [float(x) for x in (self.scene.inv_V if inv_V is None else inv_V)]
         if accum is None:

@@ -1,0 +1,160 @@
+"""Benchmark of the Nova hot path on one MI355X.
+
+Workload: the 1,048,576-triangle scene (512x512-cell fBm terrain + 64 octahedral
+spheres), 1920x1080, depth 8, Russian roulette from the 4th scatter, FP32 sky.
+A step is one frame: one path per pixel (the reference renders 1 spp per frame and
+accumulates). The timed region renders K frames (frames 0..K-1) after W warm-up
+frames; value = width*height*K / t / 1e6 Msamples/s. Scene upload and BVH build are
+outside the timed region and reported separately.
+
+roofline: the dominant kernel is the extend (closest-hit traversal) kernel. Its
+algorithmic bytes per ray are 48 (ray in, hit out) + 128 per 4-wide node visited + 48
+per triangle tested, with visits/tests counted by an instrumented render of the same
+frames; its duration comes from HIP events the library records around every extend
+launch on the render stream during the timed region.
+
+cpu_baseline: the oracle (the CPU restatement of the reference renderer) on the host
+cores, timed on a band of rows of frame 1 of the same workload.
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+METRIC = "Msamples/s (paths) at 1080p, depth 8; adjoint Msamples/s vs primal, 1 GPU"
+HBM_PEAK_GBS = 8000.0
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def host_cores():
+    try:
+        n = len(os.sched_getaffinity(0))
+    except AttributeError:
+        n = os.cpu_count() or 1
+    return max(1, min(n, int(os.environ.get("OMP_NUM_THREADS", n)), 16))
+
+
+def cpu_baseline(scene, width, height, depth, rr_start, target_s):
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import pyoracle
+    cores = host_cores()
+    t0 = time.perf_counter()
+    osc = pyoracle.OracleScene(scene)
+    build_s = time.perf_counter() - t0
+    log(f"[cpu] oracle scene + reference BVH build: {build_s:.1f} s")
+    mid = height // 2
+    rows = 2
+    t0 = time.perf_counter()
+    pyoracle.render_rows(osc, width, height, 1, mid, mid + rows, depth, rr_start, cores)
+    dt = time.perf_counter() - t0
+    rows = max(2, min(height - mid, int(rows * target_s / max(dt, 1e-3))))
+    t0 = time.perf_counter()
+    pyoracle.render_rows(osc, width, height, 1, mid, mid + rows, depth, rr_start, cores)
+    dt = time.perf_counter() - t0
+    paths = rows * width
+    return {"value": paths / dt / 1e6, "unit": "Msamples/s", "cores": cores, "kind": "port",
+            "sample": f"rows {mid}..{mid + rows - 1} of frame 1 ({paths} paths, 1 spp, depth {depth}) of the same workload, "
+                      f"{dt:.1f} s on {cores} threads; oracle BVH build {build_s:.1f} s excluded"}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=64, help="frames (1 spp each) in the timed region")
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--width", type=int, default=1920)
+    ap.add_argument("--height", type=int, default=1080)
+    ap.add_argument("--depth", type=int, default=8)
+    ap.add_argument("--rr-start", type=int, default=3)
+    ap.add_argument("--cpu-seconds", type=float, default=15.0)
+    ap.add_argument("--no-cpu", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py measures one GPU: every box here has one MI355X (asked for --gpus {args.gpus}); "
+              "the multi-GPU sharding of this path is described in DESIGN.md", file=sys.stderr)
+        sys.exit(2)
+
+    import torch
+    from axomae_amd import scenes
+    from axomae_amd.nova_hip import HipEngine, RENDER_COUNT_TRAVERSAL, RENDER_TIMING
+
+    W, H, K = args.width, args.height, args.steps
+    t0 = time.perf_counter()
+    sc = scenes.million_triangles(width=W, height=H)
+    log(f"[bench] scene {sc.triangle_count} triangles generated in {time.perf_counter() - t0:.1f} s")
+    eng = HipEngine(0)
+    t0 = time.perf_counter()
+    eng.upload(sc)
+    torch.cuda.synchronize()
+    upload_s = time.perf_counter() - t0
+    info = eng.build_bvh()
+    log(f"[bench] upload {upload_s * 1e3:.0f} ms, BVH build {info.build_ms:.1f} ms, binary nodes {info.binary_nodes} depth {info.binary_depth}, "
+        f"wide nodes {info.wide_nodes} depth {info.wide_depth}, leaves {info.leaves}")
+
+    # traversal counts of the timed frames (deterministic per ray)
+    eng.render(W, H, K, args.depth, rr_start=args.rr_start, flags=RENDER_COUNT_TRAVERSAL)
+    torch.cuda.synchronize()
+    cst = eng.render_stats()
+    log(f"[bench] instrumented: segments {cst.segments}, node visits {cst.node_visits}, triangle tests {cst.triangle_tests}, escaped {cst.escaped}")
+
+    if args.warmup > 0:
+        eng.render(W, H, args.warmup, args.depth, rr_start=args.rr_start)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    final, accum = eng.render(W, H, K, args.depth, rr_start=args.rr_start, flags=RENDER_TIMING)
+    torch.cuda.synchronize()
+    t = time.perf_counter() - t0
+    st = eng.render_stats()
+    log(f"[bench] timed {K} frames: {t * 1e3:.1f} ms (device {st.total_ms:.1f} ms; extend {st.extend_ms:.1f}, shade {st.shade_ms:.1f}, "
+        f"other {st.other_ms:.1f}), segments {st.segments}, extend launches {st.extend_launches}")
+    finite = bool(torch.isfinite(final).all().item())
+
+    paths = W * H * K
+    value = paths / t / 1e6
+    ext_bytes = 48.0 * cst.segments + 128.0 * cst.node_visits + 48.0 * cst.triangle_tests
+    ext_s = st.extend_ms / 1e3
+    achieved = ext_bytes / ext_s / 1e9 if ext_s > 0 else 0.0
+    per_launch = ext_bytes / max(st.extend_launches, 1)
+    # whole-path algorithmic bytes: 48 + 424 per segment + node/triangle bytes + 20 per escape + 16 per path
+    path_bytes = 472.0 * cst.segments + 128.0 * cst.node_visits + 48.0 * cst.triangle_tests + 20.0 * cst.escaped + 16.0 * paths
+    traffic = None
+    tfile = os.path.join(ROOT, "profiles", "extend_hbm_traffic.json")
+    if os.path.exists(tfile):
+        traffic = json.load(open(tfile)).get("bytes_per_launch")
+
+    out = {
+        "metric": METRIC, "value": round(value, 3), "unit": "Msamples/s", "n_gpus": 1, "steps": K, "warmup": args.warmup,
+        "ms_per_step": round(t * 1e3 / K, 3), "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f32",
+        "data": "synthetic",
+        "config": {"workload": "million_triangles_1080p", "triangles": sc.triangle_count, "width": W, "height": H, "spp": K,
+                   "max_depth": args.depth, "russian_roulette_from_scatter": args.rr_start, "sky": "2048x1024 RGBA32F"},
+        "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic, "kernel": "extend_kernel",
+                     "algorithmic_bytes_per_launch": round(per_launch), "avg_launch_ms": round(st.extend_ms / max(st.extend_launches, 1), 4)},
+        "path_algorithmic_GBs": round(path_bytes / t / 1e9, 1),
+        "bvh": {"build_ms": round(info.build_ms, 2), "binary_nodes": info.binary_nodes, "binary_depth": info.binary_depth,
+                "wide_nodes": info.wide_nodes, "wide_depth": info.wide_depth, "leaves": info.leaves, "upload_ms": round(upload_s * 1e3, 1)},
+        "per_segment": {"node_visits": round(cst.node_visits / max(cst.segments, 1), 2),
+                        "triangle_tests": round(cst.triangle_tests / max(cst.segments, 1), 2),
+                        "segments_per_path": round(cst.segments / paths, 3)},
+        "image_finite": finite,
+    }
+    eng.close()
+    if not args.no_cpu:
+        cb = cpu_baseline(sc, W, H, args.depth, args.rr_start, args.cpu_seconds)
+        out["cpu_baseline"] = cb
+        out["gpu_over_cpu"] = round(value / cb["value"], 1)
+    else:
+        out["cpu_baseline"] = None
+    print(json.dumps(out))
+
+
+if __name__ == "__main__":
+    main()

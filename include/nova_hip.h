@@ -145,7 +145,12 @@ typedef struct {
   int32_t vertical_invert;  /* EngineResourcesHolder::vertical_invert (Integrator.h:38-42) */
   int32_t rr_start;         /* Russian-roulette extension: first scatter index that rolls, -1 = off */
   float inv_P[16], inv_V[16]; /* CameraResourcesHolder inv_P / inv_V, column-major */
+  int32_t flags;            /* NOVA_HIP_RENDER_TIMING | NOVA_HIP_RENDER_COUNT_TRAVERSAL, 0 for none */
 } nova_hip_render_params;
+
+/* flags: time every extend/shade launch with HIP events on the render stream; count BVH node
+ * visits and triangle tests per ray (an instrumented traversal, slower). */
+enum { NOVA_HIP_RENDER_TIMING = 1, NOVA_HIP_RENDER_COUNT_TRAVERSAL = 2 };
 
 /* Renders frames [frame_start, frame_start+frame_count) of the path integrator
  * (PathIntegrator::Li, corelib/nova/integrator/Integrator.cpp:59-98, with the CPU
@@ -160,12 +165,17 @@ typedef struct {
 int nova_hip_render_frames(nova_hip_ctx *ctx, const nova_hip_render_params *params, float *accum_dev, float *final_dev,
                            float *partial_dev, void *stream);
 
-/* Per-kernel device time of the last nova_hip_render_frames call, in milliseconds, and
- * the number of path segments traced (extend launches x rays). */
+/* Statistics of the last nova_hip_render_frames call: device time of the whole call and,
+ * with NOVA_HIP_RENDER_TIMING, summed extend (traversal) and shade kernel times in
+ * milliseconds; path segments traced (rays entering an extend launch) and launches; with
+ * NOVA_HIP_RENDER_COUNT_TRAVERSAL, 4-wide node visits and triangle tests summed over rays. */
 typedef struct {
   float total_ms, extend_ms, shade_ms, other_ms;
   uint64_t segments;
   uint64_t extend_launches;
+  uint64_t node_visits;
+  uint64_t triangle_tests;
+  uint64_t escaped;          /* segments whose ray left the scene (envmap lookups after a miss) */
 } nova_hip_render_stats;
 int nova_hip_render_stats_get(nova_hip_ctx *ctx, nova_hip_render_stats *out);
 

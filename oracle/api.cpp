@@ -161,6 +161,22 @@ void oracle_render(void *sp, void *bp, int width, int height, int frame_start, i
   render(*static_cast<Scene *>(sp), *static_cast<Bvh *>(bp), p, accum, final_out);
 }
 
+// Rows [y0, y1) of frame `frame` into partial (width*height*4 floats), `threads` workers.
+void oracle_render_rows(void *sp, void *bp, int width, int height, int frame, int y0, int y1, int max_depth, int rr_start, const float *inv_P,
+                        const float *inv_V, int threads, float *partial) {
+  RenderParams p;
+  p.width = width;
+  p.height = height;
+  p.max_depth = max_depth;
+  p.rr_start = rr_start;
+  for (int i = 0; i < 16; i++) {
+    p.inv_P.m[i] = inv_P[i];
+    p.inv_V.m[i] = inv_V[i];
+  }
+  p.threads = threads;
+  render_rows(*static_cast<Scene *>(sp), *static_cast<Bvh *>(bp), p, frame, y0, y1, partial);
+}
+
 void oracle_bake_irradiance(const float *env, int w, int h, int c, int ow, int oh, int samples, int threads, float *out) {
   bake_irradiance(env, w, h, c, ow, oh, samples, out, threads);
 }

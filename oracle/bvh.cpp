@@ -1,6 +1,7 @@
 // Oracle (test infrastructure only): scene flattening, the reference's binary
 // SAH builder, its stack traversal and Moller-Trumbore triangle test.
 #include "oracle.hpp"
+#include <algorithm>
 #include <cfloat>
 #include <climits>
 #include <stdexcept>
@@ -207,13 +208,28 @@ void bvh_build(const Scene &s, Bvh &bvh) {
   b.run();
 }
 
-// BoundingBox::intersect(dir, origin, dist_min) (BoundingBox.cpp:150-160)
+// BoundingBox::intersect(dir, origin, dist_min) (BoundingBox.cpp:150-160) with one
+// deviation: a slab whose bounds give 0/0 (the ray lies in the box face's plane with a zero
+// direction component) is ignored instead of poisoning the test with NaN. The reference's
+// std::min/std::max chain drops such boxes and misses hits on the face; its default CPU
+// path (Embree) finds them, and the closest hit is defined here as the nearest triangle
+// over the whole scene, so the oracle keeps those boxes.
+static void slab(float lo_plane, float hi_plane, float o, float d, float &t1, float &t2) {
+  t1 = (lo_plane - o) / d;
+  t2 = (hi_plane - o) / d;
+  if (std::isnan(t1) || std::isnan(t2)) {  // 0/0: the ray runs inside a face plane of this slab
+    t1 = -INFINITY;
+    t2 = INFINITY;
+  }
+}
+
 static float box_intersect(const BvhNode &nd, const vec3 &d, const vec3 &o, float dist_min) {
-  float tx1 = (nd.min[0] - o.x) / d.x, tx2 = (nd.max[0] - o.x) / d.x;
+  float tx1, tx2, ty1, ty2, tz1, tz2;
+  slab(nd.min[0], nd.max[0], o.x, d.x, tx1, tx2);
+  slab(nd.min[1], nd.max[1], o.y, d.y, ty1, ty2);
+  slab(nd.min[2], nd.max[2], o.z, d.z, tz1, tz2);
   float tmin = std::min(tx1, tx2), tmax = std::max(tx1, tx2);
-  float ty1 = (nd.min[1] - o.y) / d.y, ty2 = (nd.max[1] - o.y) / d.y;
   tmin = std::max(tmin, std::min(ty1, ty2)), tmax = std::min(tmax, std::max(ty1, ty2));
-  float tz1 = (nd.min[2] - o.z) / d.z, tz2 = (nd.max[2] - o.z) / d.z;
   tmin = std::max(tmin, std::min(tz1, tz2)), tmax = std::min(tmax, std::max(tz1, tz2));
   if (tmax >= tmin && tmax > 0 && tmin <= dist_min) return tmin;
   return FLT_MAX;

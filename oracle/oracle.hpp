@@ -104,6 +104,7 @@ struct RenderParams {
 vec4 path_Li(const Scene &scene, const Bvh &bvh, vec3 o, vec3 d, int depth, SobolSampler &sampler, const RenderParams &p, int scatter_index);
 void render_frame(const Scene &scene, const Bvh &bvh, const RenderParams &p, int frame, float *partial);
 void render(const Scene &scene, const Bvh &bvh, const RenderParams &p, float *accum, float *final_out);
+void render_rows(const Scene &scene, const Bvh &bvh, const RenderParams &p, int frame, int y0, int y1, float *partial);
 
 // ---------------------------------------------------------------- bakes
 void bake_irradiance(const float *env, int env_w, int env_h, int env_c, int out_w, int out_h, int samples, float *out, int threads);

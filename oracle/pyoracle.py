@@ -51,6 +51,7 @@ def lib():
         L.oracle_closest_hit.argtypes = [P, P, f32p, C.c_size_t, C.c_int, C.c_int, i32p, f32p]
         L.oracle_render.argtypes = [P, P, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, C.c_uint32, C.c_int, C.c_int,
                                     f32p, f32p, C.c_int, f32p, f32p]
+        L.oracle_render_rows.argtypes = [P, P, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, f32p, f32p, C.c_int, f32p]
         L.oracle_bake_irradiance.argtypes = [f32p, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, f32p]
         L.oracle_bake_prefilter.argtypes = [f32p, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, C.c_float, C.c_int, C.c_int, f32p]
         _LIB = L
@@ -163,6 +164,14 @@ class OracleScene:
         lib().oracle_render(self.h, self.bvh, width, height, frame_start, frames, max_depth, max_samples_offset, int(vertical_invert),
                             rr_start, self.scene.inv_P, self.scene.inv_V, threads, acc, out)
         return out.reshape(height, width, 4), acc
+
+
+def render_rows(oscene, width, height, frame, y0, y1, max_depth, rr_start=-1, threads=8):
+    """Rows [y0, y1) of one frame (1 spp) of the oracle's CPU render; returns the partial buffer."""
+    out = np.zeros(width * height * 4, dtype=np.float32)
+    lib().oracle_render_rows(oscene.h, oscene.bvh, width, height, frame, y0, y1, max_depth, rr_start, oscene.scene.inv_P, oscene.scene.inv_V,
+                             threads, out)
+    return out.reshape(height, width, 4)
 
 
 def bake_irradiance(env, out_w, out_h, samples, threads=8):

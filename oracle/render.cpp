@@ -792,6 +792,23 @@ void render_frame(const Scene &s, const Bvh &bvh, const RenderParams &p, int fra
   for (auto &th2 : pool) th2.join();
 }
 
+// Rows [y0, y1) of one frame, one row per task (the CPU-baseline sample).
+void render_rows(const Scene &s, const Bvh &bvh, const RenderParams &p, int frame, int y0, int y1, float *partial) {
+  std::atomic<int> next{y0};
+  auto worker = [&]() {
+    for (;;) {
+      int y = next.fetch_add(1);
+      if (y >= y1) break;
+      Tile t{0, y, p.width, y + 1};
+      render_tile(s, bvh, p, frame, t, partial);
+    }
+  };
+  int nthreads = std::max(1, p.threads);
+  std::vector<std::thread> pool;
+  for (int i = 0; i < nthreads; i++) pool.emplace_back(worker);
+  for (auto &th : pool) th.join();
+}
+
 // Engine::renderCpu frame loop + writeFramebuffer (engine.cpp:351-363, 498-525)
 void render(const Scene &s, const Bvh &bvh, const RenderParams &p, float *accum, float *final_out) {
   const size_t n = (size_t)p.width * p.height * 4;
