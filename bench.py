@@ -75,6 +75,8 @@ def main():
     ap.add_argument("--rr-start", type=int, default=3)
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--copy-probe-mb", type=int, default=0,
+                    help="before the render, copy a buffer of this size 3 times (a copy of known bytes for counter calibration)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU: every box here has one MI355X (asked for --gpus {args.gpus}); "
@@ -86,6 +88,15 @@ def main():
     from axomae_amd.nova_hip import HipEngine, RENDER_COUNT_TRAVERSAL, RENDER_TIMING
 
     W, H, K = args.width, args.height, args.steps
+    if args.copy_probe_mb > 0:
+        n = args.copy_probe_mb * (1 << 20) // 4
+        src = torch.ones(n, dtype=torch.float32, device="cuda")
+        dst = torch.empty_like(src)
+        for _ in range(3):
+            dst.copy_(src)
+        torch.cuda.synchronize()
+        log(f"[bench] copy probe: 3 x {args.copy_probe_mb} MiB read + written")
+        del src, dst
     t0 = time.perf_counter()
     sc = scenes.million_triangles(width=W, height=H)
     log(f"[bench] scene {sc.triangle_count} triangles generated in {time.perf_counter() - t0:.1f} s")

@@ -186,3 +186,41 @@ void oracle_bake_prefilter(const float *env, int w, int h, int c, int ow, int oh
 }
 
 }  // extern "C"
+
+namespace oracle {
+float unit_fresnel_real(float eta, float cos_i);
+vec3 unit_correct_shading_normal(const vec3 &ng, const vec3 &wo, const vec3 &ns);
+vec3 unit_vndf_sample(float aniso, float rough, const vec3 &wo, const float u[2]);
+float unit_vndf_G1(float aniso, float rough, const vec3 &v);
+float unit_vndf_D(float aniso, float rough, const vec3 &wm);
+int unit_principled_sample(const vec3 &wo, const vec3 &albedo, float roughness, float metal, float eta, float uc, float u0, float u1, float uc2,
+                           vec3 &f, float &pdf, vec3 &wi, float &costheta, int &valid);
+}  // namespace oracle
+
+extern "C" {
+float oracle_fresnel_real(float eta, float cos_i) { return unit_fresnel_real(eta, cos_i); }
+void oracle_correct_shading_normal(const float *ng, const float *wo, const float *ns, float *out) {
+  vec3 r = unit_correct_shading_normal(vec3(ng[0], ng[1], ng[2]), vec3(wo[0], wo[1], wo[2]), vec3(ns[0], ns[1], ns[2]));
+  out[0] = r.x;
+  out[1] = r.y;
+  out[2] = r.z;
+}
+void oracle_vndf_sample(float aniso, float rough, const float *wo, const float *u, float *out) {
+  vec3 r = unit_vndf_sample(aniso, rough, vec3(wo[0], wo[1], wo[2]), u);
+  out[0] = r.x;
+  out[1] = r.y;
+  out[2] = r.z;
+}
+float oracle_vndf_G1(float aniso, float rough, const float *v) { return unit_vndf_G1(aniso, rough, vec3(v[0], v[1], v[2])); }
+float oracle_vndf_D(float aniso, float rough, const float *v) { return unit_vndf_D(aniso, rough, vec3(v[0], v[1], v[2])); }
+// out: f3, pdf, wi3, costheta, valid, lobe (10 floats)
+void oracle_principled_sample(const float *wo, const float *albedo, float roughness, float metal, float eta, const float *u4, float *out) {
+  vec3 f, wi;
+  float pdf = 0, ct = 0;
+  int valid = 0;
+  int lobe = unit_principled_sample(vec3(wo[0], wo[1], wo[2]), vec3(albedo[0], albedo[1], albedo[2]), roughness, metal, eta, u4[0], u4[1], u4[2],
+                                    u4[3], f, pdf, wi, ct, valid);
+  float vals[10] = {f.x, f.y, f.z, pdf, wi.x, wi.y, wi.z, ct, (float)valid, (float)lobe};
+  for (int i = 0; i < 10; i++) out[i] = vals[i];
+}
+}  // extern "C"

@@ -186,3 +186,53 @@ def bake_prefilter(env, out_w, out_h, roughness, samples, threads=8):
     out = np.zeros(out_w * out_h * env.channels, dtype=np.float32)
     lib().oracle_bake_prefilter(data, env.width, env.height, env.channels, out_w, out_h, roughness, samples, threads, out)
     return out.reshape(out_h, out_w, env.channels)
+
+
+def _vec(v):
+    return np.ascontiguousarray(v, dtype=np.float32)
+
+
+def fresnel_real(eta, cos_i):
+    L = lib()
+    L.oracle_fresnel_real.restype = C.c_float
+    L.oracle_fresnel_real.argtypes = [C.c_float, C.c_float]
+    return L.oracle_fresnel_real(eta, cos_i)
+
+
+def correct_shading_normal(ng, wo, ns):
+    L = lib()
+    L.oracle_correct_shading_normal.argtypes = [f32p, f32p, f32p, f32p]
+    out = np.zeros(3, dtype=np.float32)
+    L.oracle_correct_shading_normal(_vec(ng), _vec(wo), _vec(ns), out)
+    return out
+
+
+def vndf_sample(aniso, rough, wo, u):
+    L = lib()
+    L.oracle_vndf_sample.argtypes = [C.c_float, C.c_float, f32p, f32p, f32p]
+    out = np.zeros(3, dtype=np.float32)
+    L.oracle_vndf_sample(aniso, rough, _vec(wo), _vec(u), out)
+    return out
+
+
+def vndf_G1(aniso, rough, v):
+    L = lib()
+    L.oracle_vndf_G1.restype = C.c_float
+    L.oracle_vndf_G1.argtypes = [C.c_float, C.c_float, f32p]
+    return L.oracle_vndf_G1(aniso, rough, _vec(v))
+
+
+def vndf_D(aniso, rough, wm):
+    L = lib()
+    L.oracle_vndf_D.restype = C.c_float
+    L.oracle_vndf_D.argtypes = [C.c_float, C.c_float, f32p]
+    return L.oracle_vndf_D(aniso, rough, _vec(wm))
+
+
+def principled_sample(wo, albedo, roughness, metal, eta, u4):
+    """One Principled sample with ns = ng = +z; returns (lobe, valid, f, pdf, wi, costheta)."""
+    L = lib()
+    L.oracle_principled_sample.argtypes = [f32p, f32p, C.c_float, C.c_float, C.c_float, f32p, f32p]
+    out = np.zeros(10, dtype=np.float32)
+    L.oracle_principled_sample(_vec(wo), _vec(albedo), roughness, metal, eta, _vec(u4), out)
+    return int(out[9]), bool(out[8]), out[:3].copy(), float(out[3]), out[4:7].copy(), float(out[7])

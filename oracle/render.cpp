@@ -823,3 +823,42 @@ void render(const Scene &s, const Bvh &bvh, const RenderParams &p, float *accum,
 }
 
 }  // namespace oracle
+
+// ------------------------------------------------------------ unit access
+// Entry points for the BxDF property tests (the reference's BxDFTest invariants).
+namespace oracle {
+float unit_fresnel_real(float eta, float cos_i) { return fresnel_real(eta, cos_i); }
+vec3 unit_correct_shading_normal(const vec3 &ng, const vec3 &wo, const vec3 &ns) { return correct_specular_shading_normal(ng, wo, ns); }
+vec3 unit_vndf_sample(float aniso, float rough, const vec3 &wo, const float u[2]) { return VNDF(aniso, rough).sample(wo, u); }
+float unit_vndf_G1(float aniso, float rough, const vec3 &v) { return VNDF(aniso, rough).G1(v); }
+float unit_vndf_D(float aniso, float rough, const vec3 &wm) { return VNDF(aniso, rough).D(wm); }
+// One Principled sample in a frame where ns = ng = +z, dpdu = +x, dpdv = +y (world = local).
+// Returns the lobe (0 none, 1 diffuse, 2 specular, 3 dielectric) and fills f, pdf, wi, costheta
+// when the sample is valid (valid = 0 otherwise).
+int unit_principled_sample(const vec3 &wo, const vec3 &albedo, float roughness, float metal, float eta, float uc, float u0, float u1, float uc2,
+                           vec3 &f, float &pdf, vec3 &wi, float &costheta, int &valid) {
+  BsdfParams bp;
+  bp.ns = vec3(0, 0, 1);
+  bp.ng = vec3(0, 0, 1);
+  bp.dpdu = vec3(1, 0, 0);
+  bp.dpdv = vec3(0, 1, 0);
+  bp.wo_dot_ng = dot(wo, bp.ng);
+  bp.wo_dot_ns = bp.wo_dot_ng;
+  bp.albedo = albedo;
+  bp.roughness = roughness;
+  bp.metal = metal;
+  bp.eta = vec3(eta);
+  bp.transmission = eta != 0.f ? 1.f : 0.f;
+  Principled b;
+  b.configure(bp);
+  b.construct(uc);
+  const float u[2] = {u0, u1};
+  Sample s;
+  valid = b.sample_f(wo, uc2, u, s) ? 1 : 0;
+  f = s.f;
+  pdf = s.pdf;
+  wi = s.wi;
+  costheta = s.costheta;
+  return (int)b.lobe;
+}
+}  // namespace oracle
