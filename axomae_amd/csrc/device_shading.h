@@ -20,7 +20,7 @@ struct ShadeCtx {
 
 // ImageTexture<uint32_t>::sample, CPU branch: bilinear over 4 texels with ensure_bound
 // (NovaTextureInterface.h:56-145, TextureContext.h:129-167). Returns 0..255 channel values.
-__device__ __forceinline__ float4 sample_u32(const ShadeCtx &c, int id, float u, float v) {
+NOVA_HD float4 sample_u32(const ShadeCtx &c, int id, float u, float v) {
   const TexRec t = c.tex[id];
   u = wrap_uv(u);
   v = wrap_uv(v);
@@ -61,7 +61,7 @@ __device__ __forceinline__ float4 sample_u32(const ShadeCtx &c, int id, float u,
 }
 
 // FP32 material-texture extension: the same bilinear rule on float texels.
-__device__ __forceinline__ float4 sample_f32_bilinear(const ShadeCtx &c, int id, float u, float v) {
+NOVA_HD float4 sample_f32_bilinear(const ShadeCtx &c, int id, float u, float v) {
   const TexRec t = c.tex[id];
   u = wrap_uv(u);
   v = wrap_uv(v);
@@ -89,7 +89,7 @@ __device__ __forceinline__ float4 sample_f32_bilinear(const ShadeCtx &c, int id,
 }
 
 // uv2index + uvToPixel (TextureContext.h:52-58)
-__device__ __forceinline__ unsigned uv2index(float t, int dim) {
+NOVA_HD unsigned uv2index(float t, int dim) {
   float a = fabsf(t);
   float rem = a - floorf(a);
   rem = rem == 0 ? a : rem;
@@ -99,7 +99,7 @@ __device__ __forceinline__ unsigned uv2index(float t, int dim) {
 // EnvmapTexture::sample + TextureCtx::f32pixel (NovaTextureInterface.h:161-171,
 // TextureContext.h:196-223): z-up spherical coordinates with atan2_approx / acos LUT,
 // nearest texel.
-__device__ __forceinline__ float4 sample_envmap(const ShadeCtx &c, f3 dir) {
+NOVA_HD float4 sample_envmap(const ShadeCtx &c, f3 dir) {
   if (c.envmap < 0) return make_float4(0.f, 0.f, 0.f, 0.f);
   f3 n = normalize(dir);
   const float phi = atan2_approx(n.y, n.x);
@@ -130,7 +130,7 @@ struct HitRec {
 };
 
 // math::gram_shmidt 3-argument form (utils_3D.h:63-71)
-__device__ __forceinline__ m3 gram_shmidt3(f3 normal, f3 tangent, f3 bitangent) {
+NOVA_HD m3 gram_shmidt3(f3 normal, f3 tangent, f3 bitangent) {
   f3 N = normalize(normal);
   f3 T = normalize(tangent - dot(tangent, N) * N);
   f3 B = cross(N, T);
@@ -139,7 +139,7 @@ __device__ __forceinline__ m3 gram_shmidt3(f3 normal, f3 tangent, f3 bitangent) 
 }
 
 // math::make_onb (utils_3D.h:74-82)
-__device__ __forceinline__ void make_onb(f3 n, f3 &b1, f3 &b2) {
+NOVA_HD void make_onb(f3 n, f3 &b1, f3 &b2) {
   n = normalize(n);
   float sign = copysignf(1.f, n.z);
   float a = -1.f / (sign + n.z);
@@ -148,15 +148,15 @@ __device__ __forceinline__ void make_onb(f3 n, f3 &b1, f3 &b2) {
   b2 = mk3(b, sign + n.y * n.y * a, -n.y);
 }
 
-__device__ __forceinline__ f3 ld3(const float *p) { return mk3(p[0], p[1], p[2]); }
-__device__ __forceinline__ f3 bary3(f3 p0, f3 p1, f3 p2, float w, float u, float v) { return p0 * w + p1 * u + p2 * v; }
-__device__ __forceinline__ f3 mul9(const float *m, f3 v) {  // column-major 3x3
+NOVA_HD f3 ld3(const float *p) { return mk3(p[0], p[1], p[2]); }
+NOVA_HD f3 bary3(f3 p0, f3 p1, f3 p2, float w, float u, float v) { return p0 * w + p1 * u + p2 * v; }
+NOVA_HD f3 mul9(const float *m, f3 v) {  // column-major 3x3
   return mk3((m[0] * v.x + m[3] * v.y) + m[6] * v.z, (m[1] * v.x + m[4] * v.y) + m[7] * v.z, (m[2] * v.x + m[5] * v.y) + m[8] * v.z);
 }
 
 // computeDerivativesTriangle + fillShadingStructure + constructTbnFrame (embree.cpp:287-390)
 // with the closest hit's (t, u, v) standing in for Embree's.
-__device__ __forceinline__ HitRec make_hit(const ShadeCtx &c, const ShadeTri &tr, f3 ro, f3 rd, float t, float bu, float bv) {
+NOVA_HD HitRec make_hit(const ShadeCtx &c, const ShadeTri &tr, f3 ro, f3 rd, float t, float bu, float bv) {
   HitRec h;
   const float w = 1 - (bu + bv);
   h.u = tr.uv0[0] * w + tr.uv1[0] * bu + tr.uv2[0] * bv;
@@ -194,29 +194,29 @@ __device__ __forceinline__ HitRec make_hit(const ShadeCtx &c, const ShadeTri &tr
 
 // ------------------------------------------------------------------ BxDFs
 // bxdf:: helpers (BxDF_math.h:57-124)
-__device__ __forceinline__ float abscos(f3 v) { return fabsf(v.z); }
-__device__ __forceinline__ float absdot(f3 a, f3 b) { return fabsf(dot(a, b)); }
-__device__ __forceinline__ bool same_hemi(f3 a, f3 b) { return a.z * b.z > 0; }
-__device__ __forceinline__ float cos2theta(f3 v) { return sqr(v.z); }
-__device__ __forceinline__ float sin2theta(f3 v) { return smax(0.f, 1.f - cos2theta(v)); }
-__device__ __forceinline__ float sin2theta_c(float c) { return smax(0.f, 1.f - sqr(c)); }
-__device__ __forceinline__ float sintheta(f3 v) { return msqrt(sin2theta(v)); }
-__device__ __forceinline__ float tan2theta(f3 v) { return sin2theta(v) / cos2theta(v); }
-__device__ __forceinline__ float cosphi(f3 v) {
+NOVA_HD float abscos(f3 v) { return fabsf(v.z); }
+NOVA_HD float absdot(f3 a, f3 b) { return fabsf(dot(a, b)); }
+NOVA_HD bool same_hemi(f3 a, f3 b) { return a.z * b.z > 0; }
+NOVA_HD float cos2theta(f3 v) { return sqr(v.z); }
+NOVA_HD float sin2theta(f3 v) { return smax(0.f, 1.f - cos2theta(v)); }
+NOVA_HD float sin2theta_c(float c) { return smax(0.f, 1.f - sqr(c)); }
+NOVA_HD float sintheta(f3 v) { return msqrt(sin2theta(v)); }
+NOVA_HD float tan2theta(f3 v) { return sin2theta(v) / cos2theta(v); }
+NOVA_HD float cosphi(f3 v) {
   float st = sintheta(v);
   if (st == 0.f) return 1.f;
   return clampf(v.x / st, -1.f, 1.f);
 }
-__device__ __forceinline__ float sinphi(f3 v) {
+NOVA_HD float sinphi(f3 v) {
   float st = sintheta(v);
   if (st == 0.f) return 0.f;
   return clampf(v.y / st, -1.f, 1.f);
 }
-__device__ __forceinline__ float cos2phi(f3 v) { return sqr(cosphi(v)); }
-__device__ __forceinline__ float sin2phi(f3 v) { return sqr(sinphi(v)); }
-__device__ __forceinline__ float schlick1(float c, float f0) { return f0 + (1.f - f0) * pow5(1.f - c); }
+NOVA_HD float cos2phi(f3 v) { return sqr(cosphi(v)); }
+NOVA_HD float sin2phi(f3 v) { return sqr(sinphi(v)); }
+NOVA_HD float schlick1(float c, float f0) { return f0 + (1.f - f0) * pow5(1.f - c); }
 
-__device__ __forceinline__ float refract(f3 wo, f3 n, float eta, bool &valid, f3 &wt) {
+NOVA_HD float refract(f3 wo, f3 n, float eta, bool &valid, f3 &wt) {
   valid = true;
   float ci = dot(wo, n);
   if (ci < 0) {
@@ -235,7 +235,7 @@ __device__ __forceinline__ float refract(f3 wo, f3 n, float eta, bool &valid, f3
 }
 
 // Fresnel::real (BxDF_math.h:193-208)
-__device__ __forceinline__ float fresnel_real(float etar, float ci) {
+NOVA_HD float fresnel_real(float etar, float ci) {
   float eta = etar;
   ci = gmin(gmax(ci, -1.f), 1.f);
   if (ci < 0) {
@@ -251,7 +251,7 @@ __device__ __forceinline__ float fresnel_real(float etar, float ci) {
 }
 
 // shirley_chiu + hemisphere_cosine_sample (BxDF_math.h:129-157)
-__device__ __forceinline__ f3 cosine_hemisphere(float u0, float u1) {
+NOVA_HD f3 cosine_hemisphere(float u0, float u1) {
   float sx = 2 * u0 - 1, sy = 2 * u1 - 1;
   float x = 0.f, y = 0.f;
   if (!(sx == 0 && sy == 0)) {
@@ -273,7 +273,7 @@ __device__ __forceinline__ f3 cosine_hemisphere(float u0, float u1) {
 // VNDF (BxDF_math.h:231-342)
 struct VNDF {
   float ax, ay, ax_inv, ay_inv;
-  __device__ __forceinline__ VNDF(float aniso, float rough) {
+  NOVA_HD VNDF(float aniso, float rough) {
     aniso = clampf(aniso, -1.f, 1.f);
     float aspect = msqrt(1.f - 0.9f * aniso);
     float a = rough != 0 ? rough * rough : 1e-6f;
@@ -282,7 +282,7 @@ struct VNDF {
     ax_inv = 1.f / ax;
     ay_inv = 1.f / ay;
   }
-  __device__ __forceinline__ float D(f3 wm) const {
+  NOVA_HD float D(f3 wm) const {
     if (wm.z < 1e-6f) return 0.f;
     float c2 = cos2phi(wm) * sqr(ax_inv);
     float s2 = sin2phi(wm) * sqr(ay_inv);
@@ -290,19 +290,19 @@ struct VNDF {
                           (double)sqr(1.f + tan2theta(wm) * (c2 + s2)));
     return 1.f / denom;
   }
-  __device__ __forceinline__ float lambda(f3 v) const {
+  NOVA_HD float lambda(f3 v) const {
     float t2 = tan2theta(v);
     if (isinf(t2) || isnan(t2)) return 0;
     float a2 = sqr(ax) * cos2phi(v) + sqr(ay) * sin2phi(v);
     return (msqrt(1 + a2 * t2) - 1) * 0.5f;
   }
-  __device__ __forceinline__ float G1(f3 v) const {
+  NOVA_HD float G1(f3 v) const {
     v.z = smax(v.z, 1e-6f);
     return 1.f / (1.f + lambda(v));
   }
-  __device__ __forceinline__ float G(f3 wo, f3 wi) const { return 1.f / (1.f + lambda(wo) + lambda(wi)); }
-  __device__ __forceinline__ float pdf(f3 wo, f3 wm) const { return D(wm) * absdot(wo, wm) * G1(wo) / abscos(wo); }
-  __device__ __forceinline__ f3 sample(f3 wo, float u0, float u1) const {
+  NOVA_HD float G(f3 wo, f3 wi) const { return 1.f / (1.f + lambda(wo) + lambda(wi)); }
+  NOVA_HD float pdf(f3 wo, f3 wm) const { return D(wm) * absdot(wo, wm) * G1(wo) / abscos(wo); }
+  NOVA_HD f3 sample(f3 wo, float u0, float u1) const {
     f3 woh = normalize(mk3(wo.x * ax, wo.y * ay, wo.z));
     if (woh.z < 0) woh = -woh;
     f3 t = woh.z < 1.f ? normalize(cross(mk3(0, 0, 1), woh)) : mk3(1, 0, 0);
@@ -315,7 +315,7 @@ struct VNDF {
     f3 nh = px * t + py * b + msqrt(fmaxf(0.f, 1.f - sqr(px) - sqr(py))) * woh;
     return normalize(mk3(ax * nh.x, ay * nh.y, smax(1e-6f, nh.z)));
   }
-  __device__ __forceinline__ bool full_specular() const { return smax(ax, ay) < 1e-5f; }
+  NOVA_HD bool full_specular() const { return smax(ax, ay) < 1e-5f; }
 };
 
 enum : unsigned { BX_REFLECTION = 1, BX_TRANSMISSION = 2, BX_DIFFUSE = 4, BX_GLOSSY = 8, BX_SPECULAR = 16 };
@@ -337,7 +337,7 @@ struct Principled {
 
   // configureBSDFParams + generateWeightsFromParam (BSDF.h:43-97); subsurface 1, sheen 0,
   // clearcoat 0, specular 1, thin_surface false (BxDF.h:22-39 defaults).
-  __device__ __forceinline__ void configure(const BsdfIn &p) {
+  NOVA_HD void configure(const BsdfIn &p) {
     frame = gram_shmidt3(p.ns, p.dpdu, p.dpdv);
     eta = p.eta;
     albedo = p.albedo;
@@ -366,7 +366,7 @@ struct Principled {
     p_transmission = w_transmission / W;
   }
   // constructBXDF (BSDF.h:68-79)
-  __device__ __forceinline__ void construct(float uc) {
+  NOVA_HD void construct(float uc) {
     float idx = 0.f;
     lobe = LOBE_NONE;
     if (uc < (idx += p_diffuse)) {
@@ -383,7 +383,7 @@ struct Principled {
       lobe_prob = p_transmission;
     }
   }
-  __device__ __forceinline__ unsigned flags(const VNDF &g) const {
+  NOVA_HD unsigned flags(const VNDF &g) const {
     if (lobe == LOBE_DIFFUSE) return BX_DIFFUSE | BX_REFLECTION;
     if (lobe == LOBE_SPECULAR) return g.full_specular() ? (BX_SPECULAR | BX_REFLECTION) : (BX_GLOSSY | BX_REFLECTION);
     if (lobe == LOBE_DIELECTRIC) {
@@ -400,7 +400,7 @@ struct BsdfSample {
 };
 
 // DisneyDiffuseBxDF::sample_f (BxDF.h:295-325)
-__device__ __forceinline__ bool sample_disney(const Principled &b, f3 wo, float u0, float u1, BsdfSample &s) {
+NOVA_HD bool sample_disney(const Principled &b, f3 wo, float u0, float u1, BsdfSample &s) {
   f3 wi = normalize(cosine_hemisphere(u0, u1));
   f3 h = normalize(wi + wo);
   const float ci = wi.z, co = wo.z;
@@ -421,7 +421,7 @@ __device__ __forceinline__ bool sample_disney(const Principled &b, f3 wo, float 
 }
 
 // SpecularBxDF::sample_f (BxDF.h:367-397)
-__device__ __forceinline__ bool sample_specular(const Principled &b, const VNDF &g, f3 wo, float u0, float u1, BsdfSample &s) {
+NOVA_HD bool sample_specular(const Principled &b, const VNDF &g, f3 wo, float u0, float u1, BsdfSample &s) {
   f3 wm = g.sample(wo, u0, u1);
   f3 wi = reflect(-wo, wm);
   const float co = wo.z, ci = wi.z;
@@ -441,7 +441,7 @@ __device__ __forceinline__ bool sample_specular(const Principled &b, const VNDF 
 }
 
 // DielectricBxDF::sample_f and its four branches (BxDF.h:403-585)
-__device__ __forceinline__ bool sample_dielectric(const Principled &b, const VNDF &g, f3 wo, float uc, float u0, float u1, BsdfSample &s) {
+NOVA_HD bool sample_dielectric(const Principled &b, const VNDF &g, f3 wo, float uc, float u0, float u1, BsdfSample &s) {
   const float ior = b.eta.x;
   if (g.full_specular() || ior == 1.f) {
     const float R = fresnel_real(ior, wo.z), T = 1 - R;
@@ -505,7 +505,7 @@ __device__ __forceinline__ bool sample_dielectric(const Principled &b, const VND
 }
 
 // bsdf::correct_specular_shading_normal (BxDF_math.h:26-53)
-__device__ __forceinline__ f3 correct_specular_shading_normal(f3 ng, f3 wo, f3 ns) {
+NOVA_HD f3 correct_specular_shading_normal(f3 ng, f3 wo, f3 ns) {
   f3 R = reflect(-wo, ns);
   const float woz = dot(wo, ng);
   const float threshold = fminf(0.9f * woz, 0.01f);

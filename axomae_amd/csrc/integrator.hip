@@ -19,6 +19,7 @@
 #include <vector>
 
 #include "device_math.h"
+#include "device_path.h"
 #include "device_shading.h"
 #include "nova_internal.h"
 
@@ -33,28 +34,6 @@ struct TraceCtx {
   uint32_t *gstack;     // overflow entries: [level - kLdsStack][global thread]
   uint32_t gstride;     // number of threads in the grid
 };
-
-// Moller-Trumbore exactly as Triangle::hit orders it (Triangle.h:158-175); accepts
-// tnear <= t <= tbest, so a later triangle at an equal t replaces the earlier one.
-__device__ __forceinline__ bool mt_test(const TriMT &tr, f3 o, f3 d, float tnear, float tbest, float &t, float &u, float &v) {
-  const f3 v0 = mk3(tr.a.x, tr.a.y, tr.a.z), e1 = mk3(tr.b.x, tr.b.y, tr.b.z), e2 = mk3(tr.c.x, tr.c.y, tr.c.z);
-  const f3 P = cross(d, e2);
-  const float det = dot(P, e1);
-  if (det == 0.f) return false;
-  const float inv_det = 1.f / det;
-  const f3 T = o - v0;
-  const float uu = dot(P, T) * inv_det;
-  if (uu < 0 || uu > 1.f) return false;
-  const f3 Q = cross(T, e1);
-  const float vv = dot(Q, d) * inv_det;
-  if (vv < 0.f || (uu + vv) > 1.f) return false;
-  const float tt = dot(Q, e2) * inv_det;
-  if (tt < tnear || tt > tbest) return false;
-  t = tt;
-  u = uu;
-  v = vv;
-  return true;
-}
 
 // Closest hit over the 4-wide BVH. Boxes are tested with a slightly widened slab so no
 // box holding a hit the reference's division-form test accepts is culled; triangle
@@ -168,62 +147,19 @@ struct Queues {
   uint32_t cap;
 };
 
-struct FrameParams {
-  int32_t width, height, frame0, frames;
-  uint32_t offset;
-  int32_t vinvert;
-  float inv_P[16], inv_V[16];
-};
-
-__device__ __forceinline__ uint32_t pixel_offset(const FrameParams &fp, uint32_t p) {
-  const uint32_t x = p % (uint32_t)fp.width, y = p / (uint32_t)fp.width;
-  return fp.vinvert ? (((uint32_t)fp.height - 1 - y) * (uint32_t)fp.width + x) * 4u : (y * (uint32_t)fp.width + x) * 4u;
-}
-
-__device__ __forceinline__ uint32_t sample_index(const FrameParams &fp, uint32_t slot) {
-  const uint32_t npix = (uint32_t)fp.width * (uint32_t)fp.height;
-  const uint32_t f = (uint32_t)fp.frame0 + slot / npix;
-  return pixel_offset(fp, slot % npix) * f + fp.offset;  // Integrator.h:80, u32 wraparound
-}
-
-// AbstractIntegrator::render per pixel (Integrator.h:53-95) + ray_inv_mat (math_camera.h:46-58)
+// AbstractIntegrator::render per pixel (Integrator.h:53-95): one camera ray per path slot
 __global__ __launch_bounds__(256) void generate_kernel(FrameParams fp, const uint32_t *__restrict__ sobol, Queues q, uint32_t active) {
   const uint32_t n = q.cap;  // plane stride
   for (uint32_t slot = blockIdx.x * blockDim.x + threadIdx.x; slot < active; slot += gridDim.x * blockDim.x) {
-    const uint32_t npix = (uint32_t)fp.width * (uint32_t)fp.height;
-    const uint32_t p = slot % npix;
-    const int x = (int)(p % (uint32_t)fp.width), y = (int)(p / (uint32_t)fp.width);
-    const uint32_t idx = sample_index(fp, slot);
-    const uint32_t g = idx ^ (idx >> 1);
-    const float s0 = u32_to_unit(owen_scramble(sobol_xor(sobol + 0 * 32, g)));
-    const float s1 = u32_to_unit(owen_scramble(sobol_xor(sobol + 1 * 32, g)));
-    const float dx = s0 * 0.0005f, dy = s1 * 0.0005f;
-    const int sy = fp.height - y;
-    const float nx = (float)(2 * x - fp.width) / (float)fp.width + dx;
-    const float ny = (float)(fp.height - 2 * sy) / (float)fp.height + dy;
-    const float *P = fp.inv_P, *V = fp.inv_V;
-    float ox = mat4_row(P, 0, nx, ny, -1.f, 1.f), oy = mat4_row(P, 1, nx, ny, -1.f, 1.f), oz = mat4_row(P, 2, nx, ny, -1.f, 1.f),
-          ow = mat4_row(P, 3, nx, ny, -1.f, 1.f);
-    ox = ox / ow;
-    oy = oy / ow;
-    oz = oz / ow;
-    ow = ow / ow;
-    const float wx = mat4_row(V, 0, ox, oy, oz, 1.f), wy = mat4_row(V, 1, ox, oy, oz, 1.f), wz = mat4_row(V, 2, ox, oy, oz, 1.f);
-    const float ddx = mat4_row(P, 0, nx, ny, -1.f, 0.f), ddy = mat4_row(P, 1, nx, ny, -1.f, 0.f);
-    float vx = mat4_row(V, 0, ddx, ddy, -1.f, 0.f), vy = mat4_row(V, 1, ddx, ddy, -1.f, 0.f), vz = mat4_row(V, 2, ddx, ddy, -1.f, 0.f),
-          vw = mat4_row(V, 3, ddx, ddy, -1.f, 0.f);
-    const float xx = vx * vx, yy = vy * vy, zz = vz * vz, ww = vw * vw;
-    const float inv_len = 1.f / sqrtf((xx + yy) + (zz + ww));  // glm::normalize(vec4)
-    vx = vx * inv_len;
-    vy = vy * inv_len;
-    vz = vz * inv_len;
+    f3 o, d;
+    camera_ray(fp, sobol, slot, o, d);
     float *r = q.ray[0];
-    r[0 * (size_t)n + slot] = wx;
-    r[1 * (size_t)n + slot] = wy;
-    r[2 * (size_t)n + slot] = wz;
-    r[3 * (size_t)n + slot] = vx;
-    r[4 * (size_t)n + slot] = vy;
-    r[5 * (size_t)n + slot] = vz;
+    r[0 * (size_t)n + slot] = o.x;
+    r[1 * (size_t)n + slot] = o.y;
+    r[2 * (size_t)n + slot] = o.z;
+    r[3 * (size_t)n + slot] = d.x;
+    r[4 * (size_t)n + slot] = d.y;
+    r[5 * (size_t)n + slot] = d.z;
     q.slot[0][slot] = (int32_t)slot;
     q.T[slot] = 1.f;
     q.T[(size_t)n + slot] = 1.f;
@@ -264,12 +200,6 @@ __global__ __launch_bounds__(kTraceBlock) void extend_kernel(TraceCtx tc, Queues
   }
 }
 
-struct ShadeParams {
-  FrameParams fp;
-  int32_t bounce, last, rr;  // rr: this bounce rolls Russian roulette
-  uint32_t dim0;             // first sampler dimension of this scatter
-};
-
 __device__ __forceinline__ void write_partial(const Queues &q, const FrameParams &fp, uint32_t slot, float lx, float ly, float lz, float a) {
   const uint32_t npix = (uint32_t)fp.width * (uint32_t)fp.height;
   const uint32_t f = slot / npix;
@@ -283,7 +213,6 @@ __global__ __launch_bounds__(256) void shade_kernel(ShadeParams sp, ShadeCtx sc,
   const float *rin = q.ray[which];
   float *rout = q.ray[which ^ 1];
   int32_t *sout = q.slot[which ^ 1];
-  const uint32_t *dv0 = sobol + (size_t)sp.dim0 * 32;
   const uint32_t stride = gridDim.x * blockDim.x;
   // every lane of a wave runs the same number of iterations so the ballot below is uniform
   const uint32_t iters = (n + stride - 1) / stride;
@@ -300,131 +229,8 @@ __global__ __launch_bounds__(256) void shade_kernel(ShadeParams sp, ShadeCtx sc,
       f3 L = mk3(q.L[slot], q.L[cap + slot], q.L[2 * cap + slot]);
       float A = q.A[slot];
       const int prim = q.hit_prim[i];
-      bool done = true;
-      if (prim < 0) {
-        escaped = true;
-        const float4 env = sample_envmap(sc, d);
-        L = L + T * mk3(env.x, env.y, env.z);
-        A = A + env.w;
-      } else {
-        const ShadeTri tr = sc.shade[prim];
-        const HitRec h = make_hit(sc, tr, o, d, q.hit_t[i], q.hit_u[i], q.hit_v[i]);
-        const MatRec mat = sc.mats[tr.material];
-        const float ONE_255 = 1.f / 255.f;
-        // PrincipledMaterial::scatter (NovaMaterials.h:89-137)
-        f3 nrm = mk3(0.f, 0.f, 1.f);
-        if (mat.normal >= 0) {
-          float4 t4 = sample_u32(sc, mat.normal, h.u, h.v);
-          nrm = mk3(t4.x * ONE_255, t4.y * ONE_255, t4.z * ONE_255);
-        }
-        const f3 ns = normalize(mul(h.frame, nrm * 2.f - 1.f));
-        const f3 wo = normalize(-d);
-        BsdfIn bi;
-        bi.ns = ns;
-        bi.ng = h.ng;
-        bi.dpdu = h.dpdu;
-        bi.dpdv = h.dpdv;
-        bi.wo_dot_ng = dot(wo, h.ng);
-        if (mat.albedo_f32 >= 0) {
-          float4 a4 = sample_f32_bilinear(sc, mat.albedo_f32, h.u, h.v);
-          bi.albedo = mk3(a4.x, a4.y, a4.z);
-        } else if (mat.albedo >= 0) {
-          float4 a4 = sample_u32(sc, mat.albedo, h.u, h.v);
-          bi.albedo = mk3(a4.x * ONE_255, a4.y * ONE_255, a4.z * ONE_255);
-        } else {
-          bi.albedo = mk3(1.f, 1.f, 1.f);
-        }
-        f3 met = mk3(0.f, 0.f, 0.f);
-        if (mat.metallic >= 0) {
-          float4 m4 = sample_u32(sc, mat.metallic, h.u, h.v);
-          met = mk3(m4.x * ONE_255, m4.y * ONE_255, m4.z * ONE_255);
-        }
-        bi.roughness = met.y;
-        if (mat.roughness_f32 >= 0) bi.roughness = sample_f32_bilinear(sc, mat.roughness_f32, h.u, h.v).x;
-        bi.metal = met.z;
-        const bool any_eta = mat.eta[0] != 0.f || mat.eta[1] != 0.f || mat.eta[2] != 0.f;
-        const bool any_k = mat.k[0] != 0.f || mat.k[1] != 0.f || mat.k[2] != 0.f;
-        bi.transmission = (any_eta && !any_k) ? 1.f : 0.f;
-        bi.anisotropy = mat.anisotropy;
-        bi.eta = mk3(mat.eta[0], mat.eta[1], mat.eta[2]);
-
-        // sampleBSDF (NovaMaterials.h:63-79): dims dim0 (lobe), dim0+1..2 (u0), dim0+3 (uc)
-        const uint32_t idx = sample_index(sp.fp, (uint32_t)slot);
-        const uint32_t g = idx ^ (idx >> 1);
-        const float uc = u32_to_unit(owen_scramble(sobol_xor(dv0, g)));
-        const float u0 = u32_to_unit(owen_scramble(sobol_xor(dv0 + 32, g)));
-        const float u1 = u32_to_unit(owen_scramble(sobol_xor(dv0 + 64, g)));
-        Principled b;
-        b.configure(bi);
-        b.construct(uc);
-        const VNDF ggx(b.anisotropy, b.roughness);
-        if (b.flags(ggx) & (BX_GLOSSY | BX_REFLECTION)) {
-          if (bi.wo_dot_ng >= 0) {
-            const f3 nsc = normalize(correct_specular_shading_normal(bi.ng, wo, bi.ns));
-            bi.ns = nsc;
-            const int lobe = b.lobe;
-            const float lw = b.lobe_weight, lp = b.lobe_prob;
-            b.configure(bi);
-            b.lobe = lobe;
-            b.lobe_weight = lw;
-            b.lobe_prob = lp;
-          }
-        }
-        const float uc2 = u32_to_unit(owen_scramble(sobol_xor(dv0 + 96, g)));
-        // PrincipledBSDF::sample_f (BSDF.h:107-130)
-        BsdfSample smp;
-        bool ok = false;
-        if (b.lobe != LOBE_NONE) {
-          const f3 lwo = normalize(mul_t(b.frame, wo));
-          if ((b.flags(ggx) & (BX_REFLECTION | BX_TRANSMISSION)) && lwo.z != 0.f) {
-            if (b.lobe == LOBE_DIFFUSE)
-              ok = sample_disney(b, lwo, u0, u1, smp);
-            else if (b.lobe == LOBE_SPECULAR)
-              ok = sample_specular(b, ggx, lwo, u0, u1, smp);
-            else
-              ok = sample_dielectric(b, ggx, lwo, uc2, u0, u1, smp);
-            if (ok && ((smp.f.x == 0.f && smp.f.y == 0.f && smp.f.z == 0.f) || smp.pdf == 0.f || smp.wi.z == 0.f)) ok = false;
-            if (ok) smp.wi = normalize(mul(b.frame, smp.wi));
-          }
-        }
-        if (ok) {
-          f3 c = smp.f * smp.costheta / smp.pdf;
-          if (isfinite(c.x) && isfinite(c.y) && isfinite(c.z)) {
-            f3 Le = mk3(0.f, 0.f, 0.f);
-            if (mat.emissive >= 0) {
-              float4 e4 = sample_u32(sc, mat.emissive, h.u, h.v);
-              Le = mk3(e4.x * ONE_255, e4.y * ONE_255, e4.z * ONE_255) * mat.emission_scale;
-            }
-            bool survive = true;
-            if (sp.rr) {  // Russian-roulette extension: one more dimension after the scatter's four
-              const f3 tp = T * c;
-              const float qv = fminf(1.f, fmaxf(tp.x, fmaxf(tp.y, tp.z)));
-              const float ur = u32_to_unit(owen_scramble(sobol_xor(dv0 + 128, g)));
-              if (!(ur < qv))
-                survive = false;
-              else
-                c = c / qv;
-            }
-            L = L + T * Le;
-            A = A + 1.f;
-            if (survive) {
-              T = T * c;
-              // Ray::spawn (Ray.h:22-27)
-              const f3 wi = smp.wi;
-              nd = normalize(wi);
-              no = h.pos + h.ng * (1e-3f + 1e-3f * gsign(dot(h.ng, wi)));
-              if (sp.last) {  // depth budget spent: V = envmap(direction) without traversal
-                const float4 env = sample_envmap(sc, nd);
-                L = L + T * mk3(env.x, env.y, env.z);
-                A = A + env.w;
-              } else {
-                done = false;
-                cont = true;
-              }
-            }
-          }
-        }
-      }
+      cont = shade_path(sp, sc, sobol, (uint32_t)slot, o, d, prim, q.hit_t[i], q.hit_u[i], q.hit_v[i], T, L, A, no, nd, escaped);
+      const bool done = !cont;
       q.T[slot] = T.x;
       q.T[cap + slot] = T.y;
       q.T[2 * cap + slot] = T.z;

@@ -130,6 +130,63 @@ def sobol_fill(idx, dims, return_bits=False):
     return (out, bits) if return_bits else out
 
 
+def scene_desc(scene):
+    """nova_hip_scene_desc for a scenes.Scene; returns (descriptor, arrays that must outlive it)."""
+    keep = []
+
+    def ptr(a, dtype):
+        a = np.ascontiguousarray(a, dtype=dtype)
+        keep.append(a)
+        return a.ctypes.data
+
+    meshes = (MeshDesc * max(1, len(scene.meshes)))()
+    for i, m in enumerate(scene.meshes):
+        d = meshes[i]
+        d.vertices = ptr(m.vertices, np.float32)
+        d.normals = ptr(m.normals, np.float32)
+        d.tangents = ptr(m.tangents, np.float32)
+        d.bitangents = ptr(m.bitangents, np.float32)
+        d.uv = ptr(m.uv, np.float32)
+        d.indices = ptr(m.indices, np.uint32)
+        d.vertex_count = len(m.vertices) // 3
+        d.index_count = len(m.indices)
+        d.transform[:] = [float(x) for x in np.asarray(m.transform, dtype=np.float32)]
+        d.material = m.material
+    mats = (MaterialDesc * max(1, len(scene.materials)))()
+    for i, m in enumerate(scene.materials):
+        d = mats[i]
+        d.albedo, d.metallic, d.normal, d.emissive = m.albedo, m.metallic, m.normal, m.emissive
+        d.albedo_f32, d.roughness_f32 = m.albedo_f32, m.roughness_f32
+        d.eta[:] = [float(x) for x in m.eta]
+        d.k[:] = [float(x) for x in m.k]
+        d.anisotropy = m.anisotropy
+        d.emission_scale = m.emission_scale
+    texs = (TextureDesc * max(1, len(scene.textures)))()
+    for i, t in enumerate(scene.textures):
+        d = texs[i]
+        d.width, d.height, d.channels = t.width, t.height, t.channels
+        d.invert_u, d.invert_v = int(t.invert_u), int(t.invert_v)
+        d.format = 1 if t.is_f32 else 0
+        d.data = ptr(t.data, np.float32 if t.is_f32 else np.uint32)
+    keep += [meshes, mats, texs]
+    sd = SceneDesc(meshes, len(scene.meshes), mats, len(scene.materials), texs, len(scene.textures), scene.envmap)
+    return sd, keep
+
+
+def render_params(scene, width, height, frames, max_depth, frame_start=0, rr_start=-1, max_samples_offset=0, vertical_invert=False,
+                  inv_P=None, inv_V=None, flags=0):
+    p = RenderParams()
+    p.width, p.height, p.max_depth = width, height, max_depth
+    p.frame_start, p.frame_count = frame_start, frames
+    p.max_samples_offset = max_samples_offset
+    p.vertical_invert = int(vertical_invert)
+    p.rr_start = rr_start
+    p.flags = flags
+    p.inv_P[:] = [float(x) for x in (scene.inv_P if inv_P is None else inv_P)]
+    p.inv_V[:] = [float(x) for x in (scene.inv_V if inv_V is None else inv_V)]
+    return p
+
+
 class HipEngine:
     """One device context holding a scene, its BVH and render scratch."""
 
@@ -161,43 +218,7 @@ class HipEngine:
 
     # ---------------------------------------------------------------- scene
     def upload(self, scene):
-        keep = []
-
-        def ptr(a, dtype):
-            a = np.ascontiguousarray(a, dtype=dtype)
-            keep.append(a)
-            return a.ctypes.data
-
-        meshes = (MeshDesc * len(scene.meshes))()
-        for i, m in enumerate(scene.meshes):
-            d = meshes[i]
-            d.vertices = ptr(m.vertices, np.float32)
-            d.normals = ptr(m.normals, np.float32)
-            d.tangents = ptr(m.tangents, np.float32)
-            d.bitangents = ptr(m.bitangents, np.float32)
-            d.uv = ptr(m.uv, np.float32)
-            d.indices = ptr(m.indices, np.uint32)
-            d.vertex_count = len(m.vertices) // 3
-            d.index_count = len(m.indices)
-            d.transform[:] = [float(x) for x in np.asarray(m.transform, dtype=np.float32)]
-            d.material = m.material
-        mats = (MaterialDesc * max(1, len(scene.materials)))()
-        for i, m in enumerate(scene.materials):
-            d = mats[i]
-            d.albedo, d.metallic, d.normal, d.emissive = m.albedo, m.metallic, m.normal, m.emissive
-            d.albedo_f32, d.roughness_f32 = m.albedo_f32, m.roughness_f32
-            d.eta[:] = [float(x) for x in m.eta]
-            d.k[:] = [float(x) for x in m.k]
-            d.anisotropy = m.anisotropy
-            d.emission_scale = m.emission_scale
-        texs = (TextureDesc * max(1, len(scene.textures)))()
-        for i, t in enumerate(scene.textures):
-            d = texs[i]
-            d.width, d.height, d.channels = t.width, t.height, t.channels
-            d.invert_u, d.invert_v = int(t.invert_u), int(t.invert_v)
-            d.format = 1 if t.is_f32 else 0
-            d.data = ptr(t.data, np.float32 if t.is_f32 else np.uint32)
-        sd = SceneDesc(meshes, len(scene.meshes), mats, len(scene.materials), texs, len(scene.textures), scene.envmap)
+        sd, keep = scene_desc(scene)
         self._check(self.L.nova_hip_scene_upload(self.ctx, C.byref(sd), _stream(self.torch)), "scene_upload")
         self.scene = scene
         self.prim_count = scene.triangle_count
@@ -230,15 +251,8 @@ class HipEngine:
                inv_P=None, inv_V=None, flags=0):
         """Renders frames [frame_start, frame_start+frames); returns (final, accum) device tensors (H, W, 4)."""
         torch = self.torch
-        p = RenderParams()
-        p.width, p.height, p.max_depth = width, height, max_depth
-        p.frame_start, p.frame_count = frame_start, frames
-        p.max_samples_offset = max_samples_offset
-        p.vertical_invert = int(vertical_invert)
-        p.rr_start = rr_start
-        p.flags = flags
-        p.inv_P[:] = [float(x) for x in (self.scene.inv_P if inv_P is None else inv_P)]
-        p.inv_V[:] = [float(x) for x in (self.scene.inv_V if inv_V is None else inv_V)]
+        p = render_params(self.scene, width, height, frames, max_depth, frame_start, rr_start, max_samples_offset, vertical_invert, inv_P, inv_V,
+                          flags)
         if accum is None:
             accum = torch.zeros((height, width, 4), dtype=torch.float32, device="cuda")
         final = torch.empty((height, width, 4), dtype=torch.float32, device="cuda")

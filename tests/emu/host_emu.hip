@@ -1,0 +1,154 @@
+// Host build of the device path stages (test infrastructure): the scene flattening,
+// camera ray, Moller-Trumbore and per-segment shading functions the kernels run, driven
+// by a CPU loop with a brute-force closest hit. It checks the device code's arithmetic
+// and control flow against the oracle on machines without a GPU.
+#include <cmath>
+#include <cstring>
+#include <vector>
+
+#include "../../axomae_amd/csrc/device_path.h"
+#include "../../axomae_amd/csrc/device_scene.h"
+
+using namespace nova_hip;
+
+namespace {
+const uint32_t kTable[256][32] = {
+#include "../../axomae_amd/csrc/sobol_table.inc"
+};
+}
+
+extern "C" int emu_render(const nova_hip_scene_desc *sd, const nova_hip_render_params *p, float *accum, float *final_out) {
+  // ---- flatten (as nova_hip_scene_upload does, on the host)
+  uint64_t nvert = 0, ntri = 0;
+  for (uint32_t m = 0; m < sd->mesh_count; m++) {
+    nvert += sd->meshes[m].vertex_count;
+    ntri += sd->meshes[m].index_count / 3;
+  }
+  std::vector<float> pos(nvert * 3), nrm(nvert * 3), tan(nvert * 3), bit(nvert * 3), uv(nvert * 2), xf(sd->mesh_count * 16);
+  std::vector<int32_t> mesh_mat(sd->mesh_count);
+  std::vector<MeshXf> mxf(sd->mesh_count);
+  std::vector<TriRef> refs(ntri);
+  uint64_t vo = 0, to = 0;
+  for (uint32_t m = 0; m < sd->mesh_count; m++) {
+    const nova_hip_mesh_desc &md = sd->meshes[m];
+    std::memcpy(&pos[vo * 3], md.vertices, md.vertex_count * 12);
+    std::memcpy(&nrm[vo * 3], md.normals, md.vertex_count * 12);
+    std::memcpy(&tan[vo * 3], md.tangents, md.vertex_count * 12);
+    std::memcpy(&bit[vo * 3], md.bitangents, md.vertex_count * 12);
+    std::memcpy(&uv[vo * 2], md.uv, md.vertex_count * 8);
+    std::memcpy(&xf[m * 16], md.transform, 64);
+    for (int c = 0; c < 3; c++)
+      for (int r = 0; r < 3; r++) {
+        mxf[m].m3[c * 3 + r] = md.transform[c * 4 + r];
+        mxf[m].n3[c * 3 + r] = (c == r) ? 1.f : 0.f;  // identity transforms only (the test scenes)
+      }
+    mesh_mat[m] = md.material;
+    for (uint64_t i = 0; i < md.index_count; i += 3)
+      refs[to++] = TriRef{(uint32_t)(md.indices[i] + vo), (uint32_t)(md.indices[i + 1] + vo), (uint32_t)(md.indices[i + 2] + vo), m};
+    vo += md.vertex_count;
+  }
+  std::vector<TriMT> tri(ntri);
+  std::vector<ShadeTri> shade(ntri);
+  std::vector<float> aabb(ntri * 6), cent(ntri * 3);
+  for (uint32_t t = 0; t < ntri; t++)
+    flatten_triangle(t, refs[t], pos.data(), nrm.data(), tan.data(), bit.data(), uv.data(), xf.data(), mesh_mat.data(), tri.data(), shade.data(),
+                     aabb.data(), cent.data());
+  std::vector<TexRec> texrec(sd->texture_count);
+  std::vector<uint32_t> u32pool;
+  std::vector<float> f32pool;
+  for (uint32_t t = 0; t < sd->texture_count; t++) {
+    const nova_hip_texture_desc &td = sd->textures[t];
+    TexRec &r = texrec[t];
+    r.width = td.width;
+    r.height = td.height;
+    r.channels = td.channels;
+    r.invert_u = td.invert_u;
+    r.invert_v = td.invert_v;
+    r.format = td.format;
+    if (td.format == NOVA_HIP_TEX_U32) {
+      r.offset = u32pool.size();
+      u32pool.insert(u32pool.end(), (const uint32_t *)td.data, (const uint32_t *)td.data + (size_t)td.width * td.height);
+    } else {
+      r.offset = f32pool.size();
+      f32pool.insert(f32pool.end(), (const float *)td.data, (const float *)td.data + (size_t)td.width * td.height * td.channels);
+    }
+  }
+  std::vector<MatRec> mats(sd->material_count);
+  for (uint32_t m = 0; m < sd->material_count; m++) {
+    const nova_hip_material_desc &d = sd->materials[m];
+    MatRec &r = mats[m];
+    r.albedo = d.albedo;
+    r.metallic = d.metallic;
+    r.normal = d.normal;
+    r.emissive = d.emissive;
+    r.albedo_f32 = d.albedo_f32;
+    r.roughness_f32 = d.roughness_f32;
+    for (int k = 0; k < 3; k++) {
+      r.eta[k] = d.eta[k];
+      r.k[k] = d.k[k];
+    }
+    r.anisotropy = d.anisotropy;
+    r.emission_scale = d.emission_scale;
+  }
+  std::vector<float> lut(kAcosLut);
+  for (int i = 0; i < kAcosLut; i++) lut[i] = (float)std::acos(((double)i / (double)kAcosLut) * 2.0 - 1.0);
+  ShadeCtx sc{texrec.data(), u32pool.data(), f32pool.data(), mats.data(), mxf.data(), shade.data(), lut.data(), sd->envmap_texture};
+
+  // ---- frames: camera ray, then closest hit + shade per bounce, then accumulate
+  const uint32_t npix = (uint32_t)p->width * (uint32_t)p->height;
+  std::vector<float> partial((size_t)npix * 4);
+  for (int f = p->frame_start; f < p->frame_start + p->frame_count; f++) {
+    FrameParams fp;
+    fp.width = p->width;
+    fp.height = p->height;
+    fp.frame0 = f;
+    fp.frames = 1;
+    fp.offset = p->max_samples_offset;
+    fp.vinvert = p->vertical_invert;
+    std::memcpy(fp.inv_P, p->inv_P, 64);
+    std::memcpy(fp.inv_V, p->inv_V, 64);
+    for (uint32_t slot = 0; slot < npix; slot++) {
+      f3 o, d;
+      camera_ray(fp, &kTable[0][0], slot, o, d);
+      f3 T = mk3(1, 1, 1), L = mk3(0, 0, 0);
+      float A = 0.f;
+      uint32_t dim = 2;
+      for (int k = 0; k <= p->max_depth; k++) {
+        float tb = 3.402823466e+38f, bu = 0, bv = 0;
+        int best = -1;
+        for (uint32_t t = 0; t < ntri; t++) {
+          float tt, uu, vv;
+          if (mt_test(tri[t], o, d, 0.000001f, tb, tt, uu, vv)) {
+            tb = tt;
+            bu = uu;
+            bv = vv;
+            best = (int)t;
+          }
+        }
+        ShadeParams sp;
+        sp.fp = fp;
+        sp.bounce = k;
+        sp.last = k == p->max_depth;
+        sp.rr = (p->rr_start >= 0 && k >= p->rr_start) ? 1 : 0;
+        sp.dim0 = dim;
+        f3 no, nd;
+        bool esc;
+        bool cont = shade_path(sp, sc, &kTable[0][0], slot, o, d, best, tb, bu, bv, T, L, A, no, nd, esc);
+        dim += 4 + (sp.rr ? 1 : 0);
+        if (!cont) break;
+        o = no;
+        d = nd;
+      }
+      float *dst = &partial[pixel_offset(fp, slot)];
+      dst[0] = L.x;
+      dst[1] = L.y;
+      dst[2] = L.z;
+      dst[3] = A;
+    }
+    for (size_t i = 0; i < (size_t)npix * 4; i++) {
+      accum[i] += partial[i];
+      final_out[i] = accum[i] / float(f + 1);
+    }
+  }
+  return 0;
+}
