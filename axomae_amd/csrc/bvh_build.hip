@@ -85,7 +85,8 @@ __global__ __launch_bounds__(kBuildBlock) void bvh_level_kernel(const int32_t *_
                                                                 const float *__restrict__ aabb, const float *__restrict__ centroid,
                                                                 uint8_t *__restrict__ cls, int32_t *__restrict__ rk, int32_t *__restrict__ hr_pos,
                                                                 int32_t *__restrict__ tl_pos, int32_t *__restrict__ node_counter,
-                                                                int32_t *__restrict__ next_active, int32_t *__restrict__ next_count) {
+                                                                int32_t *__restrict__ next_active, int32_t *__restrict__ next_count,
+                                                                int32_t *__restrict__ parent, int32_t *__restrict__ depth) {
   __shared__ LevelShared sh;
   const int tid = threadIdx.x;
   const int32_t id = active[blockIdx.x];
@@ -274,6 +275,8 @@ __global__ __launch_bounds__(kBuildBlock) void bvh_level_kernel(const int32_t *_
   R.count = n - nL;
   nodes[li] = L;
   nodes[li + 1] = R;
+  parent[li] = parent[li + 1] = id;
+  depth[li] = depth[li + 1] = depth[id] + 1;
   BNode P = nd;
   P.left = li;
   P.count = 0;
@@ -308,7 +311,15 @@ __global__ void root_bounds_kernel(const float *__restrict__ aabb, uint32_t n, i
   }
 }
 
-__global__ void init_root_kernel(BNode *nodes, const int *bounds, uint32_t n, int32_t *prim_idx, int32_t *counters) {
+// Binary leaf holding each prim_idx position (the tie rule of the traversal needs it).
+__global__ void leaf_of_position_kernel(const BNode *__restrict__ nodes, uint32_t count, int32_t *__restrict__ leaf_of_pos) {
+  for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < count; i += gridDim.x * blockDim.x) {
+    const BNode nd = nodes[i];
+    for (int32_t k = 0; k < nd.count; k++) leaf_of_pos[nd.left + k] = (int32_t)i;
+  }
+}
+
+__global__ void init_root_kernel(BNode *nodes, const int *bounds, uint32_t n, int32_t *prim_idx, int32_t *counters, int32_t *parent, int32_t *depth) {
   uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i < n) prim_idx[i] = (int32_t)i;
   if (i == 0) {
@@ -320,6 +331,8 @@ __global__ void init_root_kernel(BNode *nodes, const int *bounds, uint32_t n, in
     r.left = 0;
     r.count = (int32_t)n;
     nodes[0] = r;
+    parent[0] = -1;
+    depth[0] = 0;
     counters[0] = 1;  // node counter
     counters[1] = 0;  // next level size
   }
@@ -420,7 +433,8 @@ extern "C" int nova_hip_bvh_build(nova_hip_ctx *ctx, int build_candidates, nova_
   NOVA_HIP_CHECK(ctx, hipSetDevice(ctx->device));
   const uint32_t n = ctx->prim_count;
   const size_t maxnodes = 2 * (size_t)n - 1;
-  for (void **p : {(void **)&ctx->prim_idx, (void **)&ctx->bnodes, (void **)&ctx->nodes4, (void **)&ctx->tri_leaf})
+  for (void **p : {(void **)&ctx->prim_idx, (void **)&ctx->bnodes, (void **)&ctx->nodes4, (void **)&ctx->tri_leaf, (void **)&ctx->bparent,
+                   (void **)&ctx->bdepth, (void **)&ctx->leaf_of_pos})
     if (*p) {
       hipFree(*p);
       *p = nullptr;
@@ -428,6 +442,9 @@ extern "C" int nova_hip_bvh_build(nova_hip_ctx *ctx, int build_candidates, nova_
   ctx->bvh_ready = false;
   NOVA_HIP_CHECK(ctx, hipMalloc(&ctx->prim_idx, n * sizeof(int32_t)));
   NOVA_HIP_CHECK(ctx, hipMalloc(&ctx->bnodes, maxnodes * sizeof(BNode)));
+  NOVA_HIP_CHECK(ctx, hipMalloc(&ctx->bparent, maxnodes * sizeof(int32_t)));
+  NOVA_HIP_CHECK(ctx, hipMalloc(&ctx->bdepth, maxnodes * sizeof(int32_t)));
+  NOVA_HIP_CHECK(ctx, hipMalloc(&ctx->leaf_of_pos, (size_t)n * sizeof(int32_t)));
 
   // temporaries
   int32_t *tmp_idx, *rk, *hr, *tl, *act0, *act1, *counters;
@@ -450,7 +467,8 @@ extern "C" int nova_hip_bvh_build(nova_hip_ctx *ctx, int build_candidates, nova_
 
   hipLaunchKernelGGL(init_bounds_kernel, dim3(1), dim3(64), 0, s, bounds);
   hipLaunchKernelGGL(root_bounds_kernel, dim3(std::min<uint32_t>(1024, (n + 255) / 256)), dim3(256), 0, s, ctx->prim_aabb, n, bounds);
-  hipLaunchKernelGGL(init_root_kernel, dim3((n + 255) / 256), dim3(256), 0, s, ctx->bnodes, bounds, n, ctx->prim_idx, counters);
+  hipLaunchKernelGGL(init_root_kernel, dim3((n + 255) / 256), dim3(256), 0, s, ctx->bnodes, bounds, n, ctx->prim_idx, counters, ctx->bparent,
+                     ctx->bdepth);
   const int32_t zero = 0;
   NOVA_HIP_CHECK(ctx, hipMemcpyAsync(act0, &zero, sizeof(int32_t), hipMemcpyHostToDevice, s));
   int32_t active = 1;
@@ -460,7 +478,7 @@ extern "C" int nova_hip_bvh_build(nova_hip_ctx *ctx, int build_candidates, nova_
     if (depth > n) return set_error(ctx, NOVA_HIP_DEVICE_ERROR, "BVH build did not converge");
     NOVA_HIP_CHECK(ctx, hipMemsetAsync(counters + 1, 0, sizeof(int32_t), s));
     hipLaunchKernelGGL(bvh_level_kernel, dim3(active), dim3(kBuildBlock), 0, s, cur, ctx->bnodes, ctx->prim_idx, tmp_idx, ctx->prim_aabb,
-                       ctx->prim_centroid, cls, rk, hr, tl, counters, nxt, counters + 1);
+                       ctx->prim_centroid, cls, rk, hr, tl, counters, nxt, counters + 1, ctx->bparent, ctx->bdepth);
     NOVA_HIP_CHECK(ctx, hipGetLastError());
     NOVA_HIP_CHECK(ctx, hipMemcpyAsync(&active, counters + 1, sizeof(int32_t), hipMemcpyDeviceToHost, s));
     NOVA_HIP_CHECK(ctx, hipStreamSynchronize(s));
@@ -496,6 +514,8 @@ extern "C" int nova_hip_bvh_build(nova_hip_ctx *ctx, int build_candidates, nova_
   NOVA_HIP_CHECK(ctx, hipMemcpy(&wide, counters + 2, sizeof(int32_t), hipMemcpyDeviceToHost));
   NOVA_HIP_CHECK(ctx, hipMalloc(&ctx->tri_leaf, (size_t)n * sizeof(TriMT)));
   hipLaunchKernelGGL(gather_leaf_tris, dim3((n + 255) / 256), dim3(256), 0, s, ctx->prim_idx, ctx->tri_world, ctx->tri_leaf, n);
+  hipLaunchKernelGGL(leaf_of_position_kernel, dim3(std::min<uint32_t>(4096, (used + 255) / 256)), dim3(256), 0, s, ctx->bnodes, (uint32_t)used,
+                     ctx->leaf_of_pos);
   NOVA_HIP_CHECK(ctx, hipGetLastError());
   NOVA_HIP_CHECK(ctx, hipEventRecord(ev1, s));
   NOVA_HIP_CHECK(ctx, hipEventSynchronize(ev1));

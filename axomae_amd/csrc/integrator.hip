@@ -33,11 +33,62 @@ struct TraceCtx {
   const TriMT *tris;
   uint32_t *gstack;     // overflow entries: [level - kLdsStack][global thread]
   uint32_t gstride;     // number of threads in the grid
+  // binary tree, only read to settle exact equal-t ties
+  const BNode *bnodes;
+  const int32_t *bparent, *bdepth, *leaf_of_pos;
 };
+
+// BoundingBox::intersect (BoundingBox.cpp:150-160) in its division form, std::min/std::max
+// operand order kept; a 0/0 slab (ray inside a face plane) spans the whole line.
+__device__ __forceinline__ float ref_box_entry(const BNode &b, f3 o, f3 d) {
+  auto slab = [](float lo, float hi, float oo, float dd, float &t1, float &t2) {
+    t1 = (lo - oo) / dd;
+    t2 = (hi - oo) / dd;
+    if (t1 != t1 || t2 != t2) {
+      t1 = -__builtin_inff();
+      t2 = __builtin_inff();
+    }
+  };
+  auto mn = [](float a, float b) { return b < a ? b : a; };
+  auto mx = [](float a, float b) { return a < b ? b : a; };
+  float tx1, tx2, ty1, ty2, tz1, tz2;
+  slab(b.mn[0], b.mx[0], o.x, d.x, tx1, tx2);
+  slab(b.mn[1], b.mx[1], o.y, d.y, ty1, ty2);
+  slab(b.mn[2], b.mx[2], o.z, d.z, tz1, tz2);
+  float t0 = mn(tx1, tx2), t1 = mx(tx1, tx2);
+  t0 = mx(t0, mn(ty1, ty2));
+  t1 = mn(t1, mx(ty1, ty2));
+  t0 = mx(t0, mn(tz1, tz2));
+  t1 = mn(t1, mx(tz1, tz2));
+  return (t1 >= t0 && t1 > 0.f) ? t0 : 3.402823466e+38f;
+}
+
+// Settles a tie between two triangles with bit-identical t the way the reference's
+// depth-first walk does (native.cpp:56-114, ties kept by `t <= tmax`): the one it reaches
+// last wins. Within a binary leaf that is the higher position; otherwise the walk enters
+// the nearer child of the two leaves' lowest common ancestor first (the left one when the
+// entry distances are equal), so the triangle under the other child wins.
+__device__ __noinline__ bool reference_visits_later(const TraceCtx &tc, int32_t pos_new, int32_t pos_old, f3 o, f3 d) {
+  int32_t a = tc.leaf_of_pos[pos_new], b = tc.leaf_of_pos[pos_old];
+  if (a == b) return pos_new > pos_old;
+  int32_t da = tc.bdepth[a], db = tc.bdepth[b];
+  for (; da > db && a > 0; da--) a = tc.bparent[a];
+  for (; db > da && b > 0; db--) b = tc.bparent[b];
+  if (a == b) return pos_new > pos_old;
+  while (a > 0 && b > 0 && tc.bparent[a] != tc.bparent[b]) {
+    a = tc.bparent[a];
+    b = tc.bparent[b];
+  }
+  if (a <= 0 || b <= 0) return pos_new > pos_old;  // malformed tree: keep the leaf order
+  const int32_t left = tc.bnodes[tc.bparent[a]].left;
+  const float li = ref_box_entry(tc.bnodes[left], o, d), ri = ref_box_entry(tc.bnodes[left + 1], o, d);
+  const int32_t first = li > ri ? left + 1 : left;
+  return a != first;
+}
 
 // Closest hit over the 4-wide BVH. Boxes are tested with a slightly widened slab so no
 // box holding a hit the reference's division-form test accepts is culled; triangle
-// tests are exact, so the hit is the reference's except at equal-t ties.
+// tests are exact, and exact equal-t ties are settled in the reference's visiting order.
 template <bool kCount>
 __device__ __forceinline__ int trace_closest(const TraceCtx &tc, uint32_t *lds, uint32_t gtid, f3 o, f3 d, float tnear, float &t_out, float &u_out,
                                              float &v_out, uint32_t &nodes_visited, uint32_t &tri_tests) {
@@ -48,6 +99,7 @@ __device__ __forceinline__ int trace_closest(const TraceCtx &tc, uint32_t *lds, 
                      d.z != 0.f ? 1.f / d.z : copysignf(3.0e38f, d.z));
   float tbest = 3.402823466e+38f;
   int best = -1;
+  int32_t bpos = -1;
   float bu = 0.f, bv = 0.f;
   int sp = 0;
   int32_t node = 0;
@@ -111,8 +163,9 @@ __device__ __forceinline__ int trace_closest(const TraceCtx &tc, uint32_t *lds, 
         const TriMT tr = tc.tris[i];
         if (kCount) tri_tests++;
         float tt, uu, vv;
-        if (mt_test(tr, o, d, tnear, tbest, tt, uu, vv)) {
+        if (mt_test(tr, o, d, tnear, tbest, tt, uu, vv) && (tt < tbest || best < 0 || reference_visits_later(tc, i, bpos, o, d))) {
           tbest = tt;
+          bpos = i;
           best = __float_as_int(tr.a.w);
           bu = uu;
           bv = vv;
@@ -336,7 +389,7 @@ extern "C" int nova_hip_closest_hit(nova_hip_ctx *ctx, const float *rays_dev, ui
   const uint32_t threads = blocks * kTraceBlock;
   int st = ensure_gstack(ctx, threads);
   if (st) return st;
-  TraceCtx tc{ctx->nodes4, ctx->tri_leaf, ctx->gstack, threads};
+  TraceCtx tc{ctx->nodes4, ctx->tri_leaf, ctx->gstack, threads, ctx->bnodes, ctx->bparent, ctx->bdepth, ctx->leaf_of_pos};
   hipLaunchKernelGGL(closest_hit_kernel, dim3(blocks), dim3(kTraceBlock), 0, (hipStream_t)stream, tc, rays_dev, n, prim_dev, tuv_dev);
   NOVA_HIP_CHECK(ctx, hipGetLastError());
   return NOVA_HIP_SUCCESS;
@@ -399,7 +452,7 @@ extern "C" int nova_hip_render_frames(nova_hip_ctx *ctx, const nova_hip_render_p
   const uint32_t trace_threads = trace_blocks * kTraceBlock;
   int st = ensure_gstack(ctx, trace_threads);
   if (st) return st;
-  TraceCtx tc{ctx->nodes4, ctx->tri_leaf, ctx->gstack, trace_threads};
+  TraceCtx tc{ctx->nodes4, ctx->tri_leaf, ctx->gstack, trace_threads, ctx->bnodes, ctx->bparent, ctx->bdepth, ctx->leaf_of_pos};
   ShadeCtx sc{ctx->textures, ctx->tex_u32, ctx->tex_f32, ctx->materials, ctx->mesh_xf, ctx->shade, ctx->acos_lut, ctx->envmap};
   const uint32_t shade_blocks = (uint32_t)std::min<size_t>((cap + 255) / 256, (size_t)cus * 8);
   NOVA_HIP_CHECK(ctx, hipMemsetAsync(counts, 0, 64, s));

@@ -100,6 +100,9 @@ struct nova_hip_ctx {
   uint32_t bnode_count = 0;
   nova_hip::Node4 *nodes4 = nullptr;
   nova_hip::TriMT *tri_leaf = nullptr;       // MT records in leaf order (prim id in a.w)
+  int32_t *bparent = nullptr;                // binary tree parent (-1 at the root)
+  int32_t *bdepth = nullptr;                 // binary tree depth
+  int32_t *leaf_of_pos = nullptr;            // binary leaf containing each prim_idx position
   nova_hip_bvh_info bvh_info{};
   bool bvh_ready = false;
 

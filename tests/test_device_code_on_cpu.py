@@ -63,3 +63,22 @@ def test_offsets_and_vertical_invert_device_code_matches_oracle(oracle, emu):
     img = emu_render(emu, sc, 24, 20, 3, 3, frame_start=5, max_samples_offset=17, vertical_invert=True)
     ref, _ = oracle.OracleScene(sc).render(24, 20, 3, 3, frame_start=5, max_samples_offset=17, vertical_invert=True)
     assert rel_rmse(img, ref) <= 1e-3
+
+
+def fp32_textured_scene(seed=0xA70AE004, tex=64):
+    """Terrain + spheres whose materials read FP32 albedo (RGB) and roughness textures."""
+    rng = np.random.default_rng(seed)
+    sc = scenes.scaled_million(scale_cells=12, spheres=3, sphere_level=2, width=32, height=20)
+    alb = sc.add_texture(scenes.Texture(tex, tex, 3, rng.uniform(0.1, 0.9, tex * tex * 3).astype(np.float32)))
+    rough = sc.add_texture(scenes.Texture(tex, tex, 1, rng.uniform(0.05, 0.9, tex * tex).astype(np.float32)))
+    for m in sc.materials:
+        m.albedo_f32 = alb
+        m.roughness_f32 = rough
+    return sc
+
+
+def test_fp32_material_textures_device_code_matches_oracle(oracle, emu):
+    sc = fp32_textured_scene()
+    img = emu_render(emu, sc, 32, 20, 2, 4)
+    ref, _ = oracle.OracleScene(sc).render(32, 20, 2, 4)
+    assert rel_rmse(img, ref) <= 1e-3

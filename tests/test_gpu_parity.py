@@ -195,3 +195,14 @@ def test_bakes_match_cpu_bake(oracle):
     for m, s, g, r in zip(maps, sizes, gpu_pf, rough):
         ref = oracle.bake_prefilter(m, s[0], s[1], r, 256)
         assert rel_rmse(g, ref) <= 1e-3
+
+
+@gpu
+def test_render_fp32_material_textures(oracle, engine):
+    from test_device_code_on_cpu import fp32_textured_scene
+    sc = fp32_textured_scene()
+    engine.upload(sc)
+    engine.build_bvh()
+    final, _ = engine.render(32, 20, frames=2, max_depth=4)
+    ref, _ = oracle.OracleScene(sc).render(32, 20, 2, 4, threads=8)
+    assert rel_rmse(final.cpu().numpy(), ref) <= 1e-3
