@@ -28,6 +28,7 @@
 #include <vector>
 
 #include "device_math.h"
+#include "device_trace.h"
 #include "nova_internal.h"
 
 namespace nova_hip {
@@ -40,11 +41,6 @@ __device__ __forceinline__ int ford(float f) {  // order-preserving float -> int
   return i >= 0 ? i : i ^ 0x7FFFFFFF;
 }
 __device__ __forceinline__ float unford(int i) { return __int_as_float(i >= 0 ? i : i ^ 0x7FFFFFFF); }
-
-__device__ __forceinline__ float half_area(float mnx, float mny, float mnz, float mxx, float mxy, float mxz) {
-  float lx = mxx - mnx, ly = mxy - mny, lz = mxz - mnz;  // BoundingBox::halfArea (BoundingBox.cpp:162-165)
-  return (lx * ly + ly * lz) + lz * lx;
-}
 
 struct BlockScan {
   int wave_tot[kBuildBlock / 64];
@@ -357,31 +353,7 @@ __global__ void collapse_kernel(const int32_t *__restrict__ frontier_bin, const 
   const int32_t b = frontier_bin[t];
   const int32_t slot = frontier_slot[t];
   int32_t ch[4];
-  int m = 0;
-  const BNode root = bn[b];
-  if (root.count > 0) {  // a leaf root: one leaf child
-    ch[m++] = b;
-  } else {
-    ch[m++] = root.left;
-    ch[m++] = root.left + 1;
-    while (m < 4) {
-      int pick = -1;
-      float best = -1.f;
-      for (int k = 0; k < m; k++) {
-        const BNode &c = bn[ch[k]];
-        if (c.count > 0) continue;
-        float ha = half_area(c.mn[0], c.mn[1], c.mn[2], c.mx[0], c.mx[1], c.mx[2]);
-        if (ha > best) {
-          best = ha;
-          pick = k;
-        }
-      }
-      if (pick < 0) break;
-      const int32_t open = ch[pick];
-      ch[pick] = bn[open].left;
-      ch[m++] = bn[open].left + 1;
-    }
-  }
+  const int m = collapse_children(bn, b, ch);
   Node4 nd;
   for (int k = 0; k < 4; k++) {
     if (k >= m) {

@@ -1,0 +1,231 @@
+// Closest-hit traversal of the 4-wide BVH and the binary-to-4-wide collapse step, written
+// once for the device kernels and for a host build of the same code.
+#pragma once
+#include "device_math.h"
+#include "device_path.h"
+#include "nova_internal.h"
+
+namespace nova_hip {
+
+struct TraceCtx {
+  const Node4 *nodes;
+  const TriMT *tris;
+  uint32_t *gstack;  // overflow entries: [level - lds levels][global thread]
+  uint32_t gstride;  // number of threads in the grid
+  // binary tree, only read to settle exact equal-t ties
+  const BNode *bnodes;
+  const int32_t *bparent, *bdepth, *leaf_of_pos;
+};
+
+// BoundingBox::halfArea (BoundingBox.cpp:162-165)
+NOVA_HD float half_area(float mnx, float mny, float mnz, float mxx, float mxy, float mxz) {
+  float lx = mxx - mnx, ly = mxy - mny, lz = mxz - mnz;
+  return (lx * ly + ly * lz) + lz * lx;
+}
+
+// Children of one 4-wide node: start from the two children of binary inner node b and
+// open the inner child with the largest half area until there are four (or only leaves
+// remain). A leaf b becomes a single leaf child. Returns the child count.
+NOVA_HD int collapse_children(const BNode *bn, int32_t b, int32_t ch[4]) {
+  const BNode root = bn[b];
+  if (root.count > 0) {
+    ch[0] = b;
+    return 1;
+  }
+  int m = 0;
+  ch[m++] = root.left;
+  ch[m++] = root.left + 1;
+  while (m < 4) {
+    int pick = -1;
+    float best = -1.f;
+    for (int k = 0; k < m; k++) {
+      const BNode &c = bn[ch[k]];
+      if (c.count > 0) continue;
+      const float ha = half_area(c.mn[0], c.mn[1], c.mn[2], c.mx[0], c.mx[1], c.mx[2]);
+      if (ha > best) {
+        best = ha;
+        pick = k;
+      }
+    }
+    if (pick < 0) break;
+    const int32_t open = ch[pick];
+    ch[pick] = bn[open].left;
+    ch[m++] = bn[open].left + 1;
+  }
+  return m;
+}
+
+// BoundingBox::intersect (BoundingBox.cpp:150-160) in its division form, std::min/std::max
+// operand order kept; a 0/0 slab (ray inside a face plane) spans the whole line.
+NOVA_HD float ref_box_entry(const BNode &b, f3 o, f3 d) {
+  float t[6];
+  const float lo[3] = {b.mn[0], b.mn[1], b.mn[2]}, hi[3] = {b.mx[0], b.mx[1], b.mx[2]};
+  const float oo[3] = {o.x, o.y, o.z}, dd[3] = {d.x, d.y, d.z};
+  for (int a = 0; a < 3; a++) {
+    float t1 = (lo[a] - oo[a]) / dd[a], t2 = (hi[a] - oo[a]) / dd[a];
+    if (t1 != t1 || t2 != t2) {
+      t1 = -__builtin_inff();
+      t2 = __builtin_inff();
+    }
+    t[2 * a] = t1;
+    t[2 * a + 1] = t2;
+  }
+  float t0 = gmin(t[0], t[1]), t1 = smax(t[0], t[1]);
+  t0 = smax(t0, gmin(t[2], t[3]));
+  t1 = gmin(t1, smax(t[2], t[3]));
+  t0 = smax(t0, gmin(t[4], t[5]));
+  t1 = gmin(t1, smax(t[4], t[5]));
+  return (t1 >= t0 && t1 > 0.f) ? t0 : 3.402823466e+38f;
+}
+
+// Settles a tie between two triangles with bit-identical t the way the reference's
+// depth-first walk does (native.cpp:56-114, ties kept by `t <= tmax`): the one it reaches
+// last wins. Within a binary leaf that is the higher position. Otherwise the walk enters
+// the nearer child of the two leaves' lowest common ancestor first (the left one when the
+// entry distances are equal) and the triangle under the other child wins -- provided the
+// walk still reaches it: below that child every box is tested against tmax = t, and a box
+// whose rounded entry distance exceeds t is culled there. Returns true when pos_new wins.
+#ifdef __HIP_DEVICE_COMPILE__
+__device__ __noinline__
+#else
+inline
+#endif
+    bool
+    reference_tie_winner_is_new(const TraceCtx &tc, int32_t pos_new, int32_t pos_old, f3 o, f3 d, float t) {
+  const int32_t leaf_new = tc.leaf_of_pos[pos_new], leaf_old = tc.leaf_of_pos[pos_old];
+  int32_t a = leaf_new, b = leaf_old;
+  if (a == b) return pos_new > pos_old;
+  int32_t da = tc.bdepth[a], db = tc.bdepth[b];
+  for (; da > db && a > 0; da--) a = tc.bparent[a];
+  for (; db > da && b > 0; db--) b = tc.bparent[b];
+  if (a == b) return pos_new > pos_old;
+  while (a > 0 && b > 0 && tc.bparent[a] != tc.bparent[b]) {
+    a = tc.bparent[a];
+    b = tc.bparent[b];
+  }
+  if (a <= 0 || b <= 0) return pos_new > pos_old;  // malformed tree: keep the leaf order
+  const int32_t left = tc.bnodes[tc.bparent[a]].left;
+  const float li = ref_box_entry(tc.bnodes[left], o, d), ri = ref_box_entry(tc.bnodes[left + 1], o, d);
+  const int32_t first = li > ri ? left + 1 : left;
+  const bool new_later = a != first;
+  // is the later triangle reached once the earlier one has set tmax = t?
+  const int32_t top = new_later ? a : b;
+  bool reached = (new_later ? (a == left ? li : ri) : (b == left ? li : ri)) < 3.402823466e+38f;
+  for (int32_t v = new_later ? leaf_new : leaf_old; reached && v != top && v > 0; v = tc.bparent[v]) reached = ref_box_entry(tc.bnodes[v], o, d) <= t;
+  return new_later ? reached : !reached;
+}
+
+// Closest hit over the 4-wide BVH. Boxes are tested with a slightly widened slab so no
+// box holding a hit the reference's division-form test accepts is culled; triangle
+// tests are exact, and exact equal-t ties are settled in the reference's visiting order.
+// Stack provides push(level, node) and at(level) for the traversal stack.
+template <bool kCount, class Stack>
+NOVA_HD int trace_closest(const TraceCtx &tc, Stack &stack, f3 o, f3 d, float tnear, float &t_out, float &u_out, float &v_out,
+                          uint32_t &nodes_visited, uint32_t &tri_tests) {
+  const float kUp = 1.000001f;
+  const float kInf = __builtin_inff();
+  const f3 inv = mk3(1.f / d.x, 1.f / d.y, 1.f / d.z);
+  // An axis whose reciprocal is infinite (zero or denormal direction component) is a
+  // containment test instead: inside the slab (boundary included) the whole line, else
+  // nothing -- what the reference's 0/0 slabs amount to in the oracle's reading.
+  const bool px = !(fabsf(inv.x) < kInf), py = !(fabsf(inv.y) < kInf), pz = !(fabsf(inv.z) < kInf);
+  const bool parallel = px || py || pz;
+  float tbest = 3.402823466e+38f;
+  int best = -1;
+  int32_t bpos = -1;
+  float bu = 0.f, bv = 0.f;
+  int sp = 0;
+  int32_t node = 0;
+  // A tree walk visits each node once; the cap only guards against a malformed tree.
+  for (uint32_t visits = 0; visits < (1u << 24); visits++) {
+    if (kCount) nodes_visited++;
+    const float4 *np = reinterpret_cast<const float4 *>(tc.nodes + node);
+    const float4 mnx = np[0], mxx = np[1], mny = np[2], mxy = np[3], mnz = np[4], mxz = np[5];
+    const int4 ch = *reinterpret_cast<const int4 *>(np + 6);
+    const int4 cn = *reinterpret_cast<const int4 *>(np + 7);
+    const float bmnx[4] = {mnx.x, mnx.y, mnx.z, mnx.w}, bmxx[4] = {mxx.x, mxx.y, mxx.z, mxx.w};
+    const float bmny[4] = {mny.x, mny.y, mny.z, mny.w}, bmxy[4] = {mxy.x, mxy.y, mxy.z, mxy.w};
+    const float bmnz[4] = {mnz.x, mnz.y, mnz.z, mnz.w}, bmxz[4] = {mxz.x, mxz.y, mxz.z, mxz.w};
+    const int chv[4] = {ch.x, ch.y, ch.z, ch.w}, cnv[4] = {cn.x, cn.y, cn.z, cn.w};
+    float tent[4];
+    int ord[4];
+    int nhit = 0;
+    for (int k = 0; k < 4; k++) {
+      float tx1 = (bmnx[k] - o.x) * inv.x, tx2 = (bmxx[k] - o.x) * inv.x;
+      float ty1 = (bmny[k] - o.y) * inv.y, ty2 = (bmxy[k] - o.y) * inv.y;
+      float tz1 = (bmnz[k] - o.z) * inv.z, tz2 = (bmxz[k] - o.z) * inv.z;
+      if (parallel) {
+        if (px) {
+          const bool in = bmnx[k] <= o.x && o.x <= bmxx[k];
+          tx1 = -kInf;
+          tx2 = in ? kInf : -kInf;
+        }
+        if (py) {
+          const bool in = bmny[k] <= o.y && o.y <= bmxy[k];
+          ty1 = -kInf;
+          ty2 = in ? kInf : -kInf;
+        }
+        if (pz) {
+          const bool in = bmnz[k] <= o.z && o.z <= bmxz[k];
+          tz1 = -kInf;
+          tz2 = in ? kInf : -kInf;
+        }
+      }
+      float t0 = fmaxf(fmaxf(fminf(tx1, tx2), fminf(ty1, ty2)), fminf(tz1, tz2));
+      float t1 = fminf(fminf(fmaxf(tx1, tx2), fmaxf(ty1, ty2)), fmaxf(tz1, tz2)) * kUp;
+      bool hit = chv[k] >= 0 && t0 <= t1 && t1 >= 0.f && t0 <= tbest * kUp;
+      if (hit) {
+        tent[nhit] = t0;
+        ord[nhit] = k;
+        nhit++;
+      }
+    }
+    // near-to-far order of the hit children
+    for (int a = 1; a < nhit; a++)
+      for (int b = a; b > 0 && tent[b] < tent[b - 1]; b--) {
+        float tf = tent[b];
+        tent[b] = tent[b - 1];
+        tent[b - 1] = tf;
+        int ti = ord[b];
+        ord[b] = ord[b - 1];
+        ord[b - 1] = ti;
+      }
+    int32_t next = -1;
+    for (int a = nhit - 1; a >= 0; a--) {  // far to near: push inner nodes, keep the nearest
+      const int k = ord[a];
+      if (cnv[k] > 0) continue;
+      if (next >= 0) stack.push(sp++, (uint32_t)next);
+      next = chv[k];
+    }
+    for (int a = 0; a < nhit; a++) {  // leaves now, near to far
+      const int k = ord[a];
+      if (cnv[k] == 0) continue;
+      if (tent[a] > tbest * kUp) continue;
+      const int32_t start = chv[k], cnt = cnv[k];
+      for (int32_t i = start; i < start + cnt; i++) {
+        const TriMT tr = tc.tris[i];
+        if (kCount) tri_tests++;
+        float tt, uu, vv;
+        if (mt_test(tr, o, d, tnear, tbest, tt, uu, vv) && (tt < tbest || best < 0 || reference_tie_winner_is_new(tc, i, bpos, o, d, tt))) {
+          tbest = tt;
+          bpos = i;
+          best = __builtin_bit_cast(int32_t, tr.a.w);
+          bu = uu;
+          bv = vv;
+        }
+      }
+    }
+    if (next >= 0) {
+      node = next;
+      continue;
+    }
+    if (sp == 0) break;
+    node = (int32_t)stack.at(--sp);
+  }
+  t_out = tbest;
+  u_out = bu;
+  v_out = bv;
+  return best;
+}
+
+}  // namespace nova_hip

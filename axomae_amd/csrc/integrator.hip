@@ -21,6 +21,7 @@
 #include "device_math.h"
 #include "device_path.h"
 #include "device_shading.h"
+#include "device_trace.h"
 #include "nova_internal.h"
 
 namespace nova_hip {
@@ -28,163 +29,22 @@ namespace nova_hip {
 constexpr int kTraceBlock = 256;
 constexpr int kLdsStack = 24;
 
-struct TraceCtx {
-  const Node4 *nodes;
-  const TriMT *tris;
-  uint32_t *gstack;     // overflow entries: [level - kLdsStack][global thread]
-  uint32_t gstride;     // number of threads in the grid
-  // binary tree, only read to settle exact equal-t ties
-  const BNode *bnodes;
-  const int32_t *bparent, *bdepth, *leaf_of_pos;
+// Traversal stack of one lane: the first kLdsStack levels in LDS (lane-interleaved), the
+// rest in a global overflow area strided by the grid size.
+struct LdsStack {
+  uint32_t *lds;
+  uint32_t *gstack;
+  uint32_t gstride, gtid, lane;
+  __device__ __forceinline__ void push(int sp, uint32_t v) {
+    if (sp < kLdsStack)
+      lds[sp * kTraceBlock + lane] = v;
+    else
+      gstack[(size_t)(sp - kLdsStack) * gstride + gtid] = v;
+  }
+  __device__ __forceinline__ uint32_t at(int sp) const {
+    return sp < kLdsStack ? lds[sp * kTraceBlock + lane] : gstack[(size_t)(sp - kLdsStack) * gstride + gtid];
+  }
 };
-
-// BoundingBox::intersect (BoundingBox.cpp:150-160) in its division form, std::min/std::max
-// operand order kept; a 0/0 slab (ray inside a face plane) spans the whole line.
-__device__ __forceinline__ float ref_box_entry(const BNode &b, f3 o, f3 d) {
-  auto slab = [](float lo, float hi, float oo, float dd, float &t1, float &t2) {
-    t1 = (lo - oo) / dd;
-    t2 = (hi - oo) / dd;
-    if (t1 != t1 || t2 != t2) {
-      t1 = -__builtin_inff();
-      t2 = __builtin_inff();
-    }
-  };
-  auto mn = [](float a, float b) { return b < a ? b : a; };
-  auto mx = [](float a, float b) { return a < b ? b : a; };
-  float tx1, tx2, ty1, ty2, tz1, tz2;
-  slab(b.mn[0], b.mx[0], o.x, d.x, tx1, tx2);
-  slab(b.mn[1], b.mx[1], o.y, d.y, ty1, ty2);
-  slab(b.mn[2], b.mx[2], o.z, d.z, tz1, tz2);
-  float t0 = mn(tx1, tx2), t1 = mx(tx1, tx2);
-  t0 = mx(t0, mn(ty1, ty2));
-  t1 = mn(t1, mx(ty1, ty2));
-  t0 = mx(t0, mn(tz1, tz2));
-  t1 = mn(t1, mx(tz1, tz2));
-  return (t1 >= t0 && t1 > 0.f) ? t0 : 3.402823466e+38f;
-}
-
-// Settles a tie between two triangles with bit-identical t the way the reference's
-// depth-first walk does (native.cpp:56-114, ties kept by `t <= tmax`): the one it reaches
-// last wins. Within a binary leaf that is the higher position; otherwise the walk enters
-// the nearer child of the two leaves' lowest common ancestor first (the left one when the
-// entry distances are equal), so the triangle under the other child wins.
-__device__ __noinline__ bool reference_visits_later(const TraceCtx &tc, int32_t pos_new, int32_t pos_old, f3 o, f3 d) {
-  int32_t a = tc.leaf_of_pos[pos_new], b = tc.leaf_of_pos[pos_old];
-  if (a == b) return pos_new > pos_old;
-  int32_t da = tc.bdepth[a], db = tc.bdepth[b];
-  for (; da > db && a > 0; da--) a = tc.bparent[a];
-  for (; db > da && b > 0; db--) b = tc.bparent[b];
-  if (a == b) return pos_new > pos_old;
-  while (a > 0 && b > 0 && tc.bparent[a] != tc.bparent[b]) {
-    a = tc.bparent[a];
-    b = tc.bparent[b];
-  }
-  if (a <= 0 || b <= 0) return pos_new > pos_old;  // malformed tree: keep the leaf order
-  const int32_t left = tc.bnodes[tc.bparent[a]].left;
-  const float li = ref_box_entry(tc.bnodes[left], o, d), ri = ref_box_entry(tc.bnodes[left + 1], o, d);
-  const int32_t first = li > ri ? left + 1 : left;
-  return a != first;
-}
-
-// Closest hit over the 4-wide BVH. Boxes are tested with a slightly widened slab so no
-// box holding a hit the reference's division-form test accepts is culled; triangle
-// tests are exact, and exact equal-t ties are settled in the reference's visiting order.
-template <bool kCount>
-__device__ __forceinline__ int trace_closest(const TraceCtx &tc, uint32_t *lds, uint32_t gtid, f3 o, f3 d, float tnear, float &t_out, float &u_out,
-                                             float &v_out, uint32_t &nodes_visited, uint32_t &tri_tests) {
-  const float kUp = 1.000001f;
-  // A zero direction component takes a huge finite reciprocal instead of infinity, so a ray
-  // running inside a box face plane gives 0 * big = 0 (inside the slab) rather than NaN.
-  const f3 inv = mk3(d.x != 0.f ? 1.f / d.x : copysignf(3.0e38f, d.x), d.y != 0.f ? 1.f / d.y : copysignf(3.0e38f, d.y),
-                     d.z != 0.f ? 1.f / d.z : copysignf(3.0e38f, d.z));
-  float tbest = 3.402823466e+38f;
-  int best = -1;
-  int32_t bpos = -1;
-  float bu = 0.f, bv = 0.f;
-  int sp = 0;
-  int32_t node = 0;
-  const int lane_off = threadIdx.x;
-  // A tree walk visits each node once; the cap only guards against a malformed tree.
-  for (uint32_t visits = 0; visits < (1u << 24); visits++) {
-    if (kCount) nodes_visited++;
-    const float4 *np = reinterpret_cast<const float4 *>(tc.nodes + node);
-    const float4 mnx = np[0], mxx = np[1], mny = np[2], mxy = np[3], mnz = np[4], mxz = np[5];
-    const int4 ch = *reinterpret_cast<const int4 *>(np + 6);
-    const int4 cn = *reinterpret_cast<const int4 *>(np + 7);
-    const float bmnx[4] = {mnx.x, mnx.y, mnx.z, mnx.w}, bmxx[4] = {mxx.x, mxx.y, mxx.z, mxx.w};
-    const float bmny[4] = {mny.x, mny.y, mny.z, mny.w}, bmxy[4] = {mxy.x, mxy.y, mxy.z, mxy.w};
-    const float bmnz[4] = {mnz.x, mnz.y, mnz.z, mnz.w}, bmxz[4] = {mxz.x, mxz.y, mxz.z, mxz.w};
-    const int chv[4] = {ch.x, ch.y, ch.z, ch.w}, cnv[4] = {cn.x, cn.y, cn.z, cn.w};
-    float tent[4];
-    int ord[4];
-    int nhit = 0;
-    for (int k = 0; k < 4; k++) {
-      float tx1 = (bmnx[k] - o.x) * inv.x, tx2 = (bmxx[k] - o.x) * inv.x;
-      float ty1 = (bmny[k] - o.y) * inv.y, ty2 = (bmxy[k] - o.y) * inv.y;
-      float tz1 = (bmnz[k] - o.z) * inv.z, tz2 = (bmxz[k] - o.z) * inv.z;
-      float t0 = fmaxf(fmaxf(fminf(tx1, tx2), fminf(ty1, ty2)), fminf(tz1, tz2));
-      float t1 = fminf(fminf(fmaxf(tx1, tx2), fmaxf(ty1, ty2)), fmaxf(tz1, tz2)) * kUp;
-      bool hit = chv[k] >= 0 && t0 <= t1 && t1 >= 0.f && t0 <= tbest * kUp;
-      if (hit) {
-        tent[nhit] = t0;
-        ord[nhit] = k;
-        nhit++;
-      }
-    }
-    // near-to-far order of the hit children
-    for (int a = 1; a < nhit; a++)
-      for (int b = a; b > 0 && tent[b] < tent[b - 1]; b--) {
-        float tf = tent[b];
-        tent[b] = tent[b - 1];
-        tent[b - 1] = tf;
-        int ti = ord[b];
-        ord[b] = ord[b - 1];
-        ord[b - 1] = ti;
-      }
-    int32_t next = -1;
-    for (int a = nhit - 1; a >= 0; a--) {  // far to near: push inner nodes, keep the nearest
-      const int k = ord[a];
-      if (cnv[k] > 0) continue;
-      if (next >= 0) {
-        if (sp < kLdsStack)
-          lds[sp * kTraceBlock + lane_off] = (uint32_t)next;
-        else
-          tc.gstack[(size_t)(sp - kLdsStack) * tc.gstride + gtid] = (uint32_t)next;
-        sp++;
-      }
-      next = chv[k];
-    }
-    for (int a = 0; a < nhit; a++) {  // leaves now, near to far
-      const int k = ord[a];
-      if (cnv[k] == 0) continue;
-      if (tent[a] > tbest * kUp) continue;
-      const int32_t start = chv[k], cnt = cnv[k];
-      for (int32_t i = start; i < start + cnt; i++) {
-        const TriMT tr = tc.tris[i];
-        if (kCount) tri_tests++;
-        float tt, uu, vv;
-        if (mt_test(tr, o, d, tnear, tbest, tt, uu, vv) && (tt < tbest || best < 0 || reference_visits_later(tc, i, bpos, o, d))) {
-          tbest = tt;
-          bpos = i;
-          best = __float_as_int(tr.a.w);
-          bu = uu;
-          bv = vv;
-        }
-      }
-    }
-    if (next >= 0) {
-      node = next;
-      continue;
-    }
-    if (sp == 0) break;
-    sp--;
-    node = (int32_t)(sp < kLdsStack ? lds[sp * kTraceBlock + lane_off] : tc.gstack[(size_t)(sp - kLdsStack) * tc.gstride + gtid]);
-  }
-  t_out = tbest;
-  u_out = bu;
-  v_out = bv;
-  return best;
-}
 
 // ------------------------------------------------------------------ kernels
 
@@ -241,7 +101,8 @@ __global__ __launch_bounds__(kTraceBlock) void extend_kernel(TraceCtx tc, Queues
     const f3 o = mk3(r[i], r[cap + i], r[2 * cap + i]);
     const f3 d = mk3(r[3 * cap + i], r[4 * cap + i], r[5 * cap + i]);
     float t, u, v;
-    const int prim = trace_closest<kCount>(tc, lds, gtid, o, d, 0.000001f, t, u, v, nv, nt);
+    LdsStack st{lds, tc.gstack, tc.gstride, gtid, threadIdx.x};
+    const int prim = trace_closest<kCount>(tc, st, o, d, 0.000001f, t, u, v, nv, nt);
     q.hit_prim[i] = prim;
     q.hit_t[i] = t;
     q.hit_u[i] = u;
@@ -349,7 +210,8 @@ __global__ __launch_bounds__(kTraceBlock) void closest_hit_kernel(TraceCtx tc, c
     const f3 d = mk3(rays[i * 6 + 3], rays[i * 6 + 4], rays[i * 6 + 5]);
     float t, u, v;
     uint32_t nv = 0, nt = 0;
-    const int p = trace_closest<false>(tc, lds, gtid, o, d, 0.000001f, t, u, v, nv, nt);
+    LdsStack st{lds, tc.gstack, tc.gstride, gtid, threadIdx.x};
+    const int p = trace_closest<false>(tc, st, o, d, 0.000001f, t, u, v, nv, nt);
     prim[i] = p;
     tuv[i * 3] = p >= 0 ? t : 0.f;
     tuv[i * 3 + 1] = u;

@@ -8,6 +8,7 @@
 
 #include "../../axomae_amd/csrc/device_path.h"
 #include "../../axomae_amd/csrc/device_scene.h"
+#include "../../axomae_amd/csrc/device_trace.h"
 
 using namespace nova_hip;
 
@@ -17,8 +18,17 @@ const uint32_t kTable[256][32] = {
 };
 }
 
-extern "C" int emu_render(const nova_hip_scene_desc *sd, const nova_hip_render_params *p, float *accum, float *final_out) {
-  // ---- flatten (as nova_hip_scene_upload does, on the host)
+namespace {
+// Scene flattening as nova_hip_scene_upload does it, on the host.
+struct EmuScene {
+  uint64_t ntri = 0;
+  std::vector<TriMT> tri;
+  std::vector<ShadeTri> shade;
+  std::vector<float> aabb, cent;
+  std::vector<MeshXf> mxf;
+};
+
+void flatten_scene(const nova_hip_scene_desc *sd, EmuScene &es) {
   uint64_t nvert = 0, ntri = 0;
   for (uint32_t m = 0; m < sd->mesh_count; m++) {
     nvert += sd->meshes[m].vertex_count;
@@ -47,12 +57,111 @@ extern "C" int emu_render(const nova_hip_scene_desc *sd, const nova_hip_render_p
       refs[to++] = TriRef{(uint32_t)(md.indices[i] + vo), (uint32_t)(md.indices[i + 1] + vo), (uint32_t)(md.indices[i + 2] + vo), m};
     vo += md.vertex_count;
   }
-  std::vector<TriMT> tri(ntri);
-  std::vector<ShadeTri> shade(ntri);
-  std::vector<float> aabb(ntri * 6), cent(ntri * 3);
+  es.ntri = ntri;
+  es.tri.resize(ntri);
+  es.shade.resize(ntri);
+  es.aabb.resize(ntri * 6);
+  es.cent.resize(ntri * 3);
   for (uint32_t t = 0; t < ntri; t++)
-    flatten_triangle(t, refs[t], pos.data(), nrm.data(), tan.data(), bit.data(), uv.data(), xf.data(), mesh_mat.data(), tri.data(), shade.data(),
-                     aabb.data(), cent.data());
+    flatten_triangle(t, refs[t], pos.data(), nrm.data(), tan.data(), bit.data(), uv.data(), xf.data(), mesh_mat.data(), es.tri.data(),
+                     es.shade.data(), es.aabb.data(), es.cent.data());
+  es.mxf = mxf;
+}
+
+struct HostStack {
+  std::vector<uint32_t> v;
+  void push(int sp, uint32_t x) {
+    if ((size_t)sp >= v.size()) v.resize(sp + 64);
+    v[sp] = x;
+  }
+  uint32_t at(int sp) const { return v[sp]; }
+};
+}  // namespace
+
+// Closest hit through the device traversal code on a binary BVH given by the caller (the
+// reference builder's output, which the GPU build reproduces): parent/depth/leaf maps and
+// the 4-wide collapse are derived here as the GPU build derives them.
+extern "C" int emu_closest_hit(const nova_hip_scene_desc *sd, const BNode *bnodes, int32_t nbnodes, const int32_t *prim_idx, const float *rays,
+                               int64_t nrays, int32_t *prim_out, float *tuv_out, uint64_t *counts) {
+  EmuScene es;
+  flatten_scene(sd, es);
+  const uint32_t n = (uint32_t)es.ntri;
+  std::vector<int32_t> parent(nbnodes, -1), depth(nbnodes, 0), leaf_of_pos(n, -1);
+  std::vector<int32_t> order{0};
+  for (size_t k = 0; k < order.size(); k++) {
+    const int32_t i = order[k];
+    const BNode &b = bnodes[i];
+    if (b.count > 0) {
+      for (int32_t j = 0; j < b.count; j++) leaf_of_pos[b.left + j] = i;
+      continue;
+    }
+    for (int32_t c = b.left; c <= b.left + 1; c++) {
+      parent[c] = i;
+      depth[c] = depth[i] + 1;
+      order.push_back(c);
+    }
+  }
+  std::vector<Node4> wide(1);
+  std::vector<std::pair<int32_t, int32_t>> frontier{{0, 0}};
+  for (size_t k = 0; k < frontier.size(); k++) {
+    int32_t ch[4];
+    const int m = collapse_children(bnodes, frontier[k].first, ch);
+    Node4 nd;
+    for (int c = 0; c < 4; c++) {
+      if (c >= m) {
+        nd.minx[c] = nd.miny[c] = nd.minz[c] = 1.f;
+        nd.maxx[c] = nd.maxy[c] = nd.maxz[c] = -1.f;
+        nd.child[c] = -1;
+        nd.count[c] = 0;
+        continue;
+      }
+      const BNode &b = bnodes[ch[c]];
+      nd.minx[c] = b.mn[0];
+      nd.miny[c] = b.mn[1];
+      nd.minz[c] = b.mn[2];
+      nd.maxx[c] = b.mx[0];
+      nd.maxy[c] = b.mx[1];
+      nd.maxz[c] = b.mx[2];
+      if (b.count > 0) {
+        nd.child[c] = b.left;
+        nd.count[c] = b.count;
+      } else {
+        nd.child[c] = (int32_t)wide.size();
+        nd.count[c] = 0;
+        frontier.push_back({ch[c], (int32_t)wide.size()});
+        wide.emplace_back();
+      }
+    }
+    wide[frontier[k].second] = nd;
+  }
+  std::vector<TriMT> tri_leaf(n);
+  for (uint32_t i = 0; i < n; i++) tri_leaf[i] = es.tri[prim_idx[i]];
+  TraceCtx tc{wide.data(), tri_leaf.data(), nullptr, 0, bnodes, parent.data(), depth.data(), leaf_of_pos.data()};
+  HostStack st;
+  uint32_t nv = 0, nt = 0;
+  for (int64_t r = 0; r < nrays; r++) {
+    const float *ry = rays + r * 6;
+    float t, u, v;
+    const int p = trace_closest<true>(tc, st, mk3(ry[0], ry[1], ry[2]), mk3(ry[3], ry[4], ry[5]), 0.000001f, t, u, v, nv, nt);
+    prim_out[r] = p;
+    tuv_out[r * 3] = p >= 0 ? t : 0.f;
+    tuv_out[r * 3 + 1] = u;
+    tuv_out[r * 3 + 2] = v;
+  }
+  if (counts) {
+    counts[0] = nv;
+    counts[1] = nt;
+  }
+  return 0;
+}
+
+extern "C" int emu_render(const nova_hip_scene_desc *sd, const nova_hip_render_params *p, float *accum, float *final_out) {
+  EmuScene es;
+  flatten_scene(sd, es);
+  const uint64_t ntri = es.ntri;
+  const std::vector<TriMT> &tri = es.tri;
+  const std::vector<ShadeTri> &shade = es.shade;
+  std::vector<MeshXf> mxf = es.mxf;
   std::vector<TexRec> texrec(sd->texture_count);
   std::vector<uint32_t> u32pool;
   std::vector<float> f32pool;

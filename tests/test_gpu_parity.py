@@ -11,40 +11,9 @@ import pytest
 
 from axomae_amd import scenes
 from conftest import rel_rmse
+from scene_cases import BVH_SCENES, hit_test_rays
 
 gpu = pytest.mark.gpu
-
-
-def soup_scene(n, seed, lo=-1.0, hi=1.0, size=0.05, duplicate=0):
-    """n random triangles (one mesh) with full attributes; `duplicate` copies of the first triangle."""
-    rng = np.random.default_rng(seed)
-    c = rng.uniform(lo, hi, size=(n, 1, 3))
-    v = c + rng.uniform(-size, size, size=(n, 3, 3))
-    if duplicate:
-        v[1:1 + duplicate] = v[0]
-    v = v.reshape(-1, 3)
-    nrm = np.tile([0.0, 0.0, 1.0], (len(v), 1))
-    tan = np.tile([1.0, 0.0, 0.0], (len(v), 1))
-    bit = np.tile([0.0, 1.0, 0.0], (len(v), 1))
-    uv = rng.uniform(0, 1, size=(len(v), 2))
-    sc = scenes.Scene(name=f"soup{n}")
-    m = sc.principled((0.5, 0.5, 0.5), roughness=0.5)
-    f = lambda a: np.ascontiguousarray(np.asarray(a, dtype=np.float64).astype(np.float32).reshape(-1))
-    sc.meshes.append(scenes.Mesh(f(v), f(nrm), f(tan), f(bit), f(uv), np.arange(len(v), dtype=np.uint32), m))
-    sc.envmap = sc.add_texture(scenes.sky_envmap(64, 32, (0.3, 0.9, 0.2)))
-    scenes.set_camera(sc, (0.0, 0.0, 3.0), (0.0, 0.0, 0.0), 50.0, 1.0)
-    return sc
-
-
-BVH_SCENES = {
-    "box_room": lambda: scenes.box_room(),
-    "terrain_spheres": lambda: scenes.scaled_million(scale_cells=48, spheres=6, sphere_level=3),
-    "soup_5000": lambda: soup_scene(5000, 1),
-    "soup_duplicates": lambda: soup_scene(600, 2, duplicate=200),
-    "soup_int_max_fallback": lambda: soup_scene(3000, 3, lo=-3000.0, hi=3000.0, size=40.0),
-    "soup_int_max_single_leaf": lambda: soup_scene(2000, 4, lo=100.0, hi=3000.0, size=40.0),
-    "single_triangle": lambda: soup_scene(1, 5),
-}
 
 
 @pytest.fixture(scope="module")
@@ -91,37 +60,13 @@ def test_bvh_primitive_order_bit_exact(oracle, engine, name):
     assert key(nodes) == key(onodes)
 
 
-def camera_rays(sc, w, h, frame=0):
-    """Primary rays of the reference camera for pixel centres (no jitter), n x 6."""
-    P = np.asarray(sc.inv_P, dtype=np.float64).reshape(4, 4).T
-    V = np.asarray(sc.inv_V, dtype=np.float64).reshape(4, 4).T
-    ys, xs = np.mgrid[0:h, 0:w]
-    nx = (2 * xs - w) / w
-    ny = (h - 2 * (h - ys)) / h
-    pts = np.stack([nx, ny, -np.ones_like(nx), np.ones_like(nx)], -1) @ P.T
-    o = (pts / pts[..., 3:]) @ V.T
-    d = np.stack([nx, ny, -np.ones_like(nx), np.zeros_like(nx)], -1) @ P.T
-    d = np.concatenate([d[..., :2], -np.ones_like(nx)[..., None], np.zeros_like(nx)[..., None]], -1) @ V.T
-    d = d[..., :3] / np.linalg.norm(d[..., :3], axis=-1, keepdims=True)
-    return np.concatenate([o[..., :3], d], -1).reshape(-1, 6).astype(np.float32)
-
-
-def random_rays(n, seed, origin_box=1.5):
-    rng = np.random.default_rng(seed)
-    o = rng.uniform(-origin_box, origin_box, size=(n, 3))
-    d = rng.standard_normal((n, 3))
-    d /= np.linalg.norm(d, axis=1, keepdims=True)
-    return np.concatenate([o, d], 1).astype(np.float32)
-
-
 @gpu
-@pytest.mark.parametrize("name", ["box_room", "terrain_spheres", "soup_5000", "soup_int_max_fallback"])
+@pytest.mark.parametrize("name", ["box_room", "terrain_spheres", "soup_5000", "soup_duplicates", "soup_int_max_fallback"])
 def test_closest_hit_matches_reference_traversal(oracle, engine, name):
+    """Primitive ids, t, u and v identical to the reference walk on every ray (the gate is
+    99.99% of ids; the traversal's tie rule and parallel-axis slabs make it exact)."""
     sc = BVH_SCENES[name]()
-    scale = 1.0 if "int_max" not in name else 3000.0
-    rays = np.concatenate([camera_rays(sc, 96, 96), random_rays(40000, 7, 1.5 * scale)])
-    if "int_max" in name:
-        rays[:, :3] *= 1.0
+    rays = hit_test_rays(sc, name, 40000)
     engine.upload(sc)
     engine.build_bvh()
     prim, tuv = engine.closest_hit(rays)
@@ -129,19 +74,10 @@ def test_closest_hit_matches_reference_traversal(oracle, engine, name):
     os_ = oracle.OracleScene(sc)
     oprim, otuv = os_.closest_hit(rays)
     same = prim == oprim
-    frac = same.mean()
-    print(f"[hit] {name}: {len(rays)} rays, match {frac:.6f}, hits {np.mean(oprim >= 0):.3f}", flush=True)
-    # exact t/u/v wherever the primitive agrees and is a hit
-    h = same & (oprim >= 0)
-    np.testing.assert_array_equal(tuv[h], otuv[h])
-    # mismatches only at near ties: equal t within 1e-6 relative, or a barycentric edge
+    print(f"[hit] {name}: {len(rays)} rays, match {same.mean():.6f}, hits {np.mean(oprim >= 0):.3f}", flush=True)
     bad = np.where(~same)[0]
-    for k in bad:
-        t1, t2 = tuv[k, 0], otuv[k, 0]
-        both = prim[k] >= 0 and oprim[k] >= 0
-        edge = min(tuv[k, 1], tuv[k, 2], 1 - tuv[k, 1] - tuv[k, 2], otuv[k, 1], otuv[k, 2], 1 - otuv[k, 1] - otuv[k, 2]) < 1e-6
-        assert both and (abs(t1 - t2) <= 1e-6 * max(abs(t2), 1e-30) or edge), (k, prim[k], oprim[k], tuv[k], otuv[k])
-    assert frac >= 0.9999
+    assert len(bad) == 0, [(int(k), int(prim[k]), int(oprim[k]), tuv[k].tolist(), otuv[k].tolist()) for k in bad[:8]]
+    np.testing.assert_array_equal(tuv, otuv)
 
 
 @gpu

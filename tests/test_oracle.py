@@ -13,7 +13,7 @@ import numpy as np
 import pytest
 
 from axomae_amd import scenes
-from test_gpu_parity import BVH_SCENES, camera_rays, random_rays, soup_scene
+from scene_cases import BVH_SCENES, camera_rays, random_rays, soup_scene
 
 
 # ------------------------------------------------------------ partition
