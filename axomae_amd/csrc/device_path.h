@@ -85,13 +85,48 @@ NOVA_HD bool mt_test(const TriMT &tr, f3 o, f3 d, float tnear, float tbest, floa
   return true;
 }
 
+// Path-replay adjoint inputs: the primal radiance of every path slot (the batch's partial
+// buffers), the loss gradient per pixel (scaled) and the gradient pool (FP32 texel layout).
+struct AdjointCtx {
+  const float *ltot;
+  const float *dimg;
+  float scale;
+  float *grad;
+};
+
+NOVA_HD void add_grad(float *p, float v) {
+#ifdef __HIP_DEVICE_COMPILE__
+  unsafeAtomicAdd(p, v);
+#else
+  *p += v;
+#endif
+}
+
+// Adds g[ch] * (bilinear weight) to the gradient of channel ch of the four texels an FP32
+// texture lookup blended.
+NOVA_HD void splat_texture_grad(const ShadeCtx &sc, float *grad, int tex, float u, float v, const float *g, int nch) {
+  unsigned idx[4];
+  float w[4];
+  bilinear_taps(sc, tex, u, v, idx, w);
+  const TexRec t = sc.tex[tex];
+  float *base = grad + t.offset;
+  for (int k = 0; k < 4; k++)
+    for (int ch = 0; ch < nch && ch < t.channels; ch++)
+      if (g[ch] != 0.f) add_grad(base + (size_t)idx[k] * t.channels + ch, w[k] * g[ch]);
+}
+
 // One path segment after its closest hit: the miss/envmap rule, the hit record, the
 // Principled scatter, the Russian-roulette extension and Ray::spawn. Updates the path
 // state (T, L, A) front to back (see integrator.hip) and returns true when the path
 // continues with the spawned ray (no, nd); on false the path is finished and L, A hold
 // its radiance and alpha.
+// With kAdj the segment is replayed: after a scatter the radiance still to come,
+// R = L_path - L (everything after this vertex, T c V_next), gives the loss gradient of
+// the vertex's weight c; it is pushed to the albedo and roughness texels through
+// d ln c / d(param) and the bilinear weights.
+template <bool kAdj = false>
 NOVA_HD bool shade_path(const ShadeParams &sp, const ShadeCtx &sc, const uint32_t *sobol, uint32_t slot, f3 o, f3 d, int prim, float ht, float hu,
-                        float hv, f3 &T, f3 &L, float &A, f3 &no, f3 &nd, bool &escaped) {
+                        float hv, f3 &T, f3 &L, float &A, f3 &no, f3 &nd, bool &escaped, const AdjointCtx *adj = nullptr) {
   escaped = false;
   if (prim < 0) {  // Li on a miss: envmap(ray.direction) (Integrator.cpp:93-97)
     escaped = true;
@@ -150,7 +185,10 @@ NOVA_HD bool shade_path(const ShadeParams &sp, const ShadeCtx &sc, const uint32_
   Principled b;
   b.configure(bi);
   b.construct(uc);
-  const VNDF ggx(b.anisotropy, b.roughness);
+  float r_sampling = b.roughness;
+  if (sc.sampling_roughness >= 0 && mat.roughness_f32 >= 0) r_sampling = sample_f32_bilinear(sc, sc.sampling_roughness, h.u, h.v).x;
+  const VNDF ggx(b.anisotropy, r_sampling);  // samples and weights by its pdf
+  const VNDF ggx_e(b.anisotropy, b.roughness);  // BSDF value
   if ((b.flags(ggx) & (BX_GLOSSY | BX_REFLECTION)) && bi.wo_dot_ng >= 0) {
     // the corrected shading normal reconfigures the frame; the chosen lobe and its weights stay
     bi.ns = normalize(correct_specular_shading_normal(bi.ng, wo, bi.ns));
@@ -165,16 +203,19 @@ NOVA_HD bool shade_path(const ShadeParams &sp, const ShadeCtx &sc, const uint32_
   // PrincipledBSDF::sample_f (BSDF.h:107-130)
   BsdfSample smp;
   bool ok = false;
+  f3 dlna = mk3(0.f, 0.f, 0.f);
+  float dlnr = 0.f;
   if (b.lobe != LOBE_NONE) {
     const f3 lwo = normalize(mul_t(b.frame, wo));
     if ((b.flags(ggx) & (BX_REFLECTION | BX_TRANSMISSION)) && lwo.z != 0.f) {
       if (b.lobe == LOBE_DIFFUSE)
         ok = sample_disney(b, lwo, u0, u1, smp);
       else if (b.lobe == LOBE_SPECULAR)
-        ok = sample_specular(b, ggx, lwo, u0, u1, smp);
+        ok = sample_specular(b, ggx, ggx_e, lwo, u0, u1, smp);
       else
-        ok = sample_dielectric(b, ggx, lwo, uc2, u0, u1, smp);
+        ok = sample_dielectric(b, ggx, ggx_e, lwo, uc2, u0, u1, smp);
       if (ok && ((smp.f.x == 0.f && smp.f.y == 0.f && smp.f.z == 0.f) || smp.pdf == 0.f || smp.wi.z == 0.f)) ok = false;
+      if (kAdj && ok) bsdf_log_derivatives(b, ggx_e, lwo, smp, dlna, dlnr);
       if (ok) smp.wi = normalize(mul(b.frame, smp.wi));
     }
   }
@@ -187,18 +228,38 @@ NOVA_HD bool shade_path(const ShadeParams &sp, const ShadeCtx &sc, const uint32_
     Le = mk3(e4.x * ONE_255, e4.y * ONE_255, e4.z * ONE_255) * mat.emission_scale;
   }
   bool survive = true;
+  int rr_channel = -1;  // channel whose T c set q < 1 (the rescale c / q depends on it)
   if (sp.rr) {  // Russian-roulette extension: one more dimension after the scatter's four
     const f3 tp = T * c;
     const float qv = fminf(1.f, fmaxf(tp.x, fmaxf(tp.y, tp.z)));
     const float ur = sobol_dim(sobol, sp.dim0 + 4, g);
-    if (!(ur < qv))
+    if (!(ur < qv)) {
       survive = false;
-    else
+    } else {
       c = c / qv;
+      if (qv < 1.f) rr_channel = (tp.x == qv) ? 0 : (tp.y == qv) ? 1 : 2;
+    }
   }
   L = L + T * Le;
   A = A + 1.f;
   if (!survive) return false;
+  if (kAdj) {
+    const uint32_t npix = (uint32_t)sp.fp.width * (uint32_t)sp.fp.height;
+    const uint32_t px = pixel_offset(sp.fp, slot % npix);
+    const float *lt = adj->ltot + (size_t)(slot / npix) * npix * 4 + px;
+    const float *di = adj->dimg + px;
+    // loss gradient of this vertex's weight, per channel: dimg * (L_path - L)
+    const float G[3] = {adj->scale * di[0] * (lt[0] - L.x), adj->scale * di[1] * (lt[1] - L.y), adj->scale * di[2] * (lt[2] - L.z)};
+    float ga[3] = {G[0] * dlna.x, G[1] * dlna.y, G[2] * dlna.z};
+    float gr[1] = {(G[0] + G[1] + G[2]) * dlnr};
+    if (rr_channel >= 0) {  // c_ch / q with q = T_m c_m: ln q moves with channel m only
+      const float dm[3] = {dlna.x, dlna.y, dlna.z};
+      ga[rr_channel] -= (G[0] + G[1] + G[2]) * dm[rr_channel];
+      gr[0] = 0.f;
+    }
+    if (mat.albedo_f32 >= 0) splat_texture_grad(sc, adj->grad, mat.albedo_f32, h.u, h.v, ga, 3);
+    if (mat.roughness_f32 >= 0) splat_texture_grad(sc, adj->grad, mat.roughness_f32, h.u, h.v, gr, 1);
+  }
   T = T * c;
   // Ray::spawn (Ray.h:22-27)
   const f3 wi = smp.wi;

@@ -16,6 +16,7 @@ struct ShadeCtx {
   const ShadeTri *shade;
   const float *acos_lut;
   int32_t envmap;
+  int32_t sampling_roughness;  // -1, or the FP32 texture that drives sampling (gradient checks)
 };
 
 // ImageTexture<uint32_t>::sample, CPU branch: bilinear over 4 texels with ensure_bound
@@ -86,6 +87,30 @@ NOVA_HD float4 sample_f32_bilinear(const ShadeCtx &c, int id, float u, float v) 
     out[q] = vy * top + ty * bot;
   }
   return make_float4(out[0], out[1], out[2], out[3]);
+}
+
+// The four texel indices and weights sample_f32_bilinear blends (for scattering gradients).
+NOVA_HD void bilinear_taps(const ShadeCtx &c, int id, float u, float v, unsigned idx[4], float w[4]) {
+  const TexRec t = c.tex[id];
+  u = wrap_uv(u);
+  v = wrap_uv(v);
+  u = t.invert_u ? 1 - u : u;
+  v = t.invert_v ? 1 - v : v;
+  const float pi = fabsf(u * (float)(t.width - 1.f));
+  const float pj = fabsf(v * (float)(t.height - 1.f));
+  unsigned i = (unsigned)floorf(pi), j = (unsigned)floorf(pj);
+  if (i + 1 >= (unsigned)t.width) i = t.width - 2;
+  if (j + 1 >= (unsigned)t.height) j = t.height - 2;
+  const float tx = pi - (float)i, ty = pj - (float)j;
+  const float hd = 1 - tx, vy = 1 - ty;
+  idx[0] = j * t.width + i;
+  idx[1] = j * t.width + i + 1;
+  idx[2] = (j + 1) * t.width + i;
+  idx[3] = (j + 1) * t.width + i + 1;
+  w[0] = vy * hd;
+  w[1] = vy * tx;
+  w[2] = ty * hd;
+  w[3] = ty * tx;
 }
 
 // uv2index + uvToPixel (TextureContext.h:52-58)
@@ -273,10 +298,12 @@ NOVA_HD f3 cosine_hemisphere(float u0, float u1) {
 // VNDF (BxDF_math.h:231-342)
 struct VNDF {
   float ax, ay, ax_inv, ay_inv;
+  float a, da_dr;  // alpha = roughness^2 (1e-6 at roughness 0) and its derivative
   NOVA_HD VNDF(float aniso, float rough) {
     aniso = clampf(aniso, -1.f, 1.f);
     float aspect = msqrt(1.f - 0.9f * aniso);
-    float a = rough != 0 ? rough * rough : 1e-6f;
+    a = rough != 0 ? rough * rough : 1e-6f;
+    da_dr = rough != 0 ? 2.f * rough : 0.f;
     ax = a / aspect;
     ay = a * aspect;
     ax_inv = 1.f / ax;
@@ -316,6 +343,20 @@ struct VNDF {
     return normalize(mk3(ax * nh.x, ay * nh.y, smax(1e-6f, nh.z)));
   }
   NOVA_HD bool full_specular() const { return smax(ax, ay) < 1e-5f; }
+  // d ln D / d alpha: with ax, ay both proportional to alpha, D = 1 / (pi a^2 cos^4 (1+e)^2)
+  // where e = tan^2 (cos^2phi/ax^2 + sin^2phi/ay^2) scales as 1/a^2.
+  NOVA_HD float dlnD_da(f3 wm) const {
+    const float e = tan2theta(wm) * (cos2phi(wm) * sqr(ax_inv) + sin2phi(wm) * sqr(ay_inv));
+    return 2.f * (e - 1.f) / (a * (1.f + e));
+  }
+  // d lambda / d alpha: lambda = (sqrt(1 + a2 t2) - 1) / 2 with a2 scaling as alpha^2.
+  NOVA_HD float dlambda_da(f3 v) const {
+    const float t2 = tan2theta(v);
+    if (isinf(t2) || isnan(t2)) return 0.f;
+    const float a2 = sqr(ax) * cos2phi(v) + sqr(ay) * sin2phi(v);
+    return a2 * t2 / (2.f * a * msqrt(1 + a2 * t2));
+  }
+  NOVA_HD float dlnG_da(f3 wo, f3 wi) const { return -(dlambda_da(wo) + dlambda_da(wi)) / (1.f + lambda(wo) + lambda(wi)); }
 };
 
 enum : unsigned { BX_REFLECTION = 1, BX_TRANSMISSION = 2, BX_DIFFUSE = 4, BX_GLOSSY = 8, BX_SPECULAR = 16 };
@@ -394,9 +435,13 @@ struct Principled {
   }
 };
 
+enum : int { SAMPLE_DIFFUSE = 0, SAMPLE_MICROFACET = 1, SAMPLE_DELTA = 2 };
+
 struct BsdfSample {
   f3 f, wi;
   float pdf, costheta;
+  f3 wm;     // microfacet normal of a microfacet sample (local frame)
+  int kind;  // SAMPLE_*
 };
 
 // DisneyDiffuseBxDF::sample_f (BxDF.h:295-325)
@@ -417,11 +462,14 @@ NOVA_HD bool sample_disney(const Principled &b, f3 wo, float u0, float u1, BsdfS
   s.pdf = b.lobe_prob * (ci * 0.318309886183790671538f);
   s.costheta = ci;
   s.wi = wi;
+  s.kind = SAMPLE_DIFFUSE;
   return true;
 }
 
 // SpecularBxDF::sample_f (BxDF.h:367-397)
-NOVA_HD bool sample_specular(const Principled &b, const VNDF &g, f3 wo, float u0, float u1, BsdfSample &s) {
+// g samples and weights by its pdf; ge gives the BSDF value (the same distribution except
+// in the detached-sampling gradient checks).
+NOVA_HD bool sample_specular(const Principled &b, const VNDF &g, const VNDF &ge, f3 wo, float u0, float u1, BsdfSample &s) {
   f3 wm = g.sample(wo, u0, u1);
   f3 wi = reflect(-wo, wm);
   const float co = wo.z, ci = wi.z;
@@ -431,17 +479,19 @@ NOVA_HD bool sample_specular(const Principled &b, const VNDF &g, f3 wo, float u0
   f3 F0 = mk3(0.04f * (1.f - m) + m * b.albedo.x, 0.04f * (1.f - m) + m * b.albedo.y, 0.04f * (1.f - m) + m * b.albedo.z);
   const float p = pow5(1.f - wmo);
   F0 = mk3(F0.x + (1.f - F0.x) * p, F0.y + (1.f - F0.y) * p, F0.z + (1.f - F0.z) * p);
-  const f3 f = F0 * g.D(wm) * g.G(wo, wi) / (4 * co * ci);
+  const f3 f = F0 * ge.D(wm) * ge.G(wo, wi) / (4 * co * ci);
   const float pdf = g.pdf(wo, wm) / (4 * wmo);
   s.f = b.lobe_weight * f;
   s.pdf = b.lobe_prob * pdf;
   s.wi = wi;
   s.costheta = ci;
+  s.wm = wm;
+  s.kind = SAMPLE_MICROFACET;
   return true;
 }
 
 // DielectricBxDF::sample_f and its four branches (BxDF.h:403-585)
-NOVA_HD bool sample_dielectric(const Principled &b, const VNDF &g, f3 wo, float uc, float u0, float u1, BsdfSample &s) {
+NOVA_HD bool sample_dielectric(const Principled &b, const VNDF &g, const VNDF &ge, f3 wo, float uc, float u0, float u1, BsdfSample &s) {
   const float ior = b.eta.x;
   if (g.full_specular() || ior == 1.f) {
     const float R = fresnel_real(ior, wo.z), T = 1 - R;
@@ -454,6 +504,7 @@ NOVA_HD bool sample_dielectric(const Principled &b, const VNDF &g, f3 wo, float 
       s.f = b.lobe_weight * b.albedo * R / ci;
       s.pdf = b.lobe_prob * pr / (pr + pt);
       s.wi = wi;
+      s.kind = SAMPLE_DELTA;
       return true;
     }
     bool ok = true;
@@ -467,6 +518,7 @@ NOVA_HD bool sample_dielectric(const Principled &b, const VNDF &g, f3 wo, float 
     s.costheta = ci;
     s.pdf = b.lobe_prob * pt / (pr + pt);
     s.wi = wi;
+    s.kind = SAMPLE_DELTA;
     return true;
   }
   f3 wm = g.sample(wo, u0, u1);
@@ -477,12 +529,14 @@ NOVA_HD bool sample_dielectric(const Principled &b, const VNDF &g, f3 wo, float 
   if (uc < (pr / (pr + pt))) {
     f3 wi = reflect(-wo, wm);
     if (!same_hemi(wo, wi)) return false;
-    const f3 f = b.albedo * g.D(wm) * g.G(wo, wi) * R / (4 * wi.z * wo.z);
+    const f3 f = b.albedo * ge.D(wm) * ge.G(wo, wi) * R / (4 * wi.z * wo.z);
     const float pdf = g.pdf(wo, wm) / (4 * absdot(wo, wm)) * pr / (pr + pt);
     s.f = b.lobe_weight * f;
     s.pdf = b.lobe_prob * pdf;
     s.wi = wi;
     s.costheta = abscos(wi);
+    s.wm = wm;
+    s.kind = SAMPLE_MICROFACET;
     return true;
   }
   f3 wi = mk3(0, 0, 0);
@@ -495,13 +549,51 @@ NOVA_HD bool sample_dielectric(const Principled &b, const VNDF &g, f3 wo, float 
   const float denom = sqr(dot(wi, wm) + dot(wo, wm) / eta);
   const float jac = absdot(wi, wm) / denom;
   const float pdf = g.pdf(wo, wm) * jac * pt / (pt + pr);
-  f3 f = b.albedo * T * g.D(wm) * g.G(wo, wi) * fabsf(dot(wi, wm) * dot(wo, wm) / (ci * co * denom));
+  f3 f = b.albedo * T * ge.D(wm) * ge.G(wo, wi) * fabsf(dot(wi, wm) * dot(wo, wm) / (ci * co * denom));
   f = f / sqr(eta);
   s.f = b.lobe_weight * f;
   s.costheta = abscos(wi);
   s.pdf = b.lobe_prob * pdf;
   s.wi = wi;
+  s.wm = wm;
+  s.kind = SAMPLE_MICROFACET;
   return true;
+}
+
+// Derivatives of ln c, c = f cos / pdf, for a sample taken with everything but the BSDF
+// value held fixed: dlna[ch] with respect to the albedo channel ch, dlnr with respect to
+// the roughness. Diffuse: f = w 1.25 albedo/pi tail(r), tail = 0.5 + fi fo (1/(co+ci) - 0.5),
+// fi = 1 + (r^2 (h.wi)^2 - 1) (1-ci)^5. Microfacet lobes: f proportional to D G in alpha = r^2;
+// the specular lobe's Fresnel F = F0 + (1-F0) p with F0 = 0.04 (1-m) + m albedo. Delta lobes
+// do not depend on the roughness. wo and s.wi are in the local frame.
+NOVA_HD void bsdf_log_derivatives(const Principled &b, const VNDF &ge, f3 wo, const BsdfSample &s, f3 &dlna, float &dlnr) {
+  const f3 inv_alb = mk3(b.albedo.x != 0.f ? 1.f / b.albedo.x : 0.f, b.albedo.y != 0.f ? 1.f / b.albedo.y : 0.f,
+                         b.albedo.z != 0.f ? 1.f / b.albedo.z : 0.f);
+  dlna = inv_alb;
+  dlnr = 0.f;
+  if (b.lobe == LOBE_DIFFUSE) {
+    const f3 wi = s.wi;
+    const f3 h = normalize(wi + wo);
+    const float ci = wi.z, co = wo.z;
+    const float hw2 = sqr(absdot(h, wi));
+    const float fss90 = sqr(b.roughness) * hw2;
+    const float pi5 = pow5(1.f - abscos(wi)), po5 = pow5(1.f - abscos(wo));
+    const float fi = 1.f + (fss90 - 1) * pi5, fo = 1.f + (fss90 - 1) * po5;
+    const float k = -0.5f + 1 / (co + ci);
+    const float tail = 0.5f + fi * fo * k;
+    const float dtail = (pi5 * fo + fi * po5) * k * 2.f * b.roughness * hw2;
+    dlnr = tail != 0.f ? dtail / tail : 0.f;
+    return;
+  }
+  if (s.kind == SAMPLE_MICROFACET) dlnr = (ge.dlnD_da(s.wm) + ge.dlnG_da(wo, s.wi)) * ge.da_dr;
+  if (b.lobe == LOBE_SPECULAR) {
+    const float m = b.metal;
+    const float p = pow5(1.f - absdot(s.wm, wo));
+    const f3 F0 = mk3(0.04f * (1.f - m) + m * b.albedo.x, 0.04f * (1.f - m) + m * b.albedo.y, 0.04f * (1.f - m) + m * b.albedo.z);
+    const f3 F = mk3(F0.x + (1.f - F0.x) * p, F0.y + (1.f - F0.y) * p, F0.z + (1.f - F0.z) * p);
+    const float dF = m * (1.f - p);
+    dlna = mk3(F.x != 0.f ? dF / F.x : 0.f, F.y != 0.f ? dF / F.y : 0.f, F.z != 0.f ? dF / F.z : 0.f);
+  }
 }
 
 // bsdf::correct_specular_shading_normal (BxDF_math.h:26-53)

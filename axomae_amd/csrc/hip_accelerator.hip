@@ -45,6 +45,7 @@ class HipIntersector final : public DeviceIntersectorInterface {
     rp.max_samples_offset = p.sample_max;
     rp.vertical_invert = p.vertical_invert ? 1 : 0;
     rp.rr_start = p.rr_start;
+    rp.sampling_roughness_texture = -1;
     for (int i = 0; i < 16; i++) {
       rp.inv_P[i] = p.inv_P[i];
       rp.inv_V[i] = p.inv_V[i];

@@ -121,7 +121,9 @@ __device__ __forceinline__ void write_partial(const Queues &q, const FrameParams
   *dst = make_float4(lx, ly, lz, a);
 }
 
-__global__ __launch_bounds__(256) void shade_kernel(ShadeParams sp, ShadeCtx sc, const uint32_t *__restrict__ sobol, Queues q, int which) {
+template <bool kAdj>
+__global__ __launch_bounds__(256) void shade_kernel(ShadeParams sp, ShadeCtx sc, const uint32_t *__restrict__ sobol, Queues q, int which,
+                                                    AdjointCtx adj) {
   const uint32_t n = q.count[which];
   const size_t cap = q.cap;
   const float *rin = q.ray[which];
@@ -143,13 +145,13 @@ __global__ __launch_bounds__(256) void shade_kernel(ShadeParams sp, ShadeCtx sc,
       f3 L = mk3(q.L[slot], q.L[cap + slot], q.L[2 * cap + slot]);
       float A = q.A[slot];
       const int prim = q.hit_prim[i];
-      cont = shade_path(sp, sc, sobol, (uint32_t)slot, o, d, prim, q.hit_t[i], q.hit_u[i], q.hit_v[i], T, L, A, no, nd, escaped);
+      cont = shade_path<kAdj>(sp, sc, sobol, (uint32_t)slot, o, d, prim, q.hit_t[i], q.hit_u[i], q.hit_v[i], T, L, A, no, nd, escaped, &adj);
       const bool done = !cont;
       q.T[slot] = T.x;
       q.T[cap + slot] = T.y;
       q.T[2 * cap + slot] = T.z;
       if (done) {
-        write_partial(q, sp.fp, (uint32_t)slot, L.x, L.y, L.z, A);
+        if (!kAdj) write_partial(q, sp.fp, (uint32_t)slot, L.x, L.y, L.z, A);  // a replay reads the primal's
       } else {
         q.L[slot] = L.x;
         q.L[cap + slot] = L.y;
@@ -257,14 +259,20 @@ extern "C" int nova_hip_closest_hit(nova_hip_ctx *ctx, const float *rays_dev, ui
   return NOVA_HIP_SUCCESS;
 }
 
-extern "C" int nova_hip_render_frames(nova_hip_ctx *ctx, const nova_hip_render_params *p, float *accum_dev, float *final_dev, float *partial_dev,
-                                      void *stream) {
+// Renders (adj == nullptr) or renders and replays for the adjoint, batch by batch.
+static int render_impl(nova_hip_ctx *ctx, const nova_hip_render_params *p, float *accum_dev, float *final_dev, float *partial_dev, void *stream,
+                       const AdjointCtx *adj) {
   if (!ctx || !p) return NOVA_HIP_INVALID_ARGUMENT;
   if (!ctx->bvh_ready) return set_error(ctx, NOVA_HIP_SCENE_NOT_PROCESSED, "scene/BVH not built");
   if (p->width <= 0 || p->height <= 0 || p->width > 65536 || p->height > 65536)
     return set_error(ctx, NOVA_HIP_INVALID_RENDERBUFFER_DIM, "Wrong buffer width / height dimension.");
   if (p->frame_count <= 0 || p->frame_start < 0 || p->max_depth < 0) return set_error(ctx, NOVA_HIP_INVALID_ARGUMENT, "bad frame range or depth");
-  if (!accum_dev || !final_dev) return set_error(ctx, NOVA_HIP_INVALID_BUFFER_STATE, "null render buffer");
+  if (!adj && (!accum_dev || !final_dev)) return set_error(ctx, NOVA_HIP_INVALID_BUFFER_STATE, "null render buffer");
+  if (p->sampling_roughness_texture >= 0) {
+    if ((size_t)p->sampling_roughness_texture >= ctx->host_textures.size() ||
+        ctx->host_textures[p->sampling_roughness_texture].format != NOVA_HIP_TEX_F32)
+      return set_error(ctx, NOVA_HIP_INVALID_ARGUMENT, "sampling_roughness_texture is not an FP32 texture of the scene");
+  }
   // sampler dimensions: 2 + sum over scatters of (4 + RR roll) must stay inside the table
   {
     uint32_t dims = 2;
@@ -315,7 +323,9 @@ extern "C" int nova_hip_render_frames(nova_hip_ctx *ctx, const nova_hip_render_p
   int st = ensure_gstack(ctx, trace_threads);
   if (st) return st;
   TraceCtx tc{ctx->nodes4, ctx->tri_leaf, ctx->gstack, trace_threads, ctx->bnodes, ctx->bparent, ctx->bdepth, ctx->leaf_of_pos};
-  ShadeCtx sc{ctx->textures, ctx->tex_u32, ctx->tex_f32, ctx->materials, ctx->mesh_xf, ctx->shade, ctx->acos_lut, ctx->envmap};
+  ShadeCtx sc{ctx->textures, ctx->tex_u32, ctx->tex_f32, ctx->materials, ctx->mesh_xf, ctx->shade, ctx->acos_lut, ctx->envmap,
+              p->sampling_roughness_texture};
+  const AdjointCtx replay = adj ? AdjointCtx{partial, adj->dimg, adj->scale, adj->grad} : AdjointCtx{nullptr, nullptr, 0.f, nullptr};
   const uint32_t shade_blocks = (uint32_t)std::min<size_t>((cap + 255) / 256, (size_t)cus * 8);
   NOVA_HIP_CHECK(ctx, hipMemsetAsync(counts, 0, 64, s));
 
@@ -348,6 +358,7 @@ extern "C" int nova_hip_render_frames(nova_hip_ctx *ctx, const nova_hip_render_p
       fp.inv_V[i] = p->inv_V[i];
     }
     const uint32_t active = (uint32_t)(npix * frames);
+    for (int pass = 0; pass < (adj ? 2 : 1); pass++) {  // pass 1: the adjoint replay of pass 0's paths
     hipLaunchKernelGGL(generate_kernel, dim3(std::min<uint32_t>((active + 255) / 256, (uint32_t)cus * 8)), dim3(256), 0, s, fp, ctx->sobol_table, q,
                        active);
     NOVA_HIP_CHECK(ctx, hipGetLastError());
@@ -368,17 +379,23 @@ extern "C" int nova_hip_render_frames(nova_hip_ctx *ctx, const nova_hip_render_p
       spp.rr = (p->rr_start >= 0 && k >= p->rr_start) ? 1 : 0;
       spp.dim0 = dim;
       if (timing && (st = stamp())) return st;
-      hipLaunchKernelGGL(shade_kernel, dim3(shade_blocks), dim3(256), 0, s, spp, sc, ctx->sobol_table, q, which);
+      if (pass == 0)
+        hipLaunchKernelGGL(shade_kernel<false>, dim3(shade_blocks), dim3(256), 0, s, spp, sc, ctx->sobol_table, q, which, replay);
+      else
+        hipLaunchKernelGGL(shade_kernel<true>, dim3(shade_blocks), dim3(256), 0, s, spp, sc, ctx->sobol_table, q, which, replay);
       if (timing && (st = stamp())) return st;
       NOVA_HIP_CHECK(ctx, hipGetLastError());
       dim += 4 + (spp.rr ? 1 : 0);
       which ^= 1;
       launches++;
     }
-    hipLaunchKernelGGL(accumulate_kernel, dim3(std::min<size_t>((npix * 4 + 255) / 256, (size_t)cus * 16)), dim3(256), 0, s, partial, frames, f0,
-                       npix * 4, accum_dev, final_dev,
-                       (f0 + frames == p->frame_start + p->frame_count) ? partial_dev : (float *)nullptr);
-    NOVA_HIP_CHECK(ctx, hipGetLastError());
+    }
+    if (!adj) {
+      hipLaunchKernelGGL(accumulate_kernel, dim3(std::min<size_t>((npix * 4 + 255) / 256, (size_t)cus * 16)), dim3(256), 0, s, partial, frames, f0,
+                         npix * 4, accum_dev, final_dev,
+                         (f0 + frames == p->frame_start + p->frame_count) ? partial_dev : (float *)nullptr);
+      NOVA_HIP_CHECK(ctx, hipGetLastError());
+    }
   }
   NOVA_HIP_CHECK(ctx, hipEventRecord(ev1, s));
   NOVA_HIP_CHECK(ctx, hipEventSynchronize(ev1));
@@ -409,6 +426,19 @@ extern "C" int nova_hip_render_frames(nova_hip_ctx *ctx, const nova_hip_render_p
   ctx->stats.triangle_tests = counting ? s64[1] : 0;
   ctx->stats.escaped = s64[2];
   return NOVA_HIP_SUCCESS;
+}
+
+extern "C" int nova_hip_render_frames(nova_hip_ctx *ctx, const nova_hip_render_params *p, float *accum_dev, float *final_dev, float *partial_dev,
+                                      void *stream) {
+  return render_impl(ctx, p, accum_dev, final_dev, partial_dev, stream, nullptr);
+}
+
+extern "C" int nova_hip_render_adjoint(nova_hip_ctx *ctx, const nova_hip_render_params *p, const float *d_image_dev, float scale,
+                                       float *grad_pool_dev, void *stream) {
+  if (!ctx || !p) return NOVA_HIP_INVALID_ARGUMENT;
+  if (!d_image_dev || !grad_pool_dev) return set_error(ctx, NOVA_HIP_INVALID_BUFFER_STATE, "null adjoint buffer");
+  const AdjointCtx adj{nullptr, d_image_dev, scale, grad_pool_dev};
+  return render_impl(ctx, p, nullptr, nullptr, nullptr, stream, &adj);
 }
 
 extern "C" int nova_hip_render_stats_get(nova_hip_ctx *ctx, nova_hip_render_stats *out) {

@@ -88,6 +88,7 @@ struct nova_hip_ctx {
   nova_hip::TexRec *textures = nullptr;
   uint32_t *tex_u32 = nullptr;
   float *tex_f32 = nullptr;
+  uint64_t tex_f32_floats = 0;               // FP32 texel pool size
   int32_t envmap = -1;
   float *acos_lut = nullptr;
   uint32_t *sobol_table = nullptr;

@@ -203,7 +203,31 @@ extern "C" int nova_hip_scene_upload(nova_hip_ctx *ctx, const nova_hip_scene_des
   ctx->mesh_count = sd->mesh_count;
   ctx->envmap = sd->envmap_texture;
   ctx->host_textures = texrec;
+  ctx->tex_f32_floats = nf32;
   ctx->scene_ready = true;
   ctx->bvh_ready = false;
+  return NOVA_HIP_SUCCESS;
+}
+
+extern "C" int nova_hip_texture_f32_info(nova_hip_ctx *ctx, int32_t texture, uint64_t *offset, uint64_t *pool_floats) {
+  if (!ctx) return NOVA_HIP_INVALID_ARGUMENT;
+  if (!ctx->scene_ready) return set_error(ctx, NOVA_HIP_SCENE_NOT_PROCESSED, "no scene uploaded");
+  if (texture < 0 || (size_t)texture >= ctx->host_textures.size() || ctx->host_textures[texture].format != NOVA_HIP_TEX_F32)
+    return set_error(ctx, NOVA_HIP_INVALID_ARGUMENT, "not an FP32 texture of the scene");
+  if (offset) *offset = ctx->host_textures[texture].offset;
+  if (pool_floats) *pool_floats = ctx->tex_f32_floats;
+  return NOVA_HIP_SUCCESS;
+}
+
+extern "C" int nova_hip_texture_f32_update(nova_hip_ctx *ctx, int32_t texture, const float *texels_dev, void *stream) {
+  if (!ctx) return NOVA_HIP_INVALID_ARGUMENT;
+  if (!texels_dev) return set_error(ctx, NOVA_HIP_INVALID_BUFFER_STATE, "null texel buffer");
+  uint64_t off = 0;
+  int st = nova_hip_texture_f32_info(ctx, texture, &off, nullptr);
+  if (st) return st;
+  const TexRec &t = ctx->host_textures[texture];
+  const size_t n = (size_t)t.width * t.height * t.channels;
+  NOVA_HIP_CHECK(ctx, hipSetDevice(ctx->device));
+  NOVA_HIP_CHECK(ctx, hipMemcpyAsync(ctx->tex_f32 + off, texels_dev, n * sizeof(float), hipMemcpyDeviceToDevice, (hipStream_t)stream));
   return NOVA_HIP_SUCCESS;
 }

@@ -21,7 +21,7 @@ ERRORS = {0: "SUCCESS", 1: "INVALID_ENGINE_STATE", 2: "INVALID_BUFFER_STATE", 4:
 
 EXPORTS = ["nova_hip_create", "nova_hip_destroy", "nova_hip_last_error", "nova_hip_scene_upload", "nova_hip_bvh_build", "nova_hip_bvh_read",
            "nova_hip_closest_hit", "nova_hip_render_frames", "nova_hip_render_stats_get", "nova_hip_sobol_fill", "nova_hip_bake_irradiance",
-           "nova_hip_bake_prefilter"]
+           "nova_hip_bake_prefilter", "nova_hip_texture_f32_info", "nova_hip_texture_f32_update", "nova_hip_render_adjoint"]
 
 
 class NovaHipError(RuntimeError):
@@ -61,7 +61,7 @@ class BvhInfo(C.Structure):
 class RenderParams(C.Structure):
     _fields_ = [("width", C.c_int32), ("height", C.c_int32), ("max_depth", C.c_int32), ("frame_start", C.c_int32), ("frame_count", C.c_int32),
                 ("max_samples_offset", C.c_uint32), ("vertical_invert", C.c_int32), ("rr_start", C.c_int32), ("inv_P", C.c_float * 16),
-                ("inv_V", C.c_float * 16), ("flags", C.c_int32)]
+                ("inv_V", C.c_float * 16), ("flags", C.c_int32), ("sampling_roughness_texture", C.c_int32)]
 
 
 RENDER_TIMING = 1
@@ -98,6 +98,9 @@ def load():
     L.nova_hip_render_frames.argtypes = [P, C.POINTER(RenderParams), P, P, P, P]
     L.nova_hip_render_stats_get.argtypes = [P, C.POINTER(RenderStats)]
     L.nova_hip_sobol_fill.argtypes = [P, P, C.c_uint64, P, P, P]
+    L.nova_hip_texture_f32_info.argtypes = [P, C.c_int32, C.POINTER(C.c_uint64), C.POINTER(C.c_uint64)]
+    L.nova_hip_texture_f32_update.argtypes = [P, C.c_int32, P, P]
+    L.nova_hip_render_adjoint.argtypes = [P, C.POINTER(RenderParams), P, C.c_float, P, P]
     L.nova_hip_bake_irradiance.argtypes = [C.POINTER(P), P, P, P, C.c_uint32, C.POINTER(P), P, P, C.c_int32, P]
     L.nova_hip_bake_prefilter.argtypes = [C.POINTER(P), P, P, P, C.c_uint32, C.POINTER(P), P, P, P, C.c_int32, P]
     _LIB = L
@@ -174,8 +177,9 @@ def scene_desc(scene):
 
 
 def render_params(scene, width, height, frames, max_depth, frame_start=0, rr_start=-1, max_samples_offset=0, vertical_invert=False,
-                  inv_P=None, inv_V=None, flags=0):
+                  inv_P=None, inv_V=None, flags=0, sampling_roughness_texture=-1):
     p = RenderParams()
+    p.sampling_roughness_texture = sampling_roughness_texture
     p.width, p.height, p.max_depth = width, height, max_depth
     p.frame_start, p.frame_count = frame_start, frames
     p.max_samples_offset = max_samples_offset
@@ -248,16 +252,56 @@ class HipEngine:
 
     # --------------------------------------------------------------- render
     def render(self, width, height, frames, max_depth, frame_start=0, rr_start=-1, max_samples_offset=0, vertical_invert=False, accum=None,
-               inv_P=None, inv_V=None, flags=0):
+               inv_P=None, inv_V=None, flags=0, sampling_roughness_texture=-1):
         """Renders frames [frame_start, frame_start+frames); returns (final, accum) device tensors (H, W, 4)."""
         torch = self.torch
         p = render_params(self.scene, width, height, frames, max_depth, frame_start, rr_start, max_samples_offset, vertical_invert, inv_P, inv_V,
-                          flags)
+                          flags, sampling_roughness_texture)
         if accum is None:
             accum = torch.zeros((height, width, 4), dtype=torch.float32, device="cuda")
         final = torch.empty((height, width, 4), dtype=torch.float32, device="cuda")
         self._check(self.L.nova_hip_render_frames(self.ctx, C.byref(p), accum.data_ptr(), final.data_ptr(), None, _stream(torch)), "render_frames")
         return final, accum
+
+    # ------------------------------------------------------ differentiation
+    def texture_info(self, tex):
+        """(offset, pool_floats) of FP32 texture `tex` in the device texel pool."""
+        off, pool = C.c_uint64(), C.c_uint64()
+        self._check(self.L.nova_hip_texture_f32_info(self.ctx, tex, C.byref(off), C.byref(pool)), "texture_f32_info")
+        return off.value, pool.value
+
+    def update_texture(self, tex, texels):
+        """Replaces FP32 texture `tex` with a device tensor of its width*height*channels floats."""
+        torch = self.torch
+        t = self.scene.textures[tex]
+        x = texels.to(device="cuda", dtype=torch.float32).contiguous()
+        if x.numel() != t.width * t.height * t.channels:
+            raise ValueError("texel count does not match the texture")
+        self._check(self.L.nova_hip_texture_f32_update(self.ctx, tex, x.data_ptr(), _stream(torch)), "texture_f32_update")
+        self._keep_texels = x  # the copy is stream-ordered: keep the source alive until the next call
+
+    def adjoint(self, width, height, frames, max_depth, d_image, scale, frame_start=0, rr_start=-1, max_samples_offset=0, vertical_invert=False,
+                sampling_roughness_texture=-1, grad=None):
+        """Path-replay adjoint: adds dLoss/d(texel) for every FP32 albedo/roughness texture to `grad` (FP32 pool layout,
+        allocated zeroed if None), where scale * d_image (H, W, 4) is the loss gradient of each frame's radiance."""
+        torch = self.torch
+        _, pool = self.texture_info(next(i for i, t in enumerate(self.scene.textures) if t.is_f32))
+        if grad is None:
+            grad = torch.zeros(pool, dtype=torch.float32, device="cuda")
+        d = d_image.to(device="cuda", dtype=torch.float32).contiguous()
+        if d.numel() != width * height * 4:
+            raise ValueError("d_image must hold width*height*4 floats")
+        p = render_params(self.scene, width, height, frames, max_depth, frame_start, rr_start, max_samples_offset, vertical_invert,
+                          sampling_roughness_texture=sampling_roughness_texture)
+        self._check(self.L.nova_hip_render_adjoint(self.ctx, C.byref(p), d.data_ptr(), float(scale), grad.data_ptr(), _stream(torch)),
+                    "render_adjoint")
+        return grad
+
+    def texture_grad(self, grad, tex):
+        """View of texture `tex`'s gradient inside a pool-layout gradient buffer, shaped (H, W, C)."""
+        off, _ = self.texture_info(tex)
+        t = self.scene.textures[tex]
+        return grad[off:off + t.width * t.height * t.channels].view(t.height, t.width, t.channels)
 
     def render_stats(self):
         s = RenderStats()

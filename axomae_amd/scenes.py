@@ -359,3 +359,40 @@ def bake_envmaps(seed=0xA70AE005, count=32):
         inten = float(np.exp(rng.uniform(np.log(1e2), np.log(1e5))))
         maps.append(sky_envmap(w, h, d, sun_intensity=inten * 1e-3, sun_size=0.002))
     return maps
+
+
+def differentiable_scene(seed=0xA70AE004, width=1280, height=720, tex=512, terrain_cells=200, spheres=10, sphere_level=4, sky=(1024, 512)):
+    """About 100k triangles (a 200x200-cell terrain, 80,000 triangles, plus 10 octahedral
+    spheres at subdivision 4, 2,048 each) whose materials all read one UV-mapped FP32
+    tex x tex RGB albedo texture (seeded noise in [0.1, 0.9]) and one FP32 roughness
+    texture (in [0.05, 0.9]). The terrain is Disney diffuse + specular (metal 0); every
+    other sphere is a conductor (metal 1), so albedo also reaches the specular Fresnel.
+    Returns (scene, albedo texture id, roughness texture id)."""
+    rng = np.random.default_rng(seed)
+    sc = Scene(name="differentiable")
+    alb = sc.add_texture(Texture(tex, tex, 3, _f32(_smooth_noise(rng, tex, 3, 0.1, 0.9))))
+    rough = sc.add_texture(Texture(tex, tex, 1, _f32(_smooth_noise(rng, tex, 1, 0.05, 0.9))))
+
+    def textured(metal):
+        m = sc.materials[sc.principled((0.5, 0.5, 0.5), roughness=0.5, metal=metal)]
+        m.albedo_f32, m.roughness_f32 = alb, rough
+        return len(sc.materials) - 1
+
+    terrain_mat = textured(0.0)
+    sc.meshes.append(terrain_mesh(rng, terrain_cells, 12.0, -3.0, 2.0, terrain_mat))
+    mats = [textured(0.0), textured(1.0)]
+    for i in range(spheres):
+        c = rng.uniform(-8, 8, size=3)
+        c[1] = rng.uniform(-1.0, 4.0)
+        sc.meshes.append(octasphere_mesh(c, rng.uniform(0.6, 1.8), sphere_level, mats[i % 2]))
+    sc.envmap = sc.add_texture(sky_envmap(sky[0], sky[1], (0.3, 0.8, 0.5)))
+    set_camera(sc, (0.0, 6.0, 18.0), (0.0, -1.0, 0.0), 50.0, width / height)
+    return sc, alb, rough
+
+
+def _smooth_noise(rng, n, channels, lo, hi):
+    """n x n x channels value noise (a few octaves, float64) mapped to [lo, hi]."""
+    out = np.stack([_fbm(rng, n - 1, octaves=5) for _ in range(channels)], -1)
+    out -= out.min()
+    out /= max(out.max(), 1e-12)
+    return (lo + (hi - lo) * out).reshape(-1)

@@ -146,6 +146,10 @@ typedef struct {
   int32_t rr_start;         /* Russian-roulette extension: first scatter index that rolls, -1 = off */
   float inv_P[16], inv_V[16]; /* CameraResourcesHolder inv_P / inv_V, column-major */
   int32_t flags;            /* NOVA_HIP_RENDER_TIMING | NOVA_HIP_RENDER_COUNT_TRAVERSAL, 0 for none */
+  int32_t sampling_roughness_texture; /* -1 for the plain renderer. Gradient-check harness only: an
+                             FP32 texture whose channel 0 replaces, for every material with a
+                             roughness_f32 texture, the roughness used to sample directions and
+                             to weight them by their pdf; the BSDF value keeps roughness_f32. */
 } nova_hip_render_params;
 
 /* flags: time every extend/shade launch with HIP events on the render stream; count BVH node
@@ -178,6 +182,32 @@ typedef struct {
   uint64_t escaped;          /* segments whose ray left the scene (envmap lookups after a miss) */
 } nova_hip_render_stats;
 int nova_hip_render_stats_get(nova_hip_ctx *ctx, nova_hip_render_stats *out);
+
+/* ---------------------------------------------------------- differentiation */
+
+/* Element offset of FP32 texture `texture` in the scene's FP32 texel pool and the pool size
+ * in floats: a gradient buffer laid out like the pool holds texture t's gradient at
+ * grad + offset(t), width*height*channels floats, texel (i,j) channel c at
+ * (j*width + i)*channels + c. No counterpart in the reference (it has no gradients). */
+int nova_hip_texture_f32_info(nova_hip_ctx *ctx, int32_t texture, uint64_t *offset, uint64_t *pool_floats);
+
+/* Replaces the texels of FP32 texture `texture` with width*height*channels floats from
+ * device memory (stream-ordered). Used by the inverse-rendering harness between renders. */
+int nova_hip_texture_f32_update(nova_hip_ctx *ctx, int32_t texture, const float *texels_dev, void *stream);
+
+/* Path-replay adjoint (Vicini et al. 2021) of the frames nova_hip_render_frames would render
+ * with the same params: for a loss whose gradient with respect to each frame's radiance
+ * buffer is scale * d_image (RGBA32F, width*height*4 floats, alpha ignored), adds
+ * dLoss/d(texel) to grad_pool_dev (laid out like the FP32 texel pool, see
+ * nova_hip_texture_f32_info) for the albedo_f32 (channels 0-2) and roughness_f32
+ * (channel 0) textures of every material. Each batch of frames is rendered once to get
+ * every path's radiance, then replayed from the same sample indices; nothing is stored per
+ * path vertex. Sampling is detached: directions, lobe choices and pdfs are constants, the
+ * gradient flows through the BSDF values and Russian-roulette rescaling. With
+ * sampling_roughness_texture >= 0 the replay samples with that texture as the renderer
+ * does. No counterpart in the reference. */
+int nova_hip_render_adjoint(nova_hip_ctx *ctx, const nova_hip_render_params *params, const float *d_image_dev, float scale,
+                            float *grad_pool_dev, void *stream);
 
 /* --------------------------------------------------------------- sampler */
 

@@ -142,6 +142,45 @@ void oracle_closest_hit(void *sp, void *bp, const float *rays, size_t n, int bru
 
 // Progressive CPU render: frames [frame_start, frame_start + frames) at 1 spp each,
 // accumulated as Engine::writeFramebuffer does. accum/final: width*height*4 floats.
+static RenderParams make_params(int width, int height, int frame_start, int frames, int max_depth, uint32_t max_samples_offset,
+                                int vertical_invert, int rr_start, const float *inv_P, const float *inv_V, int threads, int sampling_roughness) {
+  RenderParams p;
+  p.width = width;
+  p.height = height;
+  p.frame_start = frame_start;
+  p.frames = frames;
+  p.max_depth = max_depth;
+  p.max_samples_offset = max_samples_offset;
+  p.vertical_invert = vertical_invert != 0;
+  p.rr_start = rr_start;
+  for (int i = 0; i < 16; i++) {
+    p.inv_P.m[i] = inv_P[i];
+    p.inv_V.m[i] = inv_V[i];
+  }
+  p.threads = threads;
+  p.sampling_roughness = sampling_roughness;
+  return p;
+}
+
+// Renders with the sampling roughness taken from FP32 texture sampling_roughness (-1: off).
+void oracle_render_sampling(void *sp, void *bp, int width, int height, int frame_start, int frames, int max_depth, uint32_t max_samples_offset,
+                            int vertical_invert, int rr_start, const float *inv_P, const float *inv_V, int threads, int sampling_roughness,
+                            float *accum, float *final_out) {
+  RenderParams p = make_params(width, height, frame_start, frames, max_depth, max_samples_offset, vertical_invert, rr_start, inv_P, inv_V, threads,
+                               sampling_roughness);
+  render(*static_cast<Scene *>(sp), *static_cast<Bvh *>(bp), p, accum, final_out);
+}
+
+void oracle_render_adjoint(void *sp, void *bp, int width, int height, int frame_start, int frames, int max_depth, uint32_t max_samples_offset,
+                           int vertical_invert, int rr_start, const float *inv_P, const float *inv_V, int threads, int sampling_roughness,
+                           const float *d_image, float scale, double *grad_pool) {
+  RenderParams p = make_params(width, height, frame_start, frames, max_depth, max_samples_offset, vertical_invert, rr_start, inv_P, inv_V, threads,
+                               sampling_roughness);
+  render_adjoint(*static_cast<Scene *>(sp), *static_cast<Bvh *>(bp), p, d_image, scale, grad_pool);
+}
+
+uint64_t oracle_f32_pool_offset(void *sp, int texture) { return f32_pool_offset(*static_cast<Scene *>(sp), texture); }
+
 void oracle_render(void *sp, void *bp, int width, int height, int frame_start, int frames, int max_depth, uint32_t max_samples_offset,
                    int vertical_invert, int rr_start, const float *inv_P, const float *inv_V, int threads, float *accum, float *final_out) {
   RenderParams p;

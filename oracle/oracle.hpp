@@ -100,11 +100,17 @@ struct RenderParams {
   mat4 inv_P, inv_V;
   int threads = 1;
   int tiles_w = 0, tiles_h = 0;  // 0 = one tile row per thread
+  int sampling_roughness = -1;   // gradient-check harness: FP32 texture of the sampling roughness
 };
 vec4 path_Li(const Scene &scene, const Bvh &bvh, vec3 o, vec3 d, int depth, SobolSampler &sampler, const RenderParams &p, int scatter_index);
 void render_frame(const Scene &scene, const Bvh &bvh, const RenderParams &p, int frame, float *partial);
 void render(const Scene &scene, const Bvh &bvh, const RenderParams &p, float *accum, float *final_out);
 void render_rows(const Scene &scene, const Bvh &bvh, const RenderParams &p, int frame, int y0, int y1, float *partial);
+// Reverse-mode gradient of a loss with d loss / d(frame radiance) = scale * d_image (RGBA per
+// pixel, RGB used) for frames [frame_start, frame_start+frames), accumulated into grad_pool
+// (one double per FP32 texel channel, textures concatenated in scene order).
+void render_adjoint(const Scene &scene, const Bvh &bvh, const RenderParams &p, const float *d_image, float scale, double *grad_pool);
+size_t f32_pool_offset(const Scene &scene, int texture);
 
 // ---------------------------------------------------------------- bakes
 void bake_irradiance(const float *env, int env_w, int env_h, int env_c, int out_w, int out_h, int samples, float *out, int threads);

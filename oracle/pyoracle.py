@@ -51,6 +51,12 @@ def lib():
         L.oracle_closest_hit.argtypes = [P, P, f32p, C.c_size_t, C.c_int, C.c_int, i32p, f32p]
         L.oracle_render.argtypes = [P, P, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, C.c_uint32, C.c_int, C.c_int,
                                     f32p, f32p, C.c_int, f32p, f32p]
+        L.oracle_render_sampling.argtypes = [P, P, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, C.c_uint32, C.c_int, C.c_int,
+                                             f32p, f32p, C.c_int, C.c_int, f32p, f32p]
+        L.oracle_render_adjoint.argtypes = [P, P, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, C.c_uint32, C.c_int, C.c_int,
+                                            f32p, f32p, C.c_int, C.c_int, f32p, C.c_float, np.ctypeslib.ndpointer(np.float64, flags="C")]
+        L.oracle_f32_pool_offset.argtypes = [P, C.c_int]
+        L.oracle_f32_pool_offset.restype = C.c_uint64
         L.oracle_render_rows.argtypes = [P, P, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, f32p, f32p, C.c_int, f32p]
         L.oracle_bake_irradiance.argtypes = [f32p, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, f32p]
         L.oracle_bake_prefilter.argtypes = [f32p, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, C.c_float, C.c_int, C.c_int, f32p]
@@ -157,13 +163,35 @@ class OracleScene:
         return prim, tuv
 
     def render(self, width, height, frames, max_depth, frame_start=0, rr_start=-1, max_samples_offset=0, vertical_invert=False,
-               threads=8, accum=None):
+               threads=8, accum=None, sampling_roughness_texture=-1):
         n = width * height * 4
         acc = np.zeros(n, dtype=np.float32) if accum is None else accum
         out = np.zeros(n, dtype=np.float32)
-        lib().oracle_render(self.h, self.bvh, width, height, frame_start, frames, max_depth, max_samples_offset, int(vertical_invert),
-                            rr_start, self.scene.inv_P, self.scene.inv_V, threads, acc, out)
+        lib().oracle_render_sampling(self.h, self.bvh, width, height, frame_start, frames, max_depth, max_samples_offset, int(vertical_invert),
+                                     rr_start, self.scene.inv_P, self.scene.inv_V, threads, sampling_roughness_texture, acc, out)
         return out.reshape(height, width, 4), acc
+
+    def f32_pool(self):
+        """(offsets by texture id, pool size) of the FP32 texel pool layout the gradients use."""
+        offs, n = {}, 0
+        for i, t in enumerate(self.scene.textures):
+            if t.is_f32:
+                offs[i] = n
+                n += t.width * t.height * t.channels
+        return offs, n
+
+    def render_adjoint(self, width, height, frames, max_depth, d_image, scale, frame_start=0, rr_start=-1, max_samples_offset=0,
+                       vertical_invert=False, sampling_roughness_texture=-1, threads=8):
+        """CPU reverse-mode texel gradients (float64, FP32 pool layout) for dLoss/d(frame radiance) = scale * d_image."""
+        offs, n = self.f32_pool()
+        grad = np.zeros(max(n, 1), dtype=np.float64)
+        d = np.ascontiguousarray(d_image, dtype=np.float32).reshape(-1)
+        assert d.size == width * height * 4
+        lib().oracle_render_adjoint(self.h, self.bvh, width, height, frame_start, frames, max_depth, max_samples_offset, int(vertical_invert),
+                                    rr_start, self.scene.inv_P, self.scene.inv_V, threads, sampling_roughness_texture, d, scale, grad)
+        for t, off in offs.items():
+            assert lib().oracle_f32_pool_offset(self.h, t) == off
+        return grad[:n]
 
 
 def render_rows(oscene, width, height, frame, y0, y1, max_depth, rr_start=-1, threads=8):

@@ -216,6 +216,9 @@ struct Sample {  // BSDFSample (BxDF_flags.h:26-36)
   Spec f{0.f};
   vec3 wi;
   float pdf = 0, costheta = 0;
+  // for the adjoint: the local-frame directions and microfacet normal of the sample
+  vec3 lwo, lwi, wm;
+  bool microfacet = false;
 };
 
 enum : unsigned { REFLECTION = 1, TRANSMISSION = 2, DIFFUSE = 4, GLOSSY = 8, SPECULAR = 16 };
@@ -364,6 +367,7 @@ struct Principled {
   Lobe lobe = LOBE_NONE;
   float lobe_weight = 1, lobe_prob = 1;
   float vndf_aniso = 0, vndf_rough = 0;
+  float vndf_rough_eval = 0;  // roughness of the BSDF value (vndf_rough samples; they differ only in gradient checks)
 
   void configure(const BsdfParams &p) {
     params = p;
@@ -412,6 +416,7 @@ struct Principled {
     }
     vndf_aniso = params.anisotropy;
     vndf_rough = params.roughness;
+    vndf_rough_eval = params.roughness;
   }
 
   unsigned flags() const {
@@ -449,7 +454,7 @@ struct Principled {
   }
 
   bool sample_specular(const vec3 &wo, const float u[2], Sample &s) const {
-    VNDF g(vndf_aniso, vndf_rough);
+    VNDF g(vndf_aniso, vndf_rough), ge(vndf_aniso, vndf_rough_eval);
     vec3 wm = g.sample(wo, u);
     vec3 wi = reflect(-wo, wm);
     float co = costheta(wo), ci = costheta(wi);
@@ -459,17 +464,19 @@ struct Principled {
     Spec mm(params.metal);
     F0 = F0 * (Spec(1.f) - mm) + mm * params.albedo;
     F0 = schlick(wm_dot_wo, F0);
-    Spec f = F0 * g.D(wm) * g.G(wo, wi) / (4 * co * ci);
+    Spec f = F0 * ge.D(wm) * ge.G(wo, wi) / (4 * co * ci);
     float pdf = g.pdf(wo, wm) / (4 * wm_dot_wo);
     s.f = lobe_weight * f;
     s.pdf = lobe_prob * pdf;
     s.wi = wi;
     s.costheta = ci;
+    s.wm = wm;
+    s.microfacet = true;
     return true;
   }
 
   bool sample_dielectric(const vec3 &wo, float uc, const float u[2], Sample &s) const {
-    VNDF g(vndf_aniso, vndf_rough);
+    VNDF g(vndf_aniso, vndf_rough), ge(vndf_aniso, vndf_rough_eval);
     const float ior = params.eta.x;
     if (g.full_specular() || params.eta.x == 1.f) {
       float R = fresnel_real(ior, costheta(wo)), T = 1 - R;
@@ -505,12 +512,14 @@ struct Principled {
     if (uc < (pr / (pr + pt))) {  // glossyReflection
       vec3 wi = reflect(-wo, wm);
       if (!same_hemi(wo, wi)) return false;
-      Spec f = params.albedo * g.D(wm) * g.G(wo, wi) * R / (4 * costheta(wi) * costheta(wo));
+      Spec f = params.albedo * ge.D(wm) * ge.G(wo, wi) * R / (4 * costheta(wi) * costheta(wo));
       float pdf = g.pdf(wo, wm) / (4 * absdot(wo, wm)) * pr / (pr + pt);
       s.f = lobe_weight * f;
       s.pdf = lobe_prob * pdf;
       s.wi = wi;
       s.costheta = abscos(wi);
+      s.wm = wm;
+      s.microfacet = true;
       return true;
     }
     // glossyTransmission
@@ -524,12 +533,14 @@ struct Principled {
     float denom = sqr(dot(wi, wm) + dot(wo, wm) / eta);
     float jac = absdot(wi, wm) / denom;
     float pdf = g.pdf(wo, wm) * jac * pt / (pt + pr);
-    Spec f = params.albedo * T * g.D(wm) * g.G(wo, wi) * std::fabs(dot(wi, wm) * dot(wo, wm) / (ci * co * denom));
+    Spec f = params.albedo * T * ge.D(wm) * ge.G(wo, wi) * std::fabs(dot(wi, wm) * dot(wo, wm) / (ci * co * denom));
     f = f / sqr(eta);
     s.f = lobe_weight * f;
     s.costheta = abscos(wi);
     s.pdf = lobe_prob * pdf;
     s.wi = wi;
+    s.wm = wm;
+    s.microfacet = true;
     return true;
   }
 
@@ -547,6 +558,8 @@ struct Principled {
     }
     if (!ok) return false;
     if (!spec_any(s.f) || s.pdf == 0.f || s.wi.z == 0.f) return false;
+    s.lwo = lwo;
+    s.lwi = s.wi;
     s.wi = normalize(frame * s.wi);
     return true;
   }
@@ -573,11 +586,14 @@ static vec3 correct_specular_shading_normal(const vec3 &ng, const vec3 &wo, vec3
 struct ScatterResult {
   Spec attenuation, emissive;
   Sample lobe;
+  Principled bsdf;  // the configured BSDF (adjoint)
+  float u = 0, v = 0;
 };
 
 // PrincipledMaterial::scatter + sampleBSDF (NovaMaterials.h:45-137), with the
 // FP32 texture and emission-scale extensions.
-static bool scatter(const Scene &s, const Material &mat, const vec3 &in_dir, const Hit &h, SobolSampler &sampler, ScatterResult &out) {
+static bool scatter(const Scene &s, const Material &mat, const vec3 &in_dir, const Hit &h, SobolSampler &sampler, ScatterResult &out,
+                    int sampling_roughness = -1) {
   const float u = h.u, v = h.v;
   const float ONE_255 = 1.f / 255.f;
   vec3 nrm(0.f, 0.f, 1.f);
@@ -625,6 +641,8 @@ static bool scatter(const Scene &s, const Material &mat, const vec3 &in_dir, con
   Principled bsdf;
   bsdf.configure(bp);
   bsdf.construct(uc);
+  // gradient-check harness: another FP32 texture drives the sampling roughness
+  if (sampling_roughness >= 0 && mat.roughness_f32 >= 0) bsdf.vndf_rough = sample_f32_bilinear(s.textures[sampling_roughness], u, v).x;
   if (bsdf.flags() & (GLOSSY | REFLECTION)) {
     if (bp.wo_dot_ng >= 0) {
       vec3 nsc = normalize(correct_specular_shading_normal(bp.ng, wo, bp.ns));
@@ -632,13 +650,14 @@ static bool scatter(const Scene &s, const Material &mat, const vec3 &in_dir, con
       bp.wo_dot_ns = dot(wo, nsc);
       // configureBSDFParams re-runs with the corrected normal; the lobe and its weights stay.
       Lobe keep = bsdf.lobe;
-      float lw = bsdf.lobe_weight, lp = bsdf.lobe_prob, va = bsdf.vndf_aniso, vr = bsdf.vndf_rough;
+      float lw = bsdf.lobe_weight, lp = bsdf.lobe_prob, va = bsdf.vndf_aniso, vr = bsdf.vndf_rough, ve = bsdf.vndf_rough_eval;
       bsdf.configure(bp);
       bsdf.lobe = keep;
       bsdf.lobe_weight = lw;
       bsdf.lobe_prob = lp;
       bsdf.vndf_aniso = va;
       bsdf.vndf_rough = vr;
+      bsdf.vndf_rough_eval = ve;
     }
   }
   uc = sampler.sample1D();
@@ -652,6 +671,9 @@ static bool scatter(const Scene &s, const Material &mat, const vec3 &in_dir, con
   }
   out.emissive = em * mat.emission_scale;
   out.lobe = smp;
+  out.bsdf = bsdf;
+  out.u = u;
+  out.v = v;
   return valid;
 }
 
@@ -671,7 +693,7 @@ static vec4 Li_rec(const Scene &s, const Bvh &bvh, vec3 o, vec3 d, int depth, So
       Hit h = make_hit(s, prim, o, d, t, bu, bv);
       const Material &mat = s.materials[s.meshes[s.prim_mesh[prim]].material];
       ScatterResult sr;
-      if (!scatter(s, mat, d, h, sampler, sr)) return vec4(0.f);
+      if (!scatter(s, mat, d, h, sampler, sr, p.sampling_roughness)) return vec4(0.f);
       // Ray::spawn (Ray.h:22-27)
       vec3 wi = sr.lobe.wi;
       vec3 nd = normalize(wi);
@@ -820,6 +842,234 @@ void render(const Scene &s, const Bvh &bvh, const RenderParams &p, float *accum,
       final_out[i] = accum[i] / float(f + 1);
     }
   }
+}
+
+
+// --------------------------------------------------------------- adjoint
+// CPU gradient of an image loss with respect to FP32 albedo / roughness texels, written as
+// plain reverse mode over the recursion V_k = Le_k + c_k V_{k+1}: the adjoint of V_k is
+// passed down (dV_{k+1} = dV_k * c_k) and on the way back up the weight c_k receives
+// dV_k * V_{k+1}. The derivative of c_k with the sample held fixed is taken by forward-mode
+// dual numbers in double over a restatement of the lobe's BSDF value (the GPU uses
+// hand-derived closed forms), so the two are independent.
+
+namespace {
+struct Dual {
+  double v, d;
+};
+Dual operator+(Dual a, Dual b) { return {a.v + b.v, a.d + b.d}; }
+Dual operator-(Dual a, Dual b) { return {a.v - b.v, a.d - b.d}; }
+Dual operator*(Dual a, Dual b) { return {a.v * b.v, a.d * b.v + a.v * b.d}; }
+Dual operator/(Dual a, Dual b) { return {a.v / b.v, (a.d * b.v - a.v * b.d) / (b.v * b.v)}; }
+Dual cst(double x) { return {x, 0.0}; }
+Dual dsqrt(Dual a) {
+  const double r = std::sqrt(std::max(a.v, 0.0));
+  return {r, r > 0 ? a.d / (2 * r) : 0.0};
+}
+
+// GGX with alpha_x = a / aspect, alpha_y = a * aspect (BxDF_math.h:231-342), in duals.
+struct DualGGX {
+  Dual ax, ay;
+  DualGGX(float aniso, Dual r) {
+    aniso = clampf(aniso, -1.f, 1.f);
+    const double aspect = msqrt(1.f - 0.9f * aniso);
+    const Dual a = r.v != 0 ? r * r : cst(1e-6);
+    ax = a / cst(aspect);
+    ay = a * cst(aspect);
+  }
+  Dual D(const vec3 &wm) const {
+    const double c2t = (double)wm.z * wm.z, s2t = std::max(0.0, 1.0 - c2t), t2 = s2t / c2t;
+    const double c2p = cos2phi(wm), s2p = sin2phi(wm);
+    const Dual e = cst(t2) * (cst(c2p) / (ax * ax) + cst(s2p) / (ay * ay));
+    const Dual one_e = cst(1.0) + e;
+    return cst(1.0) / (cst(M_PI * c2t * c2t) * ax * ay * one_e * one_e);
+  }
+  Dual lambda(const vec3 &v) const {
+    const double c2t = (double)v.z * v.z, t2 = std::max(0.0, 1.0 - c2t) / c2t;
+    if (!std::isfinite(t2)) return cst(0.0);
+    const Dual a2 = ax * ax * cst(cos2phi(v)) + ay * ay * cst(sin2phi(v));
+    return (dsqrt(cst(1.0) + a2 * cst(t2)) - cst(1.0)) * cst(0.5);
+  }
+  Dual G(const vec3 &wo, const vec3 &wi) const { return cst(1.0) / (cst(1.0) + lambda(wo) + lambda(wi)); }
+};
+
+// The sampled lobe's BSDF value for one channel, as a function of that channel's albedo and
+// of the roughness (both duals); direction-only factors are constants.
+Dual lobe_value(const ScatterResult &sr, int ch, Dual alb, Dual r) {
+  const Principled &b = sr.bsdf;
+  const Sample &sm = sr.lobe;
+  const vec3 wo = sm.lwo, wi = sm.lwi;
+  const double co = wo.z, ci = wi.z;
+  if (b.lobe == LOBE_DIFFUSE) {
+    const vec3 h = normalize(wi + wo);
+    const double hw = std::fabs(dot(h, wi));
+    const Dual fss90 = r * r * cst(hw * hw);
+    const Dual fi = cst(1.0) + (fss90 - cst(1.0)) * cst(std::pow(1.0 - std::fabs(ci), 5));
+    const Dual fo = cst(1.0) + (fss90 - cst(1.0)) * cst(std::pow(1.0 - std::fabs(co), 5));
+    const Dual tail = cst(0.5) + fi * fo * cst(-0.5 + 1.0 / (co + ci));
+    return cst(b.lobe_weight * 1.25 * M_1_PI) * alb * tail;
+  }
+  if (!sm.microfacet) return cst(b.lobe_weight) * alb;  // delta lobes: albedo times direction terms
+  const DualGGX g(b.vndf_aniso, r);
+  const Dual dg = g.D(sm.wm) * g.G(wo, wi);
+  if (b.lobe == LOBE_SPECULAR) {
+    const double m = b.params.metal;
+    const double p = std::pow(1.0 - std::fabs((double)dot(sm.wm, wo)), 5);
+    const Dual F0 = cst(0.04 * (1 - m)) + cst(m) * alb;
+    const Dual F = F0 + (cst(1.0) - F0) * cst(p);
+    return cst(b.lobe_weight / (4 * co * ci)) * F * dg;
+  }
+  return cst(b.lobe_weight) * alb * dg;  // dielectric glossy reflection / transmission
+}
+
+void lobe_log_derivatives(const ScatterResult &sr, double dlna[3], double &dlnr) {
+  const Spec &a = sr.bsdf.params.albedo;
+  const double av[3] = {a.x, a.y, a.z};
+  const double r = sr.bsdf.vndf_rough_eval;
+  for (int ch = 0; ch < 3; ch++) {
+    const Dual f = lobe_value(sr, ch, Dual{av[ch], 1.0}, cst(r));
+    dlna[ch] = f.v != 0 ? f.d / f.v : 0.0;
+  }
+  const Dual f = lobe_value(sr, 0, cst(av[0]), Dual{r, 1.0});
+  dlnr = f.v != 0 ? f.d / f.v : 0.0;
+}
+
+// Adds g[c] * (bilinear weight) to channel c of the four texels sample_f32_bilinear blends.
+void splat(const Texture &tx, double *grad, float u, float v, const double *g, int nch) {
+  u = wrap(u);
+  v = wrap(v);
+  u = tx.invert_u ? 1 - u : u;
+  v = tx.invert_v ? 1 - v : v;
+  const int width = tx.width, height = tx.height;
+  float pi = std::fabs(u * (float)(width - 1.f));
+  float pj = std::fabs(v * (float)(height - 1.f));
+  unsigned i = (unsigned)std::floor(pi), j = (unsigned)std::floor(pj);
+  if (i + 1 >= (unsigned)width) i = width - 2;
+  if (j + 1 >= (unsigned)height) j = height - 2;
+  const float tx_ = pi - (float)i, ty_ = pj - (float)j;
+  const unsigned idx[4] = {j * width + i, j * width + i + 1, (j + 1) * width + i, (j + 1) * width + i + 1};
+  const float hd = 1 - tx_, vy = 1 - ty_;
+  const float w[4] = {vy * hd, vy * tx_, ty_ * hd, ty_ * tx_};
+  for (int k = 0; k < 4; k++)
+    for (int c = 0; c < nch && c < tx.channels; c++) grad[(size_t)idx[k] * tx.channels + c] += (double)w[k] * g[c];
+}
+
+struct AdjCtx {
+  const Scene &s;
+  const Bvh &bvh;
+  const RenderParams &p;
+  double *grad;  // f32 pool layout
+};
+
+vec4 Li_adj(const AdjCtx &A, vec3 o, vec3 d, int depth, SobolSampler &sampler, int scatter_index, Spec throughput, const double dV[3]) {
+  const Scene &s = A.s;
+  const RenderParams &p = A.p;
+  if (depth >= 0) {
+    float t = 0, bu = 0, bv = 0;
+    int prim = closest_hit(s, A.bvh, o, d, 0.000001f, t, bu, bv);
+    if (prim >= 0) {
+      Hit h = make_hit(s, prim, o, d, t, bu, bv);
+      const Material &mat = s.materials[s.meshes[s.prim_mesh[prim]].material];
+      ScatterResult sr;
+      if (!scatter(s, mat, d, h, sampler, sr, p.sampling_roughness)) return vec4(0.f);
+      vec3 wi = sr.lobe.wi;
+      vec3 nd = normalize(wi);
+      const float eps = 1e-3f;
+      vec3 no = h.position + h.ng * (eps + eps * gsign(dot(h.ng, wi)));
+      Spec color = sr.attenuation * sr.lobe.costheta / sr.lobe.pdf;
+      bool terminated = false;
+      int rr_channel = -1;
+      if (p.rr_start >= 0 && scatter_index >= p.rr_start) {
+        Spec tp = throughput * color;
+        float q = std::fmin(1.f, std::fmax(tp.x, std::fmax(tp.y, tp.z)));
+        float ur = sampler.sample1D();
+        if (!(ur < q)) {
+          terminated = true;
+        } else {
+          color = color / q;
+          throughput = throughput * color;
+          if (q < 1.f) rr_channel = (tp.x == q) ? 0 : (tp.y == q) ? 1 : 2;
+        }
+      } else {
+        throughput = throughput * color;
+      }
+      const bool finite = std::isfinite(color.x) && std::isfinite(color.y) && std::isfinite(color.z);
+      const double cv[3] = {color.x, color.y, color.z};
+      const double zero[3] = {0, 0, 0};
+      double dnext[3] = {dV[0] * cv[0], dV[1] * cv[1], dV[2] * cv[2]};
+      vec4 next = terminated ? vec4(0.f) : Li_adj(A, no, nd, depth - 1, sampler, scatter_index + 1, throughput, finite ? dnext : zero);
+      vec4 e(sr.emissive, 1.f);
+      vec4 c(color, 1.f);
+      vec4 fin = e + c * next;
+      if (vec_nonfinite(fin)) return vec4(0.f);
+      if (!terminated) {
+        // d loss / d c_k per channel, times c_k: the weight of d ln c_k / d theta
+        const double nv[3] = {next.x, next.y, next.z};
+        double wc[3], wsum = 0;
+        for (int k = 0; k < 3; k++) {
+          wc[k] = dV[k] * nv[k] * cv[k];
+          wsum += wc[k];
+        }
+        double dlna[3], dlnr;
+        lobe_log_derivatives(sr, dlna, dlnr);
+        double ga[3] = {wc[0] * dlna[0], wc[1] * dlna[1], wc[2] * dlna[2]};
+        double gr = wsum * dlnr;
+        if (rr_channel >= 0) {  // color = c / q, q = T_m c_m
+          ga[rr_channel] -= wsum * dlna[rr_channel];
+          gr = 0;
+        }
+        if (mat.albedo_f32 >= 0) splat(s.textures[mat.albedo_f32], A.grad + f32_pool_offset(s, mat.albedo_f32), sr.u, sr.v, ga, 3);
+        if (mat.roughness_f32 >= 0) splat(s.textures[mat.roughness_f32], A.grad + f32_pool_offset(s, mat.roughness_f32), sr.u, sr.v, &gr, 1);
+      }
+      return fin;
+    }
+  }
+  return sample_envmap(s, d);
+}
+}  // namespace
+
+size_t f32_pool_offset(const Scene &s, int texture) {
+  size_t off = 0;
+  for (int t = 0; t < texture; t++)
+    if (!s.textures[t].f32.empty()) off += (size_t)s.textures[t].width * s.textures[t].height * s.textures[t].channels;
+  return off;
+}
+
+void render_adjoint(const Scene &s, const Bvh &bvh, const RenderParams &p, const float *d_image, float scale, double *grad_pool) {
+  size_t pool = 0;
+  for (const Texture &t : s.textures)
+    if (!t.f32.empty()) pool += (size_t)t.width * t.height * t.channels;
+  const int nthreads = std::max(1, p.threads);
+  std::vector<std::vector<double>> part(nthreads, std::vector<double>(pool, 0.0));
+  std::atomic<int> next_row{0};
+  const int rows = p.height * p.frames;
+  auto worker = [&](int tid) {
+    AdjCtx A{s, bvh, p, part[tid].data()};
+    SobolSampler sampler;
+    for (;;) {
+      const int r = next_row.fetch_add(1);
+      if (r >= rows) break;
+      const int frame = p.frame_start + r / p.height, y = r % p.height;
+      for (int x = 0; x < p.width; x++) {  // AbstractIntegrator::render for one pixel (Integrator.h:53-95)
+        unsigned idx = p.vertical_invert ? (unsigned)(((p.height - 1 - y) * p.width + x) * 4) : (unsigned)((y * p.width + x) * 4);
+        sampler.reset(idx * (unsigned)frame + p.max_samples_offset);
+        float s0 = sampler.sample1D(), s1 = sampler.sample1D();
+        const float dx = s0 * 0.0005f, dy = s1 * 0.0005f;
+        int sy = p.height - y;
+        float ndc_x = (float)(2 * x - p.width) / (float)p.width;
+        float ndc_y = (float)(p.height - 2 * sy) / (float)p.height;
+        vec3 o, d;
+        camera_ray(p, ndc_x + dx, ndc_y + dy, o, d);
+        const double dV[3] = {(double)scale * d_image[idx], (double)scale * d_image[idx + 1], (double)scale * d_image[idx + 2]};
+        Li_adj(A, o, d, p.max_depth, sampler, 0, Spec(1.f), dV);
+      }
+    }
+  };
+  std::vector<std::thread> pool_threads;
+  for (int i = 0; i < nthreads; i++) pool_threads.emplace_back(worker, i);
+  for (auto &th : pool_threads) th.join();
+  for (int i = 0; i < nthreads; i++)
+    for (size_t k = 0; k < pool; k++) grad_pool[k] += part[i][k];
 }
 
 }  // namespace oracle

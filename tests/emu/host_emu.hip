@@ -155,7 +155,10 @@ extern "C" int emu_closest_hit(const nova_hip_scene_desc *sd, const BNode *bnode
   return 0;
 }
 
-extern "C" int emu_render(const nova_hip_scene_desc *sd, const nova_hip_render_params *p, float *accum, float *final_out) {
+// Renders (adj_grad == nullptr: accumulating into accum/final_out) or, for the adjoint,
+// renders each frame and replays it with the gradient pass of shade_path.
+static int emu_run(const nova_hip_scene_desc *sd, const nova_hip_render_params *p, float *accum, float *final_out, const float *adj_dimg,
+                   float adj_scale, float *adj_grad) {
   EmuScene es;
   flatten_scene(sd, es);
   const uint64_t ntri = es.ntri;
@@ -201,7 +204,8 @@ extern "C" int emu_render(const nova_hip_scene_desc *sd, const nova_hip_render_p
   }
   std::vector<float> lut(kAcosLut);
   for (int i = 0; i < kAcosLut; i++) lut[i] = (float)std::acos(((double)i / (double)kAcosLut) * 2.0 - 1.0);
-  ShadeCtx sc{texrec.data(), u32pool.data(), f32pool.data(), mats.data(), mxf.data(), shade.data(), lut.data(), sd->envmap_texture};
+  ShadeCtx sc{texrec.data(), u32pool.data(), f32pool.data(), mats.data(), mxf.data(), shade.data(), lut.data(), sd->envmap_texture,
+              p->sampling_roughness_texture};
 
   // ---- frames: camera ray, then closest hit + shade per bounce, then accumulate
   const uint32_t npix = (uint32_t)p->width * (uint32_t)p->height;
@@ -216,6 +220,8 @@ extern "C" int emu_render(const nova_hip_scene_desc *sd, const nova_hip_render_p
     fp.vinvert = p->vertical_invert;
     std::memcpy(fp.inv_P, p->inv_P, 64);
     std::memcpy(fp.inv_V, p->inv_V, 64);
+    const AdjointCtx adj{partial.data(), adj_dimg, adj_scale, adj_grad};
+    for (int pass = 0; pass < (adj_grad ? 2 : 1); pass++)
     for (uint32_t slot = 0; slot < npix; slot++) {
       f3 o, d;
       camera_ray(fp, &kTable[0][0], slot, o, d);
@@ -242,22 +248,34 @@ extern "C" int emu_render(const nova_hip_scene_desc *sd, const nova_hip_render_p
         sp.dim0 = dim;
         f3 no, nd;
         bool esc;
-        bool cont = shade_path(sp, sc, &kTable[0][0], slot, o, d, best, tb, bu, bv, T, L, A, no, nd, esc);
+        bool cont = pass == 0 ? shade_path(sp, sc, &kTable[0][0], slot, o, d, best, tb, bu, bv, T, L, A, no, nd, esc)
+                              : shade_path<true>(sp, sc, &kTable[0][0], slot, o, d, best, tb, bu, bv, T, L, A, no, nd, esc, &adj);
         dim += 4 + (sp.rr ? 1 : 0);
         if (!cont) break;
         o = no;
         d = nd;
       }
+      if (pass == 1) continue;
       float *dst = &partial[pixel_offset(fp, slot)];
       dst[0] = L.x;
       dst[1] = L.y;
       dst[2] = L.z;
       dst[3] = A;
     }
+    if (adj_grad) continue;
     for (size_t i = 0; i < (size_t)npix * 4; i++) {
       accum[i] += partial[i];
       final_out[i] = accum[i] / float(f + 1);
     }
   }
   return 0;
+}
+
+extern "C" int emu_render(const nova_hip_scene_desc *sd, const nova_hip_render_params *p, float *accum, float *final_out) {
+  return emu_run(sd, p, accum, final_out, nullptr, 0.f, nullptr);
+}
+
+// Texel gradients (FP32 pool layout) from the device path's replay code run on the host.
+extern "C" int emu_adjoint(const nova_hip_scene_desc *sd, const nova_hip_render_params *p, const float *d_image, float scale, float *grad) {
+  return emu_run(sd, p, nullptr, nullptr, d_image, scale, grad);
 }

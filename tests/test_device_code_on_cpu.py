@@ -29,6 +29,7 @@ def emu():
     L.emu_render.argtypes = [C.POINTER(nova_hip.SceneDesc), C.POINTER(nova_hip.RenderParams), C.c_void_p, C.c_void_p]
     L.emu_closest_hit.argtypes = [C.POINTER(nova_hip.SceneDesc), C.c_void_p, C.c_int32, C.c_void_p, C.c_void_p, C.c_int64, C.c_void_p,
                                   C.c_void_p, C.c_void_p]
+    L.emu_adjoint.argtypes = [C.POINTER(nova_hip.SceneDesc), C.POINTER(nova_hip.RenderParams), C.c_void_p, C.c_float, C.c_void_p]
     return L
 
 
