@@ -58,6 +58,24 @@ __global__ void sobol_fill_kernel(const uint32_t *__restrict__ table, const uint
   if (out_bits) out_bits[i] = v;
 }
 
+// Direction table on the current device for the context-free entry points: one copy per
+// device, kept until the process ends; nullptr if it cannot be made.
+const uint32_t *device_sobol_table() {
+  static uint32_t *tables[64] = {};
+  int dev = 0;
+  if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= 64) return nullptr;
+  if (!tables[dev]) {
+    uint32_t *t = nullptr;
+    if (hipMalloc(&t, sizeof(kSobolTable)) != hipSuccess) return nullptr;
+    if (hipMemcpy(t, kSobolTable, sizeof(kSobolTable), hipMemcpyHostToDevice) != hipSuccess) {
+      hipFree(t);
+      return nullptr;
+    }
+    tables[dev] = t;
+  }
+  return tables[dev];
+}
+
 }  // namespace nova_hip
 
 using namespace nova_hip;
@@ -119,15 +137,8 @@ const char *nova_hip_last_error(const nova_hip_ctx *ctx) { return ctx ? ctx->las
 int nova_hip_sobol_fill(const uint32_t *idx_dev, const uint32_t *dim_dev, uint64_t n, float *out_dev, uint32_t *out_bits_dev, void *stream) {
   if (n == 0) return NOVA_HIP_SUCCESS;
   if (!idx_dev || !dim_dev || !out_dev) return NOVA_HIP_INVALID_ARGUMENT;
-  static uint32_t *table = nullptr;  // one device copy per process for the context-free entry point
-  static int table_device = -1;
-  int dev = 0;
-  if (hipGetDevice(&dev) != hipSuccess) return NOVA_HIP_DEVICE_ERROR;
-  if (!table || table_device != dev) {
-    if (hipMalloc(&table, sizeof(kSobolTable)) != hipSuccess) return NOVA_HIP_OUT_OF_MEMORY;
-    if (hipMemcpy(table, kSobolTable, sizeof(kSobolTable), hipMemcpyHostToDevice) != hipSuccess) return NOVA_HIP_DEVICE_ERROR;
-    table_device = dev;
-  }
+  const uint32_t *table = device_sobol_table();
+  if (!table) return NOVA_HIP_OUT_OF_MEMORY;
   const unsigned block = 256;
   const uint64_t grid = (n + block - 1) / block;
   hipLaunchKernelGGL(sobol_fill_kernel, dim3((unsigned)grid), dim3(block), 0, (hipStream_t)stream, table, idx_dev, dim_dev, n, out_dev,

@@ -16,6 +16,7 @@ struct BakeMap {
   const float *env;
   float *out;
   int32_t w, h, c, ow, oh;
+  int32_t kind;  // NOVA_HIP_BAKE_*
   float roughness;
   uint64_t first;  // first global texel index of this map
 };
@@ -96,11 +97,13 @@ __device__ __forceinline__ int find_map(const BakeMap *maps, int count, uint64_t
   return lo;
 }
 
-__global__ void irradiance_kernel(const BakeMap *__restrict__ maps, int count, uint64_t total, const float *__restrict__ lut, int samples) {
-  for (uint64_t g = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; g < total; g += (uint64_t)gridDim.x * blockDim.x) {
-    const BakeMap m = maps[find_map(maps, count, g)];
-    const uint64_t local = g - m.first;
-    const unsigned i = (unsigned)(local % (unsigned)m.ow), j = (unsigned)(local / (unsigned)m.ow);
+// One irradiance texel. The reference draws its cosine samples from pgc3d of the texel's
+// recomputed pixel and the sample number; the Sobol variant takes Owen-scrambled Sobol
+// points (dims 0 and 1) at indices texel * samples + s, so each texel gets its own
+// stratified block of the sequence.
+__device__ __forceinline__ f3 irradiance_texel(const BakeMap &m, const float *__restrict__ lut, const uint32_t *__restrict__ sobol, unsigned i,
+                                               unsigned j, int samples) {
+  {
     const float u = (float)((double)i / (double)m.ow), v = (float)((double)j / (double)m.oh);
     const f3 nrm = sph_to_cart((float)(2 * 3.14159265358979323846 * u), (float)(3.14159265358979323846 * v));
     const float aphi = atan2_approx(nrm.y, nrm.x), atheta = acos_lt(lut, nrm.z);
@@ -112,20 +115,25 @@ __global__ void irradiance_kernel(const BakeMap *__restrict__ maps, int count, u
     if (1.0 - fabs((double)dd) > 1e-6) tangent = normalize(cross(right, nrm));
     const f3 bitangent = cross(nrm, tangent);
     f3 irr = mk3(0.f, 0.f, 0.f);
+    const uint32_t base = (j * (uint32_t)m.ow + i) * (uint32_t)samples;
     for (int s = 0; s < samples; s++) {
-      const f3 r = pgc3d(px, py, (unsigned)s);
-      const float sphi = (float)(2 * 3.14159265358979323846 * r.x);
-      const float stheta = asin_f(sqrtf(r.y));
+      float rx, ry;
+      if (m.kind == NOVA_HIP_BAKE_IRRADIANCE_SOBOL) {
+        const uint32_t idx = base + (uint32_t)s, gray = idx ^ (idx >> 1);
+        rx = u32_to_unit(owen_scramble(sobol_xor(sobol, gray)));
+        ry = u32_to_unit(owen_scramble(sobol_xor(sobol + 32, gray)));
+      } else {
+        const f3 r = pgc3d(px, py, (unsigned)s);
+        rx = r.x;
+        ry = r.y;
+      }
+      const float sphi = (float)(2 * 3.14159265358979323846 * rx);
+      const float stheta = asin_f(sqrtf(ry));
       f3 c = sph_to_cart(sphi, stheta);
       c = c.x * tangent + c.y * bitangent + c.z * nrm;
       irr = irr + dir_sample(m, lut, c);
     }
-    irr = irr / (float)samples;
-    float *o = m.out + ((size_t)j * m.ow + i) * m.c;
-    o[0] = irr.x;
-    o[1] = irr.y;
-    o[2] = irr.z;
-    for (int k = 3; k < m.c; k++) o[k] = 0.f;
+    return irr / (float)samples;
   }
 }
 
@@ -138,11 +146,9 @@ __device__ __forceinline__ float radical_inverse(unsigned bits) {
   return (float)bits * 2.3283064365386963e-10f;
 }
 
-__global__ void prefilter_kernel(const BakeMap *__restrict__ maps, int count, uint64_t total, const float *__restrict__ lut, int samples) {
-  for (uint64_t g = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; g < total; g += (uint64_t)gridDim.x * blockDim.x) {
-    const BakeMap m = maps[find_map(maps, count, g)];
-    const uint64_t local = g - m.first;
-    const unsigned i = (unsigned)(local % (unsigned)m.ow), j = (unsigned)(local / (unsigned)m.ow);
+// One prefiltered-specular texel: N = V = R, Hammersley GGX half vectors, NdotL weights.
+__device__ __forceinline__ f3 prefilter_texel(const BakeMap &m, const float *__restrict__ lut, unsigned i, unsigned j, int samples) {
+  {
     const float u = (float)((double)i / (double)m.ow), v = (float)((double)j / (double)m.oh);
     const f3 N = normalize(sph_to_cart((float)(2 * 3.14159265358979323846 * u), (float)(3.14159265358979323846 * v)));
     f3 col = mk3(0.f, 0.f, 0.f);
@@ -169,6 +175,19 @@ __global__ void prefilter_kernel(const BakeMap *__restrict__ maps, int count, ui
       }
       col = col / total_w;
     }
+    return col;
+  }
+}
+
+// Every texel of every job in one grid-stride launch; jobs are laid out back to back, so a
+// wave almost always works on a single map and kind.
+__global__ void bake_kernel(const BakeMap *__restrict__ maps, int count, uint64_t total, const float *__restrict__ lut,
+                            const uint32_t *__restrict__ sobol, int samples) {
+  for (uint64_t g = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; g < total; g += (uint64_t)gridDim.x * blockDim.x) {
+    const BakeMap m = maps[find_map(maps, count, g)];
+    const uint64_t local = g - m.first;
+    const unsigned i = (unsigned)(local % (unsigned)m.ow), j = (unsigned)(local / (unsigned)m.ow);
+    const f3 col = m.kind == NOVA_HIP_BAKE_PREFILTER ? prefilter_texel(m, lut, i, j, samples) : irradiance_texel(m, lut, sobol, i, j, samples);
     float *o = m.out + ((size_t)j * m.ow + i) * m.c;
     o[0] = col.x;
     o[1] = col.y;
@@ -177,25 +196,33 @@ __global__ void prefilter_kernel(const BakeMap *__restrict__ maps, int count, ui
   }
 }
 
-static int launch_bake(bool prefilter, const float *const *env, const int32_t *w, const int32_t *h, const int32_t *c, uint32_t count, float *const *out,
-                       const int32_t *ow, const int32_t *oh, const float *rough, int32_t samples, hipStream_t s) {
+static int launch_batch(const nova_hip_bake_job *jobs, uint32_t count, int32_t samples, hipStream_t s) {
   if (count == 0) return NOVA_HIP_SUCCESS;
-  if (!env || !w || !h || !c || !out || !ow || !oh || samples <= 0) return NOVA_HIP_INVALID_ARGUMENT;
+  if (!jobs || samples <= 0) return NOVA_HIP_INVALID_ARGUMENT;
   std::vector<BakeMap> maps(count);
   uint64_t total = 0;
   for (uint32_t i = 0; i < count; i++) {
-    if (!env[i] || !out[i] || w[i] < 1 || h[i] < 1 || c[i] < 3 || c[i] > 4 || ow[i] < 1 || oh[i] < 1) return NOVA_HIP_INVALID_ARGUMENT;
-    maps[i] = BakeMap{env[i], out[i], w[i], h[i], c[i], ow[i], oh[i], rough ? rough[i] : 0.f, total};
-    total += (uint64_t)ow[i] * oh[i];
+    const nova_hip_bake_job &jb = jobs[i];
+    if (!jb.env || !jb.out || jb.width < 1 || jb.height < 1 || jb.channels < 3 || jb.channels > 4 || jb.out_width < 1 || jb.out_height < 1)
+      return NOVA_HIP_INVALID_ARGUMENT;
+    if (jb.kind != NOVA_HIP_BAKE_IRRADIANCE && jb.kind != NOVA_HIP_BAKE_IRRADIANCE_SOBOL && jb.kind != NOVA_HIP_BAKE_PREFILTER)
+      return NOVA_HIP_INVALID_ARGUMENT;
+    // Sobol indices texel * samples + s must stay below 2^32
+    if (jb.kind == NOVA_HIP_BAKE_IRRADIANCE_SOBOL && (uint64_t)jb.out_width * jb.out_height * (uint64_t)samples > 0xFFFFFFFFull)
+      return NOVA_HIP_INVALID_ARGUMENT;
+    maps[i] = BakeMap{jb.env, jb.out, jb.width, jb.height, jb.channels, jb.out_width, jb.out_height, jb.kind, jb.roughness, total};
+    total += (uint64_t)jb.out_width * jb.out_height;
   }
-  // acos table and map descriptors live in one device allocation per call
-  std::vector<float> lut(kAcosLut);
-  for (int i = 0; i < kAcosLut; i++) lut[i] = (float)std::acos(((double)i / (double)kAcosLut) * 2.0 - 1.0);
+  const uint32_t *sobol = device_sobol_table();
+  if (!sobol) return NOVA_HIP_OUT_OF_MEMORY;
+  // acos table: one device copy per process and device
   static float *d_lut = nullptr;
   static int lut_dev = -1;
   int dev = 0;
   if (hipGetDevice(&dev) != hipSuccess) return NOVA_HIP_DEVICE_ERROR;
   if (!d_lut || lut_dev != dev) {
+    std::vector<float> lut(kAcosLut);
+    for (int i = 0; i < kAcosLut; i++) lut[i] = (float)std::acos(((double)i / (double)kAcosLut) * 2.0 - 1.0);
     if (hipMalloc(&d_lut, kAcosLut * sizeof(float)) != hipSuccess) return NOVA_HIP_OUT_OF_MEMORY;
     if (hipMemcpy(d_lut, lut.data(), kAcosLut * sizeof(float), hipMemcpyHostToDevice) != hipSuccess) return NOVA_HIP_DEVICE_ERROR;
     lut_dev = dev;
@@ -208,25 +235,35 @@ static int launch_bake(bool prefilter, const float *const *env, const int32_t *w
   }
   const unsigned block = 64;
   const unsigned grid = (unsigned)std::min<uint64_t>((total + block - 1) / block, 1u << 20);
-  if (prefilter)
-    hipLaunchKernelGGL(prefilter_kernel, dim3(grid), dim3(block), 0, s, d_maps, (int)count, total, d_lut, samples);
-  else
-    hipLaunchKernelGGL(irradiance_kernel, dim3(grid), dim3(block), 0, s, d_maps, (int)count, total, d_lut, samples);
+  hipLaunchKernelGGL(bake_kernel, dim3(grid), dim3(block), 0, s, d_maps, (int)count, total, d_lut, sobol, samples);
   hipError_t e = hipGetLastError();
   if (e == hipSuccess) e = hipStreamSynchronize(s);
   hipFree(d_maps);
   return e == hipSuccess ? NOVA_HIP_SUCCESS : NOVA_HIP_DEVICE_ERROR;
 }
 
+static int launch_legacy(int kind, const float *const *env, const int32_t *w, const int32_t *h, const int32_t *c, uint32_t count, float *const *out,
+                         const int32_t *ow, const int32_t *oh, const float *rough, int32_t samples, hipStream_t s) {
+  if (count == 0) return NOVA_HIP_SUCCESS;
+  if (!env || !w || !h || !c || !out || !ow || !oh) return NOVA_HIP_INVALID_ARGUMENT;
+  std::vector<nova_hip_bake_job> jobs(count);
+  for (uint32_t i = 0; i < count; i++) jobs[i] = nova_hip_bake_job{env[i], w[i], h[i], c[i], out[i], ow[i], oh[i], kind, rough ? rough[i] : 0.f};
+  return launch_batch(jobs.data(), count, samples, s);
+}
+
 }  // namespace nova_hip
 
 extern "C" int nova_hip_bake_irradiance(const float *const *env_dev, const int32_t *w, const int32_t *h, const int32_t *c, uint32_t count,
                                         float *const *out_dev, const int32_t *out_w, const int32_t *out_h, int32_t samples, void *stream) {
-  return nova_hip::launch_bake(false, env_dev, w, h, c, count, out_dev, out_w, out_h, nullptr, samples, (hipStream_t)stream);
+  return nova_hip::launch_legacy(NOVA_HIP_BAKE_IRRADIANCE, env_dev, w, h, c, count, out_dev, out_w, out_h, nullptr, samples, (hipStream_t)stream);
+}
+
+extern "C" int nova_hip_bake_batch(const nova_hip_bake_job *jobs, uint32_t count, int32_t samples, void *stream) {
+  return nova_hip::launch_batch(jobs, count, samples, (hipStream_t)stream);
 }
 
 extern "C" int nova_hip_bake_prefilter(const float *const *env_dev, const int32_t *w, const int32_t *h, const int32_t *c, uint32_t count,
                                        float *const *out_dev, const int32_t *out_w, const int32_t *out_h, const float *roughness, int32_t samples,
                                        void *stream) {
-  return nova_hip::launch_bake(true, env_dev, w, h, c, count, out_dev, out_w, out_h, roughness, samples, (hipStream_t)stream);
+  return nova_hip::launch_legacy(NOVA_HIP_BAKE_PREFILTER, env_dev, w, h, c, count, out_dev, out_w, out_h, roughness, samples, (hipStream_t)stream);
 }

@@ -120,6 +120,7 @@ int set_error(nova_hip_ctx *ctx, int code, const std::string &msg);
 int check_hip(nova_hip_ctx *ctx, hipError_t e, const char *what);
 void *ensure_scratch(nova_hip_ctx *ctx, int slot, size_t bytes, hipError_t *err);
 uint32_t traversal_stack_bound(const nova_hip_ctx *ctx);
+const uint32_t *device_sobol_table();
 }  // namespace nova_hip
 
 #define NOVA_HIP_CHECK(ctx, expr)                                   \

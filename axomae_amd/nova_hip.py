@@ -21,7 +21,8 @@ ERRORS = {0: "SUCCESS", 1: "INVALID_ENGINE_STATE", 2: "INVALID_BUFFER_STATE", 4:
 
 EXPORTS = ["nova_hip_create", "nova_hip_destroy", "nova_hip_last_error", "nova_hip_scene_upload", "nova_hip_bvh_build", "nova_hip_bvh_read",
            "nova_hip_closest_hit", "nova_hip_render_frames", "nova_hip_render_stats_get", "nova_hip_sobol_fill", "nova_hip_bake_irradiance",
-           "nova_hip_bake_prefilter", "nova_hip_texture_f32_info", "nova_hip_texture_f32_update", "nova_hip_render_adjoint"]
+           "nova_hip_bake_prefilter", "nova_hip_texture_f32_info", "nova_hip_texture_f32_update", "nova_hip_render_adjoint",
+           "nova_hip_bake_batch"]
 
 
 class NovaHipError(RuntimeError):
@@ -64,6 +65,13 @@ class RenderParams(C.Structure):
                 ("inv_V", C.c_float * 16), ("flags", C.c_int32), ("sampling_roughness_texture", C.c_int32)]
 
 
+class BakeJob(C.Structure):
+    _fields_ = [("env", C.c_void_p), ("width", C.c_int32), ("height", C.c_int32), ("channels", C.c_int32), ("out", C.c_void_p),
+                ("out_width", C.c_int32), ("out_height", C.c_int32), ("kind", C.c_int32), ("roughness", C.c_float)]
+
+
+BAKE_IRRADIANCE, BAKE_IRRADIANCE_SOBOL, BAKE_PREFILTER = 0, 1, 2
+
 RENDER_TIMING = 1
 RENDER_COUNT_TRAVERSAL = 2
 
@@ -98,6 +106,7 @@ def load():
     L.nova_hip_render_frames.argtypes = [P, C.POINTER(RenderParams), P, P, P, P]
     L.nova_hip_render_stats_get.argtypes = [P, C.POINTER(RenderStats)]
     L.nova_hip_sobol_fill.argtypes = [P, P, C.c_uint64, P, P, P]
+    L.nova_hip_bake_batch.argtypes = [C.POINTER(BakeJob), C.c_uint32, C.c_int32, P]
     L.nova_hip_texture_f32_info.argtypes = [P, C.c_int32, C.POINTER(C.c_uint64), C.POINTER(C.c_uint64)]
     L.nova_hip_texture_f32_update.argtypes = [P, C.c_int32, P, P]
     L.nova_hip_render_adjoint.argtypes = [P, C.POINTER(RenderParams), P, C.c_float, P, P]
@@ -334,3 +343,28 @@ def bake(kind, envmaps, out_sizes, samples, roughness=None):
     if rc:
         raise NovaHipError(rc, f"bake {kind}")
     return [o.cpu().numpy().reshape(s[1], s[0], e.channels) for o, s, e in zip(outs, out_sizes, envmaps)]
+
+
+def bake_batch(jobs, samples, keep_on_device=False):
+    """One launch for every job: jobs are (envmap Texture or device tensor with (w, h, c), out_w, out_h, kind, roughness).
+    Returns the outputs (numpy, or device tensors with keep_on_device) in job order."""
+    torch = _torch()
+    L = load()
+    arr = (BakeJob * len(jobs))()
+    keep, outs = [], []
+    for k, (env, ow, oh, kind, rough) in enumerate(jobs):
+        if isinstance(env, tuple):
+            src, (w, h, c) = env
+        else:
+            src = torch.as_tensor(np.ascontiguousarray(env.data, dtype=np.float32)).to("cuda")
+            w, h, c = env.width, env.height, env.channels
+        out = torch.zeros(oh * ow * c, dtype=torch.float32, device="cuda")
+        keep.append(src)
+        outs.append((out, ow, oh, c))
+        arr[k] = BakeJob(src.data_ptr(), w, h, c, out.data_ptr(), ow, oh, kind, float(rough))
+    rc = L.nova_hip_bake_batch(arr, len(jobs), samples, _stream(torch))
+    if rc:
+        raise NovaHipError(rc, "bake_batch")
+    if keep_on_device:
+        return [o.view(oh, ow, c) for o, ow, oh, c in outs]
+    return [o.cpu().numpy().reshape(oh, ow, c) for o, ow, oh, c in outs]

@@ -396,3 +396,40 @@ def _smooth_noise(rng, n, channels, lo, hi):
     out -= out.min()
     out /= max(out.max(), 1e-12)
     return (lo + (hi - lo) * out).reshape(-1)
+
+
+def sky_envmap_device(width, height, sun_dir, sun_intensity=50.0, sun_size=0.02):
+    """sky_envmap evaluated with torch on the GPU (float64, cast once to float32), for
+    workloads whose maps are too large to build quickly on the host. Returns a flat cuda
+    tensor of width*height*4 floats."""
+    import torch
+    dev = "cuda"
+    u = (torch.arange(width, device=dev, dtype=torch.float64) + 0.5) / width
+    v = (torch.arange(height, device=dev, dtype=torch.float64) + 0.5) / height
+    theta, phi = torch.meshgrid(math.pi * v, 2 * math.pi * u, indexing="ij")
+    d = torch.stack([torch.sin(theta) * torch.cos(phi), torch.sin(theta) * torch.sin(phi), torch.cos(theta)], -1)
+    up = d[..., 1]
+    t = torch.clamp(up * 0.5 + 0.5, 0, 1)[..., None]
+    horizon = torch.tensor([0.9, 0.85, 0.8], device=dev, dtype=torch.float64)
+    zenith = torch.tensor([0.25, 0.45, 0.9], device=dev, dtype=torch.float64)
+    ground = torch.tensor([0.2, 0.18, 0.16], device=dev, dtype=torch.float64)
+    col = torch.where(up[..., None] >= 0, horizon * (1 - t) + zenith * t, ground)
+    s = torch.tensor(np.asarray(sun_dir, dtype=np.float64) / np.linalg.norm(sun_dir), device=dev)
+    cosang = torch.clamp((d * s).sum(-1), -1, 1)
+    col = col + sun_intensity * torch.exp(-(1 - cosang) / sun_size)[..., None] * torch.tensor([1.0, 0.95, 0.85], device=dev, dtype=torch.float64)
+    rgba = torch.cat([col, torch.ones_like(col[..., :1])], -1)
+    return rgba.to(torch.float32).reshape(-1).contiguous()
+
+
+def bake_envmap_specs(seed=0xA70AE005, count=32):
+    """(width, height, sun_dir, sun_intensity, sun_size) of the IBL bake batch's maps: the same
+    draws as bake_envmaps, without building the maps."""
+    rng = np.random.default_rng(seed)
+    sizes = [(512, 256), (1024, 512), (2048, 1024), (4096, 2048)]
+    specs = []
+    for _ in range(count):
+        w, h = sizes[rng.integers(0, len(sizes))]
+        d = rng.standard_normal(3)
+        inten = float(np.exp(rng.uniform(np.log(1e2), np.log(1e5))))
+        specs.append((w, h, d, inten * 1e-3, 0.002))
+    return specs

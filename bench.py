@@ -64,6 +64,97 @@ def cpu_baseline(scene, width, height, depth, rr_start, target_s):
                       f"{dt:.1f} s on {cores} threads; oracle BVH build {build_s:.1f} s excluded"}
 
 
+def adjoint_leg(width=1280, height=720, frames=32, depth=4):
+    """Primal and path-replay adjoint throughput on the differentiation workload: ~100k
+    triangles, FP32 512x512 albedo and roughness textures, L2 loss against a render of
+    perturbed textures (sigma 0.05). Adjoint time covers the whole backward call (its
+    per-batch primal re-render and the replay)."""
+    import numpy as np
+    import torch
+    from axomae_amd import scenes
+    from axomae_amd.nova_hip import HipEngine
+    sc, alb, rough = scenes.differentiable_scene(width=width, height=height)
+    eng = HipEngine(0)
+    eng.upload(sc)
+    eng.build_bvh()
+    rng = np.random.default_rng(0xA70AE004)
+    ta = np.clip(sc.textures[alb].data + rng.normal(0, 0.05, sc.textures[alb].data.shape), 0, 1).astype(np.float32)
+    tr = np.clip(sc.textures[rough].data + rng.normal(0, 0.05, sc.textures[rough].data.shape), 0.01, 1).astype(np.float32)
+    eng.update_texture(alb, torch.from_numpy(ta))
+    eng.update_texture(rough, torch.from_numpy(tr))
+    target = eng.render(width, height, frames, depth)[0]
+    eng.update_texture(alb, torch.from_numpy(sc.textures[alb].data))
+    eng.update_texture(rough, torch.from_numpy(sc.textures[rough].data))
+    eng.render(width, height, 1, depth)  # warm-up
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    img = eng.render(width, height, frames, depth)[0]
+    torch.cuda.synchronize()
+    t_primal = time.perf_counter() - t0
+    dimg = 2 * (img - target)
+    dimg[..., 3] = 0
+    loss = float(((img[..., :3].double() - target[..., :3].double()) ** 2).sum())
+    eng.adjoint(width, height, 1, depth, dimg, 1.0)  # warm-up
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    grad = eng.adjoint(width, height, frames, depth, dimg, 1.0 / frames)
+    torch.cuda.synchronize()
+    t_adj = time.perf_counter() - t0
+    ga, gr = eng.texture_grad(grad, alb), eng.texture_grad(grad, rough)
+    paths = width * height * frames
+    primal, adjoint = paths / t_primal / 1e6, paths / t_adj / 1e6
+    log(f"[bench] adjoint leg: primal {t_primal * 1e3:.1f} ms, adjoint {t_adj * 1e3:.1f} ms, loss {loss:.4e}")
+    eng.close()
+    return {"workload": "differentiable_100k_720p", "triangles": sc.triangle_count, "width": width, "height": height, "spp": frames,
+            "max_depth": depth, "textures": "albedo 512x512x3 + roughness 512x512 FP32", "primal_msamples_s": round(primal, 3),
+            "adjoint_msamples_s": round(adjoint, 3), "adjoint_vs_primal": round(adjoint / primal, 4),
+            "grad_finite": bool(torch.isfinite(ga).all().item() and torch.isfinite(gr).all().item()),
+            "grad_nonzero_texels": int((ga.abs().sum(-1) > 0).sum().item() + (gr.abs().sum(-1) > 0).sum().item())}
+
+
+def bake_leg(samples=1024, cpu=True):
+    """The IBL bake batch: 32 FP32 equirect maps of uneven sizes (512x256 to 4096x2048,
+    seeded sun), per map a 64x32 irradiance map (Sobol sampler) and 6 GGX-prefiltered mips
+    from 256x128 at roughness i/5, `samples` samples per texel, all in one launch."""
+    import numpy as np
+    import torch
+    from axomae_amd import nova_hip, scenes
+    specs = scenes.bake_envmap_specs()
+    maps = [(scenes.sky_envmap_device(w, h, d, si, ss), (w, h, 4)) for (w, h, d, si, ss) in specs]
+    jobs = []
+    for m in maps:
+        jobs.append((m, 64, 32, nova_hip.BAKE_IRRADIANCE_SOBOL, 0.0))
+        for i in range(6):
+            jobs.append((m, 256 >> i, 128 >> i, nova_hip.BAKE_PREFILTER, i / 5.0))
+    nova_hip.bake_batch(jobs[:2], 16, keep_on_device=True)  # warm-up (tables, code objects)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    outs = nova_hip.bake_batch(jobs, samples, keep_on_device=True)
+    torch.cuda.synchronize()
+    t = time.perf_counter() - t0
+    texels = sum(j[1] * j[2] for j in jobs)
+    prefilter_r0 = sum(j[1] * j[2] for j in jobs if j[3] == nova_hip.BAKE_PREFILTER and j[4] == 0.0)
+    n_samples = (texels - prefilter_r0) * samples + prefilter_r0
+    finite = all(bool(torch.isfinite(o).all().item()) for o in outs)
+    log(f"[bench] bake batch: {len(jobs)} jobs, {texels} texels, {t * 1e3:.1f} ms")
+    res = {"workload": "ibl_bake_batch_32_maps", "maps": len(maps), "jobs": len(jobs), "irradiance": "64x32 sobol",
+           "prefilter": "6 mips from 256x128, roughness i/5", "samples_per_texel": samples, "launch_ms": round(t * 1e3, 2),
+           "texels": texels, "gsamples_s": round(n_samples / t / 1e9, 3), "maps_per_s": round(len(maps) / t, 2), "finite": finite,
+           "algorithmic_GBs": round(16.0 * n_samples / t / 1e9, 1)}
+    if cpu:
+        sys.path.insert(0, os.path.join(ROOT, "oracle"))
+        import pyoracle as po
+        threads = host_cores()
+        w, h, d, si, ss = specs[0]
+        env = scenes.sky_envmap(w, h, d, si, ss)
+        t0 = time.perf_counter()
+        po.bake_irradiance(env, 64, 32, samples, threads=threads, sobol=True)
+        tc = time.perf_counter() - t0
+        res["cpu_baseline"] = {"value": round(64 * 32 * samples / tc / 1e9, 4), "unit": "Gsamples/s", "cores": threads, "kind": "port",
+                               "sample": f"one {w}x{h} map, 64x32 irradiance, {samples} Sobol samples per texel"}
+    return res
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -75,6 +166,8 @@ def main():
     ap.add_argument("--rr-start", type=int, default=3)
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-adjoint", action="store_true", help="skip the adjoint-vs-primal leg (1280x720, 32 spp, depth 4, ~100k triangles)")
+    ap.add_argument("--no-bake", action="store_true", help="skip the IBL bake batch leg (32 maps, irradiance + 6 prefilter mips)")
     ap.add_argument("--copy-probe-mb", type=int, default=0,
                     help="before the render, copy a buffer of this size 3 times (a copy of known bytes for counter calibration)")
     args = ap.parse_args()
@@ -158,6 +251,10 @@ def main():
         "image_finite": finite,
     }
     eng.close()
+    if not args.no_adjoint:
+        out["adjoint"] = adjoint_leg()
+    if not args.no_bake:
+        out["bake"] = bake_leg(cpu=not args.no_cpu)
     if not args.no_cpu:
         cb = cpu_baseline(sc, W, H, args.depth, args.rr_start, args.cpu_seconds)
         out["cpu_baseline"] = cb

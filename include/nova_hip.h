@@ -235,6 +235,29 @@ int nova_hip_bake_prefilter(const float *const *env_dev, const int32_t *w, const
                             float *const *out_dev, const int32_t *out_w, const int32_t *out_h, const float *roughness,
                             int32_t samples, void *stream);
 
+/* One job of a batched bake: an FP32 equirect map (device pointer, width x height x channels,
+ * channels 3 or 4) and its output (device pointer, out_width x out_height x channels). */
+enum nova_hip_bake_kind {
+  NOVA_HIP_BAKE_IRRADIANCE = 0,       /* computeDiffuseIrradiance with its pgc3d hash (the reference's sampler) */
+  NOVA_HIP_BAKE_IRRADIANCE_SOBOL = 1, /* the same estimator on Owen-scrambled Sobol points, dims 0-1,
+                                         index texel * samples + s (extension) */
+  NOVA_HIP_BAKE_PREFILTER = 2         /* envmap_prefilter.frag at `roughness` */
+};
+typedef struct {
+  const float *env;
+  int32_t width, height, channels;
+  float *out;
+  int32_t out_width, out_height;
+  int32_t kind;    /* nova_hip_bake_kind */
+  float roughness; /* NOVA_HIP_BAKE_PREFILTER only */
+} nova_hip_bake_job;
+
+/* Every texel of every job in one launch (irradiance maps and prefilter mips of many
+ * environment maps together), `samples` samples per texel. Replaces the per-map
+ * launchAsyncDiffuseIrradianceCompute bands (TextureProcessing.h:155-230) and the per-mip
+ * GL passes of RenderPipeline::preFilterEnvmap (RenderPipeline.cpp:94-150). */
+int nova_hip_bake_batch(const nova_hip_bake_job *jobs, uint32_t count, int32_t samples, void *stream);
+
 #ifdef __cplusplus
 }
 #endif

@@ -220,6 +220,11 @@ void oracle_bake_irradiance(const float *env, int w, int h, int c, int ow, int o
   bake_irradiance(env, w, h, c, ow, oh, samples, out, threads);
 }
 
+// sampler: 0 pgc3d (reference), 1 Owen-scrambled Sobol
+void oracle_bake_irradiance_sampler(const float *env, int w, int h, int c, int ow, int oh, int samples, int threads, int sampler, float *out) {
+  bake_irradiance(env, w, h, c, ow, oh, samples, out, threads, sampler);
+}
+
 void oracle_bake_prefilter(const float *env, int w, int h, int c, int ow, int oh, float roughness, int samples, int threads, float *out) {
   bake_prefilter(env, w, h, c, ow, oh, roughness, samples, out, threads);
 }

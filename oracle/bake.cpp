@@ -97,7 +97,9 @@ static void parallel_rows(int n, int threads, F &&fn) {
 }
 
 // texel (i, j) of an out_w x out_h map: TextureProcessing.h:253-301
-static vec3 irradiance_texel(const EnvView &env, unsigned i, unsigned j, unsigned ow, unsigned oh, unsigned samples) {
+// sampler 0: pgc3d (the reference's); 1: Owen-scrambled Sobol dims 0-1 at index
+// texel * samples + s (the Sobol-sampled bake extension).
+static vec3 irradiance_texel(const EnvView &env, unsigned i, unsigned j, unsigned ow, unsigned oh, unsigned samples, int sampler) {
   float u = (float)((double)i / (double)ow), v = (float)((double)j / (double)oh);
   float phi = (float)(2 * M_PI * u), theta = (float)(M_PI * v);
   vec3 nrm = sph_to_cart(phi, theta);
@@ -112,7 +114,13 @@ static vec3 irradiance_texel(const EnvView &env, unsigned i, unsigned j, unsigne
   vec3 bitangent = cross(nrm, tangent);
   vec3 irr(0.f);
   for (unsigned s = 0; s < samples; s++) {
-    vec3 r = pgc3d(px, py, s);
+    vec3 r;
+    if (sampler == 1) {
+      const uint32_t idx = (j * ow + i) * samples + s;
+      r = vec3(sobol_sample(idx, 0), sobol_sample(idx, 1), 0.f);
+    } else {
+      r = pgc3d(px, py, s);
+    }
     float sphi = (float)(2 * M_PI * r.x);
     float stheta = std::asin(std::sqrt(r.y));
     vec3 c = sph_to_cart(sphi, stheta);
@@ -122,11 +130,11 @@ static vec3 irradiance_texel(const EnvView &env, unsigned i, unsigned j, unsigne
   return irr / (float)samples;
 }
 
-void bake_irradiance(const float *env, int env_w, int env_h, int env_c, int out_w, int out_h, int samples, float *out, int threads) {
+void bake_irradiance(const float *env, int env_w, int env_h, int env_c, int out_w, int out_h, int samples, float *out, int threads, int sampler) {
   EnvView ev{env, env_w, env_h, env_c};
   parallel_rows(out_w, threads, [&](int i) {
     for (int j = 0; j < out_h; j++) {
-      vec3 irr = irradiance_texel(ev, (unsigned)i, (unsigned)j, (unsigned)out_w, (unsigned)out_h, (unsigned)samples);
+      vec3 irr = irradiance_texel(ev, (unsigned)i, (unsigned)j, (unsigned)out_w, (unsigned)out_h, (unsigned)samples, sampler);
       unsigned idx = ((unsigned)j * out_w + i) * env_c;
       out[idx] = irr.x;
       out[idx + 1] = irr.y;

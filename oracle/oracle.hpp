@@ -113,7 +113,7 @@ void render_adjoint(const Scene &scene, const Bvh &bvh, const RenderParams &p, c
 size_t f32_pool_offset(const Scene &scene, int texture);
 
 // ---------------------------------------------------------------- bakes
-void bake_irradiance(const float *env, int env_w, int env_h, int env_c, int out_w, int out_h, int samples, float *out, int threads);
+void bake_irradiance(const float *env, int env_w, int env_h, int env_c, int out_w, int out_h, int samples, float *out, int threads, int sampler = 0);
 void bake_prefilter(const float *env, int env_w, int env_h, int env_c, int out_w, int out_h, float roughness, int samples, float *out, int threads);
 
 }  // namespace oracle

@@ -59,6 +59,7 @@ def lib():
         L.oracle_f32_pool_offset.restype = C.c_uint64
         L.oracle_render_rows.argtypes = [P, P, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, f32p, f32p, C.c_int, f32p]
         L.oracle_bake_irradiance.argtypes = [f32p, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, f32p]
+        L.oracle_bake_irradiance_sampler.argtypes = [f32p, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, f32p]
         L.oracle_bake_prefilter.argtypes = [f32p, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, C.c_float, C.c_int, C.c_int, f32p]
         _LIB = L
     return _LIB
@@ -202,10 +203,11 @@ def render_rows(oscene, width, height, frame, y0, y1, max_depth, rr_start=-1, th
     return out.reshape(height, width, 4)
 
 
-def bake_irradiance(env, out_w, out_h, samples, threads=8):
+def bake_irradiance(env, out_w, out_h, samples, threads=8, sobol=False):
+    """CPU irradiance bake; sobol=True draws the cosine samples from Owen-scrambled Sobol points instead of pgc3d."""
     data = np.ascontiguousarray(env.data, dtype=np.float32)
     out = np.zeros(out_w * out_h * env.channels, dtype=np.float32)
-    lib().oracle_bake_irradiance(data, env.width, env.height, env.channels, out_w, out_h, samples, threads, out)
+    lib().oracle_bake_irradiance_sampler(data, env.width, env.height, env.channels, out_w, out_h, samples, threads, int(sobol), out)
     return out.reshape(out_h, out_w, env.channels)
 
 

@@ -142,3 +142,25 @@ def test_render_fp32_material_textures(oracle, engine):
     final, _ = engine.render(32, 20, frames=2, max_depth=4)
     ref, _ = oracle.OracleScene(sc).render(32, 20, 2, 4, threads=8)
     assert rel_rmse(final.cpu().numpy(), ref) <= 1e-3
+
+
+@gpu
+def test_bake_batch_one_launch_matches_cpu_bake(oracle):
+    """Irradiance (pgc3d and Sobol samplers) and a prefilter mip chain of several maps in a
+    single nova_hip_bake_batch launch, each output against the CPU bake."""
+    from axomae_amd import nova_hip
+    maps = [scenes.sky_envmap(256, 128, (0.2, 0.9, 0.4), sun_intensity=30.0, sun_size=0.005), scenes.sky_envmap(128, 64, (-0.5, 0.3, 0.8))]
+    jobs, refs = [], []
+    for m in maps:
+        jobs.append((m, 32, 16, nova_hip.BAKE_IRRADIANCE, 0.0))
+        refs.append(lambda m=m: oracle.bake_irradiance(m, 32, 16, 256))
+        jobs.append((m, 32, 16, nova_hip.BAKE_IRRADIANCE_SOBOL, 0.0))
+        refs.append(lambda m=m: oracle.bake_irradiance(m, 32, 16, 256, sobol=True))
+        for i in range(6):
+            w, h, r = max(64 >> i, 1), max(32 >> i, 1), i / 5.0
+            jobs.append((m, w, h, nova_hip.BAKE_PREFILTER, r))
+            refs.append(lambda m=m, w=w, h=h, r=r: oracle.bake_prefilter(m, w, h, r, 256))
+    outs = nova_hip.bake_batch(jobs, 256)
+    for k, (o, ref) in enumerate(zip(outs, refs)):
+        err = rel_rmse(o, ref())
+        assert err <= 1e-3, (k, jobs[k][1:], err)

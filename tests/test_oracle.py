@@ -176,3 +176,15 @@ def test_constant_envmap_bakes_to_the_constant(oracle, c):
     for r in (0.0, 0.5, 1.0):
         pf = oracle.bake_prefilter(env, 16, 8, r, 64)
         np.testing.assert_allclose(pf[..., :3], c, rtol=1e-6)
+
+
+def test_sobol_irradiance_bake_constant_map_and_agreement_with_pgc3d(oracle):
+    """The Sobol-sampled irradiance bake keeps a constant map constant and estimates the same
+    integral as the reference's pgc3d sampler (Monte Carlo agreement on a smooth sky)."""
+    const = scenes.Texture(64, 32, 4, np.tile(np.array([0.5, 1.5, 2.5, 1.0], dtype=np.float32), 64 * 32))
+    out = oracle.bake_irradiance(const, 16, 8, 64, sobol=True)
+    np.testing.assert_allclose(out[..., :3], np.broadcast_to([0.5, 1.5, 2.5], out[..., :3].shape), rtol=2e-6)
+    sky = scenes.sky_envmap(128, 64, (0.3, 0.8, 0.2), sun_intensity=0.0)
+    a = oracle.bake_irradiance(sky, 16, 8, 2048, sobol=True)[..., :3]
+    b = oracle.bake_irradiance(sky, 16, 8, 2048)[..., :3]
+    assert np.linalg.norm(a - b) / np.linalg.norm(b) < 0.02
