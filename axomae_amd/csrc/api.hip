@@ -128,6 +128,7 @@ int nova_hip_destroy(nova_hip_ctx *ctx) {
   for (auto &b : ctx->scratch)
     if (b.ptr) hipFree(b.ptr);
   if (ctx->gstack) hipFree(ctx->gstack);
+  if (ctx->batch_stream) hipStreamDestroy(ctx->batch_stream);
   delete ctx;
   return NOVA_HIP_SUCCESS;
 }

@@ -228,4 +228,24 @@ NOVA_HD int trace_closest(const TraceCtx &tc, Stack &stack, f3 o, f3 d, float tn
   return best;
 }
 
+constexpr int kTraceBlock = 256;
+constexpr int kLdsStack = 24;
+
+// Traversal stack of one lane of a kTraceBlock-thread block: the first kLdsStack levels in
+// LDS (lane-interleaved), the rest in a global overflow area strided by the grid size.
+struct LdsStack {
+  uint32_t *lds;
+  uint32_t *gstack;
+  uint32_t gstride, gtid, lane;
+  __device__ __forceinline__ void push(int sp, uint32_t v) {
+    if (sp < kLdsStack)
+      lds[sp * kTraceBlock + lane] = v;
+    else
+      gstack[(size_t)(sp - kLdsStack) * gstride + gtid] = v;
+  }
+  __device__ __forceinline__ uint32_t at(int sp) const {
+    return sp < kLdsStack ? lds[sp * kTraceBlock + lane] : gstack[(size_t)(sp - kLdsStack) * gstride + gtid];
+  }
+};
+
 }  // namespace nova_hip

@@ -112,6 +112,7 @@ struct nova_hip_ctx {
   uint32_t *gstack = nullptr;                // overflow traversal stacks
   size_t gstack_entries = 0;
   nova_hip_render_stats stats{};
+  hipStream_t batch_stream = nullptr;        // batched renders capture their wave graph on it
 };
 
 namespace nova_hip {

@@ -22,7 +22,7 @@ ERRORS = {0: "SUCCESS", 1: "INVALID_ENGINE_STATE", 2: "INVALID_BUFFER_STATE", 4:
 EXPORTS = ["nova_hip_create", "nova_hip_destroy", "nova_hip_last_error", "nova_hip_scene_upload", "nova_hip_bvh_build", "nova_hip_bvh_read",
            "nova_hip_closest_hit", "nova_hip_render_frames", "nova_hip_render_stats_get", "nova_hip_sobol_fill", "nova_hip_bake_irradiance",
            "nova_hip_bake_prefilter", "nova_hip_texture_f32_info", "nova_hip_texture_f32_update", "nova_hip_render_adjoint",
-           "nova_hip_bake_batch"]
+           "nova_hip_bake_batch", "nova_hip_render_batch", "nova_hip_batch_autotune"]
 
 
 class NovaHipError(RuntimeError):
@@ -72,6 +72,17 @@ class BakeJob(C.Structure):
 
 BAKE_IRRADIANCE, BAKE_IRRADIANCE_SOBOL, BAKE_PREFILTER = 0, 1, 2
 
+class BatchConfig(C.Structure):
+    _fields_ = [("queue_capacity", C.c_uint32), ("trace_blocks_per_cu", C.c_uint32), ("shade_blocks_per_cu", C.c_uint32), ("use_graph", C.c_int32)]
+
+    def as_dict(self):
+        return {f: getattr(self, f) for f, _ in self._fields_}
+
+
+class BatchStats(C.Structure):
+    _fields_ = [("total_ms", C.c_float), ("paths", C.c_uint64), ("waves", C.c_uint64), ("segments", C.c_uint64), ("queue_capacity", C.c_uint32)]
+
+
 RENDER_TIMING = 1
 RENDER_COUNT_TRAVERSAL = 2
 
@@ -106,6 +117,8 @@ def load():
     L.nova_hip_render_frames.argtypes = [P, C.POINTER(RenderParams), P, P, P, P]
     L.nova_hip_render_stats_get.argtypes = [P, C.POINTER(RenderStats)]
     L.nova_hip_sobol_fill.argtypes = [P, P, C.c_uint64, P, P, P]
+    L.nova_hip_render_batch.argtypes = [P, P, C.c_uint32, C.POINTER(BatchConfig), C.POINTER(BatchStats), P, P]
+    L.nova_hip_batch_autotune.argtypes = [P, P, C.c_uint32, C.POINTER(BatchConfig), C.POINTER(C.c_float), P]
     L.nova_hip_bake_batch.argtypes = [C.POINTER(BakeJob), C.c_uint32, C.c_int32, P]
     L.nova_hip_texture_f32_info.argtypes = [P, C.c_int32, C.POINTER(C.c_uint64), C.POINTER(C.c_uint64)]
     L.nova_hip_texture_f32_update.argtypes = [P, C.c_int32, P, P]
@@ -368,3 +381,49 @@ def bake_batch(jobs, samples, keep_on_device=False):
     if keep_on_device:
         return [o.view(oh, ow, c) for o, ow, oh, c in outs]
     return [o.cpu().numpy().reshape(oh, ow, c) for o, ow, oh, c in outs]
+
+
+class BatchItem(C.Structure):
+    _fields_ = [("scene", C.c_void_p), ("params", RenderParams), ("accum_dev", C.c_void_p), ("final_dev", C.c_void_p)]
+
+
+class BatchRenderer:
+    """Renders many scenes (one HipEngine each, scene uploaded and BVH built) in shared
+    launches through nova_hip_render_batch; owns the executor context."""
+
+    def __init__(self, device=0):
+        self.exec = HipEngine(device)
+
+    def close(self):
+        self.exec.close()
+
+    def _items(self, jobs):
+        torch = self.exec.torch
+        arr = (BatchItem * len(jobs))()
+        bufs = []
+        for k, (eng, width, height, frames, max_depth, kw) in enumerate(jobs):
+            p = render_params(eng.scene, width, height, frames, max_depth, **kw)
+            acc = torch.empty((height, width, 4), dtype=torch.float32, device="cuda")
+            fin = torch.empty((height, width, 4), dtype=torch.float32, device="cuda")
+            bufs.append((acc, fin))
+            arr[k] = BatchItem(eng.ctx.value, p, acc.data_ptr(), fin.data_ptr())
+        return arr, bufs
+
+    def autotune(self, jobs):
+        """jobs: (engine, width, height, frames, max_depth, extra render_params kwargs). Returns (BatchConfig, ns per path)."""
+        arr, _ = self._items(jobs)
+        cfg, ns = BatchConfig(), C.c_float()
+        self.exec._check(self.exec.L.nova_hip_batch_autotune(self.exec.ctx, arr, len(jobs), C.byref(cfg), C.byref(ns), _stream(self.exec.torch)),
+                         "batch_autotune")
+        return cfg, ns.value
+
+    def render(self, jobs, config=None):
+        """Returns (final images, BatchStats, per-job completion ms)."""
+        if config is None:
+            config, _ = self.autotune(jobs)
+        arr, bufs = self._items(jobs)
+        st = BatchStats()
+        comp = np.zeros(len(jobs), dtype=np.float32)
+        self.exec._check(self.exec.L.nova_hip_render_batch(self.exec.ctx, arr, len(jobs), C.byref(config), C.byref(st), comp.ctypes.data,
+                                                           _stream(self.exec.torch)), "render_batch")
+        return [f for _, f in bufs], st, comp

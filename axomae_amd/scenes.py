@@ -433,3 +433,35 @@ def bake_envmap_specs(seed=0xA70AE005, count=32):
         inten = float(np.exp(rng.uniform(np.log(1e2), np.log(1e5))))
         specs.append((w, h, d, inten * 1e-3, 0.002))
     return specs
+
+
+def batch_scene_specs(seed=0xA70AE003, count=64):
+    """Draws of the batched workload: per scene (seed, triangle target log-uniform in
+    [1e3, 5e5], square resolution in {128, 256, 512, 1024}, spp in {4, 16, 64})."""
+    rng = np.random.default_rng(seed)
+    specs = []
+    for k in range(count):
+        tris = int(np.exp(rng.uniform(np.log(1e3), np.log(5e5))))
+        res = int(rng.choice([128, 256, 512, 1024]))
+        spp = int(rng.choice([4, 16, 64]))
+        specs.append((int(rng.integers(0, 2**31)), tris, res, spp))
+    return specs
+
+
+def batch_scene(seed, tris, res):
+    """A terrain + sphere mix of about `tris` triangles (about 60% terrain) under a small sky."""
+    rng = np.random.default_rng(seed)
+    sc = Scene(name=f"batch{seed}")
+    cells = max(8, int(math.sqrt(0.6 * tris / 2)))
+    sc.meshes.append(terrain_mesh(rng, cells, 15.0, -4.0, 2.5, sc.principled(rng.uniform(0.3, 0.8, 3), roughness=rng.uniform(0.3, 0.9))))
+    rest = tris - 2 * cells * cells
+    level = 3 if rest < 20000 else 4 if rest < 200000 else 5
+    n = max(1, rest // (8 * 4 ** level))
+    for _ in range(n):
+        c = rng.uniform(-12, 12, 3)
+        c[1] = rng.uniform(-2.0, 8.0)
+        m = sc.principled(rng.uniform(0.3, 0.95, 3), roughness=rng.uniform(0.05, 0.8), metal=float(rng.integers(0, 2)))
+        sc.meshes.append(octasphere_mesh(c, rng.uniform(0.3, 1.8), level, m))
+    sc.envmap = sc.add_texture(sky_envmap(128, 64, rng.standard_normal(3) * 0.3 + np.array([0.3, 0.8, 0.3])))
+    set_camera(sc, (0.0, 7.0, 26.0), (0.0, -1.0, 0.0), 50.0, 1.0)
+    return sc

@@ -183,6 +183,52 @@ typedef struct {
 } nova_hip_render_stats;
 int nova_hip_render_stats_get(nova_hip_ctx *ctx, nova_hip_render_stats *out);
 
+/* ---------------------------------------------------------------- batch */
+
+/* One scene of a batched render: a context holding an uploaded scene with its BVH built,
+ * the frames to render (as for nova_hip_render_frames; max_depth and rr_start must be the
+ * same for every item, sampling_roughness_texture -1) and device RGBA32F buffers of
+ * width*height*4 floats: accum is zeroed and receives the sum of the frames, final the
+ * image accum / (frame_start + frame_count). */
+typedef struct {
+  nova_hip_ctx *scene;
+  nova_hip_render_params params;
+  float *accum_dev, *final_dev;
+} nova_hip_batch_item;
+
+/* Launch shape of a batched render: paths in flight per wave (a multiple of 256), trace and
+ * shade workgroups per CU, and whether the wave sequence is replayed from a HIP graph. */
+typedef struct {
+  uint32_t queue_capacity;
+  uint32_t trace_blocks_per_cu, shade_blocks_per_cu;
+  int32_t use_graph;
+} nova_hip_batch_config;
+
+typedef struct {
+  float total_ms;            /* first wave start to last wave end, device time */
+  uint64_t paths, waves, segments;
+  uint32_t queue_capacity;
+} nova_hip_batch_stats;
+
+/* Renders every item in shared launches: all scenes' paths are numbered back to back and
+ * processed in waves of queue_capacity paths, each wave one generate, (extend, shade) per
+ * depth level and one accumulate launch over its paths whatever their scene. Each image
+ * equals the item's nova_hip_render_frames image bit for bit. completion_ms (optional,
+ * count floats) receives the device time at which each item's last wave ended. Items are
+ * processed in the order given (sorting them by path count gives shortest-first completion).
+ * `executor` is a context that owns the queues and the wave graph (it needs no scene).
+ * Replaces a loop of Engine renders, one per scene (engine.cpp:498-551); no counterpart in
+ * the reference's API. */
+int nova_hip_render_batch(nova_hip_ctx *executor, const nova_hip_batch_item *items, uint32_t count, const nova_hip_batch_config *config,
+                          nova_hip_batch_stats *stats, float *completion_ms, void *stream);
+
+/* Autotuner for nova_hip_render_batch: times the captured wave graph of every candidate
+ * (queue capacity 2^20, 2^21, 2^22 paths x 4 or 8 trace workgroups per CU x 4 or 8 shade
+ * workgroups per CU) on the same slots from the middle of the batch, without accumulating,
+ * and returns the fastest per path (best_ns_per_path optional). */
+int nova_hip_batch_autotune(nova_hip_ctx *executor, const nova_hip_batch_item *items, uint32_t count, nova_hip_batch_config *best,
+                            float *best_ns_per_path, void *stream);
+
 /* ---------------------------------------------------------- differentiation */
 
 /* Element offset of FP32 texture `texture` in the scene's FP32 texel pool and the pool size

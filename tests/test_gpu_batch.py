@@ -1,0 +1,61 @@
+"""Batched rendering on the MI355X: every scene of a batch equals its own
+nova_hip_render_frames image bit for bit, with the wave sequence replayed from a HIP graph
+or launched directly, and with queue capacities small enough that waves split frames and
+mix scenes; the autotuner returns one of its candidates."""
+import numpy as np
+import pytest
+
+from axomae_amd import scenes
+
+gpu = pytest.mark.gpu
+
+SPECS = [(11, 3000, 40, 3), (12, 800, 24, 5), (13, 12000, 56, 2), (14, 1500, 32, 4), (15, 5000, 20, 6)]
+DEPTH = 6
+
+
+@pytest.fixture(scope="module")
+def batch_jobs():
+    from axomae_amd.nova_hip import HipEngine
+    engines, jobs, refs = [], [], []
+    for seed, tris, res, spp in SPECS:
+        e = HipEngine(0)
+        e.upload(scenes.batch_scene(seed, tris, res))
+        e.build_bvh()
+        engines.append(e)
+        ref, _ = e.render(res, res, spp, DEPTH, rr_start=3)
+        refs.append(ref.cpu().numpy())
+        jobs.append((e, res, res, spp, DEPTH, {"rr_start": 3}))
+    yield jobs, refs
+    for e in engines:
+        e.close()
+
+
+@gpu
+@pytest.mark.parametrize("use_graph", [1, 0])
+@pytest.mark.parametrize("cap", [2048, 1 << 20])
+def test_batch_equals_single_scene_renders(batch_jobs, use_graph, cap):
+    from axomae_amd.nova_hip import BatchConfig, BatchRenderer
+    jobs, refs = batch_jobs
+    br = BatchRenderer(0)
+    finals, st, comp = br.render(jobs, BatchConfig(cap, 8, 8, use_graph))
+    print(f"[batch] cap {cap} graph {use_graph}: {st.waves} waves, {st.paths} paths, {st.total_ms:.2f} ms", flush=True)
+    assert st.paths == sum(r * r * s for _, _, r, s in SPECS)
+    for k, (f, ref) in enumerate(zip(finals, refs)):
+        np.testing.assert_array_equal(f.cpu().numpy(), ref, err_msg=f"scene {k}")
+    assert np.all(np.diff(comp) >= 0) and comp[-1] <= st.total_ms + 1e-3
+    br.close()
+
+
+@gpu
+def test_batch_autotune_picks_a_candidate(batch_jobs):
+    from axomae_amd.nova_hip import BatchRenderer
+    jobs, refs = batch_jobs
+    br = BatchRenderer(0)
+    cfg, ns = br.autotune(jobs)
+    print(f"[batch] autotune: {cfg.as_dict()} {ns:.3f} ns/path", flush=True)
+    assert cfg.trace_blocks_per_cu in (4, 8) and cfg.shade_blocks_per_cu in (4, 8) and cfg.use_graph == 1
+    assert ns > 0
+    finals, st, _ = br.render(jobs, cfg)
+    for f, ref in zip(finals, refs):
+        np.testing.assert_array_equal(f.cpu().numpy(), ref)
+    br.close()
