@@ -155,6 +155,56 @@ def bake_leg(samples=1024, cpu=True):
     return res
 
 
+def batch_leg(depth=6):
+    """64 seeded scenes (triangles log-uniform in [1e3, 5e5], resolution in {128..1024}^2,
+    spp in {4, 16, 64}, depth 6) rendered by nova_hip_render_batch with the autotuned launch
+    shape, shortest scene first; beside it the same scenes rendered one by one with
+    nova_hip_render_frames."""
+    import numpy as np
+    import torch
+    from axomae_amd import scenes
+    from axomae_amd.nova_hip import BatchRenderer, HipEngine
+    specs = scenes.batch_scene_specs()
+    t0 = time.perf_counter()
+    jobs = []
+    for seed, tris, res, spp in specs:
+        e = HipEngine(0)
+        e.upload(scenes.batch_scene(seed, tris, res))
+        e.build_bvh()
+        jobs.append((e, res, res, spp, depth, {}))
+    torch.cuda.synchronize()
+    log(f"[bench] batch: {len(jobs)} scenes generated, uploaded and built in {time.perf_counter() - t0:.1f} s")
+    jobs.sort(key=lambda j: j[1] * j[2] * j[3])
+    paths = sum(j[1] * j[2] * j[3] for j in jobs)
+    tris = sum(j[0].scene.triangle_count for j in jobs)
+    br = BatchRenderer(0)
+    cfg, ns = br.autotune(jobs)
+    br.render(jobs[:2], cfg)  # warm-up
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    finals, st, comp = br.render(jobs, cfg)
+    torch.cuda.synchronize()
+    t = time.perf_counter() - t0
+    finite = all(bool(torch.isfinite(f).all().item()) for f in finals)
+    del finals
+    br.close()
+    # the same scenes one render call each (each engine's queues freed after its call)
+    torch.cuda.synchronize()
+    t_seq = 0.0
+    for e, w, h, spp, d, _ in jobs:
+        t1 = time.perf_counter()
+        e.render(w, h, spp, d)
+        torch.cuda.synchronize()
+        t_seq += time.perf_counter() - t1
+        e.close()
+    log(f"[bench] batch: {paths / 1e6:.0f} M paths, batched {t * 1e3:.0f} ms ({st.waves} waves), one by one {t_seq * 1e3:.0f} ms")
+    return {"workload": "batch_64_scenes_depth6", "scenes": len(jobs), "triangles": tris, "paths": paths, "max_depth": depth,
+            "value": round(paths / t / 1e6, 3), "unit": "Msamples/s", "device_ms": round(st.total_ms, 2), "waves": int(st.waves),
+            "completion_ms_p50": round(float(np.percentile(comp, 50)), 2), "completion_ms_p99": round(float(np.percentile(comp, 99)), 2),
+            "order": "shortest first", "autotuned": cfg.as_dict(), "autotune_ns_per_path": round(ns, 3),
+            "one_by_one_msamples_s": round(paths / t_seq / 1e6, 3), "finite": finite}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -168,6 +218,7 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-adjoint", action="store_true", help="skip the adjoint-vs-primal leg (1280x720, 32 spp, depth 4, ~100k triangles)")
     ap.add_argument("--no-bake", action="store_true", help="skip the IBL bake batch leg (32 maps, irradiance + 6 prefilter mips)")
+    ap.add_argument("--no-batch", action="store_true", help="skip the 64-scene batched-render leg")
     ap.add_argument("--copy-probe-mb", type=int, default=0,
                     help="before the render, copy a buffer of this size 3 times (a copy of known bytes for counter calibration)")
     args = ap.parse_args()
@@ -255,6 +306,8 @@ def main():
         out["adjoint"] = adjoint_leg()
     if not args.no_bake:
         out["bake"] = bake_leg(cpu=not args.no_cpu)
+    if not args.no_batch:
+        out["batch"] = batch_leg()
     if not args.no_cpu:
         cb = cpu_baseline(sc, W, H, args.depth, args.rr_start, args.cpu_seconds)
         out["cpu_baseline"] = cb
