@@ -1,6 +1,7 @@
 // Oracle (test infrastructure only): C entry points for ctypes. Loaded only by
 // tests/, the smoke check and bench.py's CPU-baseline leg.
 #include "oracle.hpp"
+#include "fmath.hpp"
 #include <atomic>
 #include <cstring>
 #include <thread>
@@ -178,6 +179,10 @@ void oracle_render_adjoint(void *sp, void *bp, int width, int height, int frame_
                                sampling_roughness);
   render_adjoint(*static_cast<Scene *>(sp), *static_cast<Bvh *>(bp), p, d_image, scale, grad_pool);
 }
+
+// 1: the host libm's float cosf/sinf/asinf/powf as the reference's CPU build calls them;
+// 0 (default): evaluated in double and rounded once (what the HIP path computes).
+void oracle_set_float_libm(int on) { set_float_libm(on != 0); }
 
 uint64_t oracle_f32_pool_offset(void *sp, int texture) { return f32_pool_offset(*static_cast<Scene *>(sp), texture); }
 

@@ -5,6 +5,7 @@
 //    on an equirect source with level-0 sampling (no CPU version exists in the
 //    reference, so this one is pinned only by closed forms).
 #include "oracle.hpp"
+#include "fmath.hpp"
 #include <atomic>
 #include <cmath>
 #include <thread>
@@ -33,9 +34,9 @@ static unsigned uv_to_pixel(float coord, unsigned dim) { return (unsigned)(int64
 
 // sphericalToCartesian<float> (math_spherical.h:33-39)
 static vec3 sph_to_cart(float phi, float theta) {
-  const float z = std::cos(theta);
-  const float x = std::sin(theta) * std::cos(phi);
-  const float y = std::sin(theta) * std::sin(phi);
+  const float z = cos_f(theta);
+  const float x = sin_f(theta) * cos_f(phi);
+  const float y = sin_f(theta) * sin_f(phi);
   return {x, y, z};
 }
 
@@ -122,7 +123,7 @@ static vec3 irradiance_texel(const EnvView &env, unsigned i, unsigned j, unsigne
       r = pgc3d(px, py, s);
     }
     float sphi = (float)(2 * M_PI * r.x);
-    float stheta = std::asin(std::sqrt(r.y));
+    float stheta = asin_f(std::sqrt(r.y));
     vec3 c = sph_to_cart(sphi, stheta);
     c = c.x * tangent + c.y * bitangent + c.z * nrm;
     irr = irr + env.dir_sample(c);
@@ -161,7 +162,7 @@ static vec3 importance_sample_ggx(float xi0, float xi1, const vec3 &N, float r) 
   float phi = 2.0f * PI * xi0;
   float ct = std::sqrt((1.0f - xi1) / (1.0f + (a * a - 1.0f) * xi1));
   float st = std::sqrt(1.0f - ct * ct);
-  vec3 H(std::cos(phi) * st, std::sin(phi) * st, ct);
+  vec3 H(cos_f(phi) * st, sin_f(phi) * st, ct);
   vec3 up = std::fabs(N.z) < 0.999f ? vec3(0.f, 0.f, 1.f) : vec3(1.f, 0.f, 0.f);
   vec3 t = normalize(cross(up, N));
   vec3 b = cross(N, t);

@@ -15,6 +15,11 @@
 //   PathIntegrator::Li ..................... pinned by analytic scenes (constant envmap, emission only)
 //   irradiance bake ........................ pinned (constant map => constant, brute force of the texel rule)
 //   prefilter bake ......................... partial (no reference CPU implementation exists)
+//   render_adjoint (extension) ............. pinned (central finite differences of the oracle's own renders)
+//
+// Deviations, kept identical in the HIP path: a 0/0 slab in BoundingBox::intersect spans the
+// whole line (bvh.cpp), and the float cos/sin/asin/pow calls are evaluated in double and
+// rounded once (fmath.hpp; set_float_libm(true) restores the host libm's float functions).
 #pragma once
 #include "vecmath.hpp"
 #include <cstddef>

@@ -55,6 +55,7 @@ def lib():
                                              f32p, f32p, C.c_int, C.c_int, f32p, f32p]
         L.oracle_render_adjoint.argtypes = [P, P, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, C.c_uint32, C.c_int, C.c_int,
                                             f32p, f32p, C.c_int, C.c_int, f32p, C.c_float, np.ctypeslib.ndpointer(np.float64, flags="C")]
+        L.oracle_set_float_libm.argtypes = [C.c_int]
         L.oracle_f32_pool_offset.argtypes = [P, C.c_int]
         L.oracle_f32_pool_offset.restype = C.c_uint64
         L.oracle_render_rows.argtypes = [P, P, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, f32p, f32p, C.c_int, f32p]
@@ -266,3 +267,9 @@ def principled_sample(wo, albedo, roughness, metal, eta, u4):
     out = np.zeros(10, dtype=np.float32)
     L.oracle_principled_sample(_vec(wo), _vec(albedo), roughness, metal, eta, _vec(u4), out)
     return int(out[9]), bool(out[8]), out[:3].copy(), float(out[3]), out[4:7].copy(), float(out[7])
+
+
+def set_float_libm(on):
+    """True: the host libm's float cosf/sinf/asinf/powf, as the reference's CPU build calls them; False (default):
+    evaluated in double and rounded once, the correctly rounded value the HIP path computes."""
+    lib().oracle_set_float_libm(int(bool(on)))

@@ -1,6 +1,7 @@
 // Oracle (test infrastructure only): hit record (Embree-path semantics), texture
 // sampling, Principled BSDF, the recursive PathIntegrator and the frame loop.
 #include "oracle.hpp"
+#include "fmath.hpp"
 #include <algorithm>
 #include <atomic>
 #include <cfloat>
@@ -223,9 +224,9 @@ struct Sample {  // BSDFSample (BxDF_flags.h:26-36)
 
 enum : unsigned { REFLECTION = 1, TRANSMISSION = 2, DIFFUSE = 4, GLOSSY = 8, SPECULAR = 16 };
 
-static float schlick(float c, float f0) { return f0 + (1.f - f0) * powf((1.f - c), 5); }
+static float schlick(float c, float f0) { return f0 + (1.f - f0) * pow5(1.f - c); }
 static Spec schlick(float c, const Spec &f0) {
-  float p = powf((1.f - c), 5);
+  float p = pow5(1.f - c);
   return {f0.x + (1.f - f0.x) * p, f0.y + (1.f - f0.y) * p, f0.z + (1.f - f0.z) * p};
 }
 
@@ -280,7 +281,7 @@ static vec2 shirley_chiu(const float u[2]) {
     r = sy;
     theta = (float)M_PI_2 - (float)M_PI_4 * (sx / sy);
   }
-  return vec2(r * std::cos(theta), r * std::sin(theta));
+  return vec2(r * cos_f(theta), r * sin_f(theta));
 }
 static vec3 cosine_hemisphere(const float u[2]) {
   vec2 s = shirley_chiu(u);
@@ -346,7 +347,7 @@ struct VNDF {  // BxDF_math.h:231-342
     b = cross(woh, t);
     float r = msqrt(uc[0]);
     float ang = (float)(2.f * M_PI * uc[1]);  // float t = 2.f * M_PI * u[1] (BxDF_math.h:145-149)
-    vec2 p(r * std::cos(ang), r * std::sin(ang));
+    vec2 p(r * cos_f(ang), r * sin_f(ang));
     float s = 0.5f * (1.f + woh.z);
     float a = s;
     p.y = msqrt(1.f - sqr(p.x)) * (1.f - a) + a * p.y;
@@ -433,7 +434,7 @@ struct Principled {
   }
 
   bool sample_diffuse(const vec3 &wo, const float u[2], Sample &s) const {
-    auto fres = [](const vec3 &w, float value) { return 1.f + (value - 1) * powf(1.f - abscos(w), 5); };
+    auto fres = [](const vec3 &w, float value) { return 1.f + (value - 1) * pow5(1.f - abscos(w)); };
     vec3 wi = normalize(cosine_hemisphere(u));
     vec3 h = normalize(wi + wo);
     float ci = costheta(wi), co = costheta(wo);

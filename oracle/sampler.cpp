@@ -1,10 +1,12 @@
 // Oracle (test infrastructure only): Owen-scrambled Sobol generator and the
 // approximate trigonometry used by the envmap lookup.
 #include "oracle.hpp"
+#include "fmath.hpp"
 #include <algorithm>
 #include <cmath>
 
 namespace oracle {
+bool g_float_libm = false;
 
 const uint32_t kSobolTable[256][32] = {
 #include "../axomae_amd/csrc/sobol_table.inc"
