@@ -228,8 +228,11 @@ NOVA_HD int trace_closest(const TraceCtx &tc, Stack &stack, f3 o, f3 d, float tn
   return best;
 }
 
+#ifndef NOVA_LDS_STACK
+#define NOVA_LDS_STACK 24
+#endif
 constexpr int kTraceBlock = 256;
-constexpr int kLdsStack = 24;
+constexpr int kLdsStack = NOVA_LDS_STACK;  // traversal stack levels held in LDS per lane
 
 // Traversal stack of one lane of a kTraceBlock-thread block: the first kLdsStack levels in
 // LDS (lane-interleaved), the rest in a global overflow area strided by the grid size.

@@ -68,8 +68,17 @@ __global__ __launch_bounds__(256) void generate_kernel(FrameParams fp, const uin
   }
 }
 
+#ifndef NOVA_TRACE_WAVES
+#define NOVA_TRACE_WAVES 0  // waves per SIMD the traversal kernels are compiled for (0: compiler's choice)
+#endif
+#if NOVA_TRACE_WAVES > 0
+#define NOVA_TRACE_OCCUPANCY __attribute__((amdgpu_waves_per_eu(NOVA_TRACE_WAVES, NOVA_TRACE_WAVES)))
+#else
+#define NOVA_TRACE_OCCUPANCY
+#endif
+
 template <bool kCount>
-__global__ __launch_bounds__(kTraceBlock) void extend_kernel(TraceCtx tc, Queues q, int which) {
+__global__ __launch_bounds__(kTraceBlock) NOVA_TRACE_OCCUPANCY void extend_kernel(TraceCtx tc, Queues q, int which) {
   __shared__ uint32_t lds[kLdsStack * kTraceBlock];
   uint32_t nv = 0, nt = 0;
   const uint32_t n = q.count[which];

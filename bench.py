@@ -236,10 +236,10 @@ def main():
         n = args.copy_probe_mb * (1 << 20) // 4
         src = torch.ones(n, dtype=torch.float32, device="cuda")
         dst = torch.empty_like(src)
-        for _ in range(3):
-            dst.copy_(src)
+        for _ in range(3):  # an elementwise kernel of its own (NegFunctor): reads and writes copy_probe_mb MiB
+            torch.neg(src, out=dst)
         torch.cuda.synchronize()
-        log(f"[bench] copy probe: 3 x {args.copy_probe_mb} MiB read + written")
+        log(f"[bench] copy probe: 3 x torch.neg over {args.copy_probe_mb} MiB (read + written)")
         del src, dst
     t0 = time.perf_counter()
     sc = scenes.million_triangles(width=W, height=H)
