@@ -7,6 +7,23 @@
 
 namespace nova_hip {
 
+// Node and triangle records live in device global memory: loading them through the
+// global address space gives global_load instructions instead of flat ones.
+#ifdef __HIP_DEVICE_COMPILE__
+#define NOVA_GLOBAL __attribute__((address_space(1)))
+#else
+#define NOVA_GLOBAL
+#endif
+
+NOVA_HD TriMT load_tri(const TriMT *p) {
+  const NOVA_GLOBAL float4 *q = (const NOVA_GLOBAL float4 *)p;
+  TriMT t;
+  t.a = q[0];
+  t.b = q[1];
+  t.c = q[2];
+  return t;
+}
+
 struct TraceCtx {
   const Node4 *nodes;
   const TriMT *tris;
@@ -139,17 +156,18 @@ NOVA_HD int trace_closest(const TraceCtx &tc, Stack &stack, f3 o, f3 d, float tn
   // A tree walk visits each node once; the cap only guards against a malformed tree.
   for (uint32_t visits = 0; visits < (1u << 24); visits++) {
     if (kCount) nodes_visited++;
-    const float4 *np = reinterpret_cast<const float4 *>(tc.nodes + node);
+    const NOVA_GLOBAL float4 *np = (const NOVA_GLOBAL float4 *)(tc.nodes + node);
     const float4 mnx = np[0], mxx = np[1], mny = np[2], mxy = np[3], mnz = np[4], mxz = np[5];
-    const int4 ch = *reinterpret_cast<const int4 *>(np + 6);
-    const int4 cn = *reinterpret_cast<const int4 *>(np + 7);
+    const int4 ch = *(const NOVA_GLOBAL int4 *)(np + 6);
+    const int4 cn = *(const NOVA_GLOBAL int4 *)(np + 7);
     const float bmnx[4] = {mnx.x, mnx.y, mnx.z, mnx.w}, bmxx[4] = {mxx.x, mxx.y, mxx.z, mxx.w};
     const float bmny[4] = {mny.x, mny.y, mny.z, mny.w}, bmxy[4] = {mxy.x, mxy.y, mxy.z, mxy.w};
     const float bmnz[4] = {mnz.x, mnz.y, mnz.z, mnz.w}, bmxz[4] = {mxz.x, mxz.y, mxz.z, mxz.w};
     const int chv[4] = {ch.x, ch.y, ch.z, ch.w}, cnv[4] = {cn.x, cn.y, cn.z, cn.w};
-    float tent[4];
-    int ord[4];
-    int nhit = 0;
+    // per child: entry distance (+inf when culled), child reference and triangle count
+    float key[4];
+    int32_t cref[4], ccnt[4];
+#pragma unroll
     for (int k = 0; k < 4; k++) {
       float tx1 = (bmnx[k] - o.x) * inv.x, tx2 = (bmxx[k] - o.x) * inv.x;
       float ty1 = (bmny[k] - o.y) * inv.y, ty2 = (bmxy[k] - o.y) * inv.y;
@@ -171,39 +189,46 @@ NOVA_HD int trace_closest(const TraceCtx &tc, Stack &stack, f3 o, f3 d, float tn
           tz2 = in ? kInf : -kInf;
         }
       }
-      float t0 = fmaxf(fmaxf(fminf(tx1, tx2), fminf(ty1, ty2)), fminf(tz1, tz2));
-      float t1 = fminf(fminf(fmaxf(tx1, tx2), fmaxf(ty1, ty2)), fmaxf(tz1, tz2)) * kUp;
-      bool hit = chv[k] >= 0 && t0 <= t1 && t1 >= 0.f && t0 <= tbest * kUp;
-      if (hit) {
-        tent[nhit] = t0;
-        ord[nhit] = k;
-        nhit++;
-      }
+      const float t0 = fmaxf(fmaxf(fminf(tx1, tx2), fminf(ty1, ty2)), fminf(tz1, tz2));
+      const float t1 = fminf(fminf(fmaxf(tx1, tx2), fmaxf(ty1, ty2)), fmaxf(tz1, tz2)) * kUp;
+      const bool hit = chv[k] >= 0 && t0 <= t1 && t1 >= 0.f && t0 <= tbest * kUp;
+      key[k] = hit ? t0 : kInf;
+      cref[k] = hit ? chv[k] : -1;
+      ccnt[k] = cnv[k];
     }
-    // near-to-far order of the hit children
-    for (int a = 1; a < nhit; a++)
-      for (int b = a; b > 0 && tent[b] < tent[b - 1]; b--) {
-        float tf = tent[b];
-        tent[b] = tent[b - 1];
-        tent[b - 1] = tf;
-        int ti = ord[b];
-        ord[b] = ord[b - 1];
-        ord[b - 1] = ti;
-      }
+    // near-to-far order of the four children: a fixed 5-exchange sorting network, so the
+    // arrays stay in registers (culled children carry +inf and end last)
+    auto exch = [&](int i, int j) {
+      const bool sw = key[j] < key[i];
+      const float kf = sw ? key[j] : key[i], kl = sw ? key[i] : key[j];
+      const int32_t rf = sw ? cref[j] : cref[i], rl = sw ? cref[i] : cref[j];
+      const int32_t nf = sw ? ccnt[j] : ccnt[i], nl = sw ? ccnt[i] : ccnt[j];
+      key[i] = kf;
+      key[j] = kl;
+      cref[i] = rf;
+      cref[j] = rl;
+      ccnt[i] = nf;
+      ccnt[j] = nl;
+    };
+    exch(0, 1);
+    exch(2, 3);
+    exch(0, 2);
+    exch(1, 3);
+    exch(1, 2);
     int32_t next = -1;
-    for (int a = nhit - 1; a >= 0; a--) {  // far to near: push inner nodes, keep the nearest
-      const int k = ord[a];
-      if (cnv[k] > 0) continue;
+#pragma unroll
+    for (int a = 3; a >= 0; a--) {  // far to near: push inner nodes, keep the nearest
+      if (cref[a] < 0 || ccnt[a] > 0) continue;
       if (next >= 0) stack.push(sp++, (uint32_t)next);
-      next = chv[k];
+      next = cref[a];
     }
-    for (int a = 0; a < nhit; a++) {  // leaves now, near to far
-      const int k = ord[a];
-      if (cnv[k] == 0) continue;
-      if (tent[a] > tbest * kUp) continue;
-      const int32_t start = chv[k], cnt = cnv[k];
+#pragma unroll
+    for (int a = 0; a < 4; a++) {  // leaves now, near to far
+      if (cref[a] < 0 || ccnt[a] == 0) continue;
+      if (key[a] > tbest * kUp) continue;
+      const int32_t start = cref[a], cnt = ccnt[a];
       for (int32_t i = start; i < start + cnt; i++) {
-        const TriMT tr = tc.tris[i];
+        const TriMT tr = load_tri(tc.tris + i);
         if (kCount) tri_tests++;
         float tt, uu, vv;
         if (mt_test(tr, o, d, tnear, tbest, tt, uu, vv) && (tt < tbest || best < 0 || reference_tie_winner_is_new(tc, i, bpos, o, d, tt))) {
@@ -229,7 +254,7 @@ NOVA_HD int trace_closest(const TraceCtx &tc, Stack &stack, f3 o, f3 d, float tn
 }
 
 #ifndef NOVA_LDS_STACK
-#define NOVA_LDS_STACK 24
+#define NOVA_LDS_STACK 16
 #endif
 constexpr int kTraceBlock = 256;
 constexpr int kLdsStack = NOVA_LDS_STACK;  // traversal stack levels held in LDS per lane
