@@ -52,8 +52,6 @@ struct BatchState {
   uint32_t wave;
   uint32_t count[2];  // queue sizes
   uint32_t do_accum;
-  uint32_t work;      // extend work counter
-  uint32_t pad;
   unsigned long long segments;  // rays traced
 };
 
@@ -106,7 +104,6 @@ __global__ __launch_bounds__(256) void batch_generate(const BatchScene *__restri
   if (blockIdx.x == 0 && threadIdx.x == 0) {
     st->count[0] = w.count;
     st->count[1] = 0;
-    st->work = 0;
   }
 }
 
@@ -118,14 +115,7 @@ __global__ __launch_bounds__(kTraceBlock) void batch_extend(const BatchScene *__
   const size_t cap = q.cap;
   const uint32_t gtid = blockIdx.x * blockDim.x + threadIdx.x;
   if (gtid == 0) st->segments += n;
-  const uint32_t lane = threadIdx.x & 63;
-  for (;;) {  // 64 rays per wave at a time from the work counter
-    uint32_t base = 0;
-    if (lane == 0) base = atomicAdd(&st->work, 64u);
-    base = __shfl(base, 0, 64);
-    if (base >= n) break;
-    const uint32_t i = base + lane;
-    if (i >= n) continue;
+  for (uint32_t i = gtid; i < n; i += gridDim.x * blockDim.x) {
     const f3 o = mk3(r[i], r[cap + i], r[2 * cap + i]);
     const f3 d = mk3(r[3 * cap + i], r[4 * cap + i], r[5 * cap + i]);
     const TraceCtx tc = scenes[q.sid[which][i]].tc;
@@ -140,10 +130,7 @@ __global__ __launch_bounds__(kTraceBlock) void batch_extend(const BatchScene *__
   }
 }
 
-__global__ void batch_reset(BatchState *st, int which) {
-  st->count[which] = 0;
-  st->work = 0;
-}
+__global__ void batch_reset(BatchState *st, int which) { st->count[which] = 0; }
 
 __global__ __launch_bounds__(256) void batch_shade(const BatchScene *__restrict__ scenes, const WaveDesc *__restrict__ waves, BatchState *st,
                                                    const uint32_t *__restrict__ sobol, BatchQueues q, int which, int bounce, int last, int rr,
@@ -251,7 +238,6 @@ __global__ void batch_set(BatchState *st, uint32_t wave, uint32_t do_accum) {
   st->wave = wave;
   st->do_accum = do_accum;
   st->count[0] = st->count[1] = 0;
-  st->work = 0;
   st->segments = 0;
 }
 

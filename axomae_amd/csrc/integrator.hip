@@ -28,8 +28,6 @@ namespace nova_hip {
 
 // ------------------------------------------------------------------ kernels
 
-constexpr int kWorkCounter = 4;  // count[] slot of the extend work counter
-
 struct Queues {
   float *ray[2];      // 6 planes of cap floats each: ox oy oz dx dy dz
   int32_t *slot[2];
@@ -67,7 +65,6 @@ __global__ __launch_bounds__(256) void generate_kernel(FrameParams fp, const uin
   if (blockIdx.x == 0 && threadIdx.x == 0) {
     q.count[0] = active;
     q.count[1] = 0;
-    q.count[kWorkCounter] = 0;
   }
 }
 
@@ -89,17 +86,7 @@ __global__ __launch_bounds__(kTraceBlock) NOVA_TRACE_OCCUPANCY void extend_kerne
   const size_t cap = q.cap;
   const uint32_t gtid = blockIdx.x * blockDim.x + threadIdx.x;
   if (gtid == 0) q.count[2] += n;
-  // Rays are handed out 64 at a time from a counter (count[kWorkCounter], zeroed before every
-  // launch), so a wave that finishes early takes more work instead of idling while ray
-  // costs vary by orders of magnitude.
-  const uint32_t lane = threadIdx.x & 63;
-  for (;;) {
-    uint32_t base = 0;
-    if (lane == 0) base = atomicAdd(&q.count[kWorkCounter], 64u);
-    base = __shfl(base, 0, 64);
-    if (base >= n) break;
-    const uint32_t i = base + lane;
-    if (i >= n) continue;
+  for (uint32_t i = gtid; i < n; i += gridDim.x * blockDim.x) {
     const f3 o = mk3(r[i], r[cap + i], r[2 * cap + i]);
     const f3 d = mk3(r[3 * cap + i], r[4 * cap + i], r[5 * cap + i]);
     float t, u, v;
@@ -186,12 +173,7 @@ __global__ __launch_bounds__(256) void shade_kernel(ShadeParams sp, ShadeCtx sc,
   }
 }
 
-// Between an extend and a shade launch: the shade's output queue and the work counter
-// of the next extend start at zero.
-__global__ void reset_count_kernel(uint32_t *count, int which) {
-  count[which] = 0;
-  count[kWorkCounter] = 0;
-}
+__global__ void reset_count_kernel(uint32_t *count, int which) { count[which] = 0; }
 
 // Engine::writeFramebuffer per frame, in frame order (engine.cpp:351-363)
 __global__ void accumulate_kernel(const float *__restrict__ partial, int frames, int frame0, size_t npix4, float *__restrict__ accum,
