@@ -135,9 +135,9 @@ NOVA_HD bool shade_path(const ShadeParams &sp, const ShadeCtx &sc, const uint32_
     A = A + env.w;
     return false;
   }
-  const ShadeTri tr = sc.shade[prim];
+  const ShadeTri &tr = sc.shade[prim];
   const HitRec h = make_hit(sc, tr, o, d, ht, hu, hv);
-  const MatRec mat = sc.mats[tr.material];
+  const MatRec &mat = sc.mats[tr.material];
   const float ONE_255 = 1.f / 255.f;
   // PrincipledMaterial::scatter (NovaMaterials.h:89-137)
   f3 nrm = mk3(0.f, 0.f, 1.f);
