@@ -11,7 +11,7 @@ import pytest
 
 from axomae_amd import scenes
 from conftest import rel_rmse
-from scene_cases import BVH_SCENES, hit_test_rays
+from scene_cases import BVH_SCENES, hit_test_rays, soup_scene
 
 gpu = pytest.mark.gpu
 
@@ -164,3 +164,21 @@ def test_bake_batch_one_launch_matches_cpu_bake(oracle):
     for k, (o, ref) in enumerate(zip(outs, refs)):
         err = rel_rmse(o, ref())
         assert err <= 1e-3, (k, jobs[k][1:], err)
+
+
+@gpu
+@pytest.mark.parametrize("name", ["soup_200k", "million_triangles"])
+def test_bvh_primitive_order_bit_exact_large(oracle, engine, name):
+    """Scenes whose upper levels run the many-workgroup node path (nodes above 16,384
+    primitives): the same prim_idx, node count, leaves and boxes as the reference builder."""
+    sc = soup_scene(200000, 21, lo=-20.0, hi=20.0, size=0.2) if name == "soup_200k" else scenes.million_triangles()
+    engine.upload(sc)
+    info = engine.build_bvh()
+    print(f"[bvh] {name}: {sc.triangle_count} triangles, build {info.build_ms:.1f} ms", flush=True)
+    prim, nodes = engine.bvh_arrays()
+    oprim, onodes = oracle.OracleScene(sc).bvh_arrays()
+    np.testing.assert_array_equal(prim, oprim)
+    assert info.binary_nodes == len(onodes)
+    assert _leaf_set(nodes) == _leaf_set(onodes)
+    key = lambda arr: sorted(tuple(np.concatenate([a["min"], a["max"]]).tolist()) + (int(a["count"]),) for a in arr)
+    assert key(nodes) == key(onodes)
