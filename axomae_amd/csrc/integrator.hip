@@ -110,8 +110,17 @@ __device__ __forceinline__ void write_partial(const Queues &q, const FrameParams
   *dst = make_float4(lx, ly, lz, a);
 }
 
+#ifndef NOVA_SHADE_WAVES
+#define NOVA_SHADE_WAVES 0  // waves per SIMD the shade kernel is compiled for (0: compiler's choice)
+#endif
+#if NOVA_SHADE_WAVES > 0
+#define NOVA_SHADE_OCCUPANCY __attribute__((amdgpu_waves_per_eu(NOVA_SHADE_WAVES, NOVA_SHADE_WAVES)))
+#else
+#define NOVA_SHADE_OCCUPANCY
+#endif
+
 template <bool kAdj>
-__global__ __launch_bounds__(256) void shade_kernel(ShadeParams sp, ShadeCtx sc, const uint32_t *__restrict__ sobol, Queues q, int which,
+__global__ __launch_bounds__(256) NOVA_SHADE_OCCUPANCY void shade_kernel(ShadeParams sp, ShadeCtx sc, const uint32_t *__restrict__ sobol, Queues q, int which,
                                                     AdjointCtx adj) {
   const uint32_t n = q.count[which];
   const size_t cap = q.cap;
