@@ -135,6 +135,10 @@ __global__ void batch_reset(BatchState *st, int which) { st->count[which] = 0; }
 __global__ __launch_bounds__(256) void batch_shade(const BatchScene *__restrict__ scenes, const WaveDesc *__restrict__ waves, BatchState *st,
                                                    const uint32_t *__restrict__ sobol, BatchQueues q, int which, int bounce, int last, int rr,
                                                    uint32_t dim0) {
+  __shared__ uint32_t sob_tab[5 * 128];  // nibble tables of dims dim0 .. dim0+4
+  sobol_nibble_fill(sobol, dim0, 5, sob_tab, threadIdx.x, blockDim.x);
+  __syncthreads();
+  const SobolNibbles sobn{sob_tab, dim0};
   const uint32_t n = st->count[which];
   const uint64_t wstart = waves[st->wave].start;
   const size_t cap = q.cap;
@@ -163,7 +167,7 @@ __global__ __launch_bounds__(256) void batch_shade(const BatchScene *__restrict_
       sp.rr = rr;
       sp.dim0 = dim0;
       const uint32_t l = (uint32_t)(wstart + (uint64_t)slot - S.first);
-      cont = shade_path(sp, S.sc, sobol, l, o, d, q.hit_prim[i], q.hit_t[i], q.hit_u[i], q.hit_v[i], T, L, A, no, nd, escaped);
+      cont = shade_path(sp, S.sc, sobn, l, o, d, q.hit_prim[i], q.hit_t[i], q.hit_u[i], q.hit_v[i], T, L, A, no, nd, escaped);
       q.T[slot] = T.x;
       q.T[cap + slot] = T.y;
       q.T[2 * cap + slot] = T.z;

@@ -36,6 +36,37 @@ NOVA_HD uint32_t sample_index(const FrameParams &fp, uint32_t slot) {
 
 NOVA_HD float sobol_dim(const uint32_t *table, uint32_t dim, uint32_t gray) { return u32_to_unit(owen_scramble(sobol_xor(table + (size_t)dim * 32, gray))); }
 
+// The same samples from per-nibble tables of a few consecutive dimensions: entry
+// [(dim - dim0) * 128 + k * 16 + v] is the XOR of the direction numbers of dimension dim
+// selected by the bits of nibble value v at bits 4k..4k+3, so a sample is 8 table reads
+// instead of 32 conditional XORs (XOR is order-free: the bits are identical).
+struct SobolNibbles {
+  const uint32_t *tab;
+  uint32_t dim0;
+};
+NOVA_HD uint32_t sobol_nibble_xor(const uint32_t *t, uint32_t gray) {
+  uint32_t acc = 0;
+#pragma unroll
+  for (int k = 0; k < 8; k++) acc ^= t[k * 16 + ((gray >> (4 * k)) & 15u)];
+  return acc;
+}
+NOVA_HD float sobol_dim(const SobolNibbles &s, uint32_t dim, uint32_t gray) {
+  return u32_to_unit(owen_scramble(sobol_nibble_xor(s.tab + (size_t)(dim - s.dim0) * 128, gray)));
+}
+// Fills the nibble tables of dims [dim0, dim0 + ndims) (entries of dims >= 256 are 0);
+// entries e = first, first + step, ... (one thread's share).
+NOVA_HD void sobol_nibble_fill(const uint32_t *table, uint32_t dim0, int ndims, uint32_t *out, int first, int step) {
+  for (int e = first; e < ndims * 128; e += step) {
+    const uint32_t dim = dim0 + (uint32_t)(e / 128);
+    const int k = (e % 128) / 16, v = e % 16;
+    uint32_t x = 0;
+    if (dim < 256)
+      for (int i = 0; i < 4; i++)
+        if ((v >> i) & 1) x ^= table[(size_t)dim * 32 + 4 * k + i];
+    out[e] = x;
+  }
+}
+
 // Camera ray of a path slot: jitter from dims 0-1, screen2ndc(x, H - y) and ray_inv_mat
 // (Integrator.h:80-88, math_camera.h:28-58) in glm's operation order.
 NOVA_HD void camera_ray(const FrameParams &fp, const uint32_t *sobol, uint32_t slot, f3 &o, f3 &d) {
@@ -124,8 +155,8 @@ NOVA_HD void splat_texture_grad(const ShadeCtx &sc, float *grad, int tex, float 
 // R = L_path - L (everything after this vertex, T c V_next), gives the loss gradient of
 // the vertex's weight c; it is pushed to the albedo and roughness texels through
 // d ln c / d(param) and the bilinear weights.
-template <bool kAdj = false>
-NOVA_HD bool shade_path(const ShadeParams &sp, const ShadeCtx &sc, const uint32_t *sobol, uint32_t slot, f3 o, f3 d, int prim, float ht, float hu,
+template <bool kAdj = false, class Sobol = const uint32_t *>
+NOVA_HD bool shade_path(const ShadeParams &sp, const ShadeCtx &sc, Sobol sobol, uint32_t slot, f3 o, f3 d, int prim, float ht, float hu,
                         float hv, f3 &T, f3 &L, float &A, f3 &no, f3 &nd, bool &escaped, const AdjointCtx *adj = nullptr) {
   escaped = false;
   if (prim < 0) {  // Li on a miss: envmap(ray.direction) (Integrator.cpp:93-97)

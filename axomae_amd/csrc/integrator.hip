@@ -122,6 +122,11 @@ __device__ __forceinline__ void write_partial(const Queues &q, const FrameParams
 template <bool kAdj>
 __global__ __launch_bounds__(256) NOVA_SHADE_OCCUPANCY void shade_kernel(ShadeParams sp, ShadeCtx sc, const uint32_t *__restrict__ sobol, Queues q, int which,
                                                     AdjointCtx adj) {
+  // nibble tables of this scatter's sampler dimensions (dim0 .. dim0+4)
+  __shared__ uint32_t sob_tab[5 * 128];
+  sobol_nibble_fill(sobol, sp.dim0, 5, sob_tab, threadIdx.x, blockDim.x);
+  __syncthreads();
+  const SobolNibbles sobn{sob_tab, sp.dim0};
   const uint32_t n = q.count[which];
   const size_t cap = q.cap;
   const float *rin = q.ray[which];
@@ -143,7 +148,7 @@ __global__ __launch_bounds__(256) NOVA_SHADE_OCCUPANCY void shade_kernel(ShadePa
       f3 L = mk3(q.L[slot], q.L[cap + slot], q.L[2 * cap + slot]);
       float A = q.A[slot];
       const int prim = q.hit_prim[i];
-      cont = shade_path<kAdj>(sp, sc, sobol, (uint32_t)slot, o, d, prim, q.hit_t[i], q.hit_u[i], q.hit_v[i], T, L, A, no, nd, escaped, &adj);
+      cont = shade_path<kAdj>(sp, sc, sobn, (uint32_t)slot, o, d, prim, q.hit_t[i], q.hit_u[i], q.hit_v[i], T, L, A, no, nd, escaped, &adj);
       const bool done = !cont;
       q.T[slot] = T.x;
       q.T[cap + slot] = T.y;

@@ -279,3 +279,20 @@ extern "C" int emu_render(const nova_hip_scene_desc *sd, const nova_hip_render_p
 extern "C" int emu_adjoint(const nova_hip_scene_desc *sd, const nova_hip_render_params *p, const float *d_image, float scale, float *grad) {
   return emu_run(sd, p, nullptr, nullptr, d_image, scale, grad);
 }
+
+// Sobol samples of dims [dim0, dim0+5) for the given indices, from the direction table and
+// from the shade kernels' per-nibble tables (out_direct, out_nibble: n x 5 floats).
+extern "C" int emu_sobol_nibble(uint32_t dim0, const uint32_t *idx, int n, float *out_direct, float *out_nibble) {
+  uint32_t tab[5 * 128];
+  sobol_nibble_fill(&kTable[0][0], dim0, 5, tab, 0, 1);
+  const SobolNibbles s{tab, dim0};
+  for (int i = 0; i < n; i++) {
+    const uint32_t g = idx[i] ^ (idx[i] >> 1);
+    for (uint32_t d = 0; d < 5; d++) {
+      if (dim0 + d >= 256) continue;
+      out_direct[i * 5 + d] = sobol_dim(&kTable[0][0], dim0 + d, g);
+      out_nibble[i * 5 + d] = sobol_dim(s, dim0 + d, g);
+    }
+  }
+  return 0;
+}

@@ -116,3 +116,20 @@ def test_device_traversal_matches_reference_traversal(oracle, emu, name):
     print(f"[emu-hit] {name}: {len(rays)} rays, {len(bad)} mismatches, nodes/ray {counts[0] / len(rays):.1f}", flush=True)
     assert len(bad) == 0, [(int(k), int(prim[k]), int(oprim[k]), tuv[k].tolist(), otuv[k].tolist()) for k in bad[:8]]
     np.testing.assert_array_equal(tuv, otuv)
+
+
+def test_sobol_nibble_tables_match_direction_table(emu, oracle):
+    """The shade kernels read Sobol samples from per-nibble tables of the launch's five
+    dimensions; they give the direction-table (and oracle) samples bit for bit."""
+    rng = np.random.default_rng(9)
+    idx = np.concatenate([[0, 1, 2, 3, 0xFFFFFFFF, 0x80000000], rng.integers(0, 2 ** 32, 4000, dtype=np.uint64)]).astype(np.uint32)
+    emu.emu_sobol_nibble.argtypes = [C.c_uint32, C.c_void_p, C.c_int, C.c_void_p, C.c_void_p]
+    for dim0 in (2, 6, 33, 120, 251, 253):
+        a = np.full((len(idx), 5), np.nan, dtype=np.float32)
+        b = np.full((len(idx), 5), np.nan, dtype=np.float32)
+        assert emu.emu_sobol_nibble(dim0, idx.ctypes.data, len(idx), a.ctypes.data, b.ctypes.data) == 0
+        np.testing.assert_array_equal(a, b)
+        dims = [d for d in range(dim0, dim0 + 5) if d < 256]
+        I, Dm = np.meshgrid(idx, np.array(dims, dtype=np.uint32), indexing="ij")
+        ref = oracle.sobol_float(I.ravel(), Dm.ravel()).reshape(len(idx), len(dims))
+        np.testing.assert_array_equal(b[:, :len(dims)], ref)
