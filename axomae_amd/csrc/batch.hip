@@ -542,7 +542,7 @@ extern "C" int nova_hip_batch_autotune(nova_hip_ctx *ex, const nova_hip_batch_it
   if ((st = executor_stream(ex, (hipStream_t)stream, &s))) return st;
   const int cus = device_cus_of(ex->device);
   const uint32_t *sobol = ex->sobol_table;
-  const uint32_t caps[] = {1u << 20, 1u << 21, 1u << 22};
+  const uint32_t caps[] = {1u << 21, 1u << 22, 1u << 23, 1u << 24};
   const uint32_t trace_occ[] = {4, 8};
   const uint32_t shade_occ[] = {4, 8};
   // every candidate runs the same slots: one wave from the middle of the batch

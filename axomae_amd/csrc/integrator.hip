@@ -224,6 +224,11 @@ __global__ __launch_bounds__(kTraceBlock) void closest_hit_kernel(TraceCtx tc, c
   }
 }
 
+#ifndef NOVA_PATHS_PER_BATCH
+#define NOVA_PATHS_PER_BATCH (32u << 20)
+#endif
+constexpr size_t kPathsPerBatch = NOVA_PATHS_PER_BATCH;
+
 static int device_cus(nova_hip_ctx *ctx) {
   int cus = 0;
   if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, ctx->device) != hipSuccess || cus <= 0) cus = 256;
@@ -285,8 +290,10 @@ static int render_impl(nova_hip_ctx *ctx, const nova_hip_render_params *p, float
   hipStream_t s = (hipStream_t)stream;
   NOVA_HIP_CHECK(ctx, hipSetDevice(ctx->device));
   const size_t npix = (size_t)p->width * p->height;
-  // frames per batch: enough paths to fill the chip, bounded by memory
-  int fb = (int)std::max<size_t>(1, std::min<size_t>((size_t)p->frame_count, (4u << 20) / npix));
+  // Frames per batch: up to kPathsPerBatch paths in flight (about 116 B of queue and path
+  // state each, 3.9 GB at the limit). Large batches matter because the last bounces hold
+  // few rays whose launches cost their slowest ray's latency, a fixed cost per batch.
+  int fb = (int)std::max<size_t>(1, std::min<size_t>((size_t)p->frame_count, kPathsPerBatch / npix));
   const size_t cap = npix * fb;
   if (cap >= (1u << 31)) return set_error(ctx, NOVA_HIP_INVALID_ARGUMENT, "image too large");
   hipError_t e;
