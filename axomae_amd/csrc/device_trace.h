@@ -132,31 +132,47 @@ inline
   return new_later ? reached : !reached;
 }
 
-// Closest hit over the 4-wide BVH. Boxes are tested with a slightly widened slab so no
-// box holding a hit the reference's division-form test accepts is culled; triangle
-// tests are exact, and exact equal-t ties are settled in the reference's visiting order.
-// Stack provides push(level, node) and at(level) for the traversal stack.
-template <bool kCount, class Stack>
-NOVA_HD int trace_closest(const TraceCtx &tc, Stack &stack, f3 o, f3 d, float tnear, float &t_out, float &u_out, float &v_out,
-                          uint32_t &nodes_visited, uint32_t &tri_tests) {
-  const float kUp = 1.000001f;
+// Per-ray traversal state, so that a lane can be handed a new ray as soon as its ray ends.
+struct TraceState {
+  f3 o, d, inv;
+  bool px, py, pz, parallel;
+  float tbest, bu, bv;
+  int best;
+  int32_t bpos, node;
+  int sp;
+};
+
+NOVA_HD void trace_init(TraceState &st, f3 o, f3 d) {
   const float kInf = __builtin_inff();
-  const f3 inv = mk3(1.f / d.x, 1.f / d.y, 1.f / d.z);
+  st.o = o;
+  st.d = d;
+  st.inv = mk3(1.f / d.x, 1.f / d.y, 1.f / d.z);
   // An axis whose reciprocal is infinite (zero or denormal direction component) is a
   // containment test instead: inside the slab (boundary included) the whole line, else
   // nothing -- what the reference's 0/0 slabs amount to in the oracle's reading.
-  const bool px = !(fabsf(inv.x) < kInf), py = !(fabsf(inv.y) < kInf), pz = !(fabsf(inv.z) < kInf);
-  const bool parallel = px || py || pz;
-  float tbest = 3.402823466e+38f;
-  int best = -1;
-  int32_t bpos = -1;
-  float bu = 0.f, bv = 0.f;
-  int sp = 0;
-  int32_t node = 0;
-  // A tree walk visits each node once; the cap only guards against a malformed tree.
-  for (uint32_t visits = 0; visits < (1u << 24); visits++) {
+  st.px = !(fabsf(st.inv.x) < kInf);
+  st.py = !(fabsf(st.inv.y) < kInf);
+  st.pz = !(fabsf(st.inv.z) < kInf);
+  st.parallel = st.px || st.py || st.pz;
+  st.tbest = 3.402823466e+38f;
+  st.best = -1;
+  st.bpos = -1;
+  st.bu = st.bv = 0.f;
+  st.node = 0;
+  st.sp = 0;
+}
+
+// One node of the walk: test the node's four children with a slightly widened slab (no box
+// holding a hit the reference's division-form test accepts is culled), order them near to
+// far, push the inner ones, intersect the leaves (exact Moller-Trumbore; exact equal-t ties
+// settled in the reference's visiting order). Returns true when the walk is over; the hit is
+// then st.best / st.tbest / st.bu / st.bv. Stack provides push(level, node) and at(level).
+template <bool kCount, class Stack>
+NOVA_HD bool trace_step(const TraceCtx &tc, Stack &stack, TraceState &st, float tnear, uint32_t &nodes_visited, uint32_t &tri_tests) {
+  const float kUp = 1.000001f;
+  const float kInf = __builtin_inff();
     if (kCount) nodes_visited++;
-    const NOVA_GLOBAL float4 *np = (const NOVA_GLOBAL float4 *)(tc.nodes + node);
+    const NOVA_GLOBAL float4 *np = (const NOVA_GLOBAL float4 *)(tc.nodes + st.node);
     const float4 mnx = np[0], mxx = np[1], mny = np[2], mxy = np[3], mnz = np[4], mxz = np[5];
     const int4 ch = *(const NOVA_GLOBAL int4 *)(np + 6);
     const int4 cn = *(const NOVA_GLOBAL int4 *)(np + 7);
@@ -169,29 +185,29 @@ NOVA_HD int trace_closest(const TraceCtx &tc, Stack &stack, f3 o, f3 d, float tn
     int32_t cref[4], ccnt[4];
 #pragma unroll
     for (int k = 0; k < 4; k++) {
-      float tx1 = (bmnx[k] - o.x) * inv.x, tx2 = (bmxx[k] - o.x) * inv.x;
-      float ty1 = (bmny[k] - o.y) * inv.y, ty2 = (bmxy[k] - o.y) * inv.y;
-      float tz1 = (bmnz[k] - o.z) * inv.z, tz2 = (bmxz[k] - o.z) * inv.z;
-      if (parallel) {
-        if (px) {
-          const bool in = bmnx[k] <= o.x && o.x <= bmxx[k];
+      float tx1 = (bmnx[k] - st.o.x) * st.inv.x, tx2 = (bmxx[k] - st.o.x) * st.inv.x;
+      float ty1 = (bmny[k] - st.o.y) * st.inv.y, ty2 = (bmxy[k] - st.o.y) * st.inv.y;
+      float tz1 = (bmnz[k] - st.o.z) * st.inv.z, tz2 = (bmxz[k] - st.o.z) * st.inv.z;
+      if (st.parallel) {
+        if (st.px) {
+          const bool in = bmnx[k] <= st.o.x && st.o.x <= bmxx[k];
           tx1 = -kInf;
           tx2 = in ? kInf : -kInf;
         }
-        if (py) {
-          const bool in = bmny[k] <= o.y && o.y <= bmxy[k];
+        if (st.py) {
+          const bool in = bmny[k] <= st.o.y && st.o.y <= bmxy[k];
           ty1 = -kInf;
           ty2 = in ? kInf : -kInf;
         }
-        if (pz) {
-          const bool in = bmnz[k] <= o.z && o.z <= bmxz[k];
+        if (st.pz) {
+          const bool in = bmnz[k] <= st.o.z && st.o.z <= bmxz[k];
           tz1 = -kInf;
           tz2 = in ? kInf : -kInf;
         }
       }
       const float t0 = fmaxf(fmaxf(fminf(tx1, tx2), fminf(ty1, ty2)), fminf(tz1, tz2));
       const float t1 = fminf(fminf(fmaxf(tx1, tx2), fmaxf(ty1, ty2)), fmaxf(tz1, tz2)) * kUp;
-      const bool hit = chv[k] >= 0 && t0 <= t1 && t1 >= 0.f && t0 <= tbest * kUp;
+      const bool hit = chv[k] >= 0 && t0 <= t1 && t1 >= 0.f && t0 <= st.tbest * kUp;
       key[k] = hit ? t0 : kInf;
       cref[k] = hit ? chv[k] : -1;
       ccnt[k] = cnv[k];
@@ -219,38 +235,49 @@ NOVA_HD int trace_closest(const TraceCtx &tc, Stack &stack, f3 o, f3 d, float tn
 #pragma unroll
     for (int a = 3; a >= 0; a--) {  // far to near: push inner nodes, keep the nearest
       if (cref[a] < 0 || ccnt[a] > 0) continue;
-      if (next >= 0) stack.push(sp++, (uint32_t)next);
+      if (next >= 0) stack.push(st.sp++, (uint32_t)next);
       next = cref[a];
     }
 #pragma unroll
     for (int a = 0; a < 4; a++) {  // leaves now, near to far
       if (cref[a] < 0 || ccnt[a] == 0) continue;
-      if (key[a] > tbest * kUp) continue;
+      if (key[a] > st.tbest * kUp) continue;
       const int32_t start = cref[a], cnt = ccnt[a];
       for (int32_t i = start; i < start + cnt; i++) {
         const TriMT tr = load_tri(tc.tris + i);
         if (kCount) tri_tests++;
         float tt, uu, vv;
-        if (mt_test(tr, o, d, tnear, tbest, tt, uu, vv) && (tt < tbest || best < 0 || reference_tie_winner_is_new(tc, i, bpos, o, d, tt))) {
-          tbest = tt;
-          bpos = i;
-          best = __builtin_bit_cast(int32_t, tr.a.w);
-          bu = uu;
-          bv = vv;
+        if (mt_test(tr, st.o, st.d, tnear, st.tbest, tt, uu, vv) && (tt < st.tbest || st.best < 0 || reference_tie_winner_is_new(tc, i, st.bpos, st.o, st.d, tt))) {
+          st.tbest = tt;
+          st.bpos = i;
+          st.best = __builtin_bit_cast(int32_t, tr.a.w);
+          st.bu = uu;
+          st.bv = vv;
         }
       }
     }
     if (next >= 0) {
-      node = next;
-      continue;
+      st.node = next;
+      return false;
     }
-    if (sp == 0) break;
-    node = (int32_t)stack.at(--sp);
-  }
-  t_out = tbest;
-  u_out = bu;
-  v_out = bv;
-  return best;
+    if (st.sp == 0) return true;
+    st.node = (int32_t)stack.at(--st.sp);
+    return false;
+}
+
+// Closest hit of one ray: the steps until the walk ends (each node is visited once; the cap
+// only guards against a malformed tree).
+template <bool kCount, class Stack>
+NOVA_HD int trace_closest(const TraceCtx &tc, Stack &stack, f3 o, f3 d, float tnear, float &t_out, float &u_out, float &v_out,
+                          uint32_t &nodes_visited, uint32_t &tri_tests) {
+  TraceState st;
+  trace_init(st, o, d);
+  for (uint32_t visits = 0; visits < (1u << 24); visits++)
+    if (trace_step<kCount>(tc, stack, st, tnear, nodes_visited, tri_tests)) break;
+  t_out = st.tbest;
+  u_out = st.bu;
+  v_out = st.bv;
+  return st.best;
 }
 
 #ifndef NOVA_LDS_STACK

@@ -28,6 +28,8 @@ namespace nova_hip {
 
 // ------------------------------------------------------------------ kernels
 
+constexpr int kWorkCounter = 4;  // count[] slot of the extend work counter, zeroed before each extend
+
 struct Queues {
   float *ray[2];      // 6 planes of cap floats each: ox oy oz dx dy dz
   int32_t *slot[2];
@@ -65,11 +67,18 @@ __global__ __launch_bounds__(256) void generate_kernel(FrameParams fp, const uin
   if (blockIdx.x == 0 && threadIdx.x == 0) {
     q.count[0] = active;
     q.count[1] = 0;
+    q.count[kWorkCounter] = 0;
   }
 }
 
+#ifndef NOVA_TRACE_REFILL
+#define NOVA_TRACE_REFILL 56  // idle lanes that trigger a refill in the extend kernel (0: grid-stride, one ray per lane at a time)
+#endif
+// the first turn fills every lane; later turns wait for NOVA_TRACE_REFILL idle lanes
+__device__ __forceinline__ bool have_any_fill(uint32_t turn) { return turn > 0; }
+
 #ifndef NOVA_TRACE_WAVES
-#define NOVA_TRACE_WAVES 0  // waves per SIMD the traversal kernels are compiled for (0: compiler's choice)
+#define NOVA_TRACE_WAVES 6  // waves per SIMD the extend kernel is compiled for (0: compiler's choice)
 #endif
 #if NOVA_TRACE_WAVES > 0
 #define NOVA_TRACE_OCCUPANCY __attribute__((amdgpu_waves_per_eu(NOVA_TRACE_WAVES, NOVA_TRACE_WAVES)))
@@ -86,17 +95,53 @@ __global__ __launch_bounds__(kTraceBlock) NOVA_TRACE_OCCUPANCY void extend_kerne
   const size_t cap = q.cap;
   const uint32_t gtid = blockIdx.x * blockDim.x + threadIdx.x;
   if (gtid == 0) q.count[2] += n;
+  LdsStack stk{lds, tc.gstack, tc.gstride, gtid, threadIdx.x};
+#if NOVA_TRACE_REFILL > 0
+  // One traversal step per loop turn; a lane whose ray has ended takes the next unclaimed
+  // ray as soon as NOVA_TRACE_REFILL lanes of its wave are idle (one atomic per refill for
+  // the wave), so lanes do not wait for the wave's longest ray.
+  const uint32_t lane = threadIdx.x & 63;
+  TraceState ts;
+  bool have = false, exhausted = false;
+  uint32_t ray = 0;
+  for (uint32_t turns = 0; turns < (1u << 30); turns++) {
+    const uint64_t idle = __ballot(!have);
+    if (!exhausted && __popcll(idle) >= (have_any_fill(turns) ? NOVA_TRACE_REFILL : 1)) {
+      const int leader = __builtin_ffsll((long long)idle) - 1;
+      uint32_t base = 0;
+      if ((int)lane == leader) base = atomicAdd(&q.count[kWorkCounter], (uint32_t)__popcll(idle));
+      base = __shfl(base, leader, 64);
+      if (base >= n) exhausted = true;
+      if (!have) {
+        const uint32_t i = base + __builtin_amdgcn_mbcnt_hi((uint32_t)(idle >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)idle, 0));
+        if (i < n) {
+          trace_init(ts, mk3(r[i], r[cap + i], r[2 * cap + i]), mk3(r[3 * cap + i], r[4 * cap + i], r[5 * cap + i]));
+          ray = i;
+          have = true;
+        }
+      }
+    }
+    if (__ballot(have) == 0) break;
+    if (have && trace_step<kCount>(tc, stk, ts, 0.000001f, nv, nt)) {
+      q.hit_prim[ray] = ts.best;
+      q.hit_t[ray] = ts.tbest;
+      q.hit_u[ray] = ts.bu;
+      q.hit_v[ray] = ts.bv;
+      have = false;
+    }
+  }
+#else
   for (uint32_t i = gtid; i < n; i += gridDim.x * blockDim.x) {
     const f3 o = mk3(r[i], r[cap + i], r[2 * cap + i]);
     const f3 d = mk3(r[3 * cap + i], r[4 * cap + i], r[5 * cap + i]);
     float t, u, v;
-    LdsStack st{lds, tc.gstack, tc.gstride, gtid, threadIdx.x};
-    const int prim = trace_closest<kCount>(tc, st, o, d, 0.000001f, t, u, v, nv, nt);
+    const int prim = trace_closest<kCount>(tc, stk, o, d, 0.000001f, t, u, v, nv, nt);
     q.hit_prim[i] = prim;
     q.hit_t[i] = t;
     q.hit_u[i] = u;
     q.hit_v[i] = v;
   }
+#endif
   if (kCount) {
     atomicAdd(&q.stats64[0], (unsigned long long)nv);
     atomicAdd(&q.stats64[1], (unsigned long long)nt);
@@ -187,7 +232,12 @@ __global__ __launch_bounds__(256) NOVA_SHADE_OCCUPANCY void shade_kernel(ShadePa
   }
 }
 
-__global__ void reset_count_kernel(uint32_t *count, int which) { count[which] = 0; }
+// Between an extend and a shade launch: the shade's output queue and the next extend's work
+// counter start at zero.
+__global__ void reset_count_kernel(uint32_t *count, int which) {
+  count[which] = 0;
+  count[kWorkCounter] = 0;
+}
 
 // Engine::writeFramebuffer per frame, in frame order (engine.cpp:351-363)
 __global__ void accumulate_kernel(const float *__restrict__ partial, int frames, int frame0, size_t npix4, float *__restrict__ accum,
