@@ -17,6 +17,7 @@
 
 #include "device_math.h"
 #include "device_path.h"
+#include "device_queue.h"
 #include "device_shading.h"
 #include "device_trace.h"
 #include "nova_internal.h"
@@ -51,6 +52,7 @@ struct Segment {
 struct BatchState {
   uint32_t wave;
   uint32_t count[2];  // queue sizes
+  uint32_t work;      // rays of the current queue claimed by the extend launch
   uint32_t do_accum;
   unsigned long long segments;  // rays traced
 };
@@ -104,33 +106,63 @@ __global__ __launch_bounds__(256) void batch_generate(const BatchScene *__restri
   if (blockIdx.x == 0 && threadIdx.x == 0) {
     st->count[0] = w.count;
     st->count[1] = 0;
+    st->work = 0;
   }
 }
 
-__global__ __launch_bounds__(kTraceBlock) void batch_extend(const BatchScene *__restrict__ scenes, BatchState *st, BatchQueues q, int which,
-                                                            uint32_t *gstack, uint32_t gstride) {
+// Closest hits of the queue with the single-scene kernel's lane refill: a lane whose ray
+// has ended takes the next unclaimed ray once kBatchRefill lanes of its wave are idle.
+constexpr uint32_t kBatchRefill = 56;
+__global__ __launch_bounds__(kTraceBlock) __attribute__((amdgpu_waves_per_eu(6, 6))) void batch_extend(
+    const BatchScene *__restrict__ scenes, BatchState *st, BatchQueues q, int which, uint32_t *gstack, uint32_t gstride) {
   __shared__ uint32_t lds[kLdsStack * kTraceBlock];
   const uint32_t n = st->count[which];
   const float *r = q.ray[which];
   const size_t cap = q.cap;
   const uint32_t gtid = blockIdx.x * blockDim.x + threadIdx.x;
   if (gtid == 0) st->segments += n;
-  for (uint32_t i = gtid; i < n; i += gridDim.x * blockDim.x) {
-    const f3 o = mk3(r[i], r[cap + i], r[2 * cap + i]);
-    const f3 d = mk3(r[3 * cap + i], r[4 * cap + i], r[5 * cap + i]);
-    const TraceCtx tc = scenes[q.sid[which][i]].tc;
-    float t, u, v;
-    uint32_t nv = 0, nt = 0;
-    LdsStack stk{lds, gstack, gstride, gtid, threadIdx.x};
-    const int prim = trace_closest<false>(tc, stk, o, d, 0.000001f, t, u, v, nv, nt);
-    q.hit_prim[i] = prim;
-    q.hit_t[i] = t;
-    q.hit_u[i] = u;
-    q.hit_v[i] = v;
+  LdsStack stk{lds, gstack, gstride, gtid, threadIdx.x};
+  const uint32_t lane = threadIdx.x & 63;
+  TraceState ts;
+  bool have = false, exhausted = false;
+  uint32_t ray = 0;
+  int32_t sid = 0;
+  uint32_t nv = 0, nt = 0;
+  for (uint32_t turns = 0; turns < (1u << 30); turns++) {
+    const uint64_t idle = __ballot(!have);
+    if (!exhausted && (uint32_t)__popcll(idle) >= (turns > 0 ? kBatchRefill : 1u)) {
+      const int leader = __builtin_ffsll((long long)idle) - 1;
+      uint32_t base = 0;
+      if ((int)lane == leader) base = atomicAdd(&st->work, (uint32_t)__popcll(idle));
+      base = __shfl(base, leader, 64);
+      if (base >= n) exhausted = true;
+      if (!have) {
+        const uint32_t i = base + __builtin_amdgcn_mbcnt_hi((uint32_t)(idle >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)idle, 0));
+        if (i < n) {
+          trace_init(ts, mk3(r[i], r[cap + i], r[2 * cap + i]), mk3(r[3 * cap + i], r[4 * cap + i], r[5 * cap + i]));
+          ray = i;
+          sid = q.sid[which][i];
+          have = true;
+        }
+      }
+    }
+    if (__ballot(have) == 0) break;
+    if (have && trace_step<false>(scenes[sid].tc, stk, ts, 0.000001f, nv, nt)) {
+      q.hit_prim[ray] = ts.best;
+      q.hit_t[ray] = ts.tbest;
+      q.hit_u[ray] = ts.bu;
+      q.hit_v[ray] = ts.bv;
+      have = false;
+    }
   }
 }
 
-__global__ void batch_reset(BatchState *st, int which) { st->count[which] = 0; }
+// Between an extend and a shade launch: the shade's output queue and the next extend's work
+// counter start at zero.
+__global__ void batch_reset(BatchState *st, int which) {
+  st->count[which] = 0;
+  st->work = 0;
+}
 
 __global__ __launch_bounds__(256) void batch_shade(const BatchScene *__restrict__ scenes, const WaveDesc *__restrict__ waves, BatchState *st,
                                                    const uint32_t *__restrict__ sobol, BatchQueues q, int which, int bounce, int last, int rr,
@@ -145,7 +177,8 @@ __global__ __launch_bounds__(256) void batch_shade(const BatchScene *__restrict_
   const float *rin = q.ray[which];
   float *rout = q.ray[which ^ 1];
   const uint32_t stride = gridDim.x * blockDim.x;
-  const uint32_t iters = (n + stride - 1) / stride;  // uniform per wave: the ballot below needs every lane
+  const uint32_t iters = (n + stride - 1) / stride;  // uniform over the grid: the compaction below needs every thread
+  __shared__ uint32_t wave_cnt[2][4], blk_base[2];
   for (uint32_t it = 0; it < iters; it++) {
     const uint32_t i = it * stride + blockIdx.x * blockDim.x + threadIdx.x;
     bool cont = false, escaped = false;
@@ -182,12 +215,7 @@ __global__ __launch_bounds__(256) void batch_shade(const BatchScene *__restrict_
     }
     const uint64_t m = __ballot(cont);
     const uint32_t lane_rank = __builtin_amdgcn_mbcnt_hi((uint32_t)(m >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)m, 0));
-    uint32_t base = 0;
-    const int first = __builtin_ffsll((long long)m) - 1;
-    if (m != 0) {
-      if ((int)(threadIdx.x & 63) == first) base = atomicAdd(&st->count[which ^ 1], (uint32_t)__popcll(m));
-      base = __shfl(base, first, 64);
-    }
+    const uint32_t base = block_reserve<256>(m, &st->count[which ^ 1], it, wave_cnt, blk_base);
     if (cont) {
       const uint32_t j = base + lane_rank;
       rout[j] = no.x;
@@ -242,6 +270,7 @@ __global__ void batch_set(BatchState *st, uint32_t wave, uint32_t do_accum) {
   st->wave = wave;
   st->do_accum = do_accum;
   st->count[0] = st->count[1] = 0;
+  st->work = 0;
   st->segments = 0;
 }
 
@@ -543,8 +572,16 @@ extern "C" int nova_hip_batch_autotune(nova_hip_ctx *ex, const nova_hip_batch_it
   const int cus = device_cus_of(ex->device);
   const uint32_t *sobol = ex->sobol_table;
   const uint32_t caps[] = {1u << 21, 1u << 22, 1u << 23, 1u << 24};
-  const uint32_t trace_occ[] = {4, 8};
-  const uint32_t shade_occ[] = {4, 8};
+  // workgroups per CU: the extend kernel balances its lanes through a work counter, so it
+  // takes what fits at once; the shade loop splits its queue evenly, so a second round of
+  // workgroups is a candidate against one round that fits
+  int occ_t = 0, occ_s = 0;
+  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ_t, reinterpret_cast<const void *>(batch_extend), kTraceBlock, 0) != hipSuccess || occ_t <= 0)
+    occ_t = 8;
+  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ_s, reinterpret_cast<const void *>(batch_shade), 256, 0) != hipSuccess || occ_s <= 0)
+    occ_s = 4;
+  const uint32_t trace_occ[] = {(uint32_t)occ_t};
+  const uint32_t shade_occ[] = {(uint32_t)occ_s, 2u * (uint32_t)occ_s};
   // every candidate runs the same slots: one wave from the middle of the batch
   const uint64_t mid = bp.total / 2;
   float best_t = -1.f;

@@ -232,7 +232,7 @@ NOVA_HD bool shade_path(const ShadeParams &sp, const ShadeCtx &sc, Sobol sobol, 
   }
   const float uc2 = sobol_dim(sobol, sp.dim0 + 3, g);
   // PrincipledBSDF::sample_f (BSDF.h:107-130)
-  BsdfSample smp;
+  BsdfSample smp{};
   bool ok = false;
   f3 dlna = mk3(0.f, 0.f, 0.f);
   float dlnr = 0.f;

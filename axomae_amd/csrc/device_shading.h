@@ -490,73 +490,74 @@ NOVA_HD bool sample_specular(const Principled &b, const VNDF &g, const VNDF &ge,
   return true;
 }
 
-// DielectricBxDF::sample_f and its four branches (BxDF.h:403-585)
+// DielectricBxDF::sample_f and its four branches (BxDF.h:403-585). The branches fill
+// locals and the record is written once: per-branch field stores made the compiler keep
+// the record in scratch memory.
 NOVA_HD bool sample_dielectric(const Principled &b, const VNDF &g, const VNDF &ge, f3 wo, float uc, float u0, float u1, BsdfSample &s) {
   const float ior = b.eta.x;
+  f3 f, wi, wm = mk3(0.f, 0.f, 0.f);
+  float pdf, ct;
+  int kind;
   if (g.full_specular() || ior == 1.f) {
     const float R = fresnel_real(ior, wo.z), T = 1 - R;
     const float pr = R, pt = T;
     if (pr == 0.f && pt == 0.f) return false;
     if (uc < (pr / (pr + pt))) {
-      f3 wi = reflect(-wo, mk3(0, 0, 1.f));
-      const float ci = abscos(wi);
-      s.costheta = ci;
-      s.f = b.lobe_weight * b.albedo * R / ci;
-      s.pdf = b.lobe_prob * pr / (pr + pt);
-      s.wi = wi;
-      s.kind = SAMPLE_DELTA;
-      return true;
+      wi = reflect(-wo, mk3(0, 0, 1.f));
+      ct = abscos(wi);
+      f = b.lobe_weight * b.albedo * R / ct;
+      pdf = b.lobe_prob * pr / (pr + pt);
+    } else {
+      bool ok = true;
+      wi = mk3(0, 0, 0);
+      refract(wo, mk3(0.f, 0.f, 1.f), ior, ok, wi);
+      if (!ok) return false;
+      ct = abscos(wi);
+      f3 ft = T * b.albedo / ct;
+      ft = ft / sqr(ior);
+      f = b.lobe_weight * ft;
+      pdf = b.lobe_prob * pt / (pr + pt);
     }
-    bool ok = true;
-    f3 wi = mk3(0, 0, 0);
-    refract(wo, mk3(0.f, 0.f, 1.f), ior, ok, wi);
-    if (!ok) return false;
-    const float ci = abscos(wi);
-    f3 f = T * b.albedo / ci;
-    f = f / sqr(ior);
-    s.f = b.lobe_weight * f;
-    s.costheta = ci;
-    s.pdf = b.lobe_prob * pt / (pr + pt);
-    s.wi = wi;
-    s.kind = SAMPLE_DELTA;
-    return true;
+    kind = SAMPLE_DELTA;
+  } else {
+    wm = g.sample(wo, u0, u1);
+    const float wo_dot_wm = dot(wo, wm);
+    const float R = fresnel_real(ior, wo_dot_wm), T = 1 - R;
+    const float pr = R, pt = T;
+    if (pr == 0.f && pt == 0.f) return false;
+    if (uc < (pr / (pr + pt))) {
+      wi = reflect(-wo, wm);
+      if (!same_hemi(wo, wi)) return false;
+      const f3 fr = b.albedo * ge.D(wm) * ge.G(wo, wi) * R / (4 * wi.z * wo.z);
+      const float pr_ = g.pdf(wo, wm) / (4 * absdot(wo, wm)) * pr / (pr + pt);
+      f = b.lobe_weight * fr;
+      pdf = b.lobe_prob * pr_;
+      ct = abscos(wi);
+    } else {
+      wi = mk3(0, 0, 0);
+      bool no_tir = true;
+      const float eta = refract(wo, wm, ior, no_tir, wi);
+      if (!no_tir) return false;
+      wi = normalize(wi);
+      if (same_hemi(wi, wo) || wi.z == 0.f) return false;
+      const float ci = wi.z, co = wo.z;
+      const float denom = sqr(dot(wi, wm) + dot(wo, wm) / eta);
+      const float jac = absdot(wi, wm) / denom;
+      const float pt_ = g.pdf(wo, wm) * jac * pt / (pt + pr);
+      f3 ft = b.albedo * T * ge.D(wm) * ge.G(wo, wi) * fabsf(dot(wi, wm) * dot(wo, wm) / (ci * co * denom));
+      ft = ft / sqr(eta);
+      f = b.lobe_weight * ft;
+      ct = abscos(wi);
+      pdf = b.lobe_prob * pt_;
+    }
+    kind = SAMPLE_MICROFACET;
   }
-  f3 wm = g.sample(wo, u0, u1);
-  const float wo_dot_wm = dot(wo, wm);
-  const float R = fresnel_real(ior, wo_dot_wm), T = 1 - R;
-  const float pr = R, pt = T;
-  if (pr == 0.f && pt == 0.f) return false;
-  if (uc < (pr / (pr + pt))) {
-    f3 wi = reflect(-wo, wm);
-    if (!same_hemi(wo, wi)) return false;
-    const f3 f = b.albedo * ge.D(wm) * ge.G(wo, wi) * R / (4 * wi.z * wo.z);
-    const float pdf = g.pdf(wo, wm) / (4 * absdot(wo, wm)) * pr / (pr + pt);
-    s.f = b.lobe_weight * f;
-    s.pdf = b.lobe_prob * pdf;
-    s.wi = wi;
-    s.costheta = abscos(wi);
-    s.wm = wm;
-    s.kind = SAMPLE_MICROFACET;
-    return true;
-  }
-  f3 wi = mk3(0, 0, 0);
-  bool no_tir = true;
-  const float eta = refract(wo, wm, ior, no_tir, wi);
-  if (!no_tir) return false;
-  wi = normalize(wi);
-  if (same_hemi(wi, wo) || wi.z == 0.f) return false;
-  const float ci = wi.z, co = wo.z;
-  const float denom = sqr(dot(wi, wm) + dot(wo, wm) / eta);
-  const float jac = absdot(wi, wm) / denom;
-  const float pdf = g.pdf(wo, wm) * jac * pt / (pt + pr);
-  f3 f = b.albedo * T * ge.D(wm) * ge.G(wo, wi) * fabsf(dot(wi, wm) * dot(wo, wm) / (ci * co * denom));
-  f = f / sqr(eta);
-  s.f = b.lobe_weight * f;
-  s.costheta = abscos(wi);
-  s.pdf = b.lobe_prob * pdf;
+  s.f = f;
   s.wi = wi;
+  s.pdf = pdf;
+  s.costheta = ct;
   s.wm = wm;
-  s.kind = SAMPLE_MICROFACET;
+  s.kind = kind;
   return true;
 }
 

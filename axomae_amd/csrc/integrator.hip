@@ -4,8 +4,8 @@
 //   generate  : camera rays for every (frame, pixel) path slot, Sobol dims 0-1
 //   per bounce: extend (closest hit of the active queue)
 //               shade  (hit record, Principled BSDF sample, Russian roulette,
-//                       spawn; survivors are compacted into the next queue with a
-//                       wave ballot/mbcnt prefix and one atomic per wave)
+//                       spawn; survivors are compacted into the next queue with
+//                       ballot/mbcnt ranks and one atomic per workgroup and iteration)
 //   accumulate: accum += partial(frame); final = accum / (frame + 1), in frame order.
 //
 // The reference evaluates Li recursively (Integrator.cpp:59-98):
@@ -20,6 +20,7 @@
 
 #include "device_math.h"
 #include "device_path.h"
+#include "device_queue.h"
 #include "device_shading.h"
 #include "device_trace.h"
 #include "nova_internal.h"
@@ -163,9 +164,13 @@ __device__ __forceinline__ void write_partial(const Queues &q, const FrameParams
 #else
 #define NOVA_SHADE_OCCUPANCY
 #endif
+#ifndef NOVA_SHADE_BLOCKS_PER_CU
+#define NOVA_SHADE_BLOCKS_PER_CU 0  // shade workgroups per CU (0: as many as are resident at once)
+#endif
+constexpr int kShadeBlock = 256;  // 512 and 1024 measured slower (barrier waits)
 
 template <bool kAdj>
-__global__ __launch_bounds__(256) NOVA_SHADE_OCCUPANCY void shade_kernel(ShadeParams sp, ShadeCtx sc, const uint32_t *__restrict__ sobol, Queues q, int which,
+__global__ __launch_bounds__(kShadeBlock) NOVA_SHADE_OCCUPANCY void shade_kernel(ShadeParams sp, ShadeCtx sc, const uint32_t *__restrict__ sobol, Queues q, int which,
                                                     AdjointCtx adj) {
   // nibble tables of this scatter's sampler dimensions (dim0 .. dim0+4)
   __shared__ uint32_t sob_tab[5 * 128];
@@ -180,6 +185,8 @@ __global__ __launch_bounds__(256) NOVA_SHADE_OCCUPANCY void shade_kernel(ShadePa
   const uint32_t stride = gridDim.x * blockDim.x;
   // every lane of a wave runs the same number of iterations so the ballot below is uniform
   const uint32_t iters = (n + stride - 1) / stride;
+  uint32_t escapes = 0;  // wave-uniform count of escaped paths, added to the statistics once
+  __shared__ uint32_t wave_cnt[2][kShadeBlock / 64], blk_base[2];
   for (uint32_t it = 0; it < iters; it++) {
     const uint32_t i = it * stride + blockIdx.x * blockDim.x + threadIdx.x;
     bool cont = false, escaped = false;
@@ -207,18 +214,11 @@ __global__ __launch_bounds__(256) NOVA_SHADE_OCCUPANCY void shade_kernel(ShadePa
         q.A[slot] = A;
       }
     }
-    const uint64_t esc = __ballot(escaped);
-    if (esc != 0 && (int)(threadIdx.x & 63) == __builtin_ffsll((long long)esc) - 1)
-      atomicAdd(&q.stats64[2], (unsigned long long)__popcll(esc));
+    escapes += (uint32_t)__popcll(__ballot(escaped));
     // compact survivors into the next queue
     const uint64_t m = __ballot(cont);
     const uint32_t lane_rank = __builtin_amdgcn_mbcnt_hi((uint32_t)(m >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)m, 0));
-    uint32_t base = 0;
-    const int first = __builtin_ffsll((long long)m) - 1;
-    if (m != 0) {
-      if ((int)(threadIdx.x & 63) == first) base = atomicAdd(&q.count[which ^ 1], (uint32_t)__popcll(m));
-      base = __shfl(base, first, 64);
-    }
+    const uint32_t base = block_reserve<kShadeBlock>(m, &q.count[which ^ 1], it, wave_cnt, blk_base);
     if (cont) {
       const uint32_t j = base + lane_rank;
       rout[j] = no.x;
@@ -230,6 +230,7 @@ __global__ __launch_bounds__(256) NOVA_SHADE_OCCUPANCY void shade_kernel(ShadePa
       sout[j] = slot;
     }
   }
+  if (escapes != 0 && (threadIdx.x & 63) == 0) atomicAdd(&q.stats64[2], (unsigned long long)escapes);
 }
 
 // Between an extend and a shade launch: the shade's output queue and the next extend's work
@@ -386,7 +387,16 @@ static int render_impl(nova_hip_ctx *ctx, const nova_hip_render_params *p, float
   ShadeCtx sc{ctx->textures, ctx->tex_u32, ctx->tex_f32, ctx->materials, ctx->mesh_xf, ctx->shade, ctx->acos_lut, ctx->envmap,
               p->sampling_roughness_texture};
   const AdjointCtx replay = adj ? AdjointCtx{partial, adj->dimg, adj->scale, adj->grad} : AdjointCtx{nullptr, nullptr, 0.f, nullptr};
-  const uint32_t shade_blocks = (uint32_t)std::min<size_t>((cap + 255) / 256, (size_t)cus * 8);
+  // The shade loop splits the queue evenly over its grid, so the grid is what fits on the
+  // chip at once: a second round of workgroups would run after the first at lower occupancy.
+  int shade_per_cu[2] = {NOVA_SHADE_BLOCKS_PER_CU, NOVA_SHADE_BLOCKS_PER_CU};
+  for (int a = 0; a < 2; a++) {
+    if (shade_per_cu[a] > 0) continue;
+    const void *fn = a ? reinterpret_cast<const void *>(shade_kernel<true>) : reinterpret_cast<const void *>(shade_kernel<false>);
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&shade_per_cu[a], fn, kShadeBlock, 0) != hipSuccess || shade_per_cu[a] <= 0) shade_per_cu[a] = 8;
+  }
+  const uint32_t shade_blocks[2] = {(uint32_t)std::min<size_t>((cap + kShadeBlock - 1) / kShadeBlock, (size_t)cus * shade_per_cu[0]),
+                                    (uint32_t)std::min<size_t>((cap + kShadeBlock - 1) / kShadeBlock, (size_t)cus * shade_per_cu[1])};
   NOVA_HIP_CHECK(ctx, hipMemsetAsync(counts, 0, 64, s));
 
   const bool timing = (p->flags & NOVA_HIP_RENDER_TIMING) != 0;
@@ -440,9 +450,9 @@ static int render_impl(nova_hip_ctx *ctx, const nova_hip_render_params *p, float
       spp.dim0 = dim;
       if (timing && (st = stamp())) return st;
       if (pass == 0)
-        hipLaunchKernelGGL(shade_kernel<false>, dim3(shade_blocks), dim3(256), 0, s, spp, sc, ctx->sobol_table, q, which, replay);
+        hipLaunchKernelGGL(shade_kernel<false>, dim3(shade_blocks[0]), dim3(kShadeBlock), 0, s, spp, sc, ctx->sobol_table, q, which, replay);
       else
-        hipLaunchKernelGGL(shade_kernel<true>, dim3(shade_blocks), dim3(256), 0, s, spp, sc, ctx->sobol_table, q, which, replay);
+        hipLaunchKernelGGL(shade_kernel<true>, dim3(shade_blocks[1]), dim3(kShadeBlock), 0, s, spp, sc, ctx->sobol_table, q, which, replay);
       if (timing && (st = stamp())) return st;
       NOVA_HIP_CHECK(ctx, hipGetLastError());
       dim += 4 + (spp.rr ? 1 : 0);

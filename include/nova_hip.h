@@ -223,9 +223,10 @@ int nova_hip_render_batch(nova_hip_ctx *executor, const nova_hip_batch_item *ite
                           nova_hip_batch_stats *stats, float *completion_ms, void *stream);
 
 /* Autotuner for nova_hip_render_batch: times the captured wave graph of every candidate
- * (queue capacity 2^21 to 2^24 paths x 4 or 8 trace workgroups per CU x 4 or 8 shade
- * workgroups per CU) on the same slots from the middle of the batch, without accumulating,
- * and returns the fastest per path (best_ns_per_path optional). */
+ * (queue capacity 2^21 to 2^24 paths x the trace workgroups per CU that are resident at
+ * once x one or two rounds of resident shade workgroups per CU) on the same slots from the
+ * middle of the batch, without accumulating, and returns the fastest per path
+ * (best_ns_per_path optional). */
 int nova_hip_batch_autotune(nova_hip_ctx *executor, const nova_hip_batch_item *items, uint32_t count, nova_hip_batch_config *best,
                             float *best_ns_per_path, void *stream);
 
