@@ -105,10 +105,17 @@ NOVA_HD float ref_box_entry(const BNode &b, f3 o, f3 d) {
 #ifdef __HIP_DEVICE_COMPILE__
 __device__ __noinline__
 #else
-inline
+__host__ __device__ inline
 #endif
     bool
-    reference_tie_winner_is_new(const TraceCtx &tc, int32_t pos_new, int32_t pos_old, f3 o, f3 d, float t) {
+    reference_tie_winner_is_new(const BNode *bnodes, const int32_t *bparent, const int32_t *bdepth, const int32_t *leaf_of_pos, int32_t pos_new,
+                                int32_t pos_old, f3 o, f3 d, float t) {
+  // the tree arrays come as plain arguments: a context struct passed by reference to this
+  // out-of-line function would be kept in scratch memory by every caller
+  const struct {
+    const BNode *bnodes;
+    const int32_t *bparent, *bdepth, *leaf_of_pos;
+  } tc{bnodes, bparent, bdepth, leaf_of_pos};
   const int32_t leaf_new = tc.leaf_of_pos[pos_new], leaf_old = tc.leaf_of_pos[pos_old];
   int32_t a = leaf_new, b = leaf_old;
   if (a == b) return pos_new > pos_old;
@@ -183,12 +190,16 @@ NOVA_HD bool trace_step(const TraceCtx &tc, Stack &stack, TraceState &st, float 
     // per child: entry distance (+inf when culled), child reference and triangle count
     float key[4];
     int32_t cref[4], ccnt[4];
+    // Two copies of the slab test: rays with an axis-parallel direction take the
+    // containment form; the common copy keeps every min/max operand a product of the same
+    // block, which spares the NaN-quieting copies the compiler adds to values merged from
+    // branches.
+    if (st.parallel) {
 #pragma unroll
-    for (int k = 0; k < 4; k++) {
-      float tx1 = (bmnx[k] - st.o.x) * st.inv.x, tx2 = (bmxx[k] - st.o.x) * st.inv.x;
-      float ty1 = (bmny[k] - st.o.y) * st.inv.y, ty2 = (bmxy[k] - st.o.y) * st.inv.y;
-      float tz1 = (bmnz[k] - st.o.z) * st.inv.z, tz2 = (bmxz[k] - st.o.z) * st.inv.z;
-      if (st.parallel) {
+      for (int k = 0; k < 4; k++) {
+        float tx1 = (bmnx[k] - st.o.x) * st.inv.x, tx2 = (bmxx[k] - st.o.x) * st.inv.x;
+        float ty1 = (bmny[k] - st.o.y) * st.inv.y, ty2 = (bmxy[k] - st.o.y) * st.inv.y;
+        float tz1 = (bmnz[k] - st.o.z) * st.inv.z, tz2 = (bmxz[k] - st.o.z) * st.inv.z;
         if (st.px) {
           const bool in = bmnx[k] <= st.o.x && st.o.x <= bmxx[k];
           tx1 = -kInf;
@@ -204,13 +215,26 @@ NOVA_HD bool trace_step(const TraceCtx &tc, Stack &stack, TraceState &st, float 
           tz1 = -kInf;
           tz2 = in ? kInf : -kInf;
         }
+        const float t0 = fmaxf(fmaxf(fminf(tx1, tx2), fminf(ty1, ty2)), fminf(tz1, tz2));
+        const float t1 = fminf(fminf(fmaxf(tx1, tx2), fmaxf(ty1, ty2)), fmaxf(tz1, tz2)) * kUp;
+        const bool hit = chv[k] >= 0 && t0 <= t1 && t1 >= 0.f && t0 <= st.tbest * kUp;
+        key[k] = hit ? t0 : kInf;
+        cref[k] = hit ? chv[k] : -1;
+        ccnt[k] = cnv[k];
       }
-      const float t0 = fmaxf(fmaxf(fminf(tx1, tx2), fminf(ty1, ty2)), fminf(tz1, tz2));
-      const float t1 = fminf(fminf(fmaxf(tx1, tx2), fmaxf(ty1, ty2)), fmaxf(tz1, tz2)) * kUp;
-      const bool hit = chv[k] >= 0 && t0 <= t1 && t1 >= 0.f && t0 <= st.tbest * kUp;
-      key[k] = hit ? t0 : kInf;
-      cref[k] = hit ? chv[k] : -1;
-      ccnt[k] = cnv[k];
+    } else {
+#pragma unroll
+      for (int k = 0; k < 4; k++) {
+        const float tx1 = (bmnx[k] - st.o.x) * st.inv.x, tx2 = (bmxx[k] - st.o.x) * st.inv.x;
+        const float ty1 = (bmny[k] - st.o.y) * st.inv.y, ty2 = (bmxy[k] - st.o.y) * st.inv.y;
+        const float tz1 = (bmnz[k] - st.o.z) * st.inv.z, tz2 = (bmxz[k] - st.o.z) * st.inv.z;
+        const float t0 = fmaxf(fmaxf(fminf(tx1, tx2), fminf(ty1, ty2)), fminf(tz1, tz2));
+        const float t1 = fminf(fminf(fmaxf(tx1, tx2), fmaxf(ty1, ty2)), fmaxf(tz1, tz2)) * kUp;
+        const bool hit = chv[k] >= 0 && t0 <= t1 && t1 >= 0.f && t0 <= st.tbest * kUp;
+        key[k] = hit ? t0 : kInf;
+        cref[k] = hit ? chv[k] : -1;
+        ccnt[k] = cnv[k];
+      }
     }
     // near-to-far order of the four children: a fixed 5-exchange sorting network, so the
     // arrays stay in registers (culled children carry +inf and end last)
@@ -247,7 +271,7 @@ NOVA_HD bool trace_step(const TraceCtx &tc, Stack &stack, TraceState &st, float 
         const TriMT tr = load_tri(tc.tris + i);
         if (kCount) tri_tests++;
         float tt, uu, vv;
-        if (mt_test(tr, st.o, st.d, tnear, st.tbest, tt, uu, vv) && (tt < st.tbest || st.best < 0 || reference_tie_winner_is_new(tc, i, st.bpos, st.o, st.d, tt))) {
+        if (mt_test(tr, st.o, st.d, tnear, st.tbest, tt, uu, vv) && (tt < st.tbest || st.best < 0 || reference_tie_winner_is_new(tc.bnodes, tc.bparent, tc.bdepth, tc.leaf_of_pos, i, st.bpos, st.o, st.d, tt))) {
           st.tbest = tt;
           st.bpos = i;
           st.best = __builtin_bit_cast(int32_t, tr.a.w);

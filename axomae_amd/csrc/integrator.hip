@@ -276,7 +276,7 @@ __global__ __launch_bounds__(kTraceBlock) void closest_hit_kernel(TraceCtx tc, c
 }
 
 #ifndef NOVA_PATHS_PER_BATCH
-#define NOVA_PATHS_PER_BATCH (32u << 20)
+#define NOVA_PATHS_PER_BATCH (128u << 20)
 #endif
 constexpr size_t kPathsPerBatch = NOVA_PATHS_PER_BATCH;
 
