@@ -25,7 +25,8 @@
 namespace nova_hip {
 
 struct BatchScene {
-  TraceCtx tc;  // gstack fields unused (the launch passes its own)
+  TraceCtx tc;    // gstack fields and tie unused (the launch passes its own stack; tie below)
+  TieTables tie;  // the scene's tie tables, in the scene record so a lane computes their address
   ShadeCtx sc;
   FrameParams fp;
   uint64_t first;   // first global slot
@@ -113,6 +114,13 @@ __global__ __launch_bounds__(256) void batch_generate(const BatchScene *__restri
 // Closest hits of the queue with the single-scene kernel's lane refill: a lane whose ray
 // has ended takes the next unclaimed ray once kBatchRefill lanes of its wave are idle.
 constexpr uint32_t kBatchRefill = 56;
+// Tie tables of a lane's scene: an address computed from the scene index, not a pointer
+// loaded from the scene record at every leaf.
+struct SceneTie {
+  const BatchScene *scenes;
+  int32_t sid;
+  __device__ const TieTables *operator()(const TraceCtx &) const { return &scenes[sid].tie; }
+};
 __global__ __launch_bounds__(kTraceBlock) __attribute__((amdgpu_waves_per_eu(6, 6))) void batch_extend(
     const BatchScene *__restrict__ scenes, BatchState *st, BatchQueues q, int which, uint32_t *gstack, uint32_t gstride) {
   __shared__ uint32_t lds[kLdsStack * kTraceBlock];
@@ -147,7 +155,7 @@ __global__ __launch_bounds__(kTraceBlock) __attribute__((amdgpu_waves_per_eu(6, 
       }
     }
     if (__ballot(have) == 0) break;
-    if (have && trace_step<false>(scenes[sid].tc, stk, ts, 0.000001f, nv, nt)) {
+    if (have && trace_step<false>(scenes[sid].tc, stk, ts, 0.000001f, nv, nt, SceneTie{scenes, sid})) {
       q.hit_prim[ray] = ts.best;
       q.hit_t[ray] = ts.tbest;
       q.hit_u[ray] = ts.bu;
@@ -453,7 +461,8 @@ static int make_plan(nova_hip_ctx *ex, const nova_hip_batch_item *items, uint32_
     const uint64_t slots = npix * (uint64_t)p->frame_count;
     if (slots >= (1ull << 31)) return set_error(ex, NOVA_HIP_INVALID_ARGUMENT, "batch item too large");
     BatchScene &S = bp.scenes[i];
-    S.tc = TraceCtx{c->nodes4, c->tri_leaf, nullptr, 0, c->bnodes, c->bparent, c->bdepth, c->leaf_of_pos};
+    S.tc = TraceCtx{c->nodes4, c->tri_leaf, nullptr, 0, nullptr};
+    S.tie = TieTables{c->bnodes, c->bparent, c->bdepth, c->leaf_of_pos};
     S.sc = ShadeCtx{c->textures, c->tex_u32, c->tex_f32, c->materials, c->mesh_xf, c->shade, c->acos_lut, c->envmap, -1};
     S.fp.width = p->width;
     S.fp.height = p->height;
@@ -571,7 +580,7 @@ extern "C" int nova_hip_batch_autotune(nova_hip_ctx *ex, const nova_hip_batch_it
   if ((st = executor_stream(ex, (hipStream_t)stream, &s))) return st;
   const int cus = device_cus_of(ex->device);
   const uint32_t *sobol = ex->sobol_table;
-  const uint32_t caps[] = {1u << 21, 1u << 22, 1u << 23, 1u << 24};
+  const uint32_t caps[] = {1u << 22, 1u << 23, 1u << 24, 1u << 25, 1u << 26, 1u << 27};
   // workgroups per CU: the extend kernel balances its lanes through a work counter, so it
   // takes what fits at once; the shade loop splits its queue evenly, so a second round of
   // workgroups is a candidate against one round that fits

@@ -666,7 +666,7 @@ extern "C" int nova_hip_bvh_build(nova_hip_ctx *ctx, int build_candidates, nova_
   const uint32_t n = ctx->prim_count;
   const size_t maxnodes = 2 * (size_t)n - 1;
   for (void **p : {(void **)&ctx->prim_idx, (void **)&ctx->bnodes, (void **)&ctx->nodes4, (void **)&ctx->tri_leaf, (void **)&ctx->bparent,
-                   (void **)&ctx->bdepth, (void **)&ctx->leaf_of_pos})
+                   (void **)&ctx->bdepth, (void **)&ctx->leaf_of_pos, (void **)&ctx->tie})
     if (*p) {
       hipFree(*p);
       *p = nullptr;
@@ -677,6 +677,9 @@ extern "C" int nova_hip_bvh_build(nova_hip_ctx *ctx, int build_candidates, nova_
   NOVA_HIP_CHECK(ctx, hipMalloc(&ctx->bparent, maxnodes * sizeof(int32_t)));
   NOVA_HIP_CHECK(ctx, hipMalloc(&ctx->bdepth, maxnodes * sizeof(int32_t)));
   NOVA_HIP_CHECK(ctx, hipMalloc(&ctx->leaf_of_pos, (size_t)n * sizeof(int32_t)));
+  NOVA_HIP_CHECK(ctx, hipMalloc(&ctx->tie, sizeof(TieTables)));
+  const TieTables tie{ctx->bnodes, ctx->bparent, ctx->bdepth, ctx->leaf_of_pos};
+  NOVA_HIP_CHECK(ctx, hipMemcpy(ctx->tie, &tie, sizeof(TieTables), hipMemcpyHostToDevice));
 
   // temporaries
   int32_t *tmp_idx, *rk, *hr, *tl, *act0, *act1, *counters;

@@ -29,9 +29,7 @@ struct TraceCtx {
   const TriMT *tris;
   uint32_t *gstack;  // overflow entries: [level - lds levels][global thread]
   uint32_t gstride;  // number of threads in the grid
-  // binary tree, only read to settle exact equal-t ties
-  const BNode *bnodes;
-  const int32_t *bparent, *bdepth, *leaf_of_pos;
+  const TieTables *tie;  // binary tree, only read to settle exact equal-t ties
 };
 
 // BoundingBox::halfArea (BoundingBox.cpp:162-165)
@@ -108,14 +106,10 @@ __device__ __noinline__
 __host__ __device__ inline
 #endif
     bool
-    reference_tie_winner_is_new(const BNode *bnodes, const int32_t *bparent, const int32_t *bdepth, const int32_t *leaf_of_pos, int32_t pos_new,
-                                int32_t pos_old, f3 o, f3 d, float t) {
-  // the tree arrays come as plain arguments: a context struct passed by reference to this
-  // out-of-line function would be kept in scratch memory by every caller
-  const struct {
-    const BNode *bnodes;
-    const int32_t *bparent, *bdepth, *leaf_of_pos;
-  } tc{bnodes, bparent, bdepth, leaf_of_pos};
+    reference_tie_winner_is_new(const TieTables *tie, int32_t pos_new, int32_t pos_old, f3 o, f3 d, float t) {
+  // one pointer to a record in device memory: the common path keeps no tree pointers live,
+  // and no caller has to copy a context into scratch memory for this out-of-line call
+  const TieTables tc = *tie;
   const int32_t leaf_new = tc.leaf_of_pos[pos_new], leaf_old = tc.leaf_of_pos[pos_old];
   int32_t a = leaf_new, b = leaf_old;
   if (a == b) return pos_new > pos_old;
@@ -169,13 +163,20 @@ NOVA_HD void trace_init(TraceState &st, f3 o, f3 d) {
   st.sp = 0;
 }
 
+// Where a walk finds its tie tables: the context's pointer by default.
+struct TieFromCtx {
+  NOVA_HD const TieTables *operator()(const TraceCtx &tc) const { return tc.tie; }
+};
+
 // One node of the walk: test the node's four children with a slightly widened slab (no box
 // holding a hit the reference's division-form test accepts is culled), order them near to
 // far, push the inner ones, intersect the leaves (exact Moller-Trumbore; exact equal-t ties
 // settled in the reference's visiting order). Returns true when the walk is over; the hit is
-// then st.best / st.tbest / st.bu / st.bv. Stack provides push(level, node) and at(level).
-template <bool kCount, class Stack>
-NOVA_HD bool trace_step(const TraceCtx &tc, Stack &stack, TraceState &st, float tnear, uint32_t &nodes_visited, uint32_t &tri_tests) {
+// then st.best / st.tbest / st.bu / st.bv. Stack provides push(level, node) and at(level);
+// tie_of(tc) gives the tie tables.
+template <bool kCount, class Stack, class TieOf = TieFromCtx>
+NOVA_HD bool trace_step(const TraceCtx &tc, Stack &stack, TraceState &st, float tnear, uint32_t &nodes_visited, uint32_t &tri_tests,
+                        TieOf tie_of = TieOf()) {
   const float kUp = 1.000001f;
   const float kInf = __builtin_inff();
     if (kCount) nodes_visited++;
@@ -271,7 +272,7 @@ NOVA_HD bool trace_step(const TraceCtx &tc, Stack &stack, TraceState &st, float 
         const TriMT tr = load_tri(tc.tris + i);
         if (kCount) tri_tests++;
         float tt, uu, vv;
-        if (mt_test(tr, st.o, st.d, tnear, st.tbest, tt, uu, vv) && (tt < st.tbest || st.best < 0 || reference_tie_winner_is_new(tc.bnodes, tc.bparent, tc.bdepth, tc.leaf_of_pos, i, st.bpos, st.o, st.d, tt))) {
+        if (mt_test(tr, st.o, st.d, tnear, st.tbest, tt, uu, vv) && (tt < st.tbest || st.best < 0 || reference_tie_winner_is_new(tie_of(tc), i, st.bpos, st.o, st.d, tt))) {
           st.tbest = tt;
           st.bpos = i;
           st.best = __builtin_bit_cast(int32_t, tr.a.w);

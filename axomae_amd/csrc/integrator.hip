@@ -312,7 +312,7 @@ extern "C" int nova_hip_closest_hit(nova_hip_ctx *ctx, const float *rays_dev, ui
   const uint32_t threads = blocks * kTraceBlock;
   int st = ensure_gstack(ctx, threads);
   if (st) return st;
-  TraceCtx tc{ctx->nodes4, ctx->tri_leaf, ctx->gstack, threads, ctx->bnodes, ctx->bparent, ctx->bdepth, ctx->leaf_of_pos};
+  TraceCtx tc{ctx->nodes4, ctx->tri_leaf, ctx->gstack, threads, ctx->tie};
   hipLaunchKernelGGL(closest_hit_kernel, dim3(blocks), dim3(kTraceBlock), 0, (hipStream_t)stream, tc, rays_dev, n, prim_dev, tuv_dev);
   NOVA_HIP_CHECK(ctx, hipGetLastError());
   return NOVA_HIP_SUCCESS;
@@ -383,7 +383,7 @@ static int render_impl(nova_hip_ctx *ctx, const nova_hip_render_params *p, float
   const uint32_t trace_threads = trace_blocks * kTraceBlock;
   int st = ensure_gstack(ctx, trace_threads);
   if (st) return st;
-  TraceCtx tc{ctx->nodes4, ctx->tri_leaf, ctx->gstack, trace_threads, ctx->bnodes, ctx->bparent, ctx->bdepth, ctx->leaf_of_pos};
+  TraceCtx tc{ctx->nodes4, ctx->tri_leaf, ctx->gstack, trace_threads, ctx->tie};
   ShadeCtx sc{ctx->textures, ctx->tex_u32, ctx->tex_f32, ctx->materials, ctx->mesh_xf, ctx->shade, ctx->acos_lut, ctx->envmap,
               p->sampling_roughness_texture};
   const AdjointCtx replay = adj ? AdjointCtx{partial, adj->dimg, adj->scale, adj->grad} : AdjointCtx{nullptr, nullptr, 0.f, nullptr};

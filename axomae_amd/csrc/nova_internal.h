@@ -55,6 +55,13 @@ struct BNode {
   int32_t left, count;
 };
 
+// The binary tree arrays the traversal reads only to settle exact equal-t ties, gathered in
+// one device record so a kernel carries a single pointer to them.
+struct TieTables {
+  const BNode *bnodes;
+  const int32_t *bparent, *bdepth, *leaf_of_pos;
+};
+
 // 4-wide node, 128 B: child boxes SoA, then per-child reference. count[k] == 0: child[k] is
 // an inner node index; count[k] > 0: a leaf of triangles [child[k], child[k]+count[k]) in the
 // TriMT array; child[k] == -1: empty slot.
@@ -104,6 +111,7 @@ struct nova_hip_ctx {
   int32_t *bparent = nullptr;                // binary tree parent (-1 at the root)
   int32_t *bdepth = nullptr;                 // binary tree depth
   int32_t *leaf_of_pos = nullptr;            // binary leaf containing each prim_idx position
+  nova_hip::TieTables *tie = nullptr;        // device record of bnodes, bparent, bdepth, leaf_of_pos
   nova_hip_bvh_info bvh_info{};
   bool bvh_ready = false;
 

@@ -223,7 +223,7 @@ int nova_hip_render_batch(nova_hip_ctx *executor, const nova_hip_batch_item *ite
                           nova_hip_batch_stats *stats, float *completion_ms, void *stream);
 
 /* Autotuner for nova_hip_render_batch: times the captured wave graph of every candidate
- * (queue capacity 2^21 to 2^24 paths x the trace workgroups per CU that are resident at
+ * (queue capacity 2^22 to 2^27 paths x the trace workgroups per CU that are resident at
  * once x one or two rounds of resident shade workgroups per CU) on the same slots from the
  * middle of the batch, without accumulating, and returns the fastest per path
  * (best_ns_per_path optional). */

@@ -10,9 +10,9 @@ LIB = os.path.join(HERE, "libnova_emu.so")
 def build():
     src = os.path.join(HERE, "host_emu.hip")
     cmd = ["/opt/rocm/bin/hipcc", "-O2", "-std=c++17", "-fPIC", "-shared", "--offload-arch=gfx950", "-ffp-contract=off",
-           "-fhip-fp32-correctly-rounded-divide-sqrt", "-fno-fast-math", "-I", os.path.join(HERE, "..", "..", "include"), src, "-o", LIB + ".tmp"]
+           "-fhip-fp32-correctly-rounded-divide-sqrt", "-fno-fast-math", "-I", os.path.join(HERE, "..", "..", "include"), src, "-o", LIB + f".{os.getpid()}.tmp"]
     subprocess.check_call(cmd)
-    os.replace(LIB + ".tmp", LIB)
+    os.replace(LIB + f".{os.getpid()}.tmp", LIB)  # per-process temporary: parallel test workers may build at once
     return LIB
 
 

@@ -136,7 +136,8 @@ extern "C" int emu_closest_hit(const nova_hip_scene_desc *sd, const BNode *bnode
   }
   std::vector<TriMT> tri_leaf(n);
   for (uint32_t i = 0; i < n; i++) tri_leaf[i] = es.tri[prim_idx[i]];
-  TraceCtx tc{wide.data(), tri_leaf.data(), nullptr, 0, bnodes, parent.data(), depth.data(), leaf_of_pos.data()};
+  const TieTables tie{bnodes, parent.data(), depth.data(), leaf_of_pos.data()};
+  TraceCtx tc{wide.data(), tri_leaf.data(), nullptr, 0, &tie};
   HostStack st;
   uint32_t nv = 0, nt = 0;
   for (int64_t r = 0; r < nrays; r++) {
