@@ -451,6 +451,8 @@ static int make_plan(nova_hip_ctx *ex, const nova_hip_batch_item *items, uint32_
     if (p->width <= 0 || p->height <= 0 || p->width > 65536 || p->height > 65536)
       return set_error(ex, NOVA_HIP_INVALID_RENDERBUFFER_DIM, "Wrong buffer width / height dimension.");
     if (p->frame_count <= 0 || p->frame_start < 0 || p->max_depth < 0) return set_error(ex, NOVA_HIP_INVALID_ARGUMENT, "bad frame range or depth");
+    if (p->index_policy != NOVA_HIP_INDEX_CPU && p->index_policy != NOVA_HIP_INDEX_OPTIX)
+      return set_error(ex, NOVA_HIP_INVALID_ARGUMENT, "unknown index_policy");
     if (i > 0 && (p->max_depth != bp.max_depth || p->rr_start != bp.rr_start))
       return set_error(ex, NOVA_HIP_INVALID_ARGUMENT, "batch items must share max_depth and rr_start");
     if (!it.accum_dev || !it.final_dev) return set_error(ex, NOVA_HIP_INVALID_BUFFER_STATE, "null render buffer");
@@ -470,6 +472,7 @@ static int make_plan(nova_hip_ctx *ex, const nova_hip_batch_item *items, uint32_
     S.fp.frames = p->frame_count;
     S.fp.offset = p->max_samples_offset;
     S.fp.vinvert = p->vertical_invert;
+    S.fp.policy = p->index_policy;
     for (int k = 0; k < 16; k++) {
       S.fp.inv_P[k] = p->inv_P[k];
       S.fp.inv_V[k] = p->inv_V[k];

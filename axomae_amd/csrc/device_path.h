@@ -11,6 +11,7 @@ struct FrameParams {
   int32_t width, height, frame0, frames;
   uint32_t offset;
   int32_t vinvert;
+  int32_t policy;  // NOVA_HIP_INDEX_CPU or NOVA_HIP_INDEX_OPTIX
   float inv_P[16], inv_V[16];
 };
 
@@ -26,12 +27,14 @@ NOVA_HD uint32_t pixel_offset(const FrameParams &fp, uint32_t p) {
   return fp.vinvert ? (((uint32_t)fp.height - 1 - y) * (uint32_t)fp.width + x) * 4u : (y * (uint32_t)fp.width + x) * 4u;
 }
 
-// Sampler index of a path slot (frame-major slots): idx * frame + renderer_max_samples,
-// u32 wraparound (Integrator.h:80).
+// Sampler index of a path slot (frame-major slots), u32 wraparound. CPU policy:
+// idx * frame + renderer_max_samples with idx = 4 * pixel (Integrator.h:80). OptiX policy:
+// (x * H + y) * sample_max + frame (integrator_optix.cu:43).
 NOVA_HD uint32_t sample_index(const FrameParams &fp, uint32_t slot) {
   const uint32_t npix = (uint32_t)fp.width * (uint32_t)fp.height;
-  const uint32_t f = (uint32_t)fp.frame0 + slot / npix;
-  return pixel_offset(fp, slot % npix) * f + fp.offset;
+  const uint32_t f = (uint32_t)fp.frame0 + slot / npix, p = slot % npix;
+  if (fp.policy == NOVA_HIP_INDEX_OPTIX) return ((p % (uint32_t)fp.width) * (uint32_t)fp.height + p / (uint32_t)fp.width) * fp.offset + f;
+  return pixel_offset(fp, p) * f + fp.offset;
 }
 
 NOVA_HD float sobol_dim(const uint32_t *table, uint32_t dim, uint32_t gray) { return u32_to_unit(owen_scramble(sobol_xor(table + (size_t)dim * 32, gray))); }

@@ -326,6 +326,8 @@ static int render_impl(nova_hip_ctx *ctx, const nova_hip_render_params *p, float
   if (p->width <= 0 || p->height <= 0 || p->width > 65536 || p->height > 65536)
     return set_error(ctx, NOVA_HIP_INVALID_RENDERBUFFER_DIM, "Wrong buffer width / height dimension.");
   if (p->frame_count <= 0 || p->frame_start < 0 || p->max_depth < 0) return set_error(ctx, NOVA_HIP_INVALID_ARGUMENT, "bad frame range or depth");
+  if (p->index_policy != NOVA_HIP_INDEX_CPU && p->index_policy != NOVA_HIP_INDEX_OPTIX)
+    return set_error(ctx, NOVA_HIP_INVALID_ARGUMENT, "unknown index_policy");
   if (!adj && (!accum_dev || !final_dev)) return set_error(ctx, NOVA_HIP_INVALID_BUFFER_STATE, "null render buffer");
   if (p->sampling_roughness_texture >= 0) {
     if ((size_t)p->sampling_roughness_texture >= ctx->host_textures.size() ||
@@ -423,6 +425,7 @@ static int render_impl(nova_hip_ctx *ctx, const nova_hip_render_params *p, float
     fp.frames = frames;
     fp.offset = p->max_samples_offset;
     fp.vinvert = p->vertical_invert;
+    fp.policy = p->index_policy;
     for (int i = 0; i < 16; i++) {
       fp.inv_P[i] = p->inv_P[i];
       fp.inv_V[i] = p->inv_V[i];

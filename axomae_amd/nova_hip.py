@@ -62,7 +62,10 @@ class BvhInfo(C.Structure):
 class RenderParams(C.Structure):
     _fields_ = [("width", C.c_int32), ("height", C.c_int32), ("max_depth", C.c_int32), ("frame_start", C.c_int32), ("frame_count", C.c_int32),
                 ("max_samples_offset", C.c_uint32), ("vertical_invert", C.c_int32), ("rr_start", C.c_int32), ("inv_P", C.c_float * 16),
-                ("inv_V", C.c_float * 16), ("flags", C.c_int32), ("sampling_roughness_texture", C.c_int32)]
+                ("inv_V", C.c_float * 16), ("flags", C.c_int32), ("sampling_roughness_texture", C.c_int32), ("index_policy", C.c_int32)]
+
+
+INDEX_CPU, INDEX_OPTIX = 0, 1
 
 
 class BakeJob(C.Structure):
@@ -199,9 +202,10 @@ def scene_desc(scene):
 
 
 def render_params(scene, width, height, frames, max_depth, frame_start=0, rr_start=-1, max_samples_offset=0, vertical_invert=False,
-                  inv_P=None, inv_V=None, flags=0, sampling_roughness_texture=-1):
+                  inv_P=None, inv_V=None, flags=0, sampling_roughness_texture=-1, index_policy=INDEX_CPU):
     p = RenderParams()
     p.sampling_roughness_texture = sampling_roughness_texture
+    p.index_policy = index_policy
     p.width, p.height, p.max_depth = width, height, max_depth
     p.frame_start, p.frame_count = frame_start, frames
     p.max_samples_offset = max_samples_offset
@@ -274,11 +278,11 @@ class HipEngine:
 
     # --------------------------------------------------------------- render
     def render(self, width, height, frames, max_depth, frame_start=0, rr_start=-1, max_samples_offset=0, vertical_invert=False, accum=None,
-               inv_P=None, inv_V=None, flags=0, sampling_roughness_texture=-1):
+               inv_P=None, inv_V=None, flags=0, sampling_roughness_texture=-1, index_policy=INDEX_CPU):
         """Renders frames [frame_start, frame_start+frames); returns (final, accum) device tensors (H, W, 4)."""
         torch = self.torch
         p = render_params(self.scene, width, height, frames, max_depth, frame_start, rr_start, max_samples_offset, vertical_invert, inv_P, inv_V,
-                          flags, sampling_roughness_texture)
+                          flags, sampling_roughness_texture, index_policy)
         if accum is None:
             accum = torch.zeros((height, width, 4), dtype=torch.float32, device="cuda")
         final = torch.empty((height, width, 4), dtype=torch.float32, device="cuda")

@@ -150,15 +150,22 @@ typedef struct {
                              FP32 texture whose channel 0 replaces, for every material with a
                              roughness_f32 texture, the roughness used to sample directions and
                              to weight them by their pdf; the BSDF value keeps roughness_f32. */
+  int32_t index_policy;     /* Sobol sample index of pixel (x, y) in frame f:
+                             NOVA_HIP_INDEX_CPU: 4*(y*W+x)*f + max_samples_offset, row flipped
+                             with vertical_invert (Integrator.h:38-42,80; the parity default);
+                             NOVA_HIP_INDEX_OPTIX: (x*H+y)*max_samples_offset + f, the reference's
+                             GPU raygen (integrator_optix.cu:43), max_samples_offset = sample_max. */
 } nova_hip_render_params;
+
+enum { NOVA_HIP_INDEX_CPU = 0, NOVA_HIP_INDEX_OPTIX = 1 };
 
 /* flags: time every extend/shade launch with HIP events on the render stream; count BVH node
  * visits and triangle tests per ray (an instrumented traversal, slower). */
 enum { NOVA_HIP_RENDER_TIMING = 1, NOVA_HIP_RENDER_COUNT_TRAVERSAL = 2 };
 
 /* Renders frames [frame_start, frame_start+frame_count) of the path integrator
- * (PathIntegrator::Li, corelib/nova/integrator/Integrator.cpp:59-98, with the CPU
- * index policy idx = 4*(y*W+x)*frame + max_samples_offset) and accumulates them as
+ * (PathIntegrator::Li, corelib/nova/integrator/Integrator.cpp:59-98, sample indices by
+ * params->index_policy) and accumulates them as
  * Engine::writeFramebuffer does (engine.cpp:351-363): accum += partial,
  * final = accum / (frame+1). accum_dev and final_dev are device RGBA32F buffers of
  * width*height*4 floats; accum must hold the previous frames' sum (zero at frame 0).

@@ -160,6 +160,7 @@ static RenderParams make_params(int width, int height, int frame_start, int fram
   }
   p.threads = threads;
   p.sampling_roughness = sampling_roughness;
+  p.index_policy = g_index_policy;
   return p;
 }
 
@@ -184,6 +185,20 @@ void oracle_render_adjoint(void *sp, void *bp, int width, int height, int frame_
 // 0 (default): evaluated in double and rounded once (what the HIP path computes).
 void oracle_set_float_libm(int on) { set_float_libm(on != 0); }
 
+// Index policy of the renders the entry points below build (0: CPU, 1: the GPU raygen's).
+void oracle_set_index_policy(int policy) { g_index_policy = policy; }
+
+// Sobol sample index of pixel (x, y) in `frame` for a width x height render.
+uint32_t oracle_sample_index(int width, int height, int vertical_invert, int policy, uint32_t offset, int x, int y, int frame) {
+  RenderParams p;
+  p.width = width;
+  p.height = height;
+  p.vertical_invert = vertical_invert != 0;
+  p.index_policy = policy;
+  p.max_samples_offset = offset;
+  return sample_index(p, x, y, frame);
+}
+
 uint64_t oracle_f32_pool_offset(void *sp, int texture) { return f32_pool_offset(*static_cast<Scene *>(sp), texture); }
 
 void oracle_render(void *sp, void *bp, int width, int height, int frame_start, int frames, int max_depth, uint32_t max_samples_offset,
@@ -202,6 +217,7 @@ void oracle_render(void *sp, void *bp, int width, int height, int frame_start, i
     p.inv_V.m[i] = inv_V[i];
   }
   p.threads = threads;
+  p.index_policy = g_index_policy;
   render(*static_cast<Scene *>(sp), *static_cast<Bvh *>(bp), p, accum, final_out);
 }
 

@@ -106,7 +106,12 @@ struct RenderParams {
   int threads = 1;
   int tiles_w = 0, tiles_h = 0;  // 0 = one tile row per thread
   int sampling_roughness = -1;   // gradient-check harness: FP32 texture of the sampling roughness
+  int index_policy = 0;          // 0: 4*p*frame + max_samples_offset (Integrator.h:80); 1: (x*H+y)*max_samples_offset + frame (integrator_optix.cu:43)
 };
+// Sobol index of pixel (x, y) (row 0 at the bottom) in `frame` under p.index_policy.
+uint32_t sample_index(const RenderParams &p, int x, int y, int frame);
+// Index policy the C entry points give the renders they build (test harness setting).
+extern int g_index_policy;
 vec4 path_Li(const Scene &scene, const Bvh &bvh, vec3 o, vec3 d, int depth, SobolSampler &sampler, const RenderParams &p, int scatter_index);
 void render_frame(const Scene &scene, const Bvh &bvh, const RenderParams &p, int frame, float *partial);
 void render(const Scene &scene, const Bvh &bvh, const RenderParams &p, float *accum, float *final_out);

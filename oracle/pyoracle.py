@@ -56,6 +56,9 @@ def lib():
         L.oracle_render_adjoint.argtypes = [P, P, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, C.c_uint32, C.c_int, C.c_int,
                                             f32p, f32p, C.c_int, C.c_int, f32p, C.c_float, np.ctypeslib.ndpointer(np.float64, flags="C")]
         L.oracle_set_float_libm.argtypes = [C.c_int]
+        L.oracle_set_index_policy.argtypes = [C.c_int]
+        L.oracle_sample_index.argtypes = [C.c_int, C.c_int, C.c_int, C.c_int, C.c_uint32, C.c_int, C.c_int, C.c_int]
+        L.oracle_sample_index.restype = C.c_uint32
         L.oracle_f32_pool_offset.argtypes = [P, C.c_int]
         L.oracle_f32_pool_offset.restype = C.c_uint64
         L.oracle_render_rows.argtypes = [P, P, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, f32p, f32p, C.c_int, f32p]
@@ -273,3 +276,13 @@ def set_float_libm(on):
     """True: the host libm's float cosf/sinf/asinf/powf, as the reference's CPU build calls them; False (default):
     evaluated in double and rounded once, the correctly rounded value the HIP path computes."""
     lib().oracle_set_float_libm(int(bool(on)))
+
+
+def set_index_policy(policy):
+    """Sample-index policy of the renders: 0 the CPU path's 4*p*frame + offset (default),
+    1 the GPU raygen's (x*H + y)*offset + frame."""
+    lib().oracle_set_index_policy(int(policy))
+
+
+def sample_index(width, height, x, y, frame, offset=0, policy=0, vertical_invert=False):
+    return int(lib().oracle_sample_index(width, height, int(vertical_invert), int(policy), offset, x, y, frame))

@@ -768,14 +768,24 @@ static std::vector<Tile> divide_by_tiles(int W, int H, int nw, int nh) {
   return tiles;
 }
 
+int g_index_policy = 0;
+
+// Integrator.h:38-42,80 (CPU: idx = 4 * pixel, row flipped with vertical_invert) or the
+// GPU raygen's (x * H + y) * sample_max + frame (integrator_optix.cu:43); u32 wraparound.
+uint32_t sample_index(const RenderParams &p, int x, int y, int frame) {
+  if (p.index_policy == 1) return ((unsigned)x * (unsigned)p.height + (unsigned)y) * p.max_samples_offset + (unsigned)frame;
+  const unsigned idx = p.vertical_invert ? (unsigned)(((p.height - 1 - y) * p.width + x) * 4) : (unsigned)((y * p.width + x) * 4);
+  return idx * (unsigned)frame + p.max_samples_offset;
+}
+
 // AbstractIntegrator::render for one tile (Integrator.h:53-95), spp = 1 per frame.
 static void render_tile(const Scene &s, const Bvh &bvh, const RenderParams &p, int frame, const Tile &t, float *partial) {
   const float RAND_DX = 0.0005f, RAND_DY = 0.0005f;
   SobolSampler sampler;
   for (int y = t.y1 - 1; y >= t.y0; y--)
     for (int x = t.x0; x < t.x1; x++) {
-      unsigned idx = p.vertical_invert ? (unsigned)(((p.height - 1 - y) * p.width + x) * 4) : (unsigned)((y * p.width + x) * 4);
-      sampler.reset(idx * (unsigned)frame + p.max_samples_offset);
+      const unsigned idx = p.vertical_invert ? (unsigned)(((p.height - 1 - y) * p.width + x) * 4) : (unsigned)((y * p.width + x) * 4);
+      sampler.reset(sample_index(p, x, y, frame));
       float s0 = sampler.sample1D(), s1 = sampler.sample1D();
       const float dx = s0 * RAND_DX, dy = s1 * RAND_DY;
       int sy = p.height - y;
@@ -1052,8 +1062,8 @@ void render_adjoint(const Scene &s, const Bvh &bvh, const RenderParams &p, const
       if (r >= rows) break;
       const int frame = p.frame_start + r / p.height, y = r % p.height;
       for (int x = 0; x < p.width; x++) {  // AbstractIntegrator::render for one pixel (Integrator.h:53-95)
-        unsigned idx = p.vertical_invert ? (unsigned)(((p.height - 1 - y) * p.width + x) * 4) : (unsigned)((y * p.width + x) * 4);
-        sampler.reset(idx * (unsigned)frame + p.max_samples_offset);
+        const unsigned idx = p.vertical_invert ? (unsigned)(((p.height - 1 - y) * p.width + x) * 4) : (unsigned)((y * p.width + x) * 4);
+      sampler.reset(sample_index(p, x, y, frame));
         float s0 = sampler.sample1D(), s1 = sampler.sample1D();
         const float dx = s0 * 0.0005f, dy = s1 * 0.0005f;
         int sy = p.height - y;

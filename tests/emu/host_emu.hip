@@ -219,6 +219,7 @@ static int emu_run(const nova_hip_scene_desc *sd, const nova_hip_render_params *
     fp.frames = 1;
     fp.offset = p->max_samples_offset;
     fp.vinvert = p->vertical_invert;
+    fp.policy = p->index_policy;
     std::memcpy(fp.inv_P, p->inv_P, 64);
     std::memcpy(fp.inv_V, p->inv_V, 64);
     const AdjointCtx adj{partial.data(), adj_dimg, adj_scale, adj_grad};

@@ -118,6 +118,21 @@ def test_render_frame_offset_and_vertical_invert(oracle, engine):
 
 
 @gpu
+def test_render_gpu_raygen_index_policy(oracle, engine):
+    from axomae_amd.nova_hip import INDEX_OPTIX
+    sc = scenes.box_room()
+    engine.upload(sc)
+    engine.build_bvh()
+    final, _ = engine.render(48, 40, frames=4, max_depth=3, frame_start=3, max_samples_offset=64, index_policy=INDEX_OPTIX)
+    oracle.set_index_policy(1)
+    try:
+        ref, _ = oracle.OracleScene(sc).render(48, 40, 4, 3, frame_start=3, max_samples_offset=64, threads=8)
+    finally:
+        oracle.set_index_policy(0)
+    assert rel_rmse(final.cpu().numpy(), ref) <= 1e-3
+
+
+@gpu
 def test_bakes_match_cpu_bake(oracle):
     from axomae_amd import nova_hip
     maps = [scenes.sky_envmap(128, 64, (0.2, 0.9, 0.4), sun_intensity=20.0, sun_size=0.01), scenes.sky_envmap(256, 128, (-0.5, 0.3, 0.8))]

@@ -101,3 +101,24 @@ def test_atan2_approx_error_bound_and_quadrants(oracle):
     a = oracle.atan2_approx(y, x)
     # the reference formula (math_utils_approx.h:24-40) maps x<0,y<0 to atan2 - 0 and y<0,x>=0 stays negative
     assert np.max(np.abs(a - np.arctan2(y, x))) < 2e-5
+
+
+def test_sample_index_policies(oracle):
+    """CPU policy (Integrator.h:38-42,80): idx = 4*pixel, pixel row flipped by vertical_invert,
+    times the frame plus renderer_max_samples; GPU raygen policy (integrator_optix.cu:43):
+    (x*H + y)*sample_max + frame. Pinned by properties a wrong term or index would break:
+    under the GPU policy the indices of all pixels and frames 0..S-1 are exactly
+    0..W*H*S-1 (W != H, so swapping the row and column strides fails), and a pixel's S
+    frames are consecutive; under the CPU policy frame 0 gives every pixel the offset, and
+    pixel indices are multiples of 4 whose rows swap under vertical_invert; u32 wraparound."""
+    W, H, S = 7, 5, 6
+    idx = np.array([[[oracle.sample_index(W, H, x, y, f, offset=S, policy=1) for f in range(S)] for y in range(H)] for x in range(W)])
+    assert sorted(idx.ravel().tolist()) == list(range(W * H * S))
+    assert np.all(np.diff(idx, axis=2) == 1)
+    assert all(oracle.sample_index(W, H, x, y, 0, offset=17) == 17 for x in range(W) for y in range(H))
+    one = np.array([[oracle.sample_index(W, H, x, y, 1) for x in range(W)] for y in range(H)])
+    assert np.all(one % 4 == 0) and sorted((one // 4).ravel().tolist()) == list(range(W * H))
+    flip = np.array([[oracle.sample_index(W, H, x, y, 1, vertical_invert=True) for x in range(W)] for y in range(H)])
+    np.testing.assert_array_equal(flip, one[::-1])
+    assert oracle.sample_index(1920, 1080, 1919, 1079, 600) == (4 * 2073599 * 600) % (1 << 32)
+    assert oracle.sample_index(1920, 1080, 1919, 1079, 600, offset=1 << 20, policy=1) == ((1919 * 1080 + 1079) * (1 << 20) + 600) % (1 << 32)
