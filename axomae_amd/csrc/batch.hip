@@ -453,6 +453,10 @@ static int make_plan(nova_hip_ctx *ex, const nova_hip_batch_item *items, uint32_
     if (p->frame_count <= 0 || p->frame_start < 0 || p->max_depth < 0) return set_error(ex, NOVA_HIP_INVALID_ARGUMENT, "bad frame range or depth");
     if (p->index_policy != NOVA_HIP_INDEX_CPU && p->index_policy != NOVA_HIP_INDEX_OPTIX)
       return set_error(ex, NOVA_HIP_INVALID_ARGUMENT, "unknown index_policy");
+    const int32_t passes = NOVA_HIP_INTEGRATOR_NORMAL | NOVA_HIP_INTEGRATOR_DEPTH | NOVA_HIP_INTEGRATOR_EMISSIVE;
+    if (p->integrator_flags != 0 &&
+        (!(p->integrator_flags & NOVA_HIP_INTEGRATOR_PATH) || !(p->integrator_flags & NOVA_HIP_INTEGRATOR_COMBINED) || (p->integrator_flags & passes)))
+      return set_error(ex, NOVA_HIP_INVALID_ARGUMENT, "batched rendering runs the path integrator (PATH | COMBINED) only");
     if (i > 0 && (p->max_depth != bp.max_depth || p->rr_start != bp.rr_start))
       return set_error(ex, NOVA_HIP_INVALID_ARGUMENT, "batch items must share max_depth and rr_start");
     if (!it.accum_dev || !it.final_dev) return set_error(ex, NOVA_HIP_INVALID_BUFFER_STATE, "null render buffer");

@@ -72,16 +72,8 @@ NOVA_HD void sobol_nibble_fill(const uint32_t *table, uint32_t dim0, int ndims, 
 
 // Camera ray of a path slot: jitter from dims 0-1, screen2ndc(x, H - y) and ray_inv_mat
 // (Integrator.h:80-88, math_camera.h:28-58) in glm's operation order.
-NOVA_HD void camera_ray(const FrameParams &fp, const uint32_t *sobol, uint32_t slot, f3 &o, f3 &d) {
-  const uint32_t npix = (uint32_t)fp.width * (uint32_t)fp.height;
-  const uint32_t p = slot % npix;
-  const int x = (int)(p % (uint32_t)fp.width), y = (int)(p / (uint32_t)fp.width);
-  const uint32_t idx = sample_index(fp, slot);
-  const uint32_t g = idx ^ (idx >> 1);
-  const float dx = sobol_dim(sobol, 0, g) * 0.0005f, dy = sobol_dim(sobol, 1, g) * 0.0005f;
-  const int sy = fp.height - y;
-  const float nx = (float)(2 * x - fp.width) / (float)fp.width + dx;
-  const float ny = (float)(fp.height - 2 * sy) / (float)fp.height + dy;
+// ray_inv_mat at NDC (nx, ny).
+NOVA_HD void camera_ray_ndc(const FrameParams &fp, float nx, float ny, f3 &o, f3 &d) {
   const float *P = fp.inv_P, *V = fp.inv_V;
   float ox = mat4_row(P, 0, nx, ny, -1.f, 1.f), oy = mat4_row(P, 1, nx, ny, -1.f, 1.f), oz = mat4_row(P, 2, nx, ny, -1.f, 1.f),
         ow = mat4_row(P, 3, nx, ny, -1.f, 1.f);
@@ -95,6 +87,29 @@ NOVA_HD void camera_ray(const FrameParams &fp, const uint32_t *sobol, uint32_t s
   const float xx = vx * vx, yy = vy * vy, zz = vz * vz, ww = vw * vw;
   const float inv_len = 1.f / sqrtf((xx + yy) + (zz + ww));  // glm::normalize(vec4)
   d = mk3(vx * inv_len, vy * inv_len, vz * inv_len);
+}
+
+NOVA_HD void camera_ray(const FrameParams &fp, const uint32_t *sobol, uint32_t slot, f3 &o, f3 &d) {
+  const uint32_t npix = (uint32_t)fp.width * (uint32_t)fp.height;
+  const uint32_t p = slot % npix;
+  const int x = (int)(p % (uint32_t)fp.width), y = (int)(p / (uint32_t)fp.width);
+  const uint32_t idx = sample_index(fp, slot);
+  const uint32_t g = idx ^ (idx >> 1);
+  const float dx = sobol_dim(sobol, 0, g) * 0.0005f, dy = sobol_dim(sobol, 1, g) * 0.0005f;
+  const int sy = fp.height - y;
+  const float nx = (float)(2 * x - fp.width) / (float)fp.width + dx;
+  const float ny = (float)(fp.height - 2 * sy) / (float)fp.height + dy;
+  camera_ray_ndc(fp, nx, ny, o, d);
+}
+
+// Camera ray of pixel p for the render passes: the pixel corner, no jitter
+// (NormalIntegrator.cpp / DepthIntegrator.cpp / EmissiveIntegrator.cpp render loops).
+NOVA_HD void camera_ray_pass(const FrameParams &fp, uint32_t p, f3 &o, f3 &d) {
+  const int x = (int)(p % (uint32_t)fp.width), y = (int)(p / (uint32_t)fp.width);
+  const int sy = fp.height - y;
+  const float nx = (float)(2 * x - fp.width) / (float)fp.width;
+  const float ny = (float)(fp.height - 2 * sy) / (float)fp.height;
+  camera_ray_ndc(fp, nx, ny, o, d);
 }
 
 // Moller-Trumbore exactly as Triangle::hit orders it (Triangle.h:158-175); accepts
@@ -149,37 +164,35 @@ NOVA_HD void splat_texture_grad(const ShadeCtx &sc, float *grad, int tex, float 
       if (g[ch] != 0.f) add_grad(base + (size_t)idx[k] * t.channels + ch, w[k] * g[ch]);
 }
 
-// One path segment after its closest hit: the miss/envmap rule, the hit record, the
-// Principled scatter, the Russian-roulette extension and Ray::spawn. Updates the path
-// state (T, L, A) front to back (see integrator.hip) and returns true when the path
-// continues with the spawned ray (no, nd); on false the path is finished and L, A hold
-// its radiance and alpha.
-// With kAdj the segment is replayed: after a scatter the radiance still to come,
-// R = L_path - L (everything after this vertex, T c V_next), gives the loss gradient of
-// the vertex's weight c; it is pushed to the albedo and roughness texels through
-// d ln c / d(param) and the bilinear weights.
+// The surface half of PrincipledMaterial::scatter (NovaMaterials.h:89-137) at a hit: the
+// hit record, the material's textures, the shading normal and the Principled BSDF sample
+// (sampleBSDF, NovaMaterials.h:63-79: dims dim0 (lobe), dim0+1..2 (u0), dim0+3 (uc)).
+struct SurfaceSample {
+  HitRec h;
+  int32_t material;
+  f3 ns;           // shading normal from the normal map, before the specular correction (mat_rec.normal)
+  BsdfSample smp;  // wi in world space
+  bool ok;         // PrincipledBSDF::sample_f succeeded
+  f3 dlna;         // with kAdj: d ln c / d albedo channel
+  float dlnr;      // with kAdj: d ln c / d roughness
+};
+
 template <bool kAdj = false, class Sobol = const uint32_t *>
-NOVA_HD bool shade_path(const ShadeParams &sp, const ShadeCtx &sc, Sobol sobol, uint32_t slot, f3 o, f3 d, int prim, float ht, float hu,
-                        float hv, f3 &T, f3 &L, float &A, f3 &no, f3 &nd, bool &escaped, const AdjointCtx *adj = nullptr) {
-  escaped = false;
-  if (prim < 0) {  // Li on a miss: envmap(ray.direction) (Integrator.cpp:93-97)
-    escaped = true;
-    const float4 env = sample_envmap(sc, d);
-    L = L + T * mk3(env.x, env.y, env.z);
-    A = A + env.w;
-    return false;
-  }
+NOVA_HD void sample_surface(const ShadeParams &sp, const ShadeCtx &sc, Sobol sobol, uint32_t slot, f3 o, f3 d, int prim, float ht, float hu, float hv,
+                            SurfaceSample &out) {
   const ShadeTri &tr = sc.shade[prim];
-  const HitRec h = make_hit(sc, tr, o, d, ht, hu, hv);
+  out.h = make_hit(sc, tr, o, d, ht, hu, hv);
+  const HitRec &h = out.h;
+  out.material = tr.material;
   const MatRec &mat = sc.mats[tr.material];
   const float ONE_255 = 1.f / 255.f;
-  // PrincipledMaterial::scatter (NovaMaterials.h:89-137)
   f3 nrm = mk3(0.f, 0.f, 1.f);
   if (mat.normal >= 0) {
     float4 t4 = sample_u32(sc, mat.normal, h.u, h.v);
     nrm = mk3(t4.x * ONE_255, t4.y * ONE_255, t4.z * ONE_255);
   }
   const f3 ns = normalize(mul(h.frame, nrm * 2.f - 1.f));
+  out.ns = ns;
   const f3 wo = normalize(-d);
   BsdfIn bi;
   bi.ns = ns;
@@ -210,7 +223,6 @@ NOVA_HD bool shade_path(const ShadeParams &sp, const ShadeCtx &sc, Sobol sobol, 
   bi.anisotropy = mat.anisotropy;
   bi.eta = mk3(mat.eta[0], mat.eta[1], mat.eta[2]);
 
-  // sampleBSDF (NovaMaterials.h:63-79): dims dim0 (lobe), dim0+1..2 (u0), dim0+3 (uc)
   const uint32_t idx = sample_index(sp.fp, slot);
   const uint32_t g = idx ^ (idx >> 1);
   const float uc = sobol_dim(sobol, sp.dim0, g);
@@ -237,8 +249,8 @@ NOVA_HD bool shade_path(const ShadeParams &sp, const ShadeCtx &sc, Sobol sobol, 
   // PrincipledBSDF::sample_f (BSDF.h:107-130)
   BsdfSample smp{};
   bool ok = false;
-  f3 dlna = mk3(0.f, 0.f, 0.f);
-  float dlnr = 0.f;
+  out.dlna = mk3(0.f, 0.f, 0.f);
+  out.dlnr = 0.f;
   if (b.lobe != LOBE_NONE) {
     const f3 lwo = normalize(mul_t(b.frame, wo));
     if ((b.flags(ggx) & (BX_REFLECTION | BX_TRANSMISSION)) && lwo.z != 0.f) {
@@ -249,18 +261,54 @@ NOVA_HD bool shade_path(const ShadeParams &sp, const ShadeCtx &sc, Sobol sobol, 
       else
         ok = sample_dielectric(b, ggx, ggx_e, lwo, uc2, u0, u1, smp);
       if (ok && ((smp.f.x == 0.f && smp.f.y == 0.f && smp.f.z == 0.f) || smp.pdf == 0.f || smp.wi.z == 0.f)) ok = false;
-      if (kAdj && ok) bsdf_log_derivatives(b, ggx_e, lwo, smp, dlna, dlnr);
+      if (kAdj && ok) bsdf_log_derivatives(b, ggx_e, lwo, smp, out.dlna, out.dlnr);
       if (ok) smp.wi = normalize(mul(b.frame, smp.wi));
     }
   }
-  if (!ok) return false;  // scatter failure: this level contributes 0 (Integrator.cpp:72-78)
+  out.smp = smp;
+  out.ok = ok;
+}
+
+// Emitted radiance of a material at (u, v): the emissive texture / 255 (TexturePackSampler
+// emissive, NovaMaterials.h:133) times the emission-scale extension (1 by default).
+NOVA_HD f3 emitted(const ShadeCtx &sc, const MatRec &mat, float u, float v) {
+  if (mat.emissive < 0) return mk3(0.f, 0.f, 0.f);
+  const float ONE_255 = 1.f / 255.f;
+  const float4 e4 = sample_u32(sc, mat.emissive, u, v);
+  return mk3(e4.x * ONE_255, e4.y * ONE_255, e4.z * ONE_255) * mat.emission_scale;
+}
+
+// One path segment after its closest hit: the miss/envmap rule, the hit record, the
+// Principled scatter, the Russian-roulette extension and Ray::spawn. Updates the path
+// state (T, L, A) front to back (see integrator.hip) and returns true when the path
+// continues with the spawned ray (no, nd); on false the path is finished and L, A hold
+// its radiance and alpha.
+// With kAdj the segment is replayed: after a scatter the radiance still to come,
+// R = L_path - L (everything after this vertex, T c V_next), gives the loss gradient of
+// the vertex's weight c; it is pushed to the albedo and roughness texels through
+// d ln c / d(param) and the bilinear weights.
+template <bool kAdj = false, class Sobol = const uint32_t *>
+NOVA_HD bool shade_path(const ShadeParams &sp, const ShadeCtx &sc, Sobol sobol, uint32_t slot, f3 o, f3 d, int prim, float ht, float hu,
+                        float hv, f3 &T, f3 &L, float &A, f3 &no, f3 &nd, bool &escaped, const AdjointCtx *adj = nullptr) {
+  escaped = false;
+  if (prim < 0) {  // Li on a miss: envmap(ray.direction) (Integrator.cpp:93-97)
+    escaped = true;
+    const float4 env = sample_envmap(sc, d);
+    L = L + T * mk3(env.x, env.y, env.z);
+    A = A + env.w;
+    return false;
+  }
+  SurfaceSample ss;
+  sample_surface<kAdj>(sp, sc, sobol, slot, o, d, prim, ht, hu, hv, ss);
+  if (!ss.ok) return false;  // scatter failure: this level contributes 0 (Integrator.cpp:72-78)
+  const HitRec &h = ss.h;
+  const BsdfSample &smp = ss.smp;
+  const MatRec &mat = sc.mats[ss.material];
   f3 c = smp.f * smp.costheta / smp.pdf;
   if (!(isfinite(c.x) && isfinite(c.y) && isfinite(c.z))) return false;  // DENAN zeroes this level
-  f3 Le = mk3(0.f, 0.f, 0.f);
-  if (mat.emissive >= 0) {
-    float4 e4 = sample_u32(sc, mat.emissive, h.u, h.v);
-    Le = mk3(e4.x * ONE_255, e4.y * ONE_255, e4.z * ONE_255) * mat.emission_scale;
-  }
+  const f3 Le = emitted(sc, mat, h.u, h.v);
+  const uint32_t idx = sample_index(sp.fp, slot);
+  const uint32_t g = idx ^ (idx >> 1);
   bool survive = true;
   int rr_channel = -1;  // channel whose T c set q < 1 (the rescale c / q depends on it)
   if (sp.rr) {  // Russian-roulette extension: one more dimension after the scatter's four
@@ -278,6 +326,8 @@ NOVA_HD bool shade_path(const ShadeParams &sp, const ShadeCtx &sc, Sobol sobol, 
   A = A + 1.f;
   if (!survive) return false;
   if (kAdj) {
+    const f3 dlna = ss.dlna;
+    const float dlnr = ss.dlnr;
     const uint32_t npix = (uint32_t)sp.fp.width * (uint32_t)sp.fp.height;
     const uint32_t px = pixel_offset(sp.fp, slot % npix);
     const float *lt = adj->ltot + (size_t)(slot / npix) * npix * 4 + px;
@@ -306,6 +356,43 @@ NOVA_HD bool shade_path(const ShadeParams &sp, const ShadeCtx &sc, Sobol sobol, 
     return false;
   }
   return true;
+}
+
+// Render passes, one value per pixel (the same for every frame):
+//  NORMAL   (NormalIntegrator::Li): the material's shading normal at the hit, (ns, 1); 0 on a miss.
+//  DEPTH    (DepthIntegrator::render): z = 1 - (t - near)/(far - near),
+//           2 near far / (far + near - z (far - near)) * 2 - 1, divided by far, in all four
+//           channels; far in all four on a miss.
+//  EMISSIVE (EmissiveIntegrator::Li): the emissive texel / 255 and alpha 1 when the scatter
+//           succeeds, else 0.
+// The passes reset the sampler to the pixel offset (4 * pixel, no frame factor) and scatter
+// with dims 0-3; `fp` is the render's, pixel p in [0, W*H).
+enum : int { PASS_NORMAL = 1, PASS_DEPTH = 2, PASS_EMISSIVE = 3 };
+NOVA_HD float4 pass_value(int pass, const FrameParams &fp, const ShadeCtx &sc, const uint32_t *sobol, uint32_t p, f3 o, f3 d, int prim, float t,
+                          float u, float v, float z_near, float z_far) {
+  if (pass == PASS_DEPTH) {
+    if (prim < 0) return make_float4(z_far, z_far, z_far, z_far);
+    float z = 1 - (t - z_near) / (z_far - z_near);
+    z = (2.0f * z_near * z_far) / (z_far + z_near - z * (z_far - z_near)) * 2.f - 1.f;
+    const float r = z / z_far;
+    return make_float4(r, r, r, r);
+  }
+  if (prim < 0) return make_float4(0.f, 0.f, 0.f, 0.f);
+  ShadeParams sp;
+  sp.fp = fp;
+  sp.fp.frame0 = 1;  // sample index = pixel offset * 1 + 0
+  sp.fp.offset = 0;
+  sp.fp.policy = NOVA_HIP_INDEX_CPU;
+  sp.bounce = 0;
+  sp.last = 0;
+  sp.rr = 0;
+  sp.dim0 = 0;
+  SurfaceSample ss;
+  sample_surface<false>(sp, sc, sobol, p, o, d, prim, t, u, v, ss);
+  if (pass == PASS_NORMAL) return make_float4(ss.ns.x, ss.ns.y, ss.ns.z, 1.f);
+  if (!ss.ok) return make_float4(0.f, 0.f, 0.f, 0.f);
+  const f3 e = emitted(sc, sc.mats[ss.material], ss.h.u, ss.h.v);
+  return make_float4(e.x, e.y, e.z, 1.f);
 }
 
 }  // namespace nova_hip

@@ -275,6 +275,38 @@ __global__ __launch_bounds__(kTraceBlock) void closest_hit_kernel(TraceCtx tc, c
   }
 }
 
+// Render passes (NORMAL / DEPTH / EMISSIVE): one unjittered ray per pixel, the pass value
+// (pass_value) accumulated for every frame as the path integrator's frames are.
+__global__ __launch_bounds__(kTraceBlock) void pass_kernel(TraceCtx tc, ShadeCtx sc, const uint32_t *__restrict__ sobol, FrameParams fp, int pass,
+                                                           float z_near, float z_far, float *__restrict__ accum, float *__restrict__ final_out,
+                                                           float *__restrict__ partial_out) {
+  __shared__ uint32_t lds[kLdsStack * kTraceBlock];
+  const uint32_t npix = (uint32_t)fp.width * (uint32_t)fp.height;
+  const uint32_t gtid = blockIdx.x * blockDim.x + threadIdx.x;
+  LdsStack stk{lds, tc.gstack, tc.gstride, gtid, threadIdx.x};
+  for (uint32_t p = gtid; p < npix; p += gridDim.x * blockDim.x) {
+    f3 o, d;
+    camera_ray_pass(fp, p, o, d);
+    float t, u, v;
+    uint32_t nv = 0, nt = 0;
+    const int prim = trace_closest<false>(tc, stk, o, d, 0.000001f, t, u, v, nv, nt);
+    const float4 val = pass_value(pass, fp, sc, sobol, p, o, d, prim, t, u, v, z_near, z_far);
+    const uint32_t i = pixel_offset(fp, p);
+    float a[4] = {accum[i], accum[i + 1], accum[i + 2], accum[i + 3]}, fin[4] = {0.f, 0.f, 0.f, 0.f};
+    const float vv[4] = {val.x, val.y, val.z, val.w};
+    for (int f = 0; f < fp.frames; f++)  // Engine::writeFramebuffer per frame, in frame order
+      for (int c = 0; c < 4; c++) {
+        a[c] += vv[c];
+        fin[c] = a[c] / float(fp.frame0 + f + 1);
+      }
+    for (int c = 0; c < 4; c++) {
+      accum[i + c] = a[c];
+      final_out[i + c] = fin[c];
+      if (partial_out) partial_out[i + c] = vv[c];
+    }
+  }
+}
+
 #ifndef NOVA_PATHS_PER_BATCH
 #define NOVA_PATHS_PER_BATCH (128u << 20)
 #endif
@@ -318,6 +350,64 @@ extern "C" int nova_hip_closest_hit(nova_hip_ctx *ctx, const float *rays_dev, ui
   return NOVA_HIP_SUCCESS;
 }
 
+// integrator_dispatch (Integrator.cpp:17-45): 0 for the path integrator, else the PASS_* that
+// writes the frame last.
+static int resolve_integrator(nova_hip_ctx *ctx, int32_t flags, int *pass) {
+  *pass = 0;
+  if (flags == 0) return NOVA_HIP_SUCCESS;
+  if (!(flags & NOVA_HIP_INTEGRATOR_PATH)) return set_error(ctx, NOVA_HIP_RENDERER_EXCEPTION, "INVALID_INTEGRATOR: integrator_flags without PATH");
+  if (flags & NOVA_HIP_INTEGRATOR_EMISSIVE)
+    *pass = PASS_EMISSIVE;
+  else if (flags & NOVA_HIP_INTEGRATOR_DEPTH)
+    *pass = PASS_DEPTH;
+  else if (flags & NOVA_HIP_INTEGRATOR_NORMAL)
+    *pass = PASS_NORMAL;
+  else if (!(flags & NOVA_HIP_INTEGRATOR_COMBINED))
+    return set_error(ctx, NOVA_HIP_RENDERER_EXCEPTION, "INVALID_RENDER_MODE: no render pass in integrator_flags");
+  return NOVA_HIP_SUCCESS;
+}
+
+static int render_pass(nova_hip_ctx *ctx, const nova_hip_render_params *p, int pass, float *accum_dev, float *final_dev, float *partial_dev,
+                       hipStream_t s) {
+  const uint32_t npix = (uint32_t)p->width * (uint32_t)p->height;
+  const uint32_t blocks = std::min<uint32_t>((npix + kTraceBlock - 1) / kTraceBlock, (uint32_t)device_cus(ctx) * 8);
+  const uint32_t threads = blocks * kTraceBlock;
+  int st = ensure_gstack(ctx, threads);
+  if (st) return st;
+  TraceCtx tc{ctx->nodes4, ctx->tri_leaf, ctx->gstack, threads, ctx->tie};
+  ShadeCtx sc{ctx->textures, ctx->tex_u32, ctx->tex_f32, ctx->materials, ctx->mesh_xf, ctx->shade, ctx->acos_lut, ctx->envmap, -1};
+  FrameParams fp;
+  fp.width = p->width;
+  fp.height = p->height;
+  fp.frame0 = p->frame_start;
+  fp.frames = p->frame_count;
+  fp.offset = 0;
+  fp.vinvert = p->vertical_invert;
+  fp.policy = NOVA_HIP_INDEX_CPU;
+  for (int i = 0; i < 16; i++) {
+    fp.inv_P[i] = p->inv_P[i];
+    fp.inv_V[i] = p->inv_V[i];
+  }
+  hipEvent_t ev0, ev1;
+  NOVA_HIP_CHECK(ctx, hipEventCreate(&ev0));
+  NOVA_HIP_CHECK(ctx, hipEventCreate(&ev1));
+  NOVA_HIP_CHECK(ctx, hipEventRecord(ev0, s));
+  hipLaunchKernelGGL(pass_kernel, dim3(blocks), dim3(kTraceBlock), 0, s, tc, sc, ctx->sobol_table, fp, pass, p->z_near, p->z_far, accum_dev, final_dev,
+                     partial_dev);
+  NOVA_HIP_CHECK(ctx, hipGetLastError());
+  NOVA_HIP_CHECK(ctx, hipEventRecord(ev1, s));
+  NOVA_HIP_CHECK(ctx, hipEventSynchronize(ev1));
+  float ms = 0.f;
+  NOVA_HIP_CHECK(ctx, hipEventElapsedTime(&ms, ev0, ev1));
+  NOVA_HIP_CHECK(ctx, hipEventDestroy(ev0));
+  NOVA_HIP_CHECK(ctx, hipEventDestroy(ev1));
+  ctx->stats = nova_hip_render_stats{};
+  ctx->stats.total_ms = ms;
+  ctx->stats.segments = npix;
+  ctx->stats.extend_launches = 1;
+  return NOVA_HIP_SUCCESS;
+}
+
 // Renders (adj == nullptr) or renders and replays for the adjoint, batch by batch.
 static int render_impl(nova_hip_ctx *ctx, const nova_hip_render_params *p, float *accum_dev, float *final_dev, float *partial_dev, void *stream,
                        const AdjointCtx *adj) {
@@ -340,11 +430,16 @@ static int render_impl(nova_hip_ctx *ctx, const nova_hip_render_params *p, float
     for (int k = 0; k <= p->max_depth; k++) dims += 4 + ((p->rr_start >= 0 && k >= p->rr_start) ? 1 : 0);
     if (dims > 256) return set_error(ctx, NOVA_HIP_INVALID_ARGUMENT, "max_depth needs more than 256 sampler dimensions");
   }
+  int pass = 0;
+  int st = resolve_integrator(ctx, p->integrator_flags, &pass);
+  if (st) return st;
+  if (pass != 0 && adj) return set_error(ctx, NOVA_HIP_INVALID_ARGUMENT, "the adjoint differentiates the path integrator only");
   hipStream_t s = (hipStream_t)stream;
   NOVA_HIP_CHECK(ctx, hipSetDevice(ctx->device));
+  if (pass != 0) return render_pass(ctx, p, pass, accum_dev, final_dev, partial_dev, s);
   const size_t npix = (size_t)p->width * p->height;
   // Frames per batch: up to kPathsPerBatch paths in flight (about 116 B of queue and path
-  // state each, 3.9 GB at the limit). Large batches matter because the last bounces hold
+  // state each, 15.6 GB at the limit). Large batches matter because the last bounces hold
   // few rays whose launches cost their slowest ray's latency, a fixed cost per batch.
   int fb = (int)std::max<size_t>(1, std::min<size_t>((size_t)p->frame_count, kPathsPerBatch / npix));
   const size_t cap = npix * fb;
@@ -383,7 +478,7 @@ static int render_impl(nova_hip_ctx *ctx, const nova_hip_render_params *p, float
   const int cus = device_cus(ctx);
   const uint32_t trace_blocks = (uint32_t)std::min<size_t>((cap + kTraceBlock - 1) / kTraceBlock, (size_t)cus * 8);
   const uint32_t trace_threads = trace_blocks * kTraceBlock;
-  int st = ensure_gstack(ctx, trace_threads);
+  st = ensure_gstack(ctx, trace_threads);
   if (st) return st;
   TraceCtx tc{ctx->nodes4, ctx->tri_leaf, ctx->gstack, trace_threads, ctx->tie};
   ShadeCtx sc{ctx->textures, ctx->tex_u32, ctx->tex_f32, ctx->materials, ctx->mesh_xf, ctx->shade, ctx->acos_lut, ctx->envmap,

@@ -62,10 +62,13 @@ class BvhInfo(C.Structure):
 class RenderParams(C.Structure):
     _fields_ = [("width", C.c_int32), ("height", C.c_int32), ("max_depth", C.c_int32), ("frame_start", C.c_int32), ("frame_count", C.c_int32),
                 ("max_samples_offset", C.c_uint32), ("vertical_invert", C.c_int32), ("rr_start", C.c_int32), ("inv_P", C.c_float * 16),
-                ("inv_V", C.c_float * 16), ("flags", C.c_int32), ("sampling_roughness_texture", C.c_int32), ("index_policy", C.c_int32)]
+                ("inv_V", C.c_float * 16), ("flags", C.c_int32), ("sampling_roughness_texture", C.c_int32), ("index_policy", C.c_int32),
+                ("integrator_flags", C.c_int32), ("z_near", C.c_float), ("z_far", C.c_float)]
 
 
 INDEX_CPU, INDEX_OPTIX = 0, 1
+# integrator::TYPE (api_common.h:29-49)
+INTEGRATOR_PATH, INTEGRATOR_COMBINED, INTEGRATOR_NORMAL, INTEGRATOR_DEPTH, INTEGRATOR_EMISSIVE = 1, 1 << 8, 1 << 9, 1 << 10, 1 << 13
 
 
 class BakeJob(C.Structure):
@@ -202,10 +205,13 @@ def scene_desc(scene):
 
 
 def render_params(scene, width, height, frames, max_depth, frame_start=0, rr_start=-1, max_samples_offset=0, vertical_invert=False,
-                  inv_P=None, inv_V=None, flags=0, sampling_roughness_texture=-1, index_policy=INDEX_CPU):
+                  inv_P=None, inv_V=None, flags=0, sampling_roughness_texture=-1, index_policy=INDEX_CPU, integrator_flags=0, z_near=0.1,
+                  z_far=1000.0):
     p = RenderParams()
     p.sampling_roughness_texture = sampling_roughness_texture
     p.index_policy = index_policy
+    p.integrator_flags = integrator_flags
+    p.z_near, p.z_far = z_near, z_far
     p.width, p.height, p.max_depth = width, height, max_depth
     p.frame_start, p.frame_count = frame_start, frames
     p.max_samples_offset = max_samples_offset
@@ -278,11 +284,13 @@ class HipEngine:
 
     # --------------------------------------------------------------- render
     def render(self, width, height, frames, max_depth, frame_start=0, rr_start=-1, max_samples_offset=0, vertical_invert=False, accum=None,
-               inv_P=None, inv_V=None, flags=0, sampling_roughness_texture=-1, index_policy=INDEX_CPU):
-        """Renders frames [frame_start, frame_start+frames); returns (final, accum) device tensors (H, W, 4)."""
+               inv_P=None, inv_V=None, flags=0, sampling_roughness_texture=-1, index_policy=INDEX_CPU, integrator_flags=0, z_near=0.1,
+               z_far=1000.0):
+        """Renders frames [frame_start, frame_start+frames); returns (final, accum) device tensors (H, W, 4).
+        integrator_flags selects a render pass (INTEGRATOR_PATH | INTEGRATOR_NORMAL, ...); 0 is the path integrator."""
         torch = self.torch
         p = render_params(self.scene, width, height, frames, max_depth, frame_start, rr_start, max_samples_offset, vertical_invert, inv_P, inv_V,
-                          flags, sampling_roughness_texture, index_policy)
+                          flags, sampling_roughness_texture, index_policy, integrator_flags, z_near, z_far)
         if accum is None:
             accum = torch.zeros((height, width, 4), dtype=torch.float32, device="cuda")
         final = torch.empty((height, width, 4), dtype=torch.float32, device="cuda")

@@ -155,9 +155,23 @@ typedef struct {
                              with vertical_invert (Integrator.h:38-42,80; the parity default);
                              NOVA_HIP_INDEX_OPTIX: (x*H+y)*max_samples_offset + f, the reference's
                              GPU raygen (integrator_optix.cu:43), max_samples_offset = sample_max. */
+  int32_t integrator_flags; /* EngineResourcesHolder::integrator_flag (api_common.h:29-49); 0 means
+                             PATH | COMBINED. With PATH, the pass bits NORMAL, DEPTH and EMISSIVE
+                             select a render pass instead of the path integrator; when several are
+                             set the last in integrator_dispatch order wins (COMBINED, NORMAL,
+                             DEPTH, EMISSIVE), since each overwrites the frame. */
+  float z_near, z_far;      /* CameraResourcesHolder near / far, read by the DEPTH pass */
 } nova_hip_render_params;
 
 enum { NOVA_HIP_INDEX_CPU = 0, NOVA_HIP_INDEX_OPTIX = 1 };
+/* integrator::TYPE values (api_common.h:29-49) */
+enum {
+  NOVA_HIP_INTEGRATOR_PATH = 1 << 0,
+  NOVA_HIP_INTEGRATOR_COMBINED = 1 << 8,
+  NOVA_HIP_INTEGRATOR_NORMAL = 1 << 9,
+  NOVA_HIP_INTEGRATOR_DEPTH = 1 << 10,
+  NOVA_HIP_INTEGRATOR_EMISSIVE = 1 << 13
+};
 
 /* flags: time every extend/shade launch with HIP events on the render stream; count BVH node
  * visits and triangle tests per ray (an instrumented traversal, slower). */
@@ -170,6 +184,11 @@ enum { NOVA_HIP_RENDER_TIMING = 1, NOVA_HIP_RENDER_COUNT_TRAVERSAL = 2 };
  * final = accum / (frame+1). accum_dev and final_dev are device RGBA32F buffers of
  * width*height*4 floats; accum must hold the previous frames' sum (zero at frame 0).
  * partial_dev (optional) receives the last frame's radiance (HdrBufferStruct::partial_buffer).
+ * The render passes (integrator_flags NORMAL / DEPTH / EMISSIVE: NormalIntegrator.cpp,
+ * DepthIntegrator.cpp, EmissiveIntegrator.cpp) trace one unjittered ray per pixel, give the
+ * same value every frame and accumulate it the same way. A flag word without PATH returns
+ * NOVA_HIP_RENDERER_EXCEPTION (nova::exception::INVALID_INTEGRATOR), PATH without a pass
+ * bit likewise (INVALID_RENDER_MODE).
  * Replaces gpu_draw -> device_start_integrator -> OptixIntersector::traverse
  * (corelib/nova/utils/DrawEngine_GPU.cpp:14-20, integrator/cuda/job_dispatch.cpp:7-12,
  * aggregate/api/optix/device.cpp:18-21). */

@@ -185,6 +185,12 @@ void oracle_render_adjoint(void *sp, void *bp, int width, int height, int frame_
 // 0 (default): evaluated in double and rounded once (what the HIP path computes).
 void oracle_set_float_libm(int on) { set_float_libm(on != 0); }
 
+void oracle_render_pass(void *sp, void *bp, int width, int height, int frame_start, int frames, int vertical_invert, const float *inv_P,
+                        const float *inv_V, int threads, int pass, float z_near, float z_far, float *accum, float *final_out) {
+  RenderParams p = make_params(width, height, frame_start, frames, 0, 0, vertical_invert, -1, inv_P, inv_V, threads, -1);
+  render_pass(*static_cast<Scene *>(sp), *static_cast<Bvh *>(bp), p, pass, z_near, z_far, accum, final_out);
+}
+
 // Index policy of the renders the entry points below build (0: CPU, 1: the GPU raygen's).
 void oracle_set_index_policy(int policy) { g_index_policy = policy; }
 

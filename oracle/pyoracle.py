@@ -55,6 +55,8 @@ def lib():
                                              f32p, f32p, C.c_int, C.c_int, f32p, f32p]
         L.oracle_render_adjoint.argtypes = [P, P, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, C.c_uint32, C.c_int, C.c_int,
                                             f32p, f32p, C.c_int, C.c_int, f32p, C.c_float, np.ctypeslib.ndpointer(np.float64, flags="C")]
+        L.oracle_render_pass.argtypes = [P, P, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, f32p, f32p, C.c_int, C.c_int, C.c_float, C.c_float,
+                                         f32p, f32p]
         L.oracle_set_float_libm.argtypes = [C.c_int]
         L.oracle_set_index_policy.argtypes = [C.c_int]
         L.oracle_sample_index.argtypes = [C.c_int, C.c_int, C.c_int, C.c_int, C.c_uint32, C.c_int, C.c_int, C.c_int]
@@ -174,6 +176,15 @@ class OracleScene:
         out = np.zeros(n, dtype=np.float32)
         lib().oracle_render_sampling(self.h, self.bvh, width, height, frame_start, frames, max_depth, max_samples_offset, int(vertical_invert),
                                      rr_start, self.scene.inv_P, self.scene.inv_V, threads, sampling_roughness_texture, acc, out)
+        return out.reshape(height, width, 4), acc
+
+    def render_pass(self, width, height, frames, pass_, z_near=0.1, z_far=1000.0, frame_start=0, vertical_invert=False, threads=8):
+        """Render pass pass_ (1 normal, 2 depth, 3 emissive) accumulated over the frames; returns (final, accum)."""
+        n = width * height * 4
+        acc = np.zeros(n, dtype=np.float32)
+        out = np.zeros(n, dtype=np.float32)
+        lib().oracle_render_pass(self.h, self.bvh, width, height, frame_start, frames, int(vertical_invert), self.scene.inv_P, self.scene.inv_V, threads,
+                                 pass_, z_near, z_far, acc, out)
         return out.reshape(height, width, 4), acc
 
     def f32_pool(self):

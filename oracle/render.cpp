@@ -586,6 +586,7 @@ static vec3 correct_specular_shading_normal(const vec3 &ng, const vec3 &wo, vec3
 
 struct ScatterResult {
   Spec attenuation, emissive;
+  vec3 normal;  // mat_rec.normal: the shading normal from the normal map
   Sample lobe;
   Principled bsdf;  // the configured BSDF (adjoint)
   float u = 0, v = 0;
@@ -604,6 +605,7 @@ static bool scatter(const Scene &s, const Material &mat, const vec3 &in_dir, con
   }
   vec3 local_n = nrm * 2.f - 1.f;
   vec3 ns = normalize(h.frame * local_n);
+  out.normal = ns;
   BsdfParams bp;
   bp.dpdu = h.dpdu;
   bp.dpdv = h.dpdv;
@@ -855,6 +857,64 @@ void render(const Scene &s, const Bvh &bvh, const RenderParams &p, float *accum,
   }
 }
 
+
+// Render passes: NormalIntegrator / DepthIntegrator / EmissiveIntegrator (NormalIntegrator.cpp,
+// DepthIntegrator.cpp, EmissiveIntegrator.cpp). One ray per pixel through screen2ndc without
+// jitter, the sampler reset to the pixel offset (no frame factor); the frame is the same each
+// time and is accumulated as the path integrator's frames are. pass: 1 normal, 2 depth,
+// 3 emissive.
+void render_pass(const Scene &s, const Bvh &bvh, const RenderParams &p, int pass, float z_near, float z_far, float *accum, float *final_out) {
+  const size_t n = (size_t)p.width * p.height * 4;
+  std::vector<float> partial(n, 0.f);
+  std::atomic<int> next{0};
+  auto worker = [&]() {
+    SobolSampler sampler;
+    for (;;) {
+      const int y = next.fetch_add(1);
+      if (y >= p.height) break;
+      for (int x = 0; x < p.width; x++) {
+        const unsigned idx = p.vertical_invert ? (unsigned)(((p.height - 1 - y) * p.width + x) * 4) : (unsigned)((y * p.width + x) * 4);
+        sampler.reset(idx);
+        const int sy = p.height - y;
+        const float ndc_x = (float)(2 * x - p.width) / (float)p.width;
+        const float ndc_y = (float)(p.height - 2 * sy) / (float)p.height;
+        vec3 o, d;
+        camera_ray(p, ndc_x, ndc_y, o, d);
+        float t = 0, bu = 0, bv = 0;
+        const int prim = closest_hit(s, bvh, o, d, 0.000001f, t, bu, bv);
+        vec4 rgb(0.f);
+        if (pass == 2) {  // DepthIntegrator::render; Li gives MAXFLOAT on a miss
+          const float dist = prim >= 0 ? t : 3.402823466e+38f;
+          float depth = 1 - (dist - z_near) / (z_far - z_near);
+          depth = (2.0f * z_near * z_far) / (z_far + z_near - depth * (z_far - z_near)) * 2.f - 1.f;
+          rgb = vec4(depth / z_far);
+          if (dist == 3.402823466e+38f) rgb = vec4(z_far);
+        } else if (prim >= 0) {
+          Hit h = make_hit(s, prim, o, d, t, bu, bv);
+          const Material &mat = s.materials[s.meshes[s.prim_mesh[prim]].material];
+          ScatterResult sr;
+          const bool valid = scatter(s, mat, d, h, sampler, sr);
+          if (pass == 1)
+            rgb = vec4(sr.normal.x, sr.normal.y, sr.normal.z, 1.f);
+          else if (valid)
+            rgb = vec4(sr.emissive.x, sr.emissive.y, sr.emissive.z, 1.f);
+        }
+        partial[idx] = rgb.x;
+        partial[idx + 1] = rgb.y;
+        partial[idx + 2] = rgb.z;
+        partial[idx + 3] = rgb.w;
+      }
+    }
+  };
+  std::vector<std::thread> pool;
+  for (int i = 0; i < std::max(1, p.threads); i++) pool.emplace_back(worker);
+  for (auto &th : pool) th.join();
+  for (int f = p.frame_start; f < p.frame_start + p.frames; f++)
+    for (size_t i = 0; i < n; i++) {
+      accum[i] += partial[i];
+      final_out[i] = accum[i] / float(f + 1);
+    }
+}
 
 // --------------------------------------------------------------- adjoint
 // CPU gradient of an image loss with respect to FP32 albedo / roughness texels, written as

@@ -208,9 +208,62 @@ static int emu_run(const nova_hip_scene_desc *sd, const nova_hip_render_params *
   ShadeCtx sc{texrec.data(), u32pool.data(), f32pool.data(), mats.data(), mxf.data(), shade.data(), lut.data(), sd->envmap_texture,
               p->sampling_roughness_texture};
 
-  // ---- frames: camera ray, then closest hit + shade per bounce, then accumulate
   const uint32_t npix = (uint32_t)p->width * (uint32_t)p->height;
   std::vector<float> partial((size_t)npix * 4);
+  auto brute_hit = [&](f3 o, f3 d, float &tb, float &bu, float &bv) {
+    tb = 3.402823466e+38f;
+    bu = bv = 0.f;
+    int best = -1;
+    for (uint32_t t = 0; t < ntri; t++) {
+      float tt, uu, vv;
+      if (mt_test(tri[t], o, d, 0.000001f, tb, tt, uu, vv)) {
+        tb = tt;
+        bu = uu;
+        bv = vv;
+        best = (int)t;
+      }
+    }
+    return best;
+  };
+
+  // ---- render passes: one unjittered ray per pixel, the same value every frame
+  const int32_t fl = p->integrator_flags;
+  const int pass_id = (fl & NOVA_HIP_INTEGRATOR_EMISSIVE) ? PASS_EMISSIVE
+                      : (fl & NOVA_HIP_INTEGRATOR_DEPTH)  ? PASS_DEPTH
+                      : (fl & NOVA_HIP_INTEGRATOR_NORMAL) ? PASS_NORMAL
+                                                          : 0;
+  if (pass_id != 0 && !adj_grad) {
+    FrameParams fp;
+    fp.width = p->width;
+    fp.height = p->height;
+    fp.frame0 = p->frame_start;
+    fp.frames = p->frame_count;
+    fp.offset = 0;
+    fp.vinvert = p->vertical_invert;
+    fp.policy = NOVA_HIP_INDEX_CPU;
+    std::memcpy(fp.inv_P, p->inv_P, 64);
+    std::memcpy(fp.inv_V, p->inv_V, 64);
+    for (uint32_t px = 0; px < npix; px++) {
+      f3 o, d;
+      camera_ray_pass(fp, px, o, d);
+      float tb, bu, bv;
+      const int best = brute_hit(o, d, tb, bu, bv);
+      const float4 v = pass_value(pass_id, fp, sc, &kTable[0][0], px, o, d, best, tb, bu, bv, p->z_near, p->z_far);
+      float *dst = &partial[pixel_offset(fp, px)];
+      dst[0] = v.x;
+      dst[1] = v.y;
+      dst[2] = v.z;
+      dst[3] = v.w;
+    }
+    for (int f = p->frame_start; f < p->frame_start + p->frame_count; f++)
+      for (size_t i = 0; i < (size_t)npix * 4; i++) {
+        accum[i] += partial[i];
+        final_out[i] = accum[i] / float(f + 1);
+      }
+    return 0;
+  }
+
+  // ---- frames: camera ray, then closest hit + shade per bounce, then accumulate
   for (int f = p->frame_start; f < p->frame_start + p->frame_count; f++) {
     FrameParams fp;
     fp.width = p->width;
@@ -231,17 +284,8 @@ static int emu_run(const nova_hip_scene_desc *sd, const nova_hip_render_params *
       float A = 0.f;
       uint32_t dim = 2;
       for (int k = 0; k <= p->max_depth; k++) {
-        float tb = 3.402823466e+38f, bu = 0, bv = 0;
-        int best = -1;
-        for (uint32_t t = 0; t < ntri; t++) {
-          float tt, uu, vv;
-          if (mt_test(tri[t], o, d, 0.000001f, tb, tt, uu, vv)) {
-            tb = tt;
-            bu = uu;
-            bv = vv;
-            best = (int)t;
-          }
-        }
+        float tb, bu, bv;
+        const int best = brute_hit(o, d, tb, bu, bv);
         ShadeParams sp;
         sp.fp = fp;
         sp.bounce = k;

@@ -209,3 +209,53 @@ def test_float_libm_choice_only_moves_chaotic_paths(oracle):
     print(f"[libm] pixels moved by more than 1e-3 of the peak: {moved} of {d.size}", flush=True)
     assert moved <= 8
     assert np.count_nonzero(d) <= d.size // 4
+
+
+def wall_scene(emission=(0.25, 0.5, 0.75)):
+    """One emissive quad in the plane z = 0 facing +z, a camera on the +z axis; the quad
+    covers the middle of the image so both hits and misses occur."""
+    sc = scenes.Scene(name="wall")
+    m = sc.principled((0.6, 0.6, 0.6), roughness=0.5, emission=emission, emission_scale=1.0)
+    sc.meshes.append(scenes.quad_mesh([(-1, -1, 0), (1, -1, 0), (1, 1, 0), (-1, 1, 0)], m))
+    sc.envmap = sc.add_texture(scenes.Texture(2, 2, 4, np.zeros(16, dtype=np.float32)))
+    scenes.set_camera(sc, (0.1, 0.05, 4.0), (0.1, 0.05, 0.0), 40.0, 1.0)
+    return sc, m
+
+
+def test_render_passes_closed_forms(oracle):
+    """Depth pass: t of each pixel's ray is the plane distance o.z / -d.z (float64, from the
+    ray), z = 1 - (t - n)/(f - n), (2nf / (f + n - z (f - n)) * 2 - 1) / f in all channels,
+    f on a miss (DepthIntegrator.cpp). Normal pass: unit shading normal of the flat normal
+    map about +z, alpha 1; 0 on a miss. Emissive pass: the emissive texel / 255 (bilinear
+    over equal texels, so to an ulp) where the scatter succeeds, 0 on a miss."""
+    w = h = 24
+    near, far = 0.1, 20.0
+    sc, _ = wall_scene()
+    osc = oracle.OracleScene(sc)
+    rays = camera_rays(sc, w, h).reshape(h, w, 6)  # row y: ndc (2y - h)/h, as the pass renders row y
+    t = rays[..., 2].astype(np.float64) / -rays[..., 5].astype(np.float64)
+    pts = rays[..., :3] + rays[..., 3:] * t[..., None]
+    hit = (np.abs(pts[..., 0]) < 0.999) & (np.abs(pts[..., 1]) < 0.999)
+    miss = (np.abs(pts[..., 0]) > 1.001) | (np.abs(pts[..., 1]) > 1.001)
+    assert hit.sum() > 50 and miss.sum() > 50
+    dep, _ = osc.render_pass(w, h, 3, 2, z_near=near, z_far=far)
+    z = 1 - (t - near) / (far - near)
+    expect = ((2 * near * far) / (far + near - z * (far - near)) * 2 - 1) / far
+    np.testing.assert_allclose(dep[hit][:, 0], expect[hit], rtol=1e-5)
+    assert np.all(dep[hit] == dep[hit][:, :1])
+    assert np.all(dep[miss] == np.float32(far))
+    nrm, _ = osc.render_pass(w, h, 2, 1)
+    n = nrm[hit]
+    np.testing.assert_allclose(np.linalg.norm(n[:, :3], axis=1), 1.0, rtol=1e-6)
+    assert np.all(n[:, 2] > 0.9999) and np.all(n[:, 3] == 1.0)
+    assert np.all(nrm[miss] == 0.0)
+    em, _ = osc.render_pass(w, h, 4, 3)
+    e = np.float32(1) / np.float32(255)
+    texel = np.array([np.float32(64) * e, np.float32(128) * e, np.float32(191) * e, 1.0], dtype=np.float32)
+    acc = np.zeros(4, dtype=np.float32)
+    for f in range(4):  # accumulated and divided in float32, frame by frame
+        acc = (acc + texel).astype(np.float32)
+        fin = (acc / np.float32(f + 1)).astype(np.float32)
+    lit = np.all(np.abs(em - fin) <= 1e-6, axis=2)  # bilinear weights of equal texels round by an ulp
+    assert lit[hit].mean() > 0.9  # a failed scatter (sample below the surface) gives 0
+    assert np.all(em[~lit] == 0.0) and np.all(em[miss] == 0.0)
