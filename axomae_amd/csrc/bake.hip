@@ -2,7 +2,8 @@
 //  * diffuse irradiance: TextureOperations<float>::computeDiffuseIrradiance
 //    (TextureProcessing.cpp:9-64, TextureProcessing.h:253-301) - pgc3d-hashed cosine
 //    samples per texel, looked up with the nearest-texel rule uvSample degenerates to;
-//  * GGX prefiltered specular: envmap_prefilter.frag:1-116 on equirect, level 0.
+//  * GGX prefiltered specular: envmap_prefilter.frag:1-116 on equirect, level 0;
+//  * the split-sum BRDF table: brdf_lookup_table_baker.frag:1-117.
 // Each output texel is one lane; samples are summed in the reference's order.
 #include <algorithm>
 #include <vector>
@@ -179,6 +180,44 @@ __device__ __forceinline__ f3 prefilter_texel(const BakeMap &m, const float *__r
   }
 }
 
+// computeLut (brdf_lookup_table_baker.frag:74-113) at texel (i, j): NdotV = (i + 0.5) / ow,
+// roughness = (j + 0.5) / oh; the split-sum scale A and bias B over `samples` Hammersley GGX
+// half vectors about N = +z, with GeometrySmith's k = roughness^2 / 2.
+__device__ __forceinline__ float schlick_ggx_ibl(float ndotv, float roughness) {
+  const float a = roughness;
+  const float k = (a * a) / 2.0f;
+  return ndotv / (ndotv * (1.0f - k) + k);
+}
+
+__device__ __forceinline__ float2 brdf_lut_texel(const BakeMap &m, unsigned i, unsigned j, int samples) {
+  const float ndotv = ((float)i + 0.5f) / (float)m.ow, roughness = ((float)j + 0.5f) / (float)m.oh;
+  const f3 V = mk3(sqrtf(1.0f - ndotv * ndotv), 0.0f, ndotv);
+  const f3 N = mk3(0.f, 0.f, 1.f);
+  const float a = roughness * roughness;
+  const f3 up = fabsf(N.z) < 0.999f ? mk3(0.f, 0.f, 1.f) : mk3(1.f, 0.f, 0.f);
+  const f3 t = normalize(cross(up, N));
+  const f3 b = cross(N, t);
+  const float gv = schlick_ggx_ibl(fmaxf(dot(N, V), 0.0f), roughness);
+  float A = 0.f, B = 0.f;
+  for (int s = 0; s < samples; s++) {
+    const float xi0 = (float)s / (float)samples, xi1 = radical_inverse((unsigned)s);
+    const float phi = 2.0f * 3.14159265f * xi0;
+    const float ct = sqrtf((1.0f - xi1) / (1.0f + (a * a - 1.0f) * xi1));
+    const float st = sqrtf(1.0f - ct * ct);
+    const f3 H = normalize(t * (cos_f(phi) * st) + b * (sin_f(phi) * st) + N * ct);
+    const f3 L = normalize(H * (2.0f * dot(V, H)) - V);
+    const float ndl = fmaxf(L.z, 0.0f), ndh = fmaxf(H.z, 0.0f), vdh = fmaxf(dot(V, H), 0.0f);
+    if (ndl > 0.0f) {
+      const float G = schlick_ggx_ibl(ndl, roughness) * gv;
+      const float g_vis = (G * vdh) / (ndh * ndotv);
+      const float fc = pow5(1.0f - vdh);
+      A += (1.0f - fc) * g_vis;
+      B += fc * g_vis;
+    }
+  }
+  return make_float2(A / (float)samples, B / (float)samples);
+}
+
 // Every texel of every job in one grid-stride launch; jobs are laid out back to back, so a
 // wave almost always works on a single map and kind.
 __global__ void bake_kernel(const BakeMap *__restrict__ maps, int count, uint64_t total, const float *__restrict__ lut,
@@ -187,6 +226,13 @@ __global__ void bake_kernel(const BakeMap *__restrict__ maps, int count, uint64_
     const BakeMap m = maps[find_map(maps, count, g)];
     const uint64_t local = g - m.first;
     const unsigned i = (unsigned)(local % (unsigned)m.ow), j = (unsigned)(local / (unsigned)m.ow);
+    if (m.kind == NOVA_HIP_BAKE_BRDF_LUT) {
+      const float2 ab = brdf_lut_texel(m, i, j, samples);
+      float *o = m.out + ((size_t)j * m.ow + i) * 2;
+      o[0] = ab.x;
+      o[1] = ab.y;
+      continue;
+    }
     const f3 col = m.kind == NOVA_HIP_BAKE_PREFILTER ? prefilter_texel(m, lut, i, j, samples) : irradiance_texel(m, lut, sobol, i, j, samples);
     float *o = m.out + ((size_t)j * m.ow + i) * m.c;
     o[0] = col.x;
@@ -203,10 +249,14 @@ static int launch_batch(const nova_hip_bake_job *jobs, uint32_t count, int32_t s
   uint64_t total = 0;
   for (uint32_t i = 0; i < count; i++) {
     const nova_hip_bake_job &jb = jobs[i];
-    if (!jb.env || !jb.out || jb.width < 1 || jb.height < 1 || jb.channels < 3 || jb.channels > 4 || jb.out_width < 1 || jb.out_height < 1)
-      return NOVA_HIP_INVALID_ARGUMENT;
-    if (jb.kind != NOVA_HIP_BAKE_IRRADIANCE && jb.kind != NOVA_HIP_BAKE_IRRADIANCE_SOBOL && jb.kind != NOVA_HIP_BAKE_PREFILTER)
-      return NOVA_HIP_INVALID_ARGUMENT;
+    if (jb.kind == NOVA_HIP_BAKE_BRDF_LUT) {  // no environment map; two output channels
+      if (!jb.out || jb.out_width < 1 || jb.out_height < 1 || jb.channels != 2) return NOVA_HIP_INVALID_ARGUMENT;
+    } else {
+      if (!jb.env || !jb.out || jb.width < 1 || jb.height < 1 || jb.channels < 3 || jb.channels > 4 || jb.out_width < 1 || jb.out_height < 1)
+        return NOVA_HIP_INVALID_ARGUMENT;
+      if (jb.kind != NOVA_HIP_BAKE_IRRADIANCE && jb.kind != NOVA_HIP_BAKE_IRRADIANCE_SOBOL && jb.kind != NOVA_HIP_BAKE_PREFILTER)
+        return NOVA_HIP_INVALID_ARGUMENT;
+    }
     // Sobol indices texel * samples + s must stay below 2^32
     if (jb.kind == NOVA_HIP_BAKE_IRRADIANCE_SOBOL && (uint64_t)jb.out_width * jb.out_height * (uint64_t)samples > 0xFFFFFFFFull)
       return NOVA_HIP_INVALID_ARGUMENT;

@@ -76,7 +76,7 @@ class BakeJob(C.Structure):
                 ("out_width", C.c_int32), ("out_height", C.c_int32), ("kind", C.c_int32), ("roughness", C.c_float)]
 
 
-BAKE_IRRADIANCE, BAKE_IRRADIANCE_SOBOL, BAKE_PREFILTER = 0, 1, 2
+BAKE_IRRADIANCE, BAKE_IRRADIANCE_SOBOL, BAKE_PREFILTER, BAKE_BRDF_LUT = 0, 1, 2, 3
 
 class BatchConfig(C.Structure):
     _fields_ = [("queue_capacity", C.c_uint32), ("trace_blocks_per_cu", C.c_uint32), ("shade_blocks_per_cu", C.c_uint32), ("use_graph", C.c_int32)]
@@ -378,6 +378,11 @@ def bake_batch(jobs, samples, keep_on_device=False):
     arr = (BakeJob * len(jobs))()
     keep, outs = [], []
     for k, (env, ow, oh, kind, rough) in enumerate(jobs):
+        if kind == BAKE_BRDF_LUT:  # no map: two output channels (A, B)
+            out = torch.zeros(oh * ow * 2, dtype=torch.float32, device="cuda")
+            outs.append((out, ow, oh, 2))
+            arr[k] = BakeJob(None, 0, 0, 2, out.data_ptr(), ow, oh, kind, 0.0)
+            continue
         if isinstance(env, tuple):
             src, (w, h, c) = env
         else:

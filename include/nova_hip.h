@@ -314,7 +314,11 @@ enum nova_hip_bake_kind {
   NOVA_HIP_BAKE_IRRADIANCE = 0,       /* computeDiffuseIrradiance with its pgc3d hash (the reference's sampler) */
   NOVA_HIP_BAKE_IRRADIANCE_SOBOL = 1, /* the same estimator on Owen-scrambled Sobol points, dims 0-1,
                                          index texel * samples + s (extension) */
-  NOVA_HIP_BAKE_PREFILTER = 2         /* envmap_prefilter.frag at `roughness` */
+  NOVA_HIP_BAKE_PREFILTER = 2,        /* envmap_prefilter.frag at `roughness` */
+  NOVA_HIP_BAKE_BRDF_LUT = 3          /* brdf_lookup_table_baker.frag:74-113: no map (env, width, height
+                                         unused), channels 2, out_width x out_height x {A, B} with
+                                         NdotV = (i + 0.5) / out_width, roughness = (j + 0.5) / out_height;
+                                         1024 samples reproduce the shader */
 };
 typedef struct {
   const float *env;

@@ -252,6 +252,9 @@ void oracle_bake_irradiance_sampler(const float *env, int w, int h, int c, int o
   bake_irradiance(env, w, h, c, ow, oh, samples, out, threads, sampler);
 }
 
+void oracle_brdf_lut_value(float ndotv, float roughness, int samples, float *ab) { brdf_lut_value(ndotv, roughness, samples, ab[0], ab[1]); }
+void oracle_bake_brdf_lut(int ow, int oh, int samples, int threads, float *out) { bake_brdf_lut(ow, oh, samples, out, threads); }
+
 void oracle_bake_prefilter(const float *env, int w, int h, int c, int ow, int oh, float roughness, int samples, int threads, float *out) {
   bake_prefilter(env, w, h, c, ow, oh, roughness, samples, out, threads);
 }

@@ -202,3 +202,51 @@ void bake_prefilter(const float *env, int env_w, int env_h, int env_c, int out_w
 }
 
 }  // namespace oracle
+
+namespace oracle {
+
+// GeometrySchlickGGX with k = roughness^2 / 2 (brdf_lookup_table_baker.frag:9-17)
+static float schlick_ggx_ibl(float ndotv, float roughness) {
+  float a = roughness;
+  float k = (a * a) / 2.0f;
+  return ndotv / (ndotv * (1.0f - k) + k);
+}
+
+// computeLut (brdf_lookup_table_baker.frag:74-113): the split-sum scale A and bias B of
+// F0 for one (NdotV, roughness), `samples` Hammersley GGX half vectors about N = +z.
+void brdf_lut_value(float ndotv, float roughness, int samples, float &A, float &B) {
+  const vec3 V(std::sqrt(1.0f - ndotv * ndotv), 0.0f, ndotv);
+  const vec3 N(0.f, 0.f, 1.f);
+  float a = 0.f, b = 0.f;
+  for (int i = 0; i < samples; i++) {
+    const vec3 H = importance_sample_ggx((float)i / (float)samples, radical_inverse((unsigned)i), N, roughness);
+    const vec3 L = normalize(H * (2.0f * dot(V, H)) - V);
+    const float ndl = std::fmax(L.z, 0.0f);
+    const float ndh = std::fmax(H.z, 0.0f);
+    const float vdh = std::fmax(dot(V, H), 0.0f);
+    if (ndl > 0.0f) {
+      const float G = schlick_ggx_ibl(ndl, roughness) * schlick_ggx_ibl(std::fmax(dot(N, V), 0.0f), roughness);
+      const float g_vis = (G * vdh) / (ndh * ndotv);
+      const float fc = pow5(1.0f - vdh);
+      a += (1.0f - fc) * g_vis;
+      b += fc * g_vis;
+    }
+  }
+  A = a / (float)samples;
+  B = b / (float)samples;
+}
+
+// The LUT texture: texel (i, j) at NdotV = (i + 0.5) / out_w, roughness = (j + 0.5) / out_h
+// (the full-screen quad's texture coordinates at pixel centres); out is out_w x out_h x 2.
+void bake_brdf_lut(int out_w, int out_h, int samples, float *out, int threads) {
+  parallel_rows(out_h, threads, [&](int j) {
+    for (int i = 0; i < out_w; i++) {
+      float A, B;
+      brdf_lut_value(((float)i + 0.5f) / (float)out_w, ((float)j + 0.5f) / (float)out_h, samples, A, B);
+      out[((size_t)j * out_w + i) * 2] = A;
+      out[((size_t)j * out_w + i) * 2 + 1] = B;
+    }
+  });
+}
+
+}  // namespace oracle

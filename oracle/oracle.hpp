@@ -15,6 +15,8 @@
 //   PathIntegrator::Li ..................... pinned by analytic scenes (constant envmap, emission only)
 //   irradiance bake ........................ pinned (constant map => constant, brute force of the texel rule)
 //   prefilter bake ......................... partial (no reference CPU implementation exists)
+//   BRDF table (brdf_lut_value) ............. pinned (roughness 0 closed form; NdotV = 1 as a 1-D sum)
+//   render passes (render_pass) ............. pinned (wall scene: plane distances, unit normals, texel / 255)
 //   render_adjoint (extension) ............. pinned (central finite differences of the oracle's own renders)
 //
 // Deviations, kept identical in the HIP path: a 0/0 slab in BoundingBox::intersect spans the
@@ -127,5 +129,7 @@ size_t f32_pool_offset(const Scene &scene, int texture);
 // ---------------------------------------------------------------- bakes
 void bake_irradiance(const float *env, int env_w, int env_h, int env_c, int out_w, int out_h, int samples, float *out, int threads, int sampler = 0);
 void bake_prefilter(const float *env, int env_w, int env_h, int env_c, int out_w, int out_h, float roughness, int samples, float *out, int threads);
+void brdf_lut_value(float ndotv, float roughness, int samples, float &A, float &B);
+void bake_brdf_lut(int out_w, int out_h, int samples, float *out, int threads);
 
 }  // namespace oracle

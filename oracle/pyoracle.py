@@ -66,6 +66,8 @@ def lib():
         L.oracle_render_rows.argtypes = [P, P, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, f32p, f32p, C.c_int, f32p]
         L.oracle_bake_irradiance.argtypes = [f32p, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, f32p]
         L.oracle_bake_irradiance_sampler.argtypes = [f32p, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, f32p]
+        L.oracle_brdf_lut_value.argtypes = [C.c_float, C.c_float, C.c_int, f32p]
+        L.oracle_bake_brdf_lut.argtypes = [C.c_int, C.c_int, C.c_int, C.c_int, f32p]
         L.oracle_bake_prefilter.argtypes = [f32p, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, C.c_float, C.c_int, C.c_int, f32p]
         _LIB = L
     return _LIB
@@ -297,3 +299,16 @@ def set_index_policy(policy):
 
 def sample_index(width, height, x, y, frame, offset=0, policy=0, vertical_invert=False):
     return int(lib().oracle_sample_index(width, height, int(vertical_invert), int(policy), offset, x, y, frame))
+
+
+def brdf_lut_value(ndotv, roughness, samples=1024):
+    """(A, B) of brdf_lookup_table_baker.frag's computeLut."""
+    ab = np.zeros(2, dtype=np.float32)
+    lib().oracle_brdf_lut_value(ndotv, roughness, samples, ab)
+    return float(ab[0]), float(ab[1])
+
+
+def bake_brdf_lut(out_w, out_h, samples=1024, threads=8):
+    out = np.zeros(out_w * out_h * 2, dtype=np.float32)
+    lib().oracle_bake_brdf_lut(out_w, out_h, samples, threads, out)
+    return out.reshape(out_h, out_w, 2)

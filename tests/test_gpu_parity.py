@@ -170,6 +170,19 @@ def test_render_integrator_flag_errors(engine):
 
 
 @gpu
+def test_brdf_lut_bake_matches_oracle(oracle):
+    """The split-sum BRDF table (brdf_lookup_table_baker.frag) as a bake job beside an
+    irradiance job in the same launch."""
+    from axomae_amd import nova_hip as nh
+    env = scenes.sky_envmap(64, 32, (0.3, 0.9, 0.2))
+    lut, irr = nh.bake_batch([(None, 48, 40, nh.BAKE_BRDF_LUT, 0.0), (env, 16, 8, nh.BAKE_IRRADIANCE, 0.0)], 1024)
+    ref = oracle.bake_brdf_lut(48, 40, 1024)
+    print(f"[brdf-lut] max abs diff {np.abs(lut - ref).max():.3e}", flush=True)
+    np.testing.assert_allclose(lut, ref, rtol=1e-5, atol=1e-6)
+    assert rel_rmse(irr, oracle.bake_irradiance(env, 16, 8, 1024)) <= 1e-3
+
+
+@gpu
 def test_bakes_match_cpu_bake(oracle):
     from axomae_amd import nova_hip
     maps = [scenes.sky_envmap(128, 64, (0.2, 0.9, 0.4), sun_intensity=20.0, sun_size=0.01), scenes.sky_envmap(256, 128, (-0.5, 0.3, 0.8))]

@@ -259,3 +259,27 @@ def test_render_passes_closed_forms(oracle):
     lit = np.all(np.abs(em - fin) <= 1e-6, axis=2)  # bilinear weights of equal texels round by an ulp
     assert lit[hit].mean() > 0.9  # a failed scatter (sample below the surface) gives 0
     assert np.all(em[~lit] == 0.0) and np.all(em[miss] == 0.0)
+
+
+def test_brdf_lut_closed_forms(oracle):
+    """computeLut (brdf_lookup_table_baker.frag). Roughness 0: every half vector is N, so
+    L is the mirror direction, G = 1, G_Vis = 1: A = 1 - (1 - NdotV)^5, B = (1 - NdotV)^5.
+    NdotV = 1: V = N, NdotL = 2 c^2 - 1 and G_Vis = G(NdotL) with c^2 the GGX half-vector
+    cosine (1 - xi) / (1 + (a^2 - 1) xi); the 1024 radical inverses are the points k/1024, so
+    A + B is that 1-D integrand summed over them (float64 here). Fc = (1 - VdotH)^5 splits
+    it: B = sum of (1 - c)^5 G."""
+    for v in (0.1, 0.37, 0.5, 0.9):
+        A, B = oracle.brdf_lut_value(v, 0.0)
+        assert abs(B - (1 - v) ** 5) < 2e-5 and abs(A - (1 - (1 - v) ** 5)) < 2e-5  # 1024 float additions
+    for r in (0.2, 0.5, 0.8, 1.0):
+        a, k = r * r, r * r / 2
+        xi = np.arange(1024) / 1024.0
+        c = np.sqrt((1 - xi) / (1 + (a * a - 1) * xi))
+        ndl = 2 * c * c - 1
+        gs = lambda x: x / (x * (1 - k) + k)
+        g = np.where(ndl > 0, gs(np.maximum(ndl, 1e-30)), 0.0)
+        A, B = oracle.brdf_lut_value(1.0, r)
+        assert abs((A + B) - g.mean()) < 2e-5, (r, A + B, g.mean())
+        assert abs(B - np.mean((1 - c) ** 5 * g)) < 2e-5
+    lut = oracle.bake_brdf_lut(16, 16)
+    assert np.all(lut >= 0) and np.all(lut.sum(axis=2) <= 1 + 1e-5)
