@@ -562,10 +562,14 @@ __global__ void root_bounds_kernel(const float *__restrict__ aabb, uint32_t n, i
 }
 
 // Binary leaf holding each prim_idx position (the tie rule of the traversal needs it).
-__global__ void leaf_of_position_kernel(const BNode *__restrict__ nodes, uint32_t count, int32_t *__restrict__ leaf_of_pos) {
+// Per binary leaf: the leaf of each of its positions, and its size in the first traversal
+// record's b.w.
+__global__ void leaf_of_position_kernel(const BNode *__restrict__ nodes, uint32_t count, int32_t *__restrict__ leaf_of_pos,
+                                        TriMT *__restrict__ tri_leaf) {
   for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < count; i += gridDim.x * blockDim.x) {
     const BNode nd = nodes[i];
     for (int32_t k = 0; k < nd.count; k++) leaf_of_pos[nd.left + k] = (int32_t)i;
+    if (nd.count > 0) tri_leaf[nd.left].b.w = __builtin_bit_cast(float, nd.count);
   }
 }
 
@@ -620,7 +624,7 @@ __global__ void collapse_kernel(const int32_t *__restrict__ frontier_bin, const 
       nd.minx[k] = nd.miny[k] = nd.minz[k] = 1.f;
       nd.maxx[k] = nd.maxy[k] = nd.maxz[k] = -1.f;  // empty box never hit
       nd.child[k] = -1;
-      nd.count[k] = 0;
+      nd.reserved[k] = 0;
       continue;
     }
     const BNode &c = bn[ch[k]];
@@ -630,13 +634,12 @@ __global__ void collapse_kernel(const int32_t *__restrict__ frontier_bin, const 
     nd.maxx[k] = c.mx[0];
     nd.maxy[k] = c.mx[1];
     nd.maxz[k] = c.mx[2];
+    nd.reserved[k] = 0;
     if (c.count > 0) {
-      nd.child[k] = c.left;
-      nd.count[k] = c.count;
+      nd.child[k] = leaf_ref(c.left);
     } else {
       const int32_t w = atomicAdd(wide_counter, 1);
       nd.child[k] = w;
-      nd.count[k] = 0;
       const int32_t j = atomicAdd(next_count, 1);
       next_bin[j] = ch[k];
       next_slot[j] = w;
@@ -800,7 +803,7 @@ extern "C" int nova_hip_bvh_build(nova_hip_ctx *ctx, int build_candidates, nova_
   NOVA_HIP_CHECK(ctx, hipMalloc(&ctx->tri_leaf, (size_t)n * sizeof(TriMT)));
   hipLaunchKernelGGL(gather_leaf_tris, dim3((n + 255) / 256), dim3(256), 0, s, ctx->prim_idx, ctx->tri_world, ctx->tri_leaf, n);
   hipLaunchKernelGGL(leaf_of_position_kernel, dim3(std::min<uint32_t>(4096, (used + 255) / 256)), dim3(256), 0, s, ctx->bnodes, (uint32_t)used,
-                     ctx->leaf_of_pos);
+                     ctx->leaf_of_pos, ctx->tri_leaf);
   NOVA_HIP_CHECK(ctx, hipGetLastError());
   NOVA_HIP_CHECK(ctx, hipEventRecord(ev1, s));
   NOVA_HIP_CHECK(ctx, hipEventSynchronize(ev1));

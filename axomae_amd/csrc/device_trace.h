@@ -183,14 +183,13 @@ NOVA_HD bool trace_step(const TraceCtx &tc, Stack &stack, TraceState &st, float 
     const NOVA_GLOBAL float4 *np = (const NOVA_GLOBAL float4 *)(tc.nodes + st.node);
     const float4 mnx = np[0], mxx = np[1], mny = np[2], mxy = np[3], mnz = np[4], mxz = np[5];
     const int4 ch = *(const NOVA_GLOBAL int4 *)(np + 6);
-    const int4 cn = *(const NOVA_GLOBAL int4 *)(np + 7);
     const float bmnx[4] = {mnx.x, mnx.y, mnx.z, mnx.w}, bmxx[4] = {mxx.x, mxx.y, mxx.z, mxx.w};
     const float bmny[4] = {mny.x, mny.y, mny.z, mny.w}, bmxy[4] = {mxy.x, mxy.y, mxy.z, mxy.w};
     const float bmnz[4] = {mnz.x, mnz.y, mnz.z, mnz.w}, bmxz[4] = {mxz.x, mxz.y, mxz.z, mxz.w};
-    const int chv[4] = {ch.x, ch.y, ch.z, ch.w}, cnv[4] = {cn.x, cn.y, cn.z, cn.w};
-    // per child: entry distance (+inf when culled), child reference and triangle count
+    const int chv[4] = {ch.x, ch.y, ch.z, ch.w};
+    // per child: entry distance (+inf when culled) and child reference (-1 when culled)
     float key[4];
-    int32_t cref[4], ccnt[4];
+    int32_t cref[4];
     // Two copies of the slab test: rays with an axis-parallel direction take the
     // containment form; the common copy keeps every min/max operand a product of the same
     // block, which spares the NaN-quieting copies the compiler adds to values merged from
@@ -218,10 +217,9 @@ NOVA_HD bool trace_step(const TraceCtx &tc, Stack &stack, TraceState &st, float 
         }
         const float t0 = fmaxf(fmaxf(fminf(tx1, tx2), fminf(ty1, ty2)), fminf(tz1, tz2));
         const float t1 = fminf(fminf(fmaxf(tx1, tx2), fmaxf(ty1, ty2)), fmaxf(tz1, tz2)) * kUp;
-        const bool hit = chv[k] >= 0 && t0 <= t1 && t1 >= 0.f && t0 <= st.tbest * kUp;
+        const bool hit = chv[k] != -1 && t0 <= t1 && t1 >= 0.f && t0 <= st.tbest * kUp;
         key[k] = hit ? t0 : kInf;
         cref[k] = hit ? chv[k] : -1;
-        ccnt[k] = cnv[k];
       }
     } else {
 #pragma unroll
@@ -231,10 +229,9 @@ NOVA_HD bool trace_step(const TraceCtx &tc, Stack &stack, TraceState &st, float 
         const float tz1 = (bmnz[k] - st.o.z) * st.inv.z, tz2 = (bmxz[k] - st.o.z) * st.inv.z;
         const float t0 = fmaxf(fmaxf(fminf(tx1, tx2), fminf(ty1, ty2)), fminf(tz1, tz2));
         const float t1 = fminf(fminf(fmaxf(tx1, tx2), fmaxf(ty1, ty2)), fmaxf(tz1, tz2)) * kUp;
-        const bool hit = chv[k] >= 0 && t0 <= t1 && t1 >= 0.f && t0 <= st.tbest * kUp;
+        const bool hit = chv[k] != -1 && t0 <= t1 && t1 >= 0.f && t0 <= st.tbest * kUp;
         key[k] = hit ? t0 : kInf;
         cref[k] = hit ? chv[k] : -1;
-        ccnt[k] = cnv[k];
       }
     }
     // near-to-far order of the four children: a fixed 5-exchange sorting network, so the
@@ -243,13 +240,10 @@ NOVA_HD bool trace_step(const TraceCtx &tc, Stack &stack, TraceState &st, float 
       const bool sw = key[j] < key[i];
       const float kf = sw ? key[j] : key[i], kl = sw ? key[i] : key[j];
       const int32_t rf = sw ? cref[j] : cref[i], rl = sw ? cref[i] : cref[j];
-      const int32_t nf = sw ? ccnt[j] : ccnt[i], nl = sw ? ccnt[i] : ccnt[j];
       key[i] = kf;
       key[j] = kl;
       cref[i] = rf;
       cref[j] = rl;
-      ccnt[i] = nf;
-      ccnt[j] = nl;
     };
     exch(0, 1);
     exch(2, 3);
@@ -259,17 +253,19 @@ NOVA_HD bool trace_step(const TraceCtx &tc, Stack &stack, TraceState &st, float 
     int32_t next = -1;
 #pragma unroll
     for (int a = 3; a >= 0; a--) {  // far to near: push inner nodes, keep the nearest
-      if (cref[a] < 0 || ccnt[a] > 0) continue;
+      if (cref[a] < 0) continue;
       if (next >= 0) stack.push(st.sp++, (uint32_t)next);
       next = cref[a];
     }
 #pragma unroll
     for (int a = 0; a < 4; a++) {  // leaves now, near to far
-      if (cref[a] < 0 || ccnt[a] == 0) continue;
+      if (cref[a] > -2) continue;
       if (key[a] > st.tbest * kUp) continue;
-      const int32_t start = cref[a], cnt = ccnt[a];
+      const int32_t start = leaf_first(cref[a]);
+      int32_t cnt = 1;  // the leaf size comes with its first record
       for (int32_t i = start; i < start + cnt; i++) {
         const TriMT tr = load_tri(tc.tris + i);
+        if (i == start) cnt = __builtin_bit_cast(int32_t, tr.b.w);
         if (kCount) tri_tests++;
         float tt, uu, vv;
         if (mt_test(tr, st.o, st.d, tnear, st.tbest, tt, uu, vv) && (tt < st.tbest || st.best < 0 || reference_tie_winner_is_new(tie_of(tc), i, st.bpos, st.o, st.d, tt))) {

@@ -10,7 +10,8 @@
 
 namespace nova_hip {
 
-// Traversal record, 48 B, in BVH leaf order: (v0, prim id) (e1, 0) (e2, 0).
+// Traversal record, 48 B, in BVH leaf order: (v0, prim id) (e1, leaf size) (e2, 0); the
+// leaf size (int bits) is set in the first record of each leaf.
 // e1 = v1 - v0, e2 = v2 - v0 are the edges Triangle::hit forms (Triangle.h:158-160); a
 // triangle with repeated indices gets zero edges so that det == 0 rejects it, as
 // Triangle::isDegenerate does.
@@ -62,14 +63,17 @@ struct TieTables {
   const int32_t *bparent, *bdepth, *leaf_of_pos;
 };
 
-// 4-wide node, 128 B: child boxes SoA, then per-child reference. count[k] == 0: child[k] is
-// an inner node index; count[k] > 0: a leaf of triangles [child[k], child[k]+count[k]) in the
-// TriMT array; child[k] == -1: empty slot.
+// 4-wide node, 128 B: child boxes SoA, then per-child reference. child[k] >= 0: an inner
+// node index; child[k] <= -2: a leaf whose triangles start at -(child[k] + 2) in the TriMT
+// array (the size is in that first record's b.w); child[k] == -1: empty slot. The last 16
+// bytes are not read, so a visit is seven 16-byte loads.
 struct alignas(16) Node4 {
   float minx[4], maxx[4], miny[4], maxy[4], minz[4], maxz[4];
   int32_t child[4];
-  int32_t count[4];
+  int32_t reserved[4];
 };
+__host__ __device__ inline int32_t leaf_ref(int32_t first) { return -(first + 2); }
+__host__ __device__ inline int32_t leaf_first(int32_t ref) { return -(ref + 2); }
 static_assert(sizeof(Node4) == 128, "Node4 layout");
 
 struct DeviceBuffer {

@@ -8,13 +8,14 @@ frames; value = width*height*K / t / 1e6 Msamples/s. Scene upload and BVH build 
 outside the timed region and reported separately.
 
 roofline: the dominant kernel is the extend (closest-hit traversal) kernel. Its
-algorithmic bytes per ray are 48 (ray in, hit out) + 128 per 4-wide node visited + 48
-per triangle tested, with visits/tests counted by an instrumented render of the same
+algorithmic bytes per ray are 48 (ray in, hit out) + 112 per 4-wide node visited (the
+seven 16-byte quads of the 128-byte node a visit reads) + 48 per triangle tested, with
+visits/tests counted by an instrumented render of the same
 frames; its duration comes from HIP events the library records around every extend
 launch on the render stream during the timed region.
 
 cpu_baseline: the oracle (the CPU restatement of the reference renderer) on the host
-cores, timed on a band of rows of frame 1 of the same workload.
+cores, timed on whole frames 1, 2, ... of the same workload.
 """
 import argparse
 import json
@@ -27,6 +28,7 @@ sys.path.insert(0, ROOT)
 
 METRIC = "Msamples/s (paths) at 1080p, depth 8; adjoint Msamples/s vs primal, 1 GPU"
 HBM_PEAK_GBS = 8000.0
+NODE_BYTES = 112.0  # bytes of a 4-wide node one visit reads (child boxes and references)
 
 
 def log(*a):
@@ -276,12 +278,12 @@ def main():
 
     paths = W * H * K
     value = paths / t / 1e6
-    ext_bytes = 48.0 * cst.segments + 128.0 * cst.node_visits + 48.0 * cst.triangle_tests
+    ext_bytes = 48.0 * cst.segments + NODE_BYTES * cst.node_visits + 48.0 * cst.triangle_tests
     ext_s = st.extend_ms / 1e3
     achieved = ext_bytes / ext_s / 1e9 if ext_s > 0 else 0.0
     per_launch = ext_bytes / max(st.extend_launches, 1)
     # whole-path algorithmic bytes: 48 + 424 per segment + node/triangle bytes + 20 per escape + 16 per path
-    path_bytes = 472.0 * cst.segments + 128.0 * cst.node_visits + 48.0 * cst.triangle_tests + 20.0 * cst.escaped + 16.0 * paths
+    path_bytes = 472.0 * cst.segments + NODE_BYTES * cst.node_visits + 48.0 * cst.triangle_tests + 20.0 * cst.escaped + 16.0 * paths
     traffic = None
     tfile = os.path.join(ROOT, "profiles", "extend_hbm_traffic.json")
     if os.path.exists(tfile):

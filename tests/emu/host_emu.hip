@@ -112,7 +112,7 @@ extern "C" int emu_closest_hit(const nova_hip_scene_desc *sd, const BNode *bnode
         nd.minx[c] = nd.miny[c] = nd.minz[c] = 1.f;
         nd.maxx[c] = nd.maxy[c] = nd.maxz[c] = -1.f;
         nd.child[c] = -1;
-        nd.count[c] = 0;
+        nd.reserved[c] = 0;
         continue;
       }
       const BNode &b = bnodes[ch[c]];
@@ -122,12 +122,11 @@ extern "C" int emu_closest_hit(const nova_hip_scene_desc *sd, const BNode *bnode
       nd.maxx[c] = b.mx[0];
       nd.maxy[c] = b.mx[1];
       nd.maxz[c] = b.mx[2];
+      nd.reserved[c] = 0;
       if (b.count > 0) {
-        nd.child[c] = b.left;
-        nd.count[c] = b.count;
+        nd.child[c] = leaf_ref(b.left);
       } else {
         nd.child[c] = (int32_t)wide.size();
-        nd.count[c] = 0;
         frontier.push_back({ch[c], (int32_t)wide.size()});
         wide.emplace_back();
       }
@@ -136,6 +135,8 @@ extern "C" int emu_closest_hit(const nova_hip_scene_desc *sd, const BNode *bnode
   }
   std::vector<TriMT> tri_leaf(n);
   for (uint32_t i = 0; i < n; i++) tri_leaf[i] = es.tri[prim_idx[i]];
+  for (int32_t i = 0; i < nbnodes; i++)
+    if (bnodes[i].count > 0) tri_leaf[bnodes[i].left].b.w = __builtin_bit_cast(float, bnodes[i].count);
   const TieTables tie{bnodes, parent.data(), depth.data(), leaf_of_pos.data()};
   TraceCtx tc{wide.data(), tri_leaf.data(), nullptr, 0, &tie};
   HostStack st;
