@@ -172,7 +172,7 @@ __global__ void batch_reset(BatchState *st, int which) {
   st->work = 0;
 }
 
-__global__ __launch_bounds__(256) void batch_shade(const BatchScene *__restrict__ scenes, const WaveDesc *__restrict__ waves, BatchState *st,
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4, 4))) void batch_shade(const BatchScene *__restrict__ scenes, const WaveDesc *__restrict__ waves, BatchState *st,
                                                    const uint32_t *__restrict__ sobol, BatchQueues q, int which, int bounce, int last, int rr,
                                                    uint32_t dim0) {
   __shared__ uint32_t sob_tab[5 * 128];  // nibble tables of dims dim0 .. dim0+4
