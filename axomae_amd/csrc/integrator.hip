@@ -43,7 +43,9 @@ struct Queues {
   uint32_t cap;
 };
 
-// AbstractIntegrator::render per pixel (Integrator.h:53-95): one camera ray per path slot
+// AbstractIntegrator::render per pixel (Integrator.h:53-95): one camera ray per path slot.
+// The first queue is in slot order and the first shade starts every path at T = 1, L = 0,
+// A = 0, so neither slot ids nor path state are written here.
 __global__ __launch_bounds__(256) void generate_kernel(FrameParams fp, const uint32_t *__restrict__ sobol, Queues q, uint32_t active) {
   const uint32_t n = q.cap;  // plane stride
   for (uint32_t slot = blockIdx.x * blockDim.x + threadIdx.x; slot < active; slot += gridDim.x * blockDim.x) {
@@ -56,14 +58,6 @@ __global__ __launch_bounds__(256) void generate_kernel(FrameParams fp, const uin
     r[3 * (size_t)n + slot] = d.x;
     r[4 * (size_t)n + slot] = d.y;
     r[5 * (size_t)n + slot] = d.z;
-    q.slot[0][slot] = (int32_t)slot;
-    q.T[slot] = 1.f;
-    q.T[(size_t)n + slot] = 1.f;
-    q.T[2 * (size_t)n + slot] = 1.f;
-    q.L[slot] = 0.f;
-    q.L[(size_t)n + slot] = 0.f;
-    q.L[2 * (size_t)n + slot] = 0.f;
-    q.A[slot] = 0.f;
   }
   if (blockIdx.x == 0 && threadIdx.x == 0) {
     q.count[0] = active;
@@ -194,12 +188,13 @@ __global__ __launch_bounds__(kShadeBlock) NOVA_SHADE_OCCUPANCY void shade_kernel
     f3 no = mk3(0, 0, 0), nd = mk3(0, 0, 0);
     int32_t slot = -1;
     if (i < n) {
-      slot = q.slot[which][i];
+      const bool first = sp.bounce == 0;  // camera rays: slot order, fresh path state
+      slot = first ? (int32_t)i : q.slot[which][i];
       const f3 o = mk3(rin[i], rin[cap + i], rin[2 * cap + i]);
       const f3 d = mk3(rin[3 * cap + i], rin[4 * cap + i], rin[5 * cap + i]);
-      f3 T = mk3(q.T[slot], q.T[cap + slot], q.T[2 * cap + slot]);
-      f3 L = mk3(q.L[slot], q.L[cap + slot], q.L[2 * cap + slot]);
-      float A = q.A[slot];
+      f3 T = first ? mk3(1.f, 1.f, 1.f) : mk3(q.T[slot], q.T[cap + slot], q.T[2 * cap + slot]);
+      f3 L = first ? mk3(0.f, 0.f, 0.f) : mk3(q.L[slot], q.L[cap + slot], q.L[2 * cap + slot]);
+      float A = first ? 0.f : q.A[slot];
       const int prim = q.hit_prim[i];
       cont = shade_path<kAdj>(sp, sc, sobn, (uint32_t)slot, o, d, prim, q.hit_t[i], q.hit_u[i], q.hit_v[i], T, L, A, no, nd, escaped, &adj);
       const bool done = !cont;
