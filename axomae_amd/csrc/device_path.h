@@ -137,7 +137,7 @@ NOVA_HD bool mt_test(const TriMT &tr, f3 o, f3 d, float tnear, float tbest, floa
 // Path-replay adjoint inputs: the primal radiance of every path slot (the batch's partial
 // buffers), the loss gradient per pixel (scaled) and the gradient pool (FP32 texel layout).
 struct AdjointCtx {
-  const float *ltot;
+  const float *ltot;  // RGBA primal radiance of the path being replayed
   const float *dimg;
   float scale;
   float *grad;
@@ -330,7 +330,7 @@ NOVA_HD bool shade_path(const ShadeParams &sp, const ShadeCtx &sc, Sobol sobol, 
     const float dlnr = ss.dlnr;
     const uint32_t npix = (uint32_t)sp.fp.width * (uint32_t)sp.fp.height;
     const uint32_t px = pixel_offset(sp.fp, slot % npix);
-    const float *lt = adj->ltot + (size_t)(slot / npix) * npix * 4 + px;
+    const float *lt = adj->ltot;  // the path's primal radiance: the caller points ltot at it
     const float *di = adj->dimg + px;
     // loss gradient of this vertex's weight, per channel: dimg * (L_path - L)
     const float G[3] = {adj->scale * di[0] * (lt[0] - L.x), adj->scale * di[1] * (lt[1] - L.y), adj->scale * di[2] * (lt[2] - L.z)};

@@ -43,21 +43,32 @@ struct Queues {
   uint32_t cap;
 };
 
+// Paths are numbered pixel-major, frame-minor: path id = pixel * frames + frame, which is
+// also the path's entry in the first queue. So a wave traces one pixel's camera rays of
+// consecutive frames (they differ only by the jitter and walk the same nodes), and the path
+// state and the per-path radiance (partial) are indexed by path id. The sampler's slot
+// keeps the frame-major numbering, slot = frame * npix + pixel.
+__device__ __forceinline__ uint32_t path_slot(const FrameParams &fp, uint32_t pid) {
+  const uint32_t npix = (uint32_t)fp.width * (uint32_t)fp.height;
+  return (pid % (uint32_t)fp.frames) * npix + pid / (uint32_t)fp.frames;
+}
+
 // AbstractIntegrator::render per pixel (Integrator.h:53-95): one camera ray per path slot.
-// The first queue is in slot order and the first shade starts every path at T = 1, L = 0,
-// A = 0, so neither slot ids nor path state are written here.
+// The first shade takes the path id from the queue position and starts every path at T = 1,
+// L = 0, A = 0, so neither path ids nor path state are written here.
 __global__ __launch_bounds__(256) void generate_kernel(FrameParams fp, const uint32_t *__restrict__ sobol, Queues q, uint32_t active) {
   const uint32_t n = q.cap;  // plane stride
-  for (uint32_t slot = blockIdx.x * blockDim.x + threadIdx.x; slot < active; slot += gridDim.x * blockDim.x) {
+  for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < active; i += gridDim.x * blockDim.x) {
+    const uint32_t slot = path_slot(fp, i);
     f3 o, d;
     camera_ray(fp, sobol, slot, o, d);
     float *r = q.ray[0];
-    r[0 * (size_t)n + slot] = o.x;
-    r[1 * (size_t)n + slot] = o.y;
-    r[2 * (size_t)n + slot] = o.z;
-    r[3 * (size_t)n + slot] = d.x;
-    r[4 * (size_t)n + slot] = d.y;
-    r[5 * (size_t)n + slot] = d.z;
+    r[0 * (size_t)n + i] = o.x;
+    r[1 * (size_t)n + i] = o.y;
+    r[2 * (size_t)n + i] = o.z;
+    r[3 * (size_t)n + i] = d.x;
+    r[4 * (size_t)n + i] = d.y;
+    r[5 * (size_t)n + i] = d.z;
   }
   if (blockIdx.x == 0 && threadIdx.x == 0) {
     q.count[0] = active;
@@ -143,11 +154,9 @@ __global__ __launch_bounds__(kTraceBlock) NOVA_TRACE_OCCUPANCY void extend_kerne
   }
 }
 
-__device__ __forceinline__ void write_partial(const Queues &q, const FrameParams &fp, uint32_t slot, float lx, float ly, float lz, float a) {
-  const uint32_t npix = (uint32_t)fp.width * (uint32_t)fp.height;
-  const uint32_t f = slot / npix;
-  float4 *dst = reinterpret_cast<float4 *>(q.partial + (size_t)f * npix * 4 + pixel_offset(fp, slot % npix));
-  *dst = make_float4(lx, ly, lz, a);
+// A finished path's radiance and alpha, by path id.
+__device__ __forceinline__ void write_partial(const Queues &q, uint32_t pid, float lx, float ly, float lz, float a) {
+  reinterpret_cast<float4 *>(q.partial)[pid] = make_float4(lx, ly, lz, a);
 }
 
 #ifndef NOVA_SHADE_WAVES
@@ -186,28 +195,31 @@ __global__ __launch_bounds__(kShadeBlock) NOVA_SHADE_OCCUPANCY void shade_kernel
     const uint32_t i = it * stride + blockIdx.x * blockDim.x + threadIdx.x;
     bool cont = false, escaped = false;
     f3 no = mk3(0, 0, 0), nd = mk3(0, 0, 0);
-    int32_t slot = -1;
+    int32_t pid = -1;
     if (i < n) {
-      const bool first = sp.bounce == 0;  // camera rays: slot order, fresh path state
-      slot = first ? (int32_t)i : q.slot[which][i];
+      const bool first = sp.bounce == 0;  // camera rays: path id = queue position, fresh path state
+      pid = first ? (int32_t)i : q.slot[which][i];
       const f3 o = mk3(rin[i], rin[cap + i], rin[2 * cap + i]);
       const f3 d = mk3(rin[3 * cap + i], rin[4 * cap + i], rin[5 * cap + i]);
-      f3 T = first ? mk3(1.f, 1.f, 1.f) : mk3(q.T[slot], q.T[cap + slot], q.T[2 * cap + slot]);
-      f3 L = first ? mk3(0.f, 0.f, 0.f) : mk3(q.L[slot], q.L[cap + slot], q.L[2 * cap + slot]);
-      float A = first ? 0.f : q.A[slot];
+      f3 T = first ? mk3(1.f, 1.f, 1.f) : mk3(q.T[pid], q.T[cap + pid], q.T[2 * cap + pid]);
+      f3 L = first ? mk3(0.f, 0.f, 0.f) : mk3(q.L[pid], q.L[cap + pid], q.L[2 * cap + pid]);
+      float A = first ? 0.f : q.A[pid];
       const int prim = q.hit_prim[i];
-      cont = shade_path<kAdj>(sp, sc, sobn, (uint32_t)slot, o, d, prim, q.hit_t[i], q.hit_u[i], q.hit_v[i], T, L, A, no, nd, escaped, &adj);
+      AdjointCtx padj = adj;
+      if (kAdj) padj.ltot = adj.ltot + (size_t)pid * 4;  // this path's primal radiance
+      cont = shade_path<kAdj>(sp, sc, sobn, path_slot(sp.fp, (uint32_t)pid), o, d, prim, q.hit_t[i], q.hit_u[i], q.hit_v[i], T, L, A, no, nd,
+                              escaped, &padj);
       const bool done = !cont;
-      q.T[slot] = T.x;
-      q.T[cap + slot] = T.y;
-      q.T[2 * cap + slot] = T.z;
+      q.T[pid] = T.x;
+      q.T[cap + pid] = T.y;
+      q.T[2 * cap + pid] = T.z;
       if (done) {
-        if (!kAdj) write_partial(q, sp.fp, (uint32_t)slot, L.x, L.y, L.z, A);  // a replay reads the primal's
+        if (!kAdj) write_partial(q, (uint32_t)pid, L.x, L.y, L.z, A);  // a replay reads the primal's
       } else {
-        q.L[slot] = L.x;
-        q.L[cap + slot] = L.y;
-        q.L[2 * cap + slot] = L.z;
-        q.A[slot] = A;
+        q.L[pid] = L.x;
+        q.L[cap + pid] = L.y;
+        q.L[2 * cap + pid] = L.z;
+        q.A[pid] = A;
       }
     }
     escapes += (uint32_t)__popcll(__ballot(escaped));
@@ -223,7 +235,7 @@ __global__ __launch_bounds__(kShadeBlock) NOVA_SHADE_OCCUPANCY void shade_kernel
       rout[3 * cap + j] = nd.x;
       rout[4 * cap + j] = nd.y;
       rout[5 * cap + j] = nd.z;
-      sout[j] = slot;
+      sout[j] = pid;
     }
   }
   if (escapes != 0 && (threadIdx.x & 63) == 0) atomicAdd(&q.stats64[2], (unsigned long long)escapes);
@@ -236,19 +248,25 @@ __global__ void reset_count_kernel(uint32_t *count, int which) {
   count[kWorkCounter] = 0;
 }
 
-// Engine::writeFramebuffer per frame, in frame order (engine.cpp:351-363)
-__global__ void accumulate_kernel(const float *__restrict__ partial, int frames, int frame0, size_t npix4, float *__restrict__ accum,
-                                  float *__restrict__ final_out, float *__restrict__ partial_out) {
-  for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < npix4; i += (size_t)gridDim.x * blockDim.x) {
+// Engine::writeFramebuffer per frame, in frame order (engine.cpp:351-363); one thread per
+// pixel channel, the pixel's frames contiguous in the path-id-ordered partial buffer.
+__global__ void accumulate_kernel(const float *__restrict__ partial, FrameParams fp, float *__restrict__ accum, float *__restrict__ final_out,
+                                  float *__restrict__ partial_out) {
+  const uint32_t npix4 = (uint32_t)fp.width * (uint32_t)fp.height * 4u;
+  const int frames = fp.frames;
+  for (uint32_t e = blockIdx.x * blockDim.x + threadIdx.x; e < npix4; e += gridDim.x * blockDim.x) {
+    const uint32_t p = e >> 2, c = e & 3u;
+    const uint32_t i = pixel_offset(fp, p) + c;
+    const float *src = partial + (size_t)p * frames * 4 + c;
     float a = accum[i];
     float fin = 0.f;
     for (int f = 0; f < frames; f++) {
-      a += partial[(size_t)f * npix4 + i];
-      fin = a / float(frame0 + f + 1);
+      a += src[(size_t)f * 4];
+      fin = a / float(fp.frame0 + f + 1);
     }
     accum[i] = a;
     final_out[i] = fin;
-    if (partial_out) partial_out[i] = partial[(size_t)(frames - 1) * npix4 + i];
+    if (partial_out) partial_out[i] = src[(size_t)(frames - 1) * 4];
   }
 }
 
@@ -555,9 +573,8 @@ static int render_impl(nova_hip_ctx *ctx, const nova_hip_render_params *p, float
     }
     }
     if (!adj) {
-      hipLaunchKernelGGL(accumulate_kernel, dim3(std::min<size_t>((npix * 4 + 255) / 256, (size_t)cus * 16)), dim3(256), 0, s, partial, frames, f0,
-                         npix * 4, accum_dev, final_dev,
-                         (f0 + frames == p->frame_start + p->frame_count) ? partial_dev : (float *)nullptr);
+      hipLaunchKernelGGL(accumulate_kernel, dim3(std::min<size_t>((npix * 4 + 255) / 256, (size_t)cus * 16)), dim3(256), 0, s, partial, fp,
+                         accum_dev, final_dev, (f0 + frames == p->frame_start + p->frame_count) ? partial_dev : (float *)nullptr);
       NOVA_HIP_CHECK(ctx, hipGetLastError());
     }
   }

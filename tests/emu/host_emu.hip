@@ -284,6 +284,8 @@ static int emu_run(const nova_hip_scene_desc *sd, const nova_hip_render_params *
       f3 T = mk3(1, 1, 1), L = mk3(0, 0, 0);
       float A = 0.f;
       uint32_t dim = 2;
+      AdjointCtx padj = adj;
+      padj.ltot = partial.data() + pixel_offset(fp, slot);  // this path's primal radiance
       for (int k = 0; k <= p->max_depth; k++) {
         float tb, bu, bv;
         const int best = brute_hit(o, d, tb, bu, bv);
@@ -296,7 +298,7 @@ static int emu_run(const nova_hip_scene_desc *sd, const nova_hip_render_params *
         f3 no, nd;
         bool esc;
         bool cont = pass == 0 ? shade_path(sp, sc, &kTable[0][0], slot, o, d, best, tb, bu, bv, T, L, A, no, nd, esc)
-                              : shade_path<true>(sp, sc, &kTable[0][0], slot, o, d, best, tb, bu, bv, T, L, A, no, nd, esc, &adj);
+                              : shade_path<true>(sp, sc, &kTable[0][0], slot, o, d, best, tb, bu, bv, T, L, A, no, nd, esc, &padj);
         dim += 4 + (sp.rr ? 1 : 0);
         if (!cont) break;
         o = no;
