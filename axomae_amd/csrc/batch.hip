@@ -473,6 +473,7 @@ static int make_plan(nova_hip_ctx *ex, const nova_hip_batch_item *items, uint32_
     S.fp.width = p->width;
     S.fp.height = p->height;
     S.fp.frame0 = p->frame_start;
+    S.fp.frame_minor = 0;
     S.fp.frames = p->frame_count;
     S.fp.offset = p->max_samples_offset;
     S.fp.vinvert = p->vertical_invert;

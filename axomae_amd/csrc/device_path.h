@@ -12,6 +12,7 @@ struct FrameParams {
   uint32_t offset;
   int32_t vinvert;
   int32_t policy;  // NOVA_HIP_INDEX_CPU or NOVA_HIP_INDEX_OPTIX
+  int32_t frame_minor;  // integrator path numbering: 1 pixel-major / frame-minor, 0 frame-major (= slot)
   float inv_P[16], inv_V[16];
 };
 

@@ -43,12 +43,15 @@ struct Queues {
   uint32_t cap;
 };
 
-// Paths are numbered pixel-major, frame-minor: path id = pixel * frames + frame, which is
+// Renders number paths pixel-major, frame-minor: path id = pixel * frames + frame, which is
 // also the path's entry in the first queue. So a wave traces one pixel's camera rays of
 // consecutive frames (they differ only by the jitter and walk the same nodes), and the path
 // state and the per-path radiance (partial) are indexed by path id. The sampler's slot
-// keeps the frame-major numbering, slot = frame * npix + pixel.
+// keeps the frame-major numbering, slot = frame * npix + pixel. The adjoint numbers paths
+// frame-major (path id = slot): its texel-gradient atomics would otherwise collide, all 64
+// lanes of a wave adding to the same pixel's texels.
 __device__ __forceinline__ uint32_t path_slot(const FrameParams &fp, uint32_t pid) {
+  if (!fp.frame_minor) return pid;
   const uint32_t npix = (uint32_t)fp.width * (uint32_t)fp.height;
   return (pid % (uint32_t)fp.frames) * npix + pid / (uint32_t)fp.frames;
 }
@@ -395,6 +398,7 @@ static int render_pass(nova_hip_ctx *ctx, const nova_hip_render_params *p, int p
   fp.height = p->height;
   fp.frame0 = p->frame_start;
   fp.frames = p->frame_count;
+  fp.frame_minor = 0;
   fp.offset = 0;
   fp.vinvert = p->vertical_invert;
   fp.policy = NOVA_HIP_INDEX_CPU;
@@ -532,6 +536,7 @@ static int render_impl(nova_hip_ctx *ctx, const nova_hip_render_params *p, float
     fp.height = p->height;
     fp.frame0 = f0;
     fp.frames = frames;
+    fp.frame_minor = adj ? 0 : 1;
     fp.offset = p->max_samples_offset;
     fp.vinvert = p->vertical_invert;
     fp.policy = p->index_policy;

@@ -238,6 +238,7 @@ static int emu_run(const nova_hip_scene_desc *sd, const nova_hip_render_params *
     fp.width = p->width;
     fp.height = p->height;
     fp.frame0 = p->frame_start;
+    fp.frame_minor = 0;
     fp.frames = p->frame_count;
     fp.offset = 0;
     fp.vinvert = p->vertical_invert;
@@ -270,6 +271,7 @@ static int emu_run(const nova_hip_scene_desc *sd, const nova_hip_render_params *
     fp.width = p->width;
     fp.height = p->height;
     fp.frame0 = f;
+    fp.frame_minor = 0;
     fp.frames = 1;
     fp.offset = p->max_samples_offset;
     fp.vinvert = p->vertical_invert;
