@@ -47,9 +47,9 @@ struct Queues {
 // also the path's entry in the first queue. So a wave traces one pixel's camera rays of
 // consecutive frames (they differ only by the jitter and walk the same nodes), and the path
 // state and the per-path radiance (partial) are indexed by path id. The sampler's slot
-// keeps the frame-major numbering, slot = frame * npix + pixel. The adjoint numbers paths
-// frame-major (path id = slot): its texel-gradient atomics would otherwise collide, all 64
-// lanes of a wave adding to the same pixel's texels.
+// keeps the frame-major numbering, slot = frame * npix + pixel. The adjoint's replay numbers
+// paths frame-major (path id = slot): its texel-gradient atomics would otherwise collide,
+// all 64 lanes of a wave adding to the same pixel's texels.
 __device__ __forceinline__ uint32_t path_slot(const FrameParams &fp, uint32_t pid) {
   if (!fp.frame_minor) return pid;
   const uint32_t npix = (uint32_t)fp.width * (uint32_t)fp.height;
@@ -209,7 +209,10 @@ __global__ __launch_bounds__(kShadeBlock) NOVA_SHADE_OCCUPANCY void shade_kernel
       float A = first ? 0.f : q.A[pid];
       const int prim = q.hit_prim[i];
       AdjointCtx padj = adj;
-      if (kAdj) padj.ltot = adj.ltot + (size_t)pid * 4;  // this path's primal radiance
+      if (kAdj) {  // this path's primal radiance, left by pass 0 under its frame-minor path id
+        const uint32_t slot = path_slot(sp.fp, (uint32_t)pid), npix = (uint32_t)sp.fp.width * (uint32_t)sp.fp.height;
+        padj.ltot = adj.ltot + ((size_t)(slot % npix) * (uint32_t)sp.fp.frames + slot / npix) * 4;
+      }
       cont = shade_path<kAdj>(sp, sc, sobn, path_slot(sp.fp, (uint32_t)pid), o, d, prim, q.hit_t[i], q.hit_u[i], q.hit_v[i], T, L, A, no, nd,
                               escaped, &padj);
       const bool done = !cont;
@@ -536,7 +539,6 @@ static int render_impl(nova_hip_ctx *ctx, const nova_hip_render_params *p, float
     fp.height = p->height;
     fp.frame0 = f0;
     fp.frames = frames;
-    fp.frame_minor = adj ? 0 : 1;
     fp.offset = p->max_samples_offset;
     fp.vinvert = p->vertical_invert;
     fp.policy = p->index_policy;
@@ -546,6 +548,7 @@ static int render_impl(nova_hip_ctx *ctx, const nova_hip_render_params *p, float
     }
     const uint32_t active = (uint32_t)(npix * frames);
     for (int pass = 0; pass < (adj ? 2 : 1); pass++) {  // pass 1: the adjoint replay of pass 0's paths
+    fp.frame_minor = pass == 0 ? 1 : 0;
     hipLaunchKernelGGL(generate_kernel, dim3(std::min<uint32_t>((active + 255) / 256, (uint32_t)cus * 8)), dim3(256), 0, s, fp, ctx->sobol_table, q,
                        active);
     NOVA_HIP_CHECK(ctx, hipGetLastError());
