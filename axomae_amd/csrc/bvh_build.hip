@@ -619,22 +619,13 @@ __global__ void collapse_kernel(const int32_t *__restrict__ frontier_bin, const 
   int32_t ch[4];
   const int m = collapse_children(bn, b, ch);
   Node4 nd;
+  set_node_boxes(nd, bn, ch, m);
   for (int k = 0; k < 4; k++) {
     if (k >= m) {
-      nd.minx[k] = nd.miny[k] = nd.minz[k] = 1.f;
-      nd.maxx[k] = nd.maxy[k] = nd.maxz[k] = -1.f;  // empty box never hit
-      nd.child[k] = -1;
-      nd.reserved[k] = 0;
+      nd.child[k] = -1;  // empty slot, never entered
       continue;
     }
     const BNode &c = bn[ch[k]];
-    nd.minx[k] = c.mn[0];
-    nd.miny[k] = c.mn[1];
-    nd.minz[k] = c.mn[2];
-    nd.maxx[k] = c.mx[0];
-    nd.maxy[k] = c.mx[1];
-    nd.maxz[k] = c.mx[2];
-    nd.reserved[k] = 0;
     if (c.count > 0) {
       nd.child[k] = leaf_ref(c.left);
     } else {

@@ -70,6 +70,75 @@ NOVA_HD int collapse_children(const BNode *bn, int32_t b, int32_t ch[4]) {
   return m;
 }
 
+// Plane value of grid byte q on an axis with origin org and step s (a power of two).
+NOVA_HD float grid_plane(uint32_t q, float s, float org) { return fmaf((float)q, s, org); }
+
+// Grid step of one axis of a node whose children span [lo, hi]: the smallest power of two
+// with 250 steps covering the span, and at least twice the float spacing at the span's
+// largest magnitude (so each step moves the decoded value).
+NOVA_HD float grid_step(float lo, float hi) {
+  int e_span = -160, e_mag = 0;
+  const float span = hi - lo;
+  if (span > 0.f) (void)frexpf(span / 250.f, &e_span);  // span / 250 <= 2^e_span
+  (void)frexpf(fmaxf(fabsf(lo), fabsf(hi)), &e_mag);     // magnitude < 2^e_mag, spacing 2^(e_mag - 24)
+  const int e = e_span > e_mag - 23 ? e_span : e_mag - 23;
+  return ldexpf(1.f, e < -149 ? -149 : e);
+}
+
+// Largest q in [0, 255] whose plane is <= v (q = 0 is the origin itself, which is <= v).
+NOVA_HD uint32_t grid_floor(float v, float s, float org) {
+  const float f = floorf((v - org) / s);
+  int q = f >= 0.f ? (f <= 255.f ? (int)f : 255) : 0;
+  while (q > 0 && grid_plane((uint32_t)q, s, org) > v) q--;
+  while (q < 255 && grid_plane((uint32_t)q + 1, s, org) <= v) q++;
+  return (uint32_t)q;
+}
+
+// Smallest q in [0, 255] whose plane is >= v; the step is chosen so that plane 255 is.
+NOVA_HD uint32_t grid_ceil(float v, float s, float org) {
+  const float f = ceilf((v - org) / s);
+  int q = f >= 0.f ? (f <= 255.f ? (int)f : 255) : 0;
+  while (q < 255 && grid_plane((uint32_t)q, s, org) < v) q++;
+  while (q > 0 && grid_plane((uint32_t)q - 1, s, org) >= v) q--;
+  return (uint32_t)q;
+}
+
+// Boxes of a 4-wide node from the binary nodes ch[0 .. m-1]: grid origin at the low corner of
+// their union, one step per axis, each child's planes rounded outward onto the grid. Empty
+// slots get planes 0. The caller sets the child references.
+NOVA_HD void set_node_boxes(Node4 &nd, const BNode *bn, const int32_t ch[4], int m) {
+  float lo[3], hi[3], org[3], step[3];
+  uint32_t qmn[3] = {0, 0, 0}, qmx[3] = {0, 0, 0};
+  for (int a = 0; a < 3; a++) {
+    lo[a] = bn[ch[0]].mn[a];
+    hi[a] = bn[ch[0]].mx[a];
+    for (int k = 1; k < m; k++) {
+      lo[a] = fminf(lo[a], bn[ch[k]].mn[a]);
+      hi[a] = fmaxf(hi[a], bn[ch[k]].mx[a]);
+    }
+    org[a] = lo[a];
+    step[a] = grid_step(lo[a], hi[a]);
+    while (grid_plane(255u, step[a], org[a]) < hi[a]) step[a] *= 2.f;  // not taken for finite boxes; a guard
+  }
+  for (int k = 0; k < m; k++)
+    for (int a = 0; a < 3; a++) {
+      qmn[a] |= grid_floor(bn[ch[k]].mn[a], step[a], org[a]) << (8 * k);
+      qmx[a] |= grid_ceil(bn[ch[k]].mx[a], step[a], org[a]) << (8 * k);
+    }
+  nd.orgx = org[0];
+  nd.orgy = org[1];
+  nd.orgz = org[2];
+  nd.stepx = step[0];
+  nd.stepy = step[1];
+  nd.stepz = step[2];
+  nd.qminx = qmn[0];
+  nd.qmaxx = qmx[0];
+  nd.qminy = qmn[1];
+  nd.qmaxy = qmx[1];
+  nd.qminz = qmn[2];
+  nd.qmaxz = qmx[2];
+}
+
 // BoundingBox::intersect (BoundingBox.cpp:150-160) in its division form, std::min/std::max
 // operand order kept; a 0/0 slab (ray inside a face plane) spans the whole line.
 NOVA_HD float ref_box_entry(const BNode &b, f3 o, f3 d) {
@@ -181,11 +250,12 @@ NOVA_HD bool trace_step(const TraceCtx &tc, Stack &stack, TraceState &st, float 
   const float kInf = __builtin_inff();
     if (kCount) nodes_visited++;
     const NOVA_GLOBAL float4 *np = (const NOVA_GLOBAL float4 *)(tc.nodes + st.node);
-    const float4 mnx = np[0], mxx = np[1], mny = np[2], mxy = np[3], mnz = np[4], mxz = np[5];
-    const int4 ch = *(const NOVA_GLOBAL int4 *)(np + 6);
-    const float bmnx[4] = {mnx.x, mnx.y, mnx.z, mnx.w}, bmxx[4] = {mxx.x, mxx.y, mxx.z, mxx.w};
-    const float bmny[4] = {mny.x, mny.y, mny.z, mny.w}, bmxy[4] = {mxy.x, mxy.y, mxy.z, mxy.w};
-    const float bmnz[4] = {mnz.x, mnz.y, mnz.z, mnz.w}, bmxz[4] = {mxz.x, mxz.y, mxz.z, mxz.w};
+    const float4 g0 = np[0], g1 = np[1];
+    const uint4 g2 = *(const NOVA_GLOBAL uint4 *)(np + 2);
+    const int4 ch = *(const NOVA_GLOBAL int4 *)(np + 3);
+    const float orgx = g0.x, orgy = g0.y, orgz = g0.z, sx = g0.w, sy = g1.x, sz = g1.y;
+    const uint32_t qmnx = __builtin_bit_cast(uint32_t, g1.z), qmxx = __builtin_bit_cast(uint32_t, g1.w);
+    const uint32_t qmny = g2.x, qmxy = g2.y, qmnz = g2.z, qmxz = g2.w;
     const int chv[4] = {ch.x, ch.y, ch.z, ch.w};
     // per child: entry distance (+inf when culled) and child reference (-1 when culled)
     float key[4];
@@ -197,21 +267,24 @@ NOVA_HD bool trace_step(const TraceCtx &tc, Stack &stack, TraceState &st, float 
     if (st.parallel) {
 #pragma unroll
       for (int k = 0; k < 4; k++) {
-        float tx1 = (bmnx[k] - st.o.x) * st.inv.x, tx2 = (bmxx[k] - st.o.x) * st.inv.x;
-        float ty1 = (bmny[k] - st.o.y) * st.inv.y, ty2 = (bmxy[k] - st.o.y) * st.inv.y;
-        float tz1 = (bmnz[k] - st.o.z) * st.inv.z, tz2 = (bmxz[k] - st.o.z) * st.inv.z;
+        const float bmnx = grid_plane((qmnx >> (8 * k)) & 255u, sx, orgx), bmxx = grid_plane((qmxx >> (8 * k)) & 255u, sx, orgx);
+        const float bmny = grid_plane((qmny >> (8 * k)) & 255u, sy, orgy), bmxy = grid_plane((qmxy >> (8 * k)) & 255u, sy, orgy);
+        const float bmnz = grid_plane((qmnz >> (8 * k)) & 255u, sz, orgz), bmxz = grid_plane((qmxz >> (8 * k)) & 255u, sz, orgz);
+        float tx1 = (bmnx - st.o.x) * st.inv.x, tx2 = (bmxx - st.o.x) * st.inv.x;
+        float ty1 = (bmny - st.o.y) * st.inv.y, ty2 = (bmxy - st.o.y) * st.inv.y;
+        float tz1 = (bmnz - st.o.z) * st.inv.z, tz2 = (bmxz - st.o.z) * st.inv.z;
         if (st.px) {
-          const bool in = bmnx[k] <= st.o.x && st.o.x <= bmxx[k];
+          const bool in = bmnx <= st.o.x && st.o.x <= bmxx;
           tx1 = -kInf;
           tx2 = in ? kInf : -kInf;
         }
         if (st.py) {
-          const bool in = bmny[k] <= st.o.y && st.o.y <= bmxy[k];
+          const bool in = bmny <= st.o.y && st.o.y <= bmxy;
           ty1 = -kInf;
           ty2 = in ? kInf : -kInf;
         }
         if (st.pz) {
-          const bool in = bmnz[k] <= st.o.z && st.o.z <= bmxz[k];
+          const bool in = bmnz <= st.o.z && st.o.z <= bmxz;
           tz1 = -kInf;
           tz2 = in ? kInf : -kInf;
         }
@@ -222,13 +295,23 @@ NOVA_HD bool trace_step(const TraceCtx &tc, Stack &stack, TraceState &st, float 
         cref[k] = hit ? chv[k] : -1;
       }
     } else {
+      // With finite reciprocals the nearer plane of an axis is the min plane when the
+      // direction is positive and the max plane when it is negative (a decoded min plane is
+      // <= its max plane and the products keep that order), so the per-axis min/max of the
+      // two slab distances becomes a choice of plane bytes, made once per node.
+      const uint32_t nqx = st.inv.x < 0.f ? qmxx : qmnx, fqx = st.inv.x < 0.f ? qmnx : qmxx;
+      const uint32_t nqy = st.inv.y < 0.f ? qmxy : qmny, fqy = st.inv.y < 0.f ? qmny : qmxy;
+      const uint32_t nqz = st.inv.z < 0.f ? qmxz : qmnz, fqz = st.inv.z < 0.f ? qmnz : qmxz;
 #pragma unroll
       for (int k = 0; k < 4; k++) {
-        const float tx1 = (bmnx[k] - st.o.x) * st.inv.x, tx2 = (bmxx[k] - st.o.x) * st.inv.x;
-        const float ty1 = (bmny[k] - st.o.y) * st.inv.y, ty2 = (bmxy[k] - st.o.y) * st.inv.y;
-        const float tz1 = (bmnz[k] - st.o.z) * st.inv.z, tz2 = (bmxz[k] - st.o.z) * st.inv.z;
-        const float t0 = fmaxf(fmaxf(fminf(tx1, tx2), fminf(ty1, ty2)), fminf(tz1, tz2));
-        const float t1 = fminf(fminf(fmaxf(tx1, tx2), fmaxf(ty1, ty2)), fmaxf(tz1, tz2)) * kUp;
+        const float tnx = (grid_plane((nqx >> (8 * k)) & 255u, sx, orgx) - st.o.x) * st.inv.x;
+        const float tfx = (grid_plane((fqx >> (8 * k)) & 255u, sx, orgx) - st.o.x) * st.inv.x;
+        const float tny = (grid_plane((nqy >> (8 * k)) & 255u, sy, orgy) - st.o.y) * st.inv.y;
+        const float tfy = (grid_plane((fqy >> (8 * k)) & 255u, sy, orgy) - st.o.y) * st.inv.y;
+        const float tnz = (grid_plane((nqz >> (8 * k)) & 255u, sz, orgz) - st.o.z) * st.inv.z;
+        const float tfz = (grid_plane((fqz >> (8 * k)) & 255u, sz, orgz) - st.o.z) * st.inv.z;
+        const float t0 = fmaxf(fmaxf(tnx, tny), tnz);
+        const float t1 = fminf(fminf(tfx, tfy), tfz) * kUp;
         const bool hit = chv[k] != -1 && t0 <= t1 && t1 >= 0.f && t0 <= st.tbest * kUp;
         key[k] = hit ? t0 : kInf;
         cref[k] = hit ? chv[k] : -1;

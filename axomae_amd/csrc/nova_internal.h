@@ -63,18 +63,24 @@ struct TieTables {
   const int32_t *bparent, *bdepth, *leaf_of_pos;
 };
 
-// 4-wide node, 128 B: child boxes SoA, then per-child reference. child[k] >= 0: an inner
-// node index; child[k] <= -2: a leaf whose triangles start at -(child[k] + 2) in the TriMT
-// array (the size is in that first record's b.w); child[k] == -1: empty slot. The last 16
-// bytes are not read, so a visit is seven 16-byte loads.
+// 4-wide node, 64 B, four 16-byte loads per visit. The child boxes are on a per-node grid:
+// on axis a, plane value q (one byte) stands for the float fmaf(q, step_a, org_a), and the
+// build picks each min plane's q so that this value is <= the child's box minimum and each
+// max plane's q so that it is >= the maximum (checked in that same float arithmetic), so a
+// decoded box holds the child's exact box. Byte k of qmin*/qmax* belongs to child k.
+// child[k] >= 0: an inner node index; child[k] <= -2: a leaf whose triangles start at
+// -(child[k] + 2) in the TriMT array (the size is in that first record's b.w);
+// child[k] == -1: empty slot.
 struct alignas(16) Node4 {
-  float minx[4], maxx[4], miny[4], maxy[4], minz[4], maxz[4];
+  float orgx, orgy, orgz, stepx;
+  float stepy, stepz;
+  uint32_t qminx, qmaxx;
+  uint32_t qminy, qmaxy, qminz, qmaxz;
   int32_t child[4];
-  int32_t reserved[4];
 };
 __host__ __device__ inline int32_t leaf_ref(int32_t first) { return -(first + 2); }
 __host__ __device__ inline int32_t leaf_first(int32_t ref) { return -(ref + 2); }
-static_assert(sizeof(Node4) == 128, "Node4 layout");
+static_assert(sizeof(Node4) == 64, "Node4 layout");
 
 struct DeviceBuffer {
   void *ptr = nullptr;

@@ -8,8 +8,9 @@ frames; value = width*height*K / t / 1e6 Msamples/s. Scene upload and BVH build 
 outside the timed region and reported separately.
 
 roofline: the dominant kernel is the extend (closest-hit traversal) kernel. Its
-algorithmic bytes per ray are 48 (ray in, hit out) + 112 per 4-wide node visited (the
-seven 16-byte quads of the 128-byte node a visit reads) + 48 per triangle tested, with
+algorithmic bytes per ray are 48 (ray in, hit out) + 64 per 4-wide node visited (the
+whole 64-byte node: grid origin and steps, 8-bit child planes, child references) + 48 per
+triangle tested, with
 visits/tests counted by an instrumented render of the same
 frames; its duration comes from HIP events the library records around every extend
 launch on the render stream during the timed region.
@@ -28,7 +29,7 @@ sys.path.insert(0, ROOT)
 
 METRIC = "Msamples/s (paths) at 1080p, depth 8; adjoint Msamples/s vs primal, 1 GPU"
 HBM_PEAK_GBS = 8000.0
-NODE_BYTES = 112.0  # bytes of a 4-wide node one visit reads (child boxes and references)
+NODE_BYTES = 64.0  # bytes of a 4-wide node one visit reads (child boxes and references)
 
 
 def log(*a):

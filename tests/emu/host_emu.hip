@@ -107,22 +107,13 @@ extern "C" int emu_closest_hit(const nova_hip_scene_desc *sd, const BNode *bnode
     int32_t ch[4];
     const int m = collapse_children(bnodes, frontier[k].first, ch);
     Node4 nd;
+    set_node_boxes(nd, bnodes, ch, m);
     for (int c = 0; c < 4; c++) {
       if (c >= m) {
-        nd.minx[c] = nd.miny[c] = nd.minz[c] = 1.f;
-        nd.maxx[c] = nd.maxy[c] = nd.maxz[c] = -1.f;
         nd.child[c] = -1;
-        nd.reserved[c] = 0;
         continue;
       }
       const BNode &b = bnodes[ch[c]];
-      nd.minx[c] = b.mn[0];
-      nd.miny[c] = b.mn[1];
-      nd.minz[c] = b.mn[2];
-      nd.maxx[c] = b.mx[0];
-      nd.maxy[c] = b.mx[1];
-      nd.maxz[c] = b.mx[2];
-      nd.reserved[c] = 0;
       if (b.count > 0) {
         nd.child[c] = leaf_ref(b.left);
       } else {
