@@ -269,6 +269,7 @@ float unit_vndf_G1(float aniso, float rough, const vec3 &v);
 float unit_vndf_D(float aniso, float rough, const vec3 &wm);
 int unit_principled_sample(const vec3 &wo, const vec3 &albedo, float roughness, float metal, float eta, float uc, float u0, float u1, float uc2,
                            vec3 &f, float &pdf, vec3 &wi, float &costheta, int &valid);
+int unit_principled_eval(const vec3 &wo, const vec3 &wi, const vec3 &albedo, float roughness, float metal, float eta, float uc, vec3 &f, float &pdf);
 }  // namespace oracle
 
 extern "C" {
@@ -296,5 +297,17 @@ void oracle_principled_sample(const float *wo, const float *albedo, float roughn
                                     u4[3], f, pdf, wi, ct, valid);
   float vals[10] = {f.x, f.y, f.z, pdf, wi.x, wi.y, wi.z, ct, (float)valid, (float)lobe};
   for (int i = 0; i < 10; i++) out[i] = vals[i];
+}
+// out: f3, pdf, lobe (5 floats)
+void oracle_principled_eval(const float *wo, const float *wi, const float *albedo, float roughness, float metal, float eta, float uc, float *out) {
+  vec3 f;
+  float pdf = 0;
+  int lobe = unit_principled_eval(vec3(wo[0], wo[1], wo[2]), vec3(wi[0], wi[1], wi[2]), vec3(albedo[0], albedo[1], albedo[2]), roughness, metal, eta, uc,
+                                  f, pdf);
+  out[0] = f.x;
+  out[1] = f.y;
+  out[2] = f.z;
+  out[3] = pdf;
+  out[4] = (float)lobe;
 }
 }  // extern "C"

@@ -11,7 +11,9 @@
 //   closest hit (MT + native traversal) .... pinned (brute force over all triangles)
 //   hit record (Embree path) ............... partial: Embree's own interpolation is third-party (parity unpinned
 //                                            against Embree); pinned by planar-triangle closed forms
-//   BxDFs / PrincipledBSDF ................. pinned by the reference BxDFTest invariants + closed forms
+//   BxDFs / PrincipledBSDF ................. pinned by the reference BxDFTest invariants (sampled f/pdf =
+//                                            separately written f/pdf, reciprocity, albedo <= 1, pdf closed
+//                                            form, eta-1 transmission) + closed forms
 //   PathIntegrator::Li ..................... pinned by analytic scenes (constant envmap, emission only)
 //   irradiance bake ........................ pinned (constant map => constant, brute force of the texel rule)
 //   prefilter bake ......................... partial (no reference CPU implementation exists)

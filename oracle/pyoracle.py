@@ -285,6 +285,16 @@ def principled_sample(wo, albedo, roughness, metal, eta, u4):
     return int(out[9]), bool(out[8]), out[:3].copy(), float(out[3]), out[4:7].copy(), float(out[7])
 
 
+def principled_eval(wo, wi, albedo, roughness, metal, eta, uc):
+    """The lobe uc selects (same parameters as principled_sample), evaluated at (wo, wi):
+    returns (lobe, f, pdf) with the lobe weight and selection probability applied."""
+    L = lib()
+    L.oracle_principled_eval.argtypes = [f32p, f32p, f32p, C.c_float, C.c_float, C.c_float, C.c_float, f32p]
+    out = np.zeros(5, dtype=np.float32)
+    L.oracle_principled_eval(_vec(wo), _vec(wi), _vec(albedo), roughness, metal, eta, uc, out)
+    return int(out[4]), out[:3].copy(), float(out[3])
+
+
 def set_float_libm(on):
     """True: the host libm's float cosf/sinf/asinf/powf, as the reference's CPU build calls them; False (default):
     evaluated in double and rounded once, the correctly rounded value the HIP path computes."""

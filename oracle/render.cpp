@@ -545,6 +545,72 @@ struct Principled {
     return true;
   }
 
+  // Evaluation of the selected lobe for a given pair of local directions, the lobe's f and
+  // pdf weighted like weightLobe / weightPdf. Not on the integrator's path (Li only samples);
+  // written apart from the samplers above so the unit tests can check sample_f against it.
+  // DisneyDiffuseBxDF::f / pdf (BxDF.h:257-286)
+  void eval_diffuse(const vec3 &wo, const vec3 &wi, Spec &f, float &pdf) const {
+    auto fres = [](const vec3 &w, float value) { return 1.f + (value - 1) * pow5(1.f - abscos(w)); };
+    f = Spec(0.f);
+    pdf = same_hemi(wo, wi) ? lobe_prob * (abscos(wi) * (float)M_1_PI) : 0.f;
+    vec3 h = normalize(wi + wo);
+    if (!(same_hemi(wo, wi) && same_hemi(wo, h))) return;
+    float rough = sqr(params.roughness);
+    float fd90 = 0.5f + 2 * rough * sqr(absdot(wi, h));
+    Spec fd = params.albedo * (float)M_1_PI * fres(wi, fd90) * fres(wo, fd90);
+    float fss90 = rough * sqr(absdot(h, wi));
+    Spec fss = 1.25f * params.albedo * (float)M_1_PI * (0.5f + fres(wi, fss90) * fres(wo, fss90) * (-0.5f + 1 / (abscos(wo) + abscos(wi))));
+    f = lobe_weight * ((1 - params.subsurface) * fd + params.subsurface * fss);
+  }
+  // SpecularBxDF::f / pdf (BxDF.h:343-364)
+  void eval_specular(const vec3 &wo, const vec3 &wi, Spec &f, float &pdf) const {
+    f = Spec(0.f);
+    pdf = 0.f;
+    if (!same_hemi(wi, wo)) return;
+    VNDF g(vndf_aniso, vndf_rough);
+    vec3 wm = normalize(wi + wo);
+    float wm_dot_wo = absdot(wm, wo);
+    Spec mm(params.metal);
+    Spec F0 = Spec(0.04f) * (Spec(1.f) - mm) + mm * params.albedo;
+    F0 = schlick(wm_dot_wo, F0);
+    f = lobe_weight * (F0 * g.D(wm) * g.G(wo, wi) / (4 * abscos(wo) * abscos(wi)));
+    pdf = lobe_prob * (g.pdf(wo, wm) / (4 * wm_dot_wo));
+  }
+  // DielectricBxDF::f / pdf (BxDF.h:497-551), radiance transport
+  void eval_dielectric(const vec3 &wo, const vec3 &wi, Spec &f, float &pdf) const {
+    VNDF g(vndf_aniso, vndf_rough);
+    f = Spec(0.f);
+    pdf = 0.f;
+    const float eta = params.eta.x;
+    if (g.full_specular() || eta == 1.f) return;
+    if (same_hemi(wi, wo)) {
+      vec3 wm = normalize(wi + wo);
+      float wo_dot_wm = absdot(wo, wm);
+      float R = fresnel_real(eta, wo_dot_wm), T = 1.f - R;
+      f = lobe_weight * (params.albedo * g.D(wm) * g.G(wo, wi) * R / (4 * costheta(wi) * costheta(wo)));
+      pdf = lobe_prob * (g.pdf(wo, wm) / (4 * wo_dot_wm) * R / (R + T));
+      return;
+    }
+    vec3 wm = normalize(wi * eta + wo);
+    if (dot(wo, wm) < 0) wm = -wm;
+    float wo_dot_wm = absdot(wo, wm);
+    float R = fresnel_real(eta, wo_dot_wm), T = 1.f - R;
+    float denom = sqr(dot(wi, wm) + dot(wo, wm) / eta);
+    Spec v = params.albedo * T * g.D(wm) * g.G(wo, wi) * std::fabs(dot(wi, wm) * dot(wo, wm) / (costheta(wi) * costheta(wo) * denom));
+    f = lobe_weight * (v / sqr(eta));
+    pdf = lobe_prob * (g.pdf(wo, wm) * (absdot(wi, wm) / denom) * T / (T + R));
+  }
+  void eval(const vec3 &lwo, const vec3 &lwi, Spec &f, float &pdf) const {
+    f = Spec(0.f);
+    pdf = 0.f;
+    switch (lobe) {
+      case LOBE_DIFFUSE: eval_diffuse(lwo, lwi, f, pdf); break;
+      case LOBE_SPECULAR: eval_specular(lwo, lwi, f, pdf); break;
+      case LOBE_DIELECTRIC: eval_dielectric(lwo, lwi, f, pdf); break;
+      default: break;
+    }
+  }
+
   // PrincipledBSDF::sample_f (BSDF.h:107-130)
   bool sample_f(const vec3 &wo, float uc, const float u[2], Sample &s) const {
     if (lobe == LOBE_NONE) return false;
@@ -1156,8 +1222,7 @@ float unit_vndf_D(float aniso, float rough, const vec3 &wm) { return VNDF(aniso,
 // One Principled sample in a frame where ns = ng = +z, dpdu = +x, dpdv = +y (world = local).
 // Returns the lobe (0 none, 1 diffuse, 2 specular, 3 dielectric) and fills f, pdf, wi, costheta
 // when the sample is valid (valid = 0 otherwise).
-int unit_principled_sample(const vec3 &wo, const vec3 &albedo, float roughness, float metal, float eta, float uc, float u0, float u1, float uc2,
-                           vec3 &f, float &pdf, vec3 &wi, float &costheta, int &valid) {
+static BsdfParams unit_params(const vec3 &wo, const vec3 &albedo, float roughness, float metal, float eta) {
   BsdfParams bp;
   bp.ns = vec3(0, 0, 1);
   bp.ng = vec3(0, 0, 1);
@@ -1170,8 +1235,12 @@ int unit_principled_sample(const vec3 &wo, const vec3 &albedo, float roughness, 
   bp.metal = metal;
   bp.eta = vec3(eta);
   bp.transmission = eta != 0.f ? 1.f : 0.f;
+  return bp;
+}
+int unit_principled_sample(const vec3 &wo, const vec3 &albedo, float roughness, float metal, float eta, float uc, float u0, float u1, float uc2,
+                           vec3 &f, float &pdf, vec3 &wi, float &costheta, int &valid) {
   Principled b;
-  b.configure(bp);
+  b.configure(unit_params(wo, albedo, roughness, metal, eta));
   b.construct(uc);
   const float u[2] = {u0, u1};
   Sample s;
@@ -1180,6 +1249,15 @@ int unit_principled_sample(const vec3 &wo, const vec3 &albedo, float roughness, 
   pdf = s.pdf;
   wi = s.wi;
   costheta = s.costheta;
+  return (int)b.lobe;
+}
+// The lobe that uc selects, evaluated at (wo, wi) in the same frame: f and pdf of the lobe,
+// weighted by its lobe weight and selection probability as sample_f's are.
+int unit_principled_eval(const vec3 &wo, const vec3 &wi, const vec3 &albedo, float roughness, float metal, float eta, float uc, vec3 &f, float &pdf) {
+  Principled b;
+  b.configure(unit_params(wo, albedo, roughness, metal, eta));
+  b.construct(uc);
+  b.eval(wo, wi, f, pdf);
   return (int)b.lobe;
 }
 }  // namespace oracle
