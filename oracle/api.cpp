@@ -270,6 +270,7 @@ float unit_vndf_D(float aniso, float rough, const vec3 &wm);
 int unit_principled_sample(const vec3 &wo, const vec3 &albedo, float roughness, float metal, float eta, float uc, float u0, float u1, float uc2,
                            vec3 &f, float &pdf, vec3 &wi, float &costheta, int &valid);
 int unit_principled_eval(const vec3 &wo, const vec3 &wi, const vec3 &albedo, float roughness, float metal, float eta, float uc, vec3 &f, float &pdf);
+void unit_make_hit(const Scene &s, int prim, const vec3 &ro, const vec3 &rd, float t, float bu, float bv, float *out);
 }  // namespace oracle
 
 extern "C" {
@@ -297,6 +298,9 @@ void oracle_principled_sample(const float *wo, const float *albedo, float roughn
                                     u4[3], f, pdf, wi, ct, valid);
   float vals[10] = {f.x, f.y, f.z, pdf, wi.x, wi.y, wi.z, ct, (float)valid, (float)lobe};
   for (int i = 0; i < 10; i++) out[i] = vals[i];
+}
+void oracle_make_hit(void *sp, int prim, const float *ray, float t, float bu, float bv, float *out) {
+  unit_make_hit(*static_cast<Scene *>(sp), prim, vec3(ray[0], ray[1], ray[2]), vec3(ray[3], ray[4], ray[5]), t, bu, bv, out);
 }
 // out: f3, pdf, lobe (5 floats)
 void oracle_principled_eval(const float *wo, const float *wi, const float *albedo, float roughness, float metal, float eta, float uc, float *out) {

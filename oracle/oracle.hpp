@@ -9,14 +9,17 @@
 //   atan2_approx ........................... pinned (error bound vs atan2, quadrant identities)
 //   bvh_build (BvhtlBuilder) ............... pinned (brute-force partition, leaf invariants, SAH exhaustive small n)
 //   closest hit (MT + native traversal) .... pinned (brute force over all triangles)
-//   hit record (Embree path) ............... partial: Embree's own interpolation is third-party (parity unpinned
-//                                            against Embree); pinned by planar-triangle closed forms
+//   hit record (Embree path) ............... pinned by closed forms (unit-UV derivatives, the UV solve, hit
+//                                            point, transformed normals, degenerate-UV basis); Embree supplies
+//                                            only the barycentrics, with the same u->v1, v->v2 convention as MT
 //   BxDFs / PrincipledBSDF ................. pinned by the reference BxDFTest invariants (sampled f/pdf =
 //                                            separately written f/pdf, reciprocity, albedo <= 1, pdf closed
 //                                            form, eta-1 transmission) + closed forms
 //   PathIntegrator::Li ..................... pinned by analytic scenes (constant envmap, emission only)
 //   irradiance bake ........................ pinned (constant map => constant, brute force of the texel rule)
-//   prefilter bake ......................... partial (no reference CPU implementation exists)
+//   prefilter bake ......................... pinned (no reference CPU implementation exists): linearity,
+//                                            roughness 0 = the map, narrow lobes stay in a region, a cosine sky
+//                                            against the GGX integral from the distribution
 //   BRDF table (brdf_lut_value) ............. pinned (roughness 0 closed form; NdotV = 1 as a 1-D sum)
 //   render passes (render_pass) ............. pinned (wall scene: plane distances, unit normals, texel / 255)
 //   render_adjoint (extension) ............. pinned (central finite differences of the oracle's own renders)

@@ -171,6 +171,15 @@ class OracleScene:
         lib().oracle_closest_hit(self.h, self.bvh, rays, n, int(brute), threads, prim, tuv)
         return prim, tuv
 
+    def make_hit(self, prim, ray, t, u, v):
+        """Hit record of primitive prim at (t, u, v) on ray (ox, oy, oz, dx, dy, dz): a dict of
+        ng, dpdu, dpdv, position, uv and the shading frame columns T, B, N."""
+        L = lib()
+        L.oracle_make_hit.argtypes = [C.c_void_p, C.c_int, f32p, C.c_float, C.c_float, C.c_float, f32p]
+        out = np.zeros(23, dtype=np.float32)
+        L.oracle_make_hit(self.h, int(prim), _vec(ray), t, u, v, out)
+        return dict(ng=out[0:3], dpdu=out[3:6], dpdv=out[6:9], position=out[9:12], uv=out[12:14], T=out[14:17], B=out[17:20], N=out[20:23])
+
     def render(self, width, height, frames, max_depth, frame_start=0, rr_start=-1, max_samples_offset=0, vertical_invert=False,
                threads=8, accum=None, sampling_roughness_texture=-1):
         n = width * height * 4

@@ -1251,6 +1251,24 @@ int unit_principled_sample(const vec3 &wo, const vec3 &albedo, float roughness, 
   costheta = s.costheta;
   return (int)b.lobe;
 }
+// The hit record of primitive prim at (t, bu, bv) on ray (ro, rd): ng, dpdu, dpdv, position,
+// u, v, then the shading frame's T, B, N columns (23 floats).
+void unit_make_hit(const Scene &s, int prim, const vec3 &ro, const vec3 &rd, float t, float bu, float bv, float *out) {
+  const Hit h = make_hit(s, prim, ro, rd, t, bu, bv);
+  const vec3 vs[4] = {h.ng, h.dpdu, h.dpdv, h.position};
+  for (int k = 0; k < 4; k++) {
+    out[3 * k] = vs[k].x;
+    out[3 * k + 1] = vs[k].y;
+    out[3 * k + 2] = vs[k].z;
+  }
+  out[12] = h.u;
+  out[13] = h.v;
+  for (int c = 0; c < 3; c++) {
+    out[14 + 3 * c] = h.frame.c[c].x;
+    out[15 + 3 * c] = h.frame.c[c].y;
+    out[16 + 3 * c] = h.frame.c[c].z;
+  }
+}
 // The lobe that uc selects, evaluated at (wo, wi) in the same frame: f and pdf of the lobe,
 // weighted by its lobe weight and selection probability as sample_f's are.
 int unit_principled_eval(const vec3 &wo, const vec3 &wi, const vec3 &albedo, float roughness, float metal, float eta, float uc, vec3 &f, float &pdf) {
