@@ -83,6 +83,10 @@ __device__ __forceinline__ int find_scene(const BatchScene *__restrict__ scenes,
 
 __global__ __launch_bounds__(256) void batch_generate(const BatchScene *__restrict__ scenes, int nscenes, const WaveDesc *__restrict__ waves,
                                                       BatchState *st, const uint32_t *__restrict__ sobol, BatchQueues q) {
+  __shared__ uint32_t sob_tab[2 * 128];  // jitter dims 0-1 as per-nibble tables
+  sobol_nibble_fill(sobol, 0, 2, sob_tab, threadIdx.x, blockDim.x);
+  __syncthreads();
+  const SobolNibbles sobn{sob_tab, 0};
   const WaveDesc w = waves[st->wave];
   const size_t n = q.cap;
   for (uint32_t r = blockIdx.x * blockDim.x + threadIdx.x; r < w.count; r += gridDim.x * blockDim.x) {
@@ -90,7 +94,7 @@ __global__ __launch_bounds__(256) void batch_generate(const BatchScene *__restri
     const int s = find_scene(scenes, nscenes, g);
     const uint32_t l = (uint32_t)(g - scenes[s].first);
     f3 o, d;
-    camera_ray(scenes[s].fp, sobol, l, o, d);
+    camera_ray(scenes[s].fp, sobn, l, o, d);
     float *ray = q.ray[0];
     ray[r] = o.x;
     ray[n + r] = o.y;

@@ -90,7 +90,9 @@ NOVA_HD void camera_ray_ndc(const FrameParams &fp, float nx, float ny, f3 &o, f3
   d = mk3(vx * inv_len, vy * inv_len, vz * inv_len);
 }
 
-NOVA_HD void camera_ray(const FrameParams &fp, const uint32_t *sobol, uint32_t slot, f3 &o, f3 &d) {
+// Sob: the direction table (const uint32_t *) or nibble tables holding dims 0-1 (SobolNibbles).
+template <class Sob>
+NOVA_HD void camera_ray(const FrameParams &fp, const Sob &sobol, uint32_t slot, f3 &o, f3 &d) {
   const uint32_t npix = (uint32_t)fp.width * (uint32_t)fp.height;
   const uint32_t p = slot % npix;
   const int x = (int)(p % (uint32_t)fp.width), y = (int)(p / (uint32_t)fp.width);

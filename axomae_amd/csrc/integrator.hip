@@ -60,11 +60,17 @@ __device__ __forceinline__ uint32_t path_slot(const FrameParams &fp, uint32_t pi
 // The first shade takes the path id from the queue position and starts every path at T = 1,
 // L = 0, A = 0, so neither path ids nor path state are written here.
 __global__ __launch_bounds__(256) void generate_kernel(FrameParams fp, const uint32_t *__restrict__ sobol, Queues q, uint32_t active) {
+  // the jitter dimensions 0-1 from per-nibble tables: 8 LDS reads per sample instead of a
+  // 32-step conditional XOR over the index bits
+  __shared__ uint32_t sob_tab[2 * 128];
+  sobol_nibble_fill(sobol, 0, 2, sob_tab, threadIdx.x, blockDim.x);
+  __syncthreads();
+  const SobolNibbles sobn{sob_tab, 0};
   const uint32_t n = q.cap;  // plane stride
   for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < active; i += gridDim.x * blockDim.x) {
     const uint32_t slot = path_slot(fp, i);
     f3 o, d;
-    camera_ray(fp, sobol, slot, o, d);
+    camera_ray(fp, sobn, slot, o, d);
     float *r = q.ray[0];
     r[0 * (size_t)n + i] = o.x;
     r[1 * (size_t)n + i] = o.y;
