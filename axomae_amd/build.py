@@ -13,7 +13,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
 BUILD = os.path.join(HERE, "_build")
 LIB = os.path.join(HERE, "libnova_hip.so")
-SOURCES = ["api.hip", "scene.hip", "bvh_build.hip", "integrator.hip", "batch.hip", "bake.hip", "hip_accelerator.hip"]
+SOURCES = ["api.hip", "scene.hip", "bvh_build.hip", "integrator.hip", "batch.hip", "bake.hip", "renderbuffer.hip", "hip_accelerator.hip"]
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 ARCH = os.environ.get("NOVA_HIP_ARCH", "gfx950")
 FLAGS = [

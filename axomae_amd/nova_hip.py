@@ -22,7 +22,18 @@ ERRORS = {0: "SUCCESS", 1: "INVALID_ENGINE_STATE", 2: "INVALID_BUFFER_STATE", 4:
 EXPORTS = ["nova_hip_create", "nova_hip_destroy", "nova_hip_last_error", "nova_hip_scene_upload", "nova_hip_bvh_build", "nova_hip_bvh_read",
            "nova_hip_closest_hit", "nova_hip_render_frames", "nova_hip_render_stats_get", "nova_hip_sobol_fill", "nova_hip_bake_irradiance",
            "nova_hip_bake_prefilter", "nova_hip_texture_f32_info", "nova_hip_texture_f32_update", "nova_hip_render_adjoint",
-           "nova_hip_bake_batch", "nova_hip_render_batch", "nova_hip_batch_autotune"]
+           "nova_hip_bake_batch", "nova_hip_render_batch", "nova_hip_batch_autotune", "nova_hip_renderbuffer_create",
+           "nova_hip_renderbuffer_destroy", "nova_hip_renderbuffer_preallocate", "nova_hip_renderbuffer_resize", "nova_hip_renderbuffer_reset",
+           "nova_hip_renderbuffer_clean_canvas", "nova_hip_renderbuffer_swap", "nova_hip_renderbuffer_get", "nova_hip_render_to_renderbuffer",
+           "nova_hip_renderbuffer_read"]
+
+RB_FRONT, RB_BACK, RB_PARTIAL, RB_ACCUMULATOR, RB_DEPTH, RB_NORMAL = range(6)
+RB_CHANNELS = {RB_FRONT: 4, RB_BACK: 4, RB_PARTIAL: 4, RB_ACCUMULATOR: 4, RB_DEPTH: 1, RB_NORMAL: 3}
+
+
+class RenderBufferView(C.Structure):
+    _fields_ = [("front", C.c_void_p), ("back", C.c_void_p), ("partial", C.c_void_p), ("accumulator", C.c_void_p), ("depth", C.c_void_p),
+                ("normal", C.c_void_p), ("width", C.c_uint32), ("height", C.c_uint32), ("capacity", C.c_uint64)]
 
 
 class NovaHipError(RuntimeError):
@@ -131,6 +142,16 @@ def load():
     L.nova_hip_render_adjoint.argtypes = [P, C.POINTER(RenderParams), P, C.c_float, P, P]
     L.nova_hip_bake_irradiance.argtypes = [C.POINTER(P), P, P, P, C.c_uint32, C.POINTER(P), P, P, C.c_int32, P]
     L.nova_hip_bake_prefilter.argtypes = [C.POINTER(P), P, P, P, C.c_uint32, C.POINTER(P), P, P, P, C.c_int32, P]
+    L.nova_hip_renderbuffer_create.argtypes = [P, C.POINTER(P)]
+    L.nova_hip_renderbuffer_destroy.argtypes = [P]
+    L.nova_hip_renderbuffer_preallocate.argtypes = [P, C.c_uint, C.c_uint]
+    L.nova_hip_renderbuffer_resize.argtypes = [P, C.c_uint, C.c_uint]
+    L.nova_hip_renderbuffer_reset.argtypes = [P, P]
+    L.nova_hip_renderbuffer_clean_canvas.argtypes = [P, P]
+    L.nova_hip_renderbuffer_swap.argtypes = [P]
+    L.nova_hip_renderbuffer_get.argtypes = [P, C.POINTER(RenderBufferView)]
+    L.nova_hip_render_to_renderbuffer.argtypes = [P, C.POINTER(RenderParams), P, P]
+    L.nova_hip_renderbuffer_read.argtypes = [P, C.c_int, P, P]
     _LIB = L
     return L
 
@@ -297,6 +318,13 @@ class HipEngine:
         self._check(self.L.nova_hip_render_frames(self.ctx, C.byref(p), accum.data_ptr(), final.data_ptr(), None, _stream(torch)), "render_frames")
         return final, accum
 
+    def render_to(self, rb, frames, max_depth, frame_start=0, rr_start=-1, max_samples_offset=0, vertical_invert=False, inv_P=None,
+                  inv_V=None, integrator_flags=0, index_policy=INDEX_CPU):
+        """The engine's frame loop into a :class:`RenderBuffer` (render, accumulate, swap) at its dimensions."""
+        p = render_params(self.scene, rb.width, rb.height, frames, max_depth, frame_start, rr_start, max_samples_offset, vertical_invert, inv_P,
+                          inv_V, 0, -1, index_policy, integrator_flags, 0.1, 1000.0)
+        self._check(self.L.nova_hip_render_to_renderbuffer(self.ctx, C.byref(p), rb.rb, _stream(self.torch)), "render_to_renderbuffer")
+
     # ------------------------------------------------------ differentiation
     def texture_info(self, tex):
         """(offset, pool_floats) of FP32 texture `tex` in the device texel pool."""
@@ -444,3 +472,59 @@ class BatchRenderer:
         self.exec._check(self.exec.L.nova_hip_render_batch(self.exec.ctx, arr, len(jobs), C.byref(config), C.byref(st), comp.ctypes.data,
                                                            _stream(self.exec.torch)), "render_batch")
         return [f for _, f in bufs], st, comp
+
+
+class RenderBuffer:
+    """Device render buffers of an engine's context (nova_hip_renderbuffer_*): front/back
+    display buffers, partial, accumulator, depth and normal."""
+
+    def __init__(self, engine, width, height, capacity=None):
+        self.eng = engine
+        self.rb = C.c_void_p()
+        engine._check(engine.L.nova_hip_renderbuffer_create(engine.ctx, C.byref(self.rb)), "renderbuffer_create")
+        if capacity is not None:
+            engine._check(engine.L.nova_hip_renderbuffer_preallocate(self.rb, capacity[0], capacity[1]), "renderbuffer_preallocate")
+        self.resize(width, height)
+
+    def resize(self, width, height):
+        self.eng._check(self.eng.L.nova_hip_renderbuffer_resize(self.rb, width, height), "renderbuffer_resize")
+
+    def view(self):
+        v = RenderBufferView()
+        self.eng._check(self.eng.L.nova_hip_renderbuffer_get(self.rb, C.byref(v)), "renderbuffer_get")
+        return v
+
+    @property
+    def width(self):
+        return self.view().width
+
+    @property
+    def height(self):
+        return self.view().height
+
+    def reset(self):
+        self.eng._check(self.eng.L.nova_hip_renderbuffer_reset(self.rb, _stream(self.eng.torch)), "renderbuffer_reset")
+
+    def clean_canvas(self):
+        self.eng._check(self.eng.L.nova_hip_renderbuffer_clean_canvas(self.rb, _stream(self.eng.torch)), "renderbuffer_clean_canvas")
+
+    def swap(self):
+        self.eng._check(self.eng.L.nova_hip_renderbuffer_swap(self.rb), "renderbuffer_swap")
+
+    def read(self, which=RB_FRONT):
+        """One buffer copied to the host as (H, W, channels) float32."""
+        v = self.view()
+        out = np.empty((v.height, v.width, RB_CHANNELS[which]), dtype=np.float32)
+        self.eng._check(self.eng.L.nova_hip_renderbuffer_read(self.rb, which, out.ctypes.data, _stream(self.eng.torch)), "renderbuffer_read")
+        return out
+
+    def close(self):
+        if self.rb:
+            self.eng.L.nova_hip_renderbuffer_destroy(self.rb)
+            self.rb = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

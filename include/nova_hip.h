@@ -209,6 +209,51 @@ typedef struct {
 } nova_hip_render_stats;
 int nova_hip_render_stats_get(nova_hip_ctx *ctx, nova_hip_render_stats *out);
 
+/* --------------------------------------------------------- render buffers */
+
+/* RenderBufferPack + NvRenderBuffer (corelib/nova/api/renderbuffer.cpp:8-230) in device
+ * memory: front and back display buffers, partial and accumulator (RGBA32F), depth (one
+ * float) and normal (three floats) per pixel. Allocated for a capacity, used at
+ * width x height <= capacity; all buffers start zeroed. The depth and normal buffers are
+ * allocated and cleared like the reference's, which writes neither on this path. */
+typedef struct nova_hip_renderbuffer nova_hip_renderbuffer;
+typedef struct {
+  float *front, *back, *partial, *accumulator, *depth, *normal;  /* device pointers */
+  uint32_t width, height;
+  uint64_t capacity;                                              /* pixels */
+} nova_hip_renderbuffer_view;
+enum { NOVA_HIP_RB_FRONT = 0, NOVA_HIP_RB_BACK = 1, NOVA_HIP_RB_PARTIAL = 2, NOVA_HIP_RB_ACCUMULATOR = 3, NOVA_HIP_RB_DEPTH = 4, NOVA_HIP_RB_NORMAL = 5 };
+
+/* create_renderbuffer (renderbuffer.cpp:232): an empty render buffer on ctx's device;
+ * errors of the calls below are reported on ctx (nova_hip_last_error). */
+int nova_hip_renderbuffer_create(nova_hip_ctx *ctx, nova_hip_renderbuffer **out);
+int nova_hip_renderbuffer_destroy(nova_hip_renderbuffer *rb);
+/* RenderBuffer::preallocate (renderbuffer.cpp:126-136, RenderBufferPack::allocate :32-42):
+ * capacity max_width * max_height; NOVA_HIP_OUT_OF_MEMORY when the allocation fails. */
+int nova_hip_renderbuffer_preallocate(nova_hip_renderbuffer *rb, unsigned max_width, unsigned max_height);
+/* RenderBuffer::resize (renderbuffer.cpp:138-146): reallocates (zeroed) only when the
+ * capacity is below width * height, then sets the dimensions. */
+int nova_hip_renderbuffer_resize(nova_hip_renderbuffer *rb, unsigned width, unsigned height);
+/* resetBuffers -> cleanBuffers (renderbuffer.cpp:119-124, 44-57): zeroes the whole capacity;
+ * cleanCanvas (:60-73): zeroes width * height pixels. Stream-ordered. */
+int nova_hip_renderbuffer_reset(nova_hip_renderbuffer *rb, void *stream);
+int nova_hip_renderbuffer_clean_canvas(nova_hip_renderbuffer *rb, void *stream);
+/* swapBackBuffer (renderbuffer.cpp:163-166): exchanges the front and back buffers. */
+int nova_hip_renderbuffer_swap(nova_hip_renderbuffer *rb);
+/* getRenderBuffers / getAccumulator / frontBuffer / backBuffer (renderbuffer.cpp:148-173). */
+int nova_hip_renderbuffer_get(const nova_hip_renderbuffer *rb, nova_hip_renderbuffer_view *out);
+/* The engine's frame (engine.cpp:498-525): nova_hip_render_frames into the partial buffer,
+ * writeFramebuffer (accumulator += partial, back = accumulator / (frame + 1), engine.cpp:351-363)
+ * and swapBackBuffer, all on the device. params->width/height must equal the render buffer's
+ * dimensions (NOVA_HIP_INVALID_RENDERBUFFER_DIM otherwise). With frame_count 1 a call is one
+ * iteration of the engine's loop; with more, the frames are rendered in one batch and the
+ * buffers are swapped once, so the front buffer holds the last frame's image and the
+ * accumulator all frames' sum. */
+int nova_hip_render_to_renderbuffer(nova_hip_ctx *ctx, const nova_hip_render_params *params, nova_hip_renderbuffer *rb, void *stream);
+/* Copies one buffer (NOVA_HIP_RB_*; width * height pixels) to host memory, for display:
+ * getFramebuffer / sampleColor (renderbuffer.cpp:175-220) without the per-frame copy. */
+int nova_hip_renderbuffer_read(const nova_hip_renderbuffer *rb, int which, float *host, void *stream);
+
 /* ---------------------------------------------------------------- batch */
 
 /* One scene of a batched render: a context holding an uploaded scene with its BVH built,
