@@ -384,6 +384,181 @@ NOVA_HD int trace_closest(const TraceCtx &tc, Stack &stack, f3 o, f3 d, float tn
   return st.best;
 }
 
+// ----------------------------------------------------------- wave-coherent walk
+//
+// For waves whose 64 rays nearly coincide (one pixel's camera rays of 64 frames: the jitter
+// is under half a pixel), the wave walks the tree once for all of them. The node to visit is
+// wave-uniform and read through the scalar unit (a 64-byte node is two scalar loads, no
+// vector gathers); each lane runs the slab test of its own ray, and the wave visits a child
+// when any lane enters it, carrying the mask of those lanes. A lane thus tests exactly the
+// boxes and triangles its own walk would reach -- only the order is the wave's (near to far
+// for the first lane of the mask) -- and the closest hit with the reference's tie rule does
+// not depend on the order. The stack is the wave's: (node, lane mask) entries in LDS.
+#ifdef __HIP_DEVICE_COMPILE__
+typedef const __attribute__((address_space(4))) Node4 ConstNode4;  // scalar-loadable: uniform address, never written by the walk
+typedef const __attribute__((address_space(4))) TriMT ConstTriMT;
+#else
+typedef const Node4 ConstNode4;
+typedef const TriMT ConstTriMT;
+#endif
+
+// a[k] for a wave-uniform k, as selects (an indexed private array would live in scratch)
+template <class T>
+__device__ __forceinline__ T pick4(const T (&a)[4], int k) {
+  return k == 0 ? a[0] : k == 1 ? a[1] : k == 2 ? a[2] : a[3];
+}
+
+constexpr int kWaveStack = 64;  // (node, mask) entries per wave; usable when 3 * wide depth + 4 <= kWaveStack
+
+template <bool kCount>
+__device__ void coherent_walk(const TraceCtx &tc, TraceState &st, uint64_t lanes, uint32_t *stk_node, uint64_t *stk_mask, float tnear,
+                              uint32_t &nodes_visited, uint32_t &tri_tests) {
+  const float kUp = 1.000001f;
+  const float kInf = __builtin_inff();
+  ConstNode4 *nodes = (ConstNode4 *)tc.nodes;
+  ConstTriMT *tris = (ConstTriMT *)tc.tris;
+  const uint32_t lane = __lane_id();
+  int32_t node = 0;
+  uint64_t mask = lanes;  // lanes that entered this node's box
+  int sp = 0;
+  for (uint32_t guard = 0; guard < (1u << 24) && mask; guard++) {
+    node = __builtin_amdgcn_readfirstlane(node);
+    const Node4 nd = nodes[node];
+    const bool in = (mask >> lane) & 1u;
+    float key[4] = {kInf, kInf, kInf, kInf};
+    bool hit[4] = {false, false, false, false};
+    if (in) {
+      if (kCount) nodes_visited++;
+      if (st.parallel) {
+#pragma unroll
+        for (int k = 0; k < 4; k++) {
+          const float bmnx = grid_plane((nd.qminx >> (8 * k)) & 255u, nd.stepx, nd.orgx), bmxx = grid_plane((nd.qmaxx >> (8 * k)) & 255u, nd.stepx, nd.orgx);
+          const float bmny = grid_plane((nd.qminy >> (8 * k)) & 255u, nd.stepy, nd.orgy), bmxy = grid_plane((nd.qmaxy >> (8 * k)) & 255u, nd.stepy, nd.orgy);
+          const float bmnz = grid_plane((nd.qminz >> (8 * k)) & 255u, nd.stepz, nd.orgz), bmxz = grid_plane((nd.qmaxz >> (8 * k)) & 255u, nd.stepz, nd.orgz);
+          float tx1 = (bmnx - st.o.x) * st.inv.x, tx2 = (bmxx - st.o.x) * st.inv.x;
+          float ty1 = (bmny - st.o.y) * st.inv.y, ty2 = (bmxy - st.o.y) * st.inv.y;
+          float tz1 = (bmnz - st.o.z) * st.inv.z, tz2 = (bmxz - st.o.z) * st.inv.z;
+          if (st.px) {
+            const bool inside = bmnx <= st.o.x && st.o.x <= bmxx;
+            tx1 = -kInf;
+            tx2 = inside ? kInf : -kInf;
+          }
+          if (st.py) {
+            const bool inside = bmny <= st.o.y && st.o.y <= bmxy;
+            ty1 = -kInf;
+            ty2 = inside ? kInf : -kInf;
+          }
+          if (st.pz) {
+            const bool inside = bmnz <= st.o.z && st.o.z <= bmxz;
+            tz1 = -kInf;
+            tz2 = inside ? kInf : -kInf;
+          }
+          const float t0 = fmaxf(fmaxf(fminf(tx1, tx2), fminf(ty1, ty2)), fminf(tz1, tz2));
+          const float t1 = fminf(fminf(fmaxf(tx1, tx2), fmaxf(ty1, ty2)), fmaxf(tz1, tz2)) * kUp;
+          hit[k] = nd.child[k] != -1 && t0 <= t1 && t1 >= 0.f && t0 <= st.tbest * kUp;
+          key[k] = hit[k] ? t0 : kInf;
+        }
+      } else {
+        const uint32_t nqx = st.inv.x < 0.f ? nd.qmaxx : nd.qminx, fqx = st.inv.x < 0.f ? nd.qminx : nd.qmaxx;
+        const uint32_t nqy = st.inv.y < 0.f ? nd.qmaxy : nd.qminy, fqy = st.inv.y < 0.f ? nd.qminy : nd.qmaxy;
+        const uint32_t nqz = st.inv.z < 0.f ? nd.qmaxz : nd.qminz, fqz = st.inv.z < 0.f ? nd.qminz : nd.qmaxz;
+#pragma unroll
+        for (int k = 0; k < 4; k++) {
+          const float tnx = (grid_plane((nqx >> (8 * k)) & 255u, nd.stepx, nd.orgx) - st.o.x) * st.inv.x;
+          const float tfx = (grid_plane((fqx >> (8 * k)) & 255u, nd.stepx, nd.orgx) - st.o.x) * st.inv.x;
+          const float tny = (grid_plane((nqy >> (8 * k)) & 255u, nd.stepy, nd.orgy) - st.o.y) * st.inv.y;
+          const float tfy = (grid_plane((fqy >> (8 * k)) & 255u, nd.stepy, nd.orgy) - st.o.y) * st.inv.y;
+          const float tnz = (grid_plane((nqz >> (8 * k)) & 255u, nd.stepz, nd.orgz) - st.o.z) * st.inv.z;
+          const float tfz = (grid_plane((fqz >> (8 * k)) & 255u, nd.stepz, nd.orgz) - st.o.z) * st.inv.z;
+          const float t0 = fmaxf(fmaxf(tnx, tny), tnz);
+          const float t1 = fminf(fminf(tfx, tfy), tfz) * kUp;
+          hit[k] = nd.child[k] != -1 && t0 <= t1 && t1 >= 0.f && t0 <= st.tbest * kUp;
+          key[k] = hit[k] ? t0 : kInf;
+        }
+      }
+    }
+    // the wave's view of each child: the lanes that enter it, and the first lane's entry
+    // distance as the ordering key (an orderable integer, so the sort runs on scalars)
+    const int leader = __builtin_ffsll((long long)mask) - 1;
+    uint64_t m[4];
+    int32_t ok[4];
+#pragma unroll
+    for (int k = 0; k < 4; k++) {
+      m[k] = __ballot(hit[k]);
+      const int32_t b = __builtin_amdgcn_readlane(__builtin_bit_cast(int32_t, key[k]), leader);
+      ok[k] = m[k] ? (b ^ ((b >> 31) & 0x7fffffff)) : 0x7fffffff;
+    }
+    int ord[4] = {0, 1, 2, 3};
+    auto exch = [&](int i, int j) {
+      const bool sw = pick4(ok, ord[j]) < pick4(ok, ord[i]);
+      const int a = sw ? ord[j] : ord[i], b = sw ? ord[i] : ord[j];
+      ord[i] = a;
+      ord[j] = b;
+    };
+    exch(0, 1);
+    exch(2, 3);
+    exch(0, 2);
+    exch(1, 3);
+    exch(1, 2);
+    // leaves near to far, each by the lanes that enter its box
+#pragma unroll
+    for (int a = 0; a < 4; a++) {
+      const int k = ord[a];
+      const int32_t c = pick4(nd.child, k);
+      const uint64_t mk = pick4(m, k);
+      if (c > -2 || !mk) continue;
+      const bool mine = ((mk >> lane) & 1u) && pick4(key, k) <= st.tbest * kUp;
+      const int32_t start = leaf_first(c);
+      const int32_t cnt = __builtin_bit_cast(int32_t, tris[start].b.w);
+      for (int32_t i = start; i < start + cnt; i++) {
+        const TriMT tr = tris[i];
+        if (!mine) continue;
+        if (kCount) tri_tests++;
+        float tt, uu, vv;
+        if (mt_test(tr, st.o, st.d, tnear, st.tbest, tt, uu, vv) &&
+            (tt < st.tbest || st.best < 0 || reference_tie_winner_is_new(tc.tie, i, st.bpos, st.o, st.d, tt))) {
+          st.tbest = tt;
+          st.bpos = i;
+          st.best = __builtin_bit_cast(int32_t, tr.a.w);
+          st.bu = uu;
+          st.bv = vv;
+        }
+      }
+    }
+    // inner children far to near onto the wave's stack, the nearest visited next
+    int32_t next = -1;
+    uint64_t next_mask = 0;
+#pragma unroll
+    for (int a = 3; a >= 0; a--) {
+      const int k = ord[a];
+      const int32_t c = pick4(nd.child, k);
+      const uint64_t mk = pick4(m, k);
+      if (c < 0 || !mk) continue;
+      if (next >= 0) {
+        if (lane == 0) {
+          stk_node[sp] = (uint32_t)next;
+          stk_mask[sp] = next_mask;
+        }
+        sp++;
+      }
+      next = c;
+      next_mask = mk;
+    }
+    if (next >= 0) {
+      node = next;
+      mask = next_mask;
+      continue;
+    }
+    if (sp == 0) break;
+    sp--;
+    __builtin_amdgcn_wave_barrier();
+    node = (int32_t)stk_node[sp];
+    mask = stk_mask[sp];
+    mask = ((uint64_t)(uint32_t)__builtin_amdgcn_readfirstlane((int32_t)(uint32_t)(mask >> 32)) << 32) |
+           (uint32_t)__builtin_amdgcn_readfirstlane((int32_t)(uint32_t)mask);
+  }
+}
+
 #ifndef NOVA_LDS_STACK
 #define NOVA_LDS_STACK 16
 #endif

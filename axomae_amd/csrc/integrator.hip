@@ -163,6 +163,39 @@ __global__ __launch_bounds__(kTraceBlock) NOVA_TRACE_OCCUPANCY void extend_kerne
   }
 }
 
+// Camera rays of pixel-major batches: each wave takes 64 consecutive rays (one pixel's frames
+// when the batch has 64 frames) and walks the tree once for all of them (coherent_walk).
+constexpr int kCoherentBlock = 256;
+template <bool kCount>
+__global__ __launch_bounds__(kCoherentBlock) void extend_coherent_kernel(TraceCtx tc, Queues q, int which) {
+  __shared__ uint32_t stk_node[kCoherentBlock / 64][kWaveStack];
+  __shared__ uint64_t stk_mask[kCoherentBlock / 64][kWaveStack];
+  uint32_t nv = 0, nt = 0;
+  const uint32_t n = q.count[which];
+  const float *r = q.ray[which];
+  const size_t cap = q.cap;
+  const uint32_t lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  if (blockIdx.x == 0 && threadIdx.x == 0) q.count[2] += n;
+  for (uint32_t base = (blockIdx.x * (kCoherentBlock / 64) + wv) * 64u; base < n; base += gridDim.x * kCoherentBlock) {
+    const uint32_t i = base + lane;
+    const bool valid = i < n;
+    TraceState ts;
+    trace_init(ts, valid ? mk3(r[i], r[cap + i], r[2 * cap + i]) : mk3(0.f, 0.f, 0.f),
+               valid ? mk3(r[3 * cap + i], r[4 * cap + i], r[5 * cap + i]) : mk3(0.f, 0.f, 1.f));
+    coherent_walk<kCount>(tc, ts, __ballot(valid), stk_node[wv], stk_mask[wv], 0.000001f, nv, nt);
+    if (valid) {
+      q.hit_prim[i] = ts.best;
+      q.hit_t[i] = ts.tbest;
+      q.hit_u[i] = ts.bu;
+      q.hit_v[i] = ts.bv;
+    }
+  }
+  if (kCount) {
+    atomicAdd(&q.stats64[0], (unsigned long long)nv);
+    atomicAdd(&q.stats64[1], (unsigned long long)nt);
+  }
+}
+
 // A finished path's radiance and alpha, by path id.
 __device__ __forceinline__ void write_partial(const Queues &q, uint32_t pid, float lx, float ly, float lz, float a) {
   reinterpret_cast<float4 *>(q.partial)[pid] = make_float4(lx, ly, lz, a);
@@ -560,12 +593,23 @@ static int render_impl(nova_hip_ctx *ctx, const nova_hip_render_params *p, float
     NOVA_HIP_CHECK(ctx, hipGetLastError());
     uint32_t dim = 2;
     int which = 0;
+    // camera rays of a pixel-major batch with at least 16 frames: a wave holds at most 4
+    // pixels' rays, close enough to walk the tree together
+    const bool coherent0 = fp.frame_minor && frames >= 16 && !(p->flags & NOVA_HIP_RENDER_PER_RAY_TRAVERSAL) &&
+                           traversal_stack_bound(ctx) <= (uint32_t)kWaveStack;
     for (int k = 0; k <= p->max_depth; k++) {
       if (timing && (st = stamp())) return st;
-      if (counting)
+      if (k == 0 && coherent0) {
+        const uint32_t cblocks = std::min<uint32_t>((active + kCoherentBlock - 1) / kCoherentBlock, (uint32_t)cus * 8);
+        if (counting)
+          hipLaunchKernelGGL(extend_coherent_kernel<true>, dim3(cblocks), dim3(kCoherentBlock), 0, s, tc, q, which);
+        else
+          hipLaunchKernelGGL(extend_coherent_kernel<false>, dim3(cblocks), dim3(kCoherentBlock), 0, s, tc, q, which);
+      } else if (counting) {
         hipLaunchKernelGGL(extend_kernel<true>, dim3(trace_blocks), dim3(kTraceBlock), 0, s, tc, q, which);
-      else
+      } else {
         hipLaunchKernelGGL(extend_kernel<false>, dim3(trace_blocks), dim3(kTraceBlock), 0, s, tc, q, which);
+      }
       if (timing && (st = stamp())) return st;
       hipLaunchKernelGGL(reset_count_kernel, dim3(1), dim3(1), 0, s, counts, which ^ 1);
       ShadeParams spp;

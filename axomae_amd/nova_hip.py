@@ -102,6 +102,7 @@ class BatchStats(C.Structure):
 
 RENDER_TIMING = 1
 RENDER_COUNT_TRAVERSAL = 2
+RENDER_PER_RAY_TRAVERSAL = 4  # camera rays one walk per ray (testing and tuning)
 
 
 class RenderStats(C.Structure):

@@ -175,7 +175,9 @@ enum {
 
 /* flags: time every extend/shade launch with HIP events on the render stream; count BVH node
  * visits and triangle tests per ray (an instrumented traversal, slower). */
-enum { NOVA_HIP_RENDER_TIMING = 1, NOVA_HIP_RENDER_COUNT_TRAVERSAL = 2 };
+/* NOVA_HIP_RENDER_PER_RAY_TRAVERSAL: trace camera rays one walk per ray even where a wave
+ * would walk the tree once for its rays (the result is the same; for tests and tuning). */
+enum { NOVA_HIP_RENDER_TIMING = 1, NOVA_HIP_RENDER_COUNT_TRAVERSAL = 2, NOVA_HIP_RENDER_PER_RAY_TRAVERSAL = 4 };
 
 /* Renders frames [frame_start, frame_start+frame_count) of the path integrator
  * (PathIntegrator::Li, corelib/nova/integrator/Integrator.cpp:59-98, sample indices by
