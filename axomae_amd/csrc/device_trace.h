@@ -393,7 +393,8 @@ NOVA_HD int trace_closest(const TraceCtx &tc, Stack &stack, f3 o, f3 d, float tn
 // when any lane enters it, carrying the mask of those lanes. A lane thus tests exactly the
 // boxes and triangles its own walk would reach -- only the order is the wave's (near to far
 // for the first lane of the mask) -- and the closest hit with the reference's tie rule does
-// not depend on the order. The stack is the wave's: (node, lane mask) entries in LDS.
+// not depend on the order. The stack is the wave's: (node, lane mask) entries, entry e held
+// in lane e of three registers, so a push is a compare-and-select and a pop a lane read.
 #ifdef __HIP_DEVICE_COMPILE__
 typedef const __attribute__((address_space(4))) Node4 ConstNode4;  // scalar-loadable: uniform address, never written by the walk
 typedef const __attribute__((address_space(4))) TriMT ConstTriMT;
@@ -402,17 +403,10 @@ typedef const Node4 ConstNode4;
 typedef const TriMT ConstTriMT;
 #endif
 
-// a[k] for a wave-uniform k, as selects (an indexed private array would live in scratch)
-template <class T>
-__device__ __forceinline__ T pick4(const T (&a)[4], int k) {
-  return k == 0 ? a[0] : k == 1 ? a[1] : k == 2 ? a[2] : a[3];
-}
-
-constexpr int kWaveStack = 64;  // (node, mask) entries per wave; usable when 3 * wide depth + 4 <= kWaveStack
+constexpr int kWaveStack = 64;  // (node, mask) entries per wave, one per lane; usable when 3 * wide depth + 4 < kWaveStack
 
 template <bool kCount>
-__device__ void coherent_walk(const TraceCtx &tc, TraceState &st, uint64_t lanes, uint32_t *stk_node, uint64_t *stk_mask, float tnear,
-                              uint32_t &nodes_visited, uint32_t &tri_tests) {
+__device__ void coherent_walk(const TraceCtx &tc, TraceState &st, uint64_t lanes, float tnear, uint32_t &nodes_visited, uint32_t &tri_tests) {
   const float kUp = 1.000001f;
   const float kInf = __builtin_inff();
   ConstNode4 *nodes = (ConstNode4 *)tc.nodes;
@@ -421,6 +415,8 @@ __device__ void coherent_walk(const TraceCtx &tc, TraceState &st, uint64_t lanes
   int32_t node = 0;
   uint64_t mask = lanes;  // lanes that entered this node's box
   int sp = 0;
+  // the wave's stack: entry e is lane e of three registers (node, mask low, mask high)
+  int32_t stk_node = 0, stk_lo = 0, stk_hi = 0;
   for (uint32_t guard = 0; guard < (1u << 24) && mask; guard++) {
     node = __builtin_amdgcn_readfirstlane(node);
     const Node4 nd = nodes[node];
@@ -477,37 +473,24 @@ __device__ void coherent_walk(const TraceCtx &tc, TraceState &st, uint64_t lanes
         }
       }
     }
-    // the wave's view of each child: the lanes that enter it, and the first lane's entry
-    // distance as the ordering key (an orderable integer, so the sort runs on scalars)
+    // The wave's view of each child: the lanes that enter it, and an ordering key -- the
+    // first lane's entry distance as an orderable integer with the child's index in its low
+    // two bits (so the four keys differ; order among near-equal keys does not matter).
     const int leader = __builtin_ffsll((long long)mask) - 1;
     uint64_t m[4];
-    int32_t ok[4];
+    int32_t pk[4];
 #pragma unroll
     for (int k = 0; k < 4; k++) {
       m[k] = __ballot(hit[k]);
       const int32_t b = __builtin_amdgcn_readlane(__builtin_bit_cast(int32_t, key[k]), leader);
-      ok[k] = m[k] ? (b ^ ((b >> 31) & 0x7fffffff)) : 0x7fffffff;
+      pk[k] = ((m[k] ? (b ^ ((b >> 31) & 0x7fffffff)) : 0x7fffffff) & ~3) | k;
     }
-    int ord[4] = {0, 1, 2, 3};
-    auto exch = [&](int i, int j) {
-      const bool sw = pick4(ok, ord[j]) < pick4(ok, ord[i]);
-      const int a = sw ? ord[j] : ord[i], b = sw ? ord[i] : ord[j];
-      ord[i] = a;
-      ord[j] = b;
-    };
-    exch(0, 1);
-    exch(2, 3);
-    exch(0, 2);
-    exch(1, 3);
-    exch(1, 2);
-    // leaves near to far, each by the lanes that enter its box
+    // leaves, in child order, by the lanes that enter each leaf's box
 #pragma unroll
-    for (int a = 0; a < 4; a++) {
-      const int k = ord[a];
-      const int32_t c = pick4(nd.child, k);
-      const uint64_t mk = pick4(m, k);
-      if (c > -2 || !mk) continue;
-      const bool mine = ((mk >> lane) & 1u) && pick4(key, k) <= st.tbest * kUp;
+    for (int k = 0; k < 4; k++) {
+      const int32_t c = nd.child[k];
+      if (c > -2 || !m[k]) continue;
+      const bool mine = ((m[k] >> lane) & 1u) && key[k] <= st.tbest * kUp;
       const int32_t start = leaf_first(c);
       const int32_t cnt = __builtin_bit_cast(int32_t, tris[start].b.w);
       for (int32_t i = start; i < start + cnt; i++) {
@@ -525,37 +508,42 @@ __device__ void coherent_walk(const TraceCtx &tc, TraceState &st, uint64_t lanes
         }
       }
     }
-    // inner children far to near onto the wave's stack, the nearest visited next
+    // Inner children entered by some lane: the nearest is visited next, the others go onto
+    // the stack far to near -- child k to slot sp + (entered inner children farther than k).
+    bool inner[4];
+    int ninner = 0;
+#pragma unroll
+    for (int k = 0; k < 4; k++) {
+      inner[k] = nd.child[k] >= 0 && m[k] != 0;
+      ninner += inner[k] ? 1 : 0;
+    }
     int32_t next = -1;
     uint64_t next_mask = 0;
 #pragma unroll
-    for (int a = 3; a >= 0; a--) {
-      const int k = ord[a];
-      const int32_t c = pick4(nd.child, k);
-      const uint64_t mk = pick4(m, k);
-      if (c < 0 || !mk) continue;
-      if (next >= 0) {
-        if (lane == 0) {
-          stk_node[sp] = (uint32_t)next;
-          stk_mask[sp] = next_mask;
-        }
-        sp++;
-      }
-      next = c;
-      next_mask = mk;
+    for (int k = 0; k < 4; k++) {
+      int farther = 0;
+#pragma unroll
+      for (int j = 0; j < 4; j++) farther += (j != k && inner[j] && pk[j] > pk[k]) ? 1 : 0;
+      const bool nearest = inner[k] && farther == ninner - 1;
+      next = nearest ? nd.child[k] : next;
+      next_mask = nearest ? m[k] : next_mask;
+      // pushed entries land at sp .. sp + ninner - 2; the others write the spare last slot
+      const int slot = (inner[k] && !nearest) ? sp + farther : kWaveStack - 1;
+      const bool here = (int)lane == slot;  // a lane write: compare and select
+      stk_node = here ? nd.child[k] : stk_node;
+      stk_lo = here ? (int32_t)(uint32_t)m[k] : stk_lo;
+      stk_hi = here ? (int32_t)(uint32_t)(m[k] >> 32) : stk_hi;
     }
     if (next >= 0) {
+      sp += ninner - 1;
       node = next;
       mask = next_mask;
       continue;
     }
     if (sp == 0) break;
     sp--;
-    __builtin_amdgcn_wave_barrier();
-    node = (int32_t)stk_node[sp];
-    mask = stk_mask[sp];
-    mask = ((uint64_t)(uint32_t)__builtin_amdgcn_readfirstlane((int32_t)(uint32_t)(mask >> 32)) << 32) |
-           (uint32_t)__builtin_amdgcn_readfirstlane((int32_t)(uint32_t)mask);
+    node = __builtin_amdgcn_readlane(stk_node, sp);
+    mask = ((uint64_t)(uint32_t)__builtin_amdgcn_readlane(stk_hi, sp) << 32) | (uint32_t)__builtin_amdgcn_readlane(stk_lo, sp);
   }
 }
 

@@ -168,8 +168,6 @@ __global__ __launch_bounds__(kTraceBlock) NOVA_TRACE_OCCUPANCY void extend_kerne
 constexpr int kCoherentBlock = 256;
 template <bool kCount>
 __global__ __launch_bounds__(kCoherentBlock) void extend_coherent_kernel(TraceCtx tc, Queues q, int which) {
-  __shared__ uint32_t stk_node[kCoherentBlock / 64][kWaveStack];
-  __shared__ uint64_t stk_mask[kCoherentBlock / 64][kWaveStack];
   uint32_t nv = 0, nt = 0;
   const uint32_t n = q.count[which];
   const float *r = q.ray[which];
@@ -182,7 +180,7 @@ __global__ __launch_bounds__(kCoherentBlock) void extend_coherent_kernel(TraceCt
     TraceState ts;
     trace_init(ts, valid ? mk3(r[i], r[cap + i], r[2 * cap + i]) : mk3(0.f, 0.f, 0.f),
                valid ? mk3(r[3 * cap + i], r[4 * cap + i], r[5 * cap + i]) : mk3(0.f, 0.f, 1.f));
-    coherent_walk<kCount>(tc, ts, __ballot(valid), stk_node[wv], stk_mask[wv], 0.000001f, nv, nt);
+    coherent_walk<kCount>(tc, ts, __ballot(valid), 0.000001f, nv, nt);
     if (valid) {
       q.hit_prim[i] = ts.best;
       q.hit_t[i] = ts.tbest;
@@ -596,7 +594,7 @@ static int render_impl(nova_hip_ctx *ctx, const nova_hip_render_params *p, float
     // camera rays of a pixel-major batch with at least 16 frames: a wave holds at most 4
     // pixels' rays, close enough to walk the tree together
     const bool coherent0 = fp.frame_minor && frames >= 16 && !(p->flags & NOVA_HIP_RENDER_PER_RAY_TRAVERSAL) &&
-                           traversal_stack_bound(ctx) <= (uint32_t)kWaveStack;
+                           traversal_stack_bound(ctx) < (uint32_t)kWaveStack;
     for (int k = 0; k <= p->max_depth; k++) {
       if (timing && (st = stamp())) return st;
       if (k == 0 && coherent0) {
