@@ -7,7 +7,8 @@ accumulates). The timed region renders K frames (frames 0..K-1) after W warm-up
 frames; value = width*height*K / t / 1e6 Msamples/s. Scene upload and BVH build are
 outside the timed region and reported separately.
 
-roofline: the dominant kernel is the extend (closest-hit traversal) kernel. Its
+roofline: the dominant kernel is the extend (closest-hit traversal) step: one walk per wave
+for the camera rays, one walk per ray for the later bounces. Its
 algorithmic bytes per ray are 48 (ray in, hit out) + 64 per 4-wide node visited (the
 whole 64-byte node: grid origin and steps, 8-bit child planes, child references) + 48 per
 triangle tested, with
@@ -297,7 +298,7 @@ def main():
         "config": {"workload": "million_triangles_1080p", "triangles": sc.triangle_count, "width": W, "height": H, "spp": K,
                    "max_depth": args.depth, "russian_roulette_from_scatter": args.rr_start, "sky": "2048x1024 RGBA32F"},
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic, "kernel": "extend_kernel",
+                     "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic, "kernel": "extend (extend_coherent_kernel for camera rays, extend_kernel for bounces 1-8)",
                      "algorithmic_bytes_per_launch": round(per_launch), "avg_launch_ms": round(st.extend_ms / max(st.extend_launches, 1), 4)},
         "path_algorithmic_GBs": round(path_bytes / t / 1e9, 1),
         "bvh": {"build_ms": round(info.build_ms, 2), "binary_nodes": info.binary_nodes, "binary_depth": info.binary_depth,
