@@ -166,8 +166,19 @@ __global__ __launch_bounds__(kTraceBlock) NOVA_TRACE_OCCUPANCY void extend_kerne
 // Camera rays of pixel-major batches: each wave takes 64 consecutive rays (one pixel's frames
 // when the batch has 64 frames) and walks the tree once for all of them (coherent_walk).
 constexpr int kCoherentBlock = 256;
+#ifndef NOVA_COHERENT_BLOCKS_PER_CU
+#define NOVA_COHERENT_BLOCKS_PER_CU 64  // many more workgroups than fit at once: a static split of the packets balances better
+#endif
+#ifndef NOVA_COHERENT_WAVES
+#define NOVA_COHERENT_WAVES 0  // waves per SIMD the coherent walk is compiled for (0: compiler's choice)
+#endif
+#if NOVA_COHERENT_WAVES > 0
+#define NOVA_COHERENT_OCCUPANCY __attribute__((amdgpu_waves_per_eu(NOVA_COHERENT_WAVES, NOVA_COHERENT_WAVES)))
+#else
+#define NOVA_COHERENT_OCCUPANCY
+#endif
 template <bool kCount>
-__global__ __launch_bounds__(kCoherentBlock) void extend_coherent_kernel(TraceCtx tc, Queues q, int which) {
+__global__ __launch_bounds__(kCoherentBlock) NOVA_COHERENT_OCCUPANCY void extend_coherent_kernel(TraceCtx tc, Queues q, int which) {
   uint32_t nv = 0, nt = 0;
   const uint32_t n = q.count[which];
   const float *r = q.ray[which];
@@ -598,7 +609,7 @@ static int render_impl(nova_hip_ctx *ctx, const nova_hip_render_params *p, float
     for (int k = 0; k <= p->max_depth; k++) {
       if (timing && (st = stamp())) return st;
       if (k == 0 && coherent0) {
-        const uint32_t cblocks = std::min<uint32_t>((active + kCoherentBlock - 1) / kCoherentBlock, (uint32_t)cus * 8);
+        const uint32_t cblocks = std::min<uint32_t>((active + kCoherentBlock - 1) / kCoherentBlock, (uint32_t)cus * NOVA_COHERENT_BLOCKS_PER_CU);
         if (counting)
           hipLaunchKernelGGL(extend_coherent_kernel<true>, dim3(cblocks), dim3(kCoherentBlock), 0, s, tc, q, which);
         else
