@@ -144,6 +144,11 @@ struct AdjointCtx {
   const float *dimg;
   float scale;
   float *grad;
+  // Hits of the primal pass by (bounce, pixel-major path id): (prim bits, t, u, v). The
+  // primal pass writes them, the replay reads them instead of tracing its rays again (they
+  // are the same rays, so the hits are the same). Null: the replay traces.
+  float4 *hit_cache;
+  uint32_t cache_paths;  // paths per bounce in the cache
 };
 
 NOVA_HD void add_grad(float *p, float v) {

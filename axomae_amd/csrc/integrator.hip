@@ -205,6 +205,8 @@ __global__ __launch_bounds__(kCoherentBlock) NOVA_COHERENT_OCCUPANCY void extend
   }
 }
 
+constexpr size_t kHitCacheBytes = (size_t)32 << 30;  // adjoint replays reuse the primal's hits up to this size
+
 // A finished path's radiance and alpha, by path id.
 __device__ __forceinline__ void write_partial(const Queues &q, uint32_t pid, float lx, float ly, float lz, float a) {
   reinterpret_cast<float4 *>(q.partial)[pid] = make_float4(lx, ly, lz, a);
@@ -255,14 +257,33 @@ __global__ __launch_bounds__(kShadeBlock) NOVA_SHADE_OCCUPANCY void shade_kernel
       f3 T = first ? mk3(1.f, 1.f, 1.f) : mk3(q.T[pid], q.T[cap + pid], q.T[2 * cap + pid]);
       f3 L = first ? mk3(0.f, 0.f, 0.f) : mk3(q.L[pid], q.L[cap + pid], q.L[2 * cap + pid]);
       float A = first ? 0.f : q.A[pid];
-      const int prim = q.hit_prim[i];
+      int prim;
+      float ht, hu, hv;
       AdjointCtx padj = adj;
-      if (kAdj) {  // this path's primal radiance, left by pass 0 under its frame-minor path id
+      if (kAdj) {  // this path's primal radiance (and hits), left by pass 0 under its frame-minor path id
         const uint32_t slot = path_slot(sp.fp, (uint32_t)pid), npix = (uint32_t)sp.fp.width * (uint32_t)sp.fp.height;
-        padj.ltot = adj.ltot + ((size_t)(slot % npix) * (uint32_t)sp.fp.frames + slot / npix) * 4;
+        const size_t pid0 = (size_t)(slot % npix) * (uint32_t)sp.fp.frames + slot / npix;
+        padj.ltot = adj.ltot + pid0 * 4;
+        if (adj.hit_cache) {
+          const float4 h = adj.hit_cache[(size_t)sp.bounce * adj.cache_paths + pid0];
+          prim = __builtin_bit_cast(int32_t, h.x);
+          ht = h.y;
+          hu = h.z;
+          hv = h.w;
+        } else {
+          prim = q.hit_prim[i];
+          ht = q.hit_t[i];
+          hu = q.hit_u[i];
+          hv = q.hit_v[i];
+        }
+      } else {
+        prim = q.hit_prim[i];
+        ht = q.hit_t[i];
+        hu = q.hit_u[i];
+        hv = q.hit_v[i];
+        if (adj.hit_cache) adj.hit_cache[(size_t)sp.bounce * adj.cache_paths + pid] = make_float4(__builtin_bit_cast(float, prim), ht, hu, hv);
       }
-      cont = shade_path<kAdj>(sp, sc, sobn, path_slot(sp.fp, (uint32_t)pid), o, d, prim, q.hit_t[i], q.hit_u[i], q.hit_v[i], T, L, A, no, nd,
-                              escaped, &padj);
+      cont = shade_path<kAdj>(sp, sc, sobn, path_slot(sp.fp, (uint32_t)pid), o, d, prim, ht, hu, hv, T, L, A, no, nd, escaped, &padj);
       const bool done = !cont;
       q.T[pid] = T.x;
       q.T[cap + pid] = T.y;
@@ -552,7 +573,17 @@ static int render_impl(nova_hip_ctx *ctx, const nova_hip_render_params *p, float
   TraceCtx tc{ctx->nodes4, ctx->tri_leaf, ctx->gstack, trace_threads, ctx->tie};
   ShadeCtx sc{ctx->textures, ctx->tex_u32, ctx->tex_f32, ctx->materials, ctx->mesh_xf, ctx->shade, ctx->acos_lut, ctx->envmap,
               p->sampling_roughness_texture};
-  const AdjointCtx replay = adj ? AdjointCtx{partial, adj->dimg, adj->scale, adj->grad} : AdjointCtx{nullptr, nullptr, 0.f, nullptr};
+  AdjointCtx replay = adj ? AdjointCtx{partial, adj->dimg, adj->scale, adj->grad, nullptr, 0u} : AdjointCtx{nullptr, nullptr, 0.f, nullptr, nullptr, 0u};
+  // The replay reuses the primal pass's hits when they fit in kHitCacheBytes (16 B per
+  // segment slot: (max_depth + 1) x paths per batch).
+  if (adj && !(p->flags & NOVA_HIP_ADJOINT_RETRACE)) {
+    const size_t bytes = (size_t)(p->max_depth + 1) * cap * sizeof(float4);
+    if (bytes <= kHitCacheBytes) {
+      replay.hit_cache = (float4 *)ensure_scratch(ctx, 6, bytes, &e);
+      if (!replay.hit_cache) return check_hip(ctx, e, "hit cache alloc");
+      replay.cache_paths = (uint32_t)cap;
+    }
+  }
   // The shade loop splits the queue evenly over its grid, so the grid is what fits on the
   // chip at once: a second round of workgroups would run after the first at lower occupancy.
   int shade_per_cu[2] = {NOVA_SHADE_BLOCKS_PER_CU, NOVA_SHADE_BLOCKS_PER_CU};
@@ -602,13 +633,19 @@ static int render_impl(nova_hip_ctx *ctx, const nova_hip_render_params *p, float
     NOVA_HIP_CHECK(ctx, hipGetLastError());
     uint32_t dim = 2;
     int which = 0;
-    // camera rays of a pixel-major batch with at least 16 frames: a wave holds at most 4
-    // pixels' rays, close enough to walk the tree together
-    const bool coherent0 = fp.frame_minor && frames >= 16 && !(p->flags & NOVA_HIP_RENDER_PER_RAY_TRAVERSAL) &&
+    // camera rays of a pixel-major batch with at least NOVA_COHERENT_MIN_FRAMES frames: a
+    // wave holds 64 / frames pixels' rays, close enough to walk the tree together (a wave of
+    // 64 pixels of one row is not: the frame-major adjoint replay keeps the per-ray walk)
+#ifndef NOVA_COHERENT_MIN_FRAMES
+#define NOVA_COHERENT_MIN_FRAMES 8
+#endif
+    const bool coherent0 = fp.frame_minor && frames >= NOVA_COHERENT_MIN_FRAMES && !(p->flags & NOVA_HIP_RENDER_PER_RAY_TRAVERSAL) &&
                            traversal_stack_bound(ctx) < (uint32_t)kWaveStack;
     for (int k = 0; k <= p->max_depth; k++) {
       if (timing && (st = stamp())) return st;
-      if (k == 0 && coherent0) {
+      if (pass == 1 && replay.hit_cache) {
+        // the replay's rays are pass 0's: their hits are in the cache
+      } else if (k == 0 && coherent0) {
         const uint32_t cblocks = std::min<uint32_t>((active + kCoherentBlock - 1) / kCoherentBlock, (uint32_t)cus * NOVA_COHERENT_BLOCKS_PER_CU);
         if (counting)
           hipLaunchKernelGGL(extend_coherent_kernel<true>, dim3(cblocks), dim3(kCoherentBlock), 0, s, tc, q, which);

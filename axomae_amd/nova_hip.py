@@ -103,6 +103,7 @@ class BatchStats(C.Structure):
 RENDER_TIMING = 1
 RENDER_COUNT_TRAVERSAL = 2
 RENDER_PER_RAY_TRAVERSAL = 4  # camera rays one walk per ray (testing and tuning)
+ADJOINT_RETRACE = 8  # the adjoint replay traces again instead of reusing the primal's hits (testing)
 
 
 class RenderStats(C.Structure):
@@ -344,9 +345,10 @@ class HipEngine:
         self._keep_texels = x  # the copy is stream-ordered: keep the source alive until the next call
 
     def adjoint(self, width, height, frames, max_depth, d_image, scale, frame_start=0, rr_start=-1, max_samples_offset=0, vertical_invert=False,
-                sampling_roughness_texture=-1, grad=None):
+                sampling_roughness_texture=-1, grad=None, flags=0):
         """Path-replay adjoint: adds dLoss/d(texel) for every FP32 albedo/roughness texture to `grad` (FP32 pool layout,
-        allocated zeroed if None), where scale * d_image (H, W, 4) is the loss gradient of each frame's radiance."""
+        allocated zeroed if None), where scale * d_image (H, W, 4) is the loss gradient of each frame's radiance.
+        flags=ADJOINT_RETRACE makes the replay trace its rays again instead of reusing the primal pass's hits."""
         torch = self.torch
         _, pool = self.texture_info(next(i for i, t in enumerate(self.scene.textures) if t.is_f32))
         if grad is None:
@@ -355,7 +357,7 @@ class HipEngine:
         if d.numel() != width * height * 4:
             raise ValueError("d_image must hold width*height*4 floats")
         p = render_params(self.scene, width, height, frames, max_depth, frame_start, rr_start, max_samples_offset, vertical_invert,
-                          sampling_roughness_texture=sampling_roughness_texture)
+                          flags=flags, sampling_roughness_texture=sampling_roughness_texture)
         self._check(self.L.nova_hip_render_adjoint(self.ctx, C.byref(p), d.data_ptr(), float(scale), grad.data_ptr(), _stream(torch)),
                     "render_adjoint")
         return grad

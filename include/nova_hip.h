@@ -177,7 +177,9 @@ enum {
  * visits and triangle tests per ray (an instrumented traversal, slower). */
 /* NOVA_HIP_RENDER_PER_RAY_TRAVERSAL: trace camera rays one walk per ray even where a wave
  * would walk the tree once for its rays (the result is the same; for tests and tuning). */
-enum { NOVA_HIP_RENDER_TIMING = 1, NOVA_HIP_RENDER_COUNT_TRAVERSAL = 2, NOVA_HIP_RENDER_PER_RAY_TRAVERSAL = 4 };
+/* NOVA_HIP_ADJOINT_RETRACE: the adjoint's replay traces its rays again instead of reusing
+ * the primal pass's hits (the same rays, so the same hits; for tests). */
+enum { NOVA_HIP_RENDER_TIMING = 1, NOVA_HIP_RENDER_COUNT_TRAVERSAL = 2, NOVA_HIP_RENDER_PER_RAY_TRAVERSAL = 4, NOVA_HIP_ADJOINT_RETRACE = 8 };
 
 /* Renders frames [frame_start, frame_start+frame_count) of the path integrator
  * (PathIntegrator::Li, corelib/nova/integrator/Integrator.cpp:59-98, sample indices by

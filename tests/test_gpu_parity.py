@@ -250,9 +250,9 @@ def test_bvh_primitive_order_bit_exact_large(oracle, engine, name):
 
 
 @gpu
-@pytest.mark.parametrize("frames", [16, 64])
+@pytest.mark.parametrize("frames", [8, 16, 64])
 def test_wave_coherent_camera_walk_matches_the_per_ray_walk(engine, frames):
-    """Camera rays of pixel-major batches (>= 16 frames) are traced by one walk per wave; the
+    """Camera rays of pixel-major batches (>= 8 frames) are traced by one walk per wave; the
     hits, and so the images and accumulators, equal the per-ray walk's bit for bit."""
     from axomae_amd.nova_hip import RENDER_COUNT_TRAVERSAL, RENDER_PER_RAY_TRAVERSAL
 
