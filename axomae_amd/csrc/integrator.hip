@@ -334,13 +334,9 @@ __global__ void accumulate_kernel(const float *__restrict__ partial, FrameParams
     const uint32_t i = pixel_offset(fp, p) + c;
     const float *src = partial + (size_t)p * frames * 4 + c;
     float a = accum[i];
-    float fin = 0.f;
-    for (int f = 0; f < frames; f++) {
-      a += src[(size_t)f * 4];
-      fin = a / float(fp.frame0 + f + 1);
-    }
+    for (int f = 0; f < frames; f++) a += src[(size_t)f * 4];  // frame order
     accum[i] = a;
-    final_out[i] = fin;
+    final_out[i] = a / float(fp.frame0 + frames);  // the last frame's average, the one displayed
     if (partial_out) partial_out[i] = src[(size_t)(frames - 1) * 4];
   }
 }
@@ -509,6 +505,8 @@ static int render_impl(nova_hip_ctx *ctx, const nova_hip_render_params *p, float
   if (p->index_policy != NOVA_HIP_INDEX_CPU && p->index_policy != NOVA_HIP_INDEX_OPTIX)
     return set_error(ctx, NOVA_HIP_INVALID_ARGUMENT, "unknown index_policy");
   if (!adj && (!accum_dev || !final_dev)) return set_error(ctx, NOVA_HIP_INVALID_BUFFER_STATE, "null render buffer");
+  if ((((uintptr_t)accum_dev | (uintptr_t)final_dev | (uintptr_t)partial_dev) & 3u) != 0)
+    return set_error(ctx, NOVA_HIP_INVALID_BUFFER_STATE, "render buffers must be 4-byte aligned float arrays");
   if (p->sampling_roughness_texture >= 0) {
     if ((size_t)p->sampling_roughness_texture >= ctx->host_textures.size() ||
         ctx->host_textures[p->sampling_roughness_texture].format != NOVA_HIP_TEX_F32)
