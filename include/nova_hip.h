@@ -229,7 +229,8 @@ typedef struct {
 enum { NOVA_HIP_RB_FRONT = 0, NOVA_HIP_RB_BACK = 1, NOVA_HIP_RB_PARTIAL = 2, NOVA_HIP_RB_ACCUMULATOR = 3, NOVA_HIP_RB_DEPTH = 4, NOVA_HIP_RB_NORMAL = 5 };
 
 /* create_renderbuffer (renderbuffer.cpp:232): an empty render buffer on ctx's device;
- * errors of the calls below are reported on ctx (nova_hip_last_error). */
+ * errors of the calls below are reported on ctx (nova_hip_last_error). The render buffer
+ * refers to ctx: destroy it before ctx. */
 int nova_hip_renderbuffer_create(nova_hip_ctx *ctx, nova_hip_renderbuffer **out);
 int nova_hip_renderbuffer_destroy(nova_hip_renderbuffer *rb);
 /* RenderBuffer::preallocate (renderbuffer.cpp:126-136, RenderBufferPack::allocate :32-42):
