@@ -1,7 +1,8 @@
 // Batched rendering of many scenes in shared launches.
 //
 // Every path of every scene gets a global slot: scenes back to back (in the order given),
-// each scene frame-major as nova_hip_render_frames numbers its slots. The slots are cut into
+// each scene pixel-major as nova_hip_render_frames numbers its paths (path = pixel * frames +
+// frame; the sampler slot of a path is its frame-major slot, path_slot). The slots are cut into
 // waves of `queue_capacity` paths; a wave runs generate -> (extend, reset, shade) per depth
 // level -> accumulate -> advance, and each of those is one grid-stride launch over the
 // wave's combined queue, whatever scenes the paths belong to (a path carries its scene id
@@ -42,12 +43,13 @@ struct WaveDesc {
   uint32_t pix_total;  // sum of the segments' pixel counts
 };
 
-// The part of one scene inside one wave: local slots [a, b), pixels [0, npix) with prefix
-// pix_base among the wave's segments.
+// The part of one scene inside one wave: local paths [a, b), which cover pixels
+// p0 .. p0 + (its pixel count) - 1, with prefix pix_base among the wave's segments.
 struct Segment {
   uint32_t scene;
   uint32_t a, b;
   uint32_t pix_base;
+  uint32_t p0;
 };
 
 struct BatchState {
@@ -94,7 +96,7 @@ __global__ __launch_bounds__(256) void batch_generate(const BatchScene *__restri
     const int s = find_scene(scenes, nscenes, g);
     const uint32_t l = (uint32_t)(g - scenes[s].first);
     f3 o, d;
-    camera_ray(scenes[s].fp, sobn, l, o, d);
+    camera_ray(scenes[s].fp, sobn, path_slot(scenes[s].fp, l), o, d);
     float *ray = q.ray[0];
     ray[r] = o.x;
     ray[n + r] = o.y;
@@ -169,6 +171,37 @@ __global__ __launch_bounds__(kTraceBlock) __attribute__((amdgpu_waves_per_eu(6, 
   }
 }
 
+// Camera rays (queue 0 right after batch_generate, in slot order) walked once per wave of 64
+// consecutive slots, as the single-scene renderer walks its pixel-major camera rays. Used
+// when every scene's path count is a multiple of 64, so the 64 slots of a packet belong to
+// one scene (waves start at multiples of the queue capacity, itself a multiple of 256).
+constexpr int kBatchCoherentBlock = 256;
+__global__ __launch_bounds__(kBatchCoherentBlock) void batch_extend_coherent(const BatchScene *__restrict__ scenes, BatchState *st, BatchQueues q) {
+  const uint32_t n = st->count[0];
+  const float *r = q.ray[0];
+  const size_t cap = q.cap;
+  const uint32_t lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  if (blockIdx.x == 0 && threadIdx.x == 0) st->segments += n;
+  uint32_t nv = 0, nt = 0;
+  for (uint32_t base = (blockIdx.x * (kBatchCoherentBlock / 64) + wv) * 64u; base < n; base += gridDim.x * kBatchCoherentBlock) {
+    const uint32_t i = base + lane;
+    const bool valid = i < n;
+    const int32_t sid = __builtin_amdgcn_readfirstlane(q.sid[0][base]);
+    TraceCtx tc = scenes[sid].tc;
+    tc.tie = &scenes[sid].tie;
+    TraceState ts;
+    trace_init(ts, valid ? mk3(r[i], r[cap + i], r[2 * cap + i]) : mk3(0.f, 0.f, 0.f),
+               valid ? mk3(r[3 * cap + i], r[4 * cap + i], r[5 * cap + i]) : mk3(0.f, 0.f, 1.f));
+    coherent_walk<false>(tc, ts, __ballot(valid), 0.000001f, nv, nt);
+    if (valid) {
+      q.hit_prim[i] = ts.best;
+      q.hit_t[i] = ts.tbest;
+      q.hit_u[i] = ts.bu;
+      q.hit_v[i] = ts.bv;
+    }
+  }
+}
+
 // Between an extend and a shade launch: the shade's output queue and the next extend's work
 // counter start at zero.
 __global__ void batch_reset(BatchState *st, int which) {
@@ -212,7 +245,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4, 4))) voi
       sp.rr = rr;
       sp.dim0 = dim0;
       const uint32_t l = (uint32_t)(wstart + (uint64_t)slot - S.first);
-      cont = shade_path(sp, S.sc, sobn, l, o, d, q.hit_prim[i], q.hit_t[i], q.hit_u[i], q.hit_v[i], T, L, A, no, nd, escaped);
+      cont = shade_path(sp, S.sc, sobn, path_slot(S.fp, l), o, d, q.hit_prim[i], q.hit_t[i], q.hit_u[i], q.hit_v[i], T, L, A, no, nd, escaped);
       q.T[slot] = T.x;
       q.T[cap + slot] = T.y;
       q.T[2 * cap + slot] = T.z;
@@ -259,13 +292,13 @@ __global__ __launch_bounds__(256) void batch_accumulate(const BatchScene *__rest
     }
     const Segment sg = segs[lo];
     const BatchScene &S = scenes[sg.scene];
-    const uint32_t p = k - sg.pix_base;
-    const uint32_t fa = sg.a / S.npix, fb = (sg.b - 1) / S.npix;
+    const uint32_t p = sg.p0 + (k - sg.pix_base);
+    const uint32_t F = (uint32_t)S.fp.frames;
+    const uint32_t fa = sg.a > p * F ? sg.a - p * F : 0u, fb = std::min(F, sg.b - p * F);  // this wave's frames of pixel p
     float4 *acc = reinterpret_cast<float4 *>(S.accum + pixel_offset(S.fp, p));
     float4 a = *acc;
-    for (uint32_t f = fa; f <= fb; f++) {
-      const uint32_t l = f * S.npix + p;
-      if (l < sg.a || l >= sg.b) continue;
+    for (uint32_t f = fa; f < fb; f++) {
+      const uint32_t l = p * F + f;
       const float4 v = reinterpret_cast<const float4 *>(partial)[S.first + l - w.start];
       a.x += v.x;
       a.y += v.y;
@@ -301,6 +334,7 @@ struct BatchPlan {
   uint64_t total = 0;
   int max_depth = 0, rr_start = -1;
   uint32_t stack_bound = 0;
+  bool coherent0 = true;  // bounce 0 walks packets of 64 camera rays (batch_extend_coherent)
 };
 
 struct WavePlan {
@@ -317,8 +351,10 @@ static WavePlan plan_waves(const BatchPlan &bp, uint64_t start, uint64_t end, ui
       const BatchScene &S = bp.scenes[s];
       const uint64_t a = std::max(w0, S.first), b = std::min(w1, S.first + S.slots);
       if (a >= b) continue;
-      wp.segs.push_back(Segment{s, (uint32_t)(a - S.first), (uint32_t)(b - S.first), wd.pix_total});
-      wd.pix_total += S.npix;
+      const uint32_t la = (uint32_t)(a - S.first), lb = (uint32_t)(b - S.first), F = (uint32_t)S.fp.frames;
+      const uint32_t p0 = la / F, p1 = (lb - 1) / F;  // pixels of paths la .. lb-1 (pixel-major)
+      wp.segs.push_back(Segment{s, la, lb, wd.pix_total, p0});
+      wd.pix_total += p1 - p0 + 1;
     }
     wd.seg_end = (uint32_t)wp.segs.size();
     wp.waves.push_back(wd);
@@ -395,8 +431,12 @@ static void launch_wave(const BatchPlan &bp, const BatchBuffers &B, const nova_h
   uint32_t dim = 2;
   int which = 0;
   for (int k = 0; k <= bp.max_depth; k++) {
-    hipLaunchKernelGGL(batch_extend, dim3(trace_blocks), dim3(kTraceBlock), 0, s, B.scenes, B.state, B.q, which, B.gstack,
-                       trace_blocks * (uint32_t)kTraceBlock);
+    if (k == 0 && bp.coherent0)
+      hipLaunchKernelGGL(batch_extend_coherent, dim3((cap + kBatchCoherentBlock - 1) / kBatchCoherentBlock), dim3(kBatchCoherentBlock), 0, s, B.scenes,
+                         B.state, B.q);
+    else
+      hipLaunchKernelGGL(batch_extend, dim3(trace_blocks), dim3(kTraceBlock), 0, s, B.scenes, B.state, B.q, which, B.gstack,
+                         trace_blocks * (uint32_t)kTraceBlock);
     hipLaunchKernelGGL(batch_reset, dim3(1), dim3(1), 0, s, B.state, which ^ 1);
     const int rr = (bp.rr_start >= 0 && k >= bp.rr_start) ? 1 : 0;
     hipLaunchKernelGGL(batch_shade, dim3(shade_blocks), dim3(256), 0, s, B.scenes, B.waves, B.state, sobol, B.q, which, k,
@@ -477,7 +517,7 @@ static int make_plan(nova_hip_ctx *ex, const nova_hip_batch_item *items, uint32_
     S.fp.width = p->width;
     S.fp.height = p->height;
     S.fp.frame0 = p->frame_start;
-    S.fp.frame_minor = 0;
+    S.fp.frame_minor = 1;
     S.fp.frames = p->frame_count;
     S.fp.offset = p->max_samples_offset;
     S.fp.vinvert = p->vertical_invert;
@@ -492,7 +532,9 @@ static int make_plan(nova_hip_ctx *ex, const nova_hip_batch_item *items, uint32_
     S.accum = it.accum_dev;
     first += slots;
     bp.stack_bound = std::max(bp.stack_bound, traversal_stack_bound(c));
+    bp.coherent0 = bp.coherent0 && slots % 64 == 0;
   }
+  bp.coherent0 = bp.coherent0 && bp.stack_bound < (uint32_t)kWaveStack;
   uint32_t dims = 2;
   for (int k = 0; k <= bp.max_depth; k++) dims += 4 + ((bp.rr_start >= 0 && k >= bp.rr_start) ? 1 : 0);
   if (dims > 256) return set_error(ex, NOVA_HIP_INVALID_ARGUMENT, "max_depth needs more than 256 sampler dimensions");

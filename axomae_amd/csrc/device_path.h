@@ -28,6 +28,19 @@ NOVA_HD uint32_t pixel_offset(const FrameParams &fp, uint32_t p) {
   return fp.vinvert ? (((uint32_t)fp.height - 1 - y) * (uint32_t)fp.width + x) * 4u : (y * (uint32_t)fp.width + x) * 4u;
 }
 
+// Renders number paths pixel-major, frame-minor: path id = pixel * frames + frame, which is
+// also the path's entry in the first queue. So a wave traces one pixel's camera rays of
+// consecutive frames (they differ only by the jitter and walk the same nodes), and the path
+// state and the per-path radiance (partial) are indexed by path id. The sampler's slot
+// keeps the frame-major numbering, slot = frame * npix + pixel. The adjoint's replay numbers
+// paths frame-major (path id = slot): its texel-gradient atomics would otherwise collide,
+// all 64 lanes of a wave adding to the same pixel's texels.
+NOVA_HD uint32_t path_slot(const FrameParams &fp, uint32_t pid) {
+  if (!fp.frame_minor) return pid;
+  const uint32_t npix = (uint32_t)fp.width * (uint32_t)fp.height;
+  return (pid % (uint32_t)fp.frames) * npix + pid / (uint32_t)fp.frames;
+}
+
 // Sampler index of a path slot (frame-major slots), u32 wraparound. CPU policy:
 // idx * frame + renderer_max_samples with idx = 4 * pixel (Integrator.h:80). OptiX policy:
 // (x * H + y) * sample_max + frame (integrator_optix.cu:43).

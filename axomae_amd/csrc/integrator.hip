@@ -43,18 +43,6 @@ struct Queues {
   uint32_t cap;
 };
 
-// Renders number paths pixel-major, frame-minor: path id = pixel * frames + frame, which is
-// also the path's entry in the first queue. So a wave traces one pixel's camera rays of
-// consecutive frames (they differ only by the jitter and walk the same nodes), and the path
-// state and the per-path radiance (partial) are indexed by path id. The sampler's slot
-// keeps the frame-major numbering, slot = frame * npix + pixel. The adjoint's replay numbers
-// paths frame-major (path id = slot): its texel-gradient atomics would otherwise collide,
-// all 64 lanes of a wave adding to the same pixel's texels.
-__device__ __forceinline__ uint32_t path_slot(const FrameParams &fp, uint32_t pid) {
-  if (!fp.frame_minor) return pid;
-  const uint32_t npix = (uint32_t)fp.width * (uint32_t)fp.height;
-  return (pid % (uint32_t)fp.frames) * npix + pid / (uint32_t)fp.frames;
-}
 
 // AbstractIntegrator::render per pixel (Integrator.h:53-95): one camera ray per path slot.
 // The first shade takes the path id from the queue position and starts every path at T = 1,

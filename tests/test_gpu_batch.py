@@ -13,11 +13,16 @@ SPECS = [(11, 3000, 40, 3), (12, 800, 24, 5), (13, 12000, 56, 2), (14, 1500, 32,
 DEPTH = 6
 
 
-@pytest.fixture(scope="module")
-def batch_jobs():
+# every path count a multiple of 64: bounce 0 walks packets of 64 camera rays
+SPECS64 = [(11, 3000, 40, 8), (13, 12000, 56, 2), (14, 1500, 32, 4), (16, 2000, 24, 16)]
+
+
+@pytest.fixture(scope="module", params=["mixed", "packets64"])
+def batch_jobs(request):
     from axomae_amd.nova_hip import HipEngine
+    specs = SPECS if request.param == "mixed" else SPECS64
     engines, jobs, refs = [], [], []
-    for seed, tris, res, spp in SPECS:
+    for seed, tris, res, spp in specs:
         e = HipEngine(0)
         e.upload(scenes.batch_scene(seed, tris, res))
         e.build_bvh()
@@ -39,7 +44,7 @@ def test_batch_equals_single_scene_renders(batch_jobs, use_graph, cap):
     br = BatchRenderer(0)
     finals, st, comp = br.render(jobs, BatchConfig(cap, 8, 8, use_graph))
     print(f"[batch] cap {cap} graph {use_graph}: {st.waves} waves, {st.paths} paths, {st.total_ms:.2f} ms", flush=True)
-    assert st.paths == sum(r * r * s for _, _, r, s in SPECS)
+    assert st.paths == sum(w * h * s for _, w, h, s, _, _ in jobs)
     for k, (f, ref) in enumerate(zip(finals, refs)):
         np.testing.assert_array_equal(f.cpu().numpy(), ref, err_msg=f"scene {k}")
     assert np.all(np.diff(comp) >= 0) and comp[-1] <= st.total_ms + 1e-3
