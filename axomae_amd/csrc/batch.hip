@@ -141,6 +141,7 @@ __global__ __launch_bounds__(kTraceBlock) __attribute__((amdgpu_waves_per_eu(6, 
   bool have = false, exhausted = false;
   uint32_t ray = 0;
   int32_t sid = 0;
+  TraceCtx tc{nullptr, nullptr, nullptr, 0, nullptr};  // the lane's scene, loaded when it takes a ray (not at every step)
   uint32_t nv = 0, nt = 0;
   for (uint32_t turns = 0; turns < (1u << 30); turns++) {
     const uint64_t idle = __ballot(!have);
@@ -156,12 +157,13 @@ __global__ __launch_bounds__(kTraceBlock) __attribute__((amdgpu_waves_per_eu(6, 
           trace_init(ts, mk3(r[i], r[cap + i], r[2 * cap + i]), mk3(r[3 * cap + i], r[4 * cap + i], r[5 * cap + i]));
           ray = i;
           sid = q.sid[which][i];
+          tc = scenes[sid].tc;
           have = true;
         }
       }
     }
     if (__ballot(have) == 0) break;
-    if (have && trace_step<false>(scenes[sid].tc, stk, ts, 0.000001f, nv, nt, SceneTie{scenes, sid})) {
+    if (have && trace_step<false>(tc, stk, ts, 0.000001f, nv, nt, SceneTie{scenes, sid})) {
       q.hit_prim[ray] = ts.best;
       q.hit_t[ray] = ts.tbest;
       q.hit_u[ray] = ts.bu;
