@@ -178,6 +178,7 @@ __global__ __launch_bounds__(kTraceBlock) __attribute__((amdgpu_waves_per_eu(6, 
 // when every scene's path count is a multiple of 64, so the 64 slots of a packet belong to
 // one scene (waves start at multiples of the queue capacity, itself a multiple of 256).
 constexpr int kBatchCoherentBlock = 256;
+constexpr uint32_t kBatchCoherentBlocksPerCu = 64;  // grid-stride over the packets, as the single-scene walk
 __global__ __launch_bounds__(kBatchCoherentBlock) void batch_extend_coherent(const BatchScene *__restrict__ scenes, BatchState *st, BatchQueues q) {
   const uint32_t n = st->count[0];
   const float *r = q.ray[0];
@@ -434,8 +435,8 @@ static void launch_wave(const BatchPlan &bp, const BatchBuffers &B, const nova_h
   int which = 0;
   for (int k = 0; k <= bp.max_depth; k++) {
     if (k == 0 && bp.coherent0)
-      hipLaunchKernelGGL(batch_extend_coherent, dim3((cap + kBatchCoherentBlock - 1) / kBatchCoherentBlock), dim3(kBatchCoherentBlock), 0, s, B.scenes,
-                         B.state, B.q);
+      hipLaunchKernelGGL(batch_extend_coherent, dim3(std::min<uint32_t>((cap + kBatchCoherentBlock - 1) / kBatchCoherentBlock, (uint32_t)cus * kBatchCoherentBlocksPerCu)),
+                         dim3(kBatchCoherentBlock), 0, s, B.scenes, B.state, B.q);
     else
       hipLaunchKernelGGL(batch_extend, dim3(trace_blocks), dim3(kTraceBlock), 0, s, B.scenes, B.state, B.q, which, B.gstack,
                          trace_blocks * (uint32_t)kTraceBlock);
