@@ -292,6 +292,8 @@ typedef struct {
  * equals the item's nova_hip_render_frames image bit for bit. completion_ms (optional,
  * count floats) receives the device time at which each item's last wave ended. Items are
  * processed in the order given (sorting them by path count gives shortest-first completion).
+ * When every item's path count (width * height * frame_count) is a multiple of 64, camera
+ * rays are traced 64 at a time by one walk per wave, which is faster; the image is the same.
  * `executor` is a context that owns the queues and the wave graph (it needs no scene).
  * Replaces a loop of Engine renders, one per scene (engine.cpp:498-551); no counterpart in
  * the reference's API. */
