@@ -108,7 +108,7 @@ static void free_scene(nova_hip_ctx *ctx) {
                    (void **)&ctx->mesh_xf,   (void **)&ctx->materials, (void **)&ctx->textures,  (void **)&ctx->tex_u32,
                    (void **)&ctx->tex_f32,   (void **)&ctx->acos_lut,  (void **)&ctx->prim_idx,  (void **)&ctx->bnodes,
                    (void **)&ctx->nodes4,    (void **)&ctx->tri_leaf,  (void **)&ctx->bparent, (void **)&ctx->bdepth,
-                   (void **)&ctx->leaf_of_pos, (void **)&ctx->tie};
+                   (void **)&ctx->leaf_of_pos, (void **)&ctx->tie,   (void **)&ctx->xf_inv};
   for (void **p : ptrs)
     if (*p) {
       hipFree(*p);

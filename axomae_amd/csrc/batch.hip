@@ -343,11 +343,14 @@ struct BatchPlan {
 struct WavePlan {
   std::vector<WaveDesc> waves;
   std::vector<Segment> segs;
+  bool ok = true;  // false when a wave would start inside a packet of the per-wave camera walk
 };
 
 static WavePlan plan_waves(const BatchPlan &bp, uint64_t start, uint64_t end, uint32_t cap) {
   WavePlan wp;
   for (uint64_t w0 = start; w0 < end; w0 += cap) {
+    // the per-wave camera walk reads one scene index per packet of 64 paths
+    if (bp.coherent0 && w0 % 64 != 0) wp.ok = false;
     const uint64_t w1 = std::min<uint64_t>(end, w0 + cap);
     WaveDesc wd{w0, (uint32_t)(w1 - w0), (uint32_t)wp.segs.size(), 0, 0};
     for (uint32_t s = 0; s < bp.scenes.size(); s++) {
@@ -507,6 +510,9 @@ static int make_plan(nova_hip_ctx *ex, const nova_hip_batch_item *items, uint32_
     if (i > 0 && (p->max_depth != bp.max_depth || p->rr_start != bp.rr_start))
       return set_error(ex, NOVA_HIP_INVALID_ARGUMENT, "batch items must share max_depth and rr_start");
     if (!it.accum_dev || !it.final_dev) return set_error(ex, NOVA_HIP_INVALID_BUFFER_STATE, "null render buffer");
+    // batch_accumulate reads and writes a pixel as one float4
+    if ((reinterpret_cast<uintptr_t>(it.accum_dev) | reinterpret_cast<uintptr_t>(it.final_dev)) & 15u)
+      return set_error(ex, NOVA_HIP_INVALID_BUFFER_STATE, "batch render buffers must be 16-byte aligned");
     if (p->sampling_roughness_texture >= 0) return set_error(ex, NOVA_HIP_INVALID_ARGUMENT, "sampling_roughness_texture is not batched");
     bp.max_depth = p->max_depth;
     bp.rr_start = p->rr_start;
@@ -515,7 +521,7 @@ static int make_plan(nova_hip_ctx *ex, const nova_hip_batch_item *items, uint32_
     if (slots >= (1ull << 31)) return set_error(ex, NOVA_HIP_INVALID_ARGUMENT, "batch item too large");
     BatchScene &S = bp.scenes[i];
     S.tc = TraceCtx{c->nodes4, c->tri_leaf, nullptr, 0, nullptr};
-    S.tie = TieTables{c->bnodes, c->bparent, c->bdepth, c->leaf_of_pos};
+    S.tie = TieTables{c->bnodes, c->bparent, c->bdepth, c->leaf_of_pos, c->xf_inv};
     S.sc = ShadeCtx{c->textures, c->tex_u32, c->tex_f32, c->materials, c->mesh_xf, c->shade, c->acos_lut, c->envmap, -1};
     S.fp.width = p->width;
     S.fp.height = p->height;
@@ -572,6 +578,7 @@ extern "C" int nova_hip_render_batch(nova_hip_ctx *ex, const nova_hip_batch_item
   const nova_hip_batch_config cfg = *config;
   const uint32_t cap = (uint32_t)std::min<uint64_t>(cfg.queue_capacity, (bp.total + 255) / 256 * 256);
   const WavePlan wp = plan_waves(bp, 0, bp.total, cap);
+  if (!wp.ok) return set_error(ex, NOVA_HIP_INVALID_ARGUMENT, "batch wave start not on a packet boundary");
   const int cus = device_cus_of(ex->device);
   const uint32_t trace_threads = std::min<uint32_t>((cap + kTraceBlock - 1) / kTraceBlock, (uint32_t)cus * cfg.trace_blocks_per_cu) * kTraceBlock;
   BatchBuffers B;
@@ -637,7 +644,8 @@ extern "C" int nova_hip_batch_autotune(nova_hip_ctx *ex, const nova_hip_batch_it
   if ((st = executor_stream(ex, (hipStream_t)stream, &s))) return st;
   const int cus = device_cus_of(ex->device);
   const uint32_t *sobol = ex->sobol_table;
-  const uint32_t caps[] = {1u << 22, 1u << 23, 1u << 24, 1u << 25, 1u << 26, 1u << 27};
+  // the same limit check_config enforces for nova_hip_render_batch (2^26)
+  const uint32_t caps[] = {1u << 22, 1u << 23, 1u << 24, 1u << 25, 1u << 26};
   // workgroups per CU: the extend kernel balances its lanes through a work counter, so it
   // takes what fits at once; the shade loop splits its queue evenly, so a second round of
   // workgroups is a candidate against one round that fits
@@ -648,8 +656,10 @@ extern "C" int nova_hip_batch_autotune(nova_hip_ctx *ex, const nova_hip_batch_it
     occ_s = 4;
   const uint32_t trace_occ[] = {(uint32_t)occ_t};
   const uint32_t shade_occ[] = {(uint32_t)occ_s, 2u * (uint32_t)occ_s};
-  // every candidate runs the same slots: one wave from the middle of the batch
-  const uint64_t mid = bp.total / 2;
+  // every candidate runs the same slots: one wave from the middle of the batch. A wave must
+  // start on a multiple of 64 paths (a packet of the per-wave camera walk never straddles
+  // two scenes), so the start is rounded down to a multiple of 256, as capacities are.
+  const uint64_t mid = bp.total / 2 / 256 * 256;
   float best_t = -1.f;
   hipEvent_t e0, e1;
   NOVA_HIP_CHECK(ex, hipEventCreate(&e0));
@@ -658,8 +668,9 @@ extern "C" int nova_hip_batch_autotune(nova_hip_ctx *ex, const nova_hip_batch_it
     for (uint32_t to : trace_occ)
       for (uint32_t so : shade_occ) {
         const uint32_t cap = (uint32_t)std::min<uint64_t>(cap0, (bp.total + 255) / 256 * 256);
-        const uint64_t start = bp.total - mid >= cap ? mid : (bp.total > cap ? bp.total - cap : 0);
+        const uint64_t start = bp.total - mid >= cap ? mid : (bp.total > cap ? (bp.total - cap) / 256 * 256 : 0);
         const WavePlan wp = plan_waves(bp, start, std::min<uint64_t>(bp.total, start + cap), cap);
+        if (!wp.ok) return set_error(ex, NOVA_HIP_INVALID_ARGUMENT, "batch wave start not on a packet boundary");
         nova_hip_batch_config cfg{cap, to, so, 1};
         const uint32_t trace_threads = std::min<uint32_t>((cap + kTraceBlock - 1) / kTraceBlock, (uint32_t)cus * to) * kTraceBlock;
         BatchBuffers B;

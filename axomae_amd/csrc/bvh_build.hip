@@ -672,7 +672,7 @@ extern "C" int nova_hip_bvh_build(nova_hip_ctx *ctx, int build_candidates, nova_
   NOVA_HIP_CHECK(ctx, hipMalloc(&ctx->bdepth, maxnodes * sizeof(int32_t)));
   NOVA_HIP_CHECK(ctx, hipMalloc(&ctx->leaf_of_pos, (size_t)n * sizeof(int32_t)));
   NOVA_HIP_CHECK(ctx, hipMalloc(&ctx->tie, sizeof(TieTables)));
-  const TieTables tie{ctx->bnodes, ctx->bparent, ctx->bdepth, ctx->leaf_of_pos};
+  const TieTables tie{ctx->bnodes, ctx->bparent, ctx->bdepth, ctx->leaf_of_pos, ctx->xf_inv};
   NOVA_HIP_CHECK(ctx, hipMemcpy(ctx->tie, &tie, sizeof(TieTables), hipMemcpyHostToDevice));
 
   // temporaries

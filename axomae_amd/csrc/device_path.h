@@ -150,6 +150,36 @@ NOVA_HD bool mt_test(const TriMT &tr, f3 o, f3 d, float tnear, float tbest, floa
   return true;
 }
 
+#ifdef __HIP_DEVICE_COMPILE__
+#define NOVA_RARE __attribute__((noinline))
+#else
+#define NOVA_RARE
+#endif
+
+// Triangle::hit for a mesh with a non-identity transform (Triangle.h:150-175): the ray is
+// taken to object space by the inverse model matrix (glm mat4 * vec4 order,
+// (m0 x + m1 y) + (m2 z + m3 w)), then tested against the object-space record. t is the
+// same ray parameter in both spaces. Out of line: the common identity case keeps its
+// registers.
+NOVA_HD NOVA_RARE bool mt_test_object(const TriMT &tr, const float4 *xf_inv, f3 o, f3 d, float tnear, float tbest, float &t, float &u, float &v) {
+  const uint32_t k = (uint32_t)__builtin_bit_cast(int32_t, tr.c.w) - 1u;
+  const float4 r0 = xf_inv[3 * k], r1 = xf_inv[3 * k + 1], r2 = xf_inv[3 * k + 2];
+  const f3 oo = mk3((r0.x * o.x + r0.y * o.y) + (r0.z * o.z + r0.w * 1.f), (r1.x * o.x + r1.y * o.y) + (r1.z * o.z + r1.w * 1.f),
+                    (r2.x * o.x + r2.y * o.y) + (r2.z * o.z + r2.w * 1.f));
+  const f3 dd = mk3((r0.x * d.x + r0.y * d.y) + (r0.z * d.z + r0.w * 0.f), (r1.x * d.x + r1.y * d.y) + (r1.z * d.z + r1.w * 0.f),
+                    (r2.x * d.x + r2.y * d.y) + (r2.z * d.z + r2.w * 0.f));
+  return mt_test(tr, oo, dd, tnear, tbest, t, u, v);
+}
+
+// The triangle test of a traversal record: world-space MT for meshes with the identity
+// transform, object-space MT otherwise. tie() returns the TieTables record holding the
+// inverse matrices (read only in the second case).
+template <class TieGet>
+NOVA_HD bool tri_hit(const TriMT &tr, TieGet tie, f3 o, f3 d, float tnear, float tbest, float &t, float &u, float &v) {
+  if (__builtin_bit_cast(int32_t, tr.c.w) != 0) return mt_test_object(tr, tie()->xf_inv, o, d, tnear, tbest, t, u, v);
+  return mt_test(tr, o, d, tnear, tbest, t, u, v);
+}
+
 // Path-replay adjoint inputs: the primal radiance of every path slot (the batch's partial
 // buffers), the loss gradient per pixel (scaled) and the gradient pool (FP32 texel layout).
 struct AdjointCtx {
@@ -157,11 +187,13 @@ struct AdjointCtx {
   const float *dimg;
   float scale;
   float *grad;
-  // Hits of the primal pass by (bounce, pixel-major path id): (prim bits, t, u, v). The
-  // primal pass writes them, the replay reads them instead of tracing its rays again (they
-  // are the same rays, so the hits are the same). Null: the replay traces.
+  // Hits of the primal pass by (bounce, pixel-major path id): (prim bits, t, u, v), for
+  // bounces below cache_bounces. The primal pass writes them, the replay reads them instead
+  // of tracing its rays again (they are the same rays, so the hits are the same); deeper
+  // bounces are traced again. Null: the replay traces every bounce.
   float4 *hit_cache;
-  uint32_t cache_paths;  // paths per bounce in the cache
+  uint32_t cache_paths;    // paths per bounce in the cache
+  uint32_t cache_bounces;  // bounces 0 .. cache_bounces-1 are cached
 };
 
 NOVA_HD void add_grad(float *p, float v) {

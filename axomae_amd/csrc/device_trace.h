@@ -351,7 +351,8 @@ NOVA_HD bool trace_step(const TraceCtx &tc, Stack &stack, TraceState &st, float 
         if (i == start) cnt = __builtin_bit_cast(int32_t, tr.b.w);
         if (kCount) tri_tests++;
         float tt, uu, vv;
-        if (mt_test(tr, st.o, st.d, tnear, st.tbest, tt, uu, vv) && (tt < st.tbest || st.best < 0 || reference_tie_winner_is_new(tie_of(tc), i, st.bpos, st.o, st.d, tt))) {
+        if (tri_hit(tr, [&]() { return tie_of(tc); }, st.o, st.d, tnear, st.tbest, tt, uu, vv) &&
+            (tt < st.tbest || st.best < 0 || reference_tie_winner_is_new(tie_of(tc), i, st.bpos, st.o, st.d, tt))) {
           st.tbest = tt;
           st.bpos = i;
           st.best = __builtin_bit_cast(int32_t, tr.a.w);
@@ -498,7 +499,7 @@ __device__ void coherent_walk(const TraceCtx &tc, TraceState &st, uint64_t lanes
         if (!mine) continue;
         if (kCount) tri_tests++;
         float tt, uu, vv;
-        if (mt_test(tr, st.o, st.d, tnear, st.tbest, tt, uu, vv) &&
+        if (tri_hit(tr, [&]() { return tc.tie; }, st.o, st.d, tnear, st.tbest, tt, uu, vv) &&
             (tt < st.tbest || st.best < 0 || reference_tie_winner_is_new(tc.tie, i, st.bpos, st.o, st.d, tt))) {
           st.tbest = tt;
           st.bpos = i;

@@ -193,7 +193,9 @@ __global__ __launch_bounds__(kCoherentBlock) NOVA_COHERENT_OCCUPANCY void extend
   }
 }
 
-constexpr size_t kHitCacheBytes = (size_t)32 << 30;  // adjoint replays reuse the primal's hits up to this size
+// The adjoint replay reuses the primal pass's hits of its first bounces, in a cache of at
+// most this size whatever the job (16 B per path and cached bounce).
+constexpr size_t kHitCacheBytes = (size_t)1 << 30;
 
 // A finished path's radiance and alpha, by path id.
 __device__ __forceinline__ void write_partial(const Queues &q, uint32_t pid, float lx, float ly, float lz, float a) {
@@ -252,7 +254,8 @@ __global__ __launch_bounds__(kShadeBlock) NOVA_SHADE_OCCUPANCY void shade_kernel
         const uint32_t slot = path_slot(sp.fp, (uint32_t)pid), npix = (uint32_t)sp.fp.width * (uint32_t)sp.fp.height;
         const size_t pid0 = (size_t)(slot % npix) * (uint32_t)sp.fp.frames + slot / npix;
         padj.ltot = adj.ltot + pid0 * 4;
-        if (adj.hit_cache) {
+        if (adj.hit_cache && (uint32_t)sp.bounce < adj.cache_bounces) {
+          // entries a primal path did not reach hold prim -1 (a miss): always in bounds
           const float4 h = adj.hit_cache[(size_t)sp.bounce * adj.cache_paths + pid0];
           prim = __builtin_bit_cast(int32_t, h.x);
           ht = h.y;
@@ -269,7 +272,8 @@ __global__ __launch_bounds__(kShadeBlock) NOVA_SHADE_OCCUPANCY void shade_kernel
         ht = q.hit_t[i];
         hu = q.hit_u[i];
         hv = q.hit_v[i];
-        if (adj.hit_cache) adj.hit_cache[(size_t)sp.bounce * adj.cache_paths + pid] = make_float4(__builtin_bit_cast(float, prim), ht, hu, hv);
+        if (adj.hit_cache && (uint32_t)sp.bounce < adj.cache_bounces)
+          adj.hit_cache[(size_t)sp.bounce * adj.cache_paths + pid] = make_float4(__builtin_bit_cast(float, prim), ht, hu, hv);
       }
       cont = shade_path<kAdj>(sp, sc, sobn, path_slot(sp.fp, (uint32_t)pid), o, d, prim, ht, hu, hv, T, L, A, no, nd, escaped, &padj);
       const bool done = !cont;
@@ -559,15 +563,17 @@ static int render_impl(nova_hip_ctx *ctx, const nova_hip_render_params *p, float
   TraceCtx tc{ctx->nodes4, ctx->tri_leaf, ctx->gstack, trace_threads, ctx->tie};
   ShadeCtx sc{ctx->textures, ctx->tex_u32, ctx->tex_f32, ctx->materials, ctx->mesh_xf, ctx->shade, ctx->acos_lut, ctx->envmap,
               p->sampling_roughness_texture};
-  AdjointCtx replay = adj ? AdjointCtx{partial, adj->dimg, adj->scale, adj->grad, nullptr, 0u} : AdjointCtx{nullptr, nullptr, 0.f, nullptr, nullptr, 0u};
-  // The replay reuses the primal pass's hits when they fit in kHitCacheBytes (16 B per
-  // segment slot: (max_depth + 1) x paths per batch).
+  AdjointCtx replay = adj ? AdjointCtx{partial, adj->dimg, adj->scale, adj->grad, nullptr, 0u, 0u}
+                          : AdjointCtx{nullptr, nullptr, 0.f, nullptr, nullptr, 0u, 0u};
+  // The replay reuses the primal pass's hits of as many leading bounces as fit in
+  // kHitCacheBytes (16 B per path of the batch and bounce); it traces the others again.
   if (adj && !(p->flags & NOVA_HIP_ADJOINT_RETRACE)) {
-    const size_t bytes = (size_t)(p->max_depth + 1) * cap * sizeof(float4);
-    if (bytes <= kHitCacheBytes) {
-      replay.hit_cache = (float4 *)ensure_scratch(ctx, 6, bytes, &e);
+    const size_t bounces = std::min<size_t>((size_t)p->max_depth + 1, kHitCacheBytes / (cap * sizeof(float4)));
+    if (bounces > 0) {
+      replay.hit_cache = (float4 *)ensure_scratch(ctx, 6, bounces * cap * sizeof(float4), &e);
       if (!replay.hit_cache) return check_hip(ctx, e, "hit cache alloc");
       replay.cache_paths = (uint32_t)cap;
+      replay.cache_bounces = (uint32_t)bounces;
     }
   }
   // The shade loop splits the queue evenly over its grid, so the grid is what fits on the
@@ -614,6 +620,8 @@ static int render_impl(nova_hip_ctx *ctx, const nova_hip_render_params *p, float
     const uint32_t active = (uint32_t)(npix * frames);
     for (int pass = 0; pass < (adj ? 2 : 1); pass++) {  // pass 1: the adjoint replay of pass 0's paths
     fp.frame_minor = pass == 0 ? 1 : 0;
+    if (pass == 0 && replay.hit_cache)  // every entry starts as a miss (prim -1)
+      NOVA_HIP_CHECK(ctx, hipMemsetAsync(replay.hit_cache, 0xFF, (size_t)replay.cache_bounces * cap * sizeof(float4), s));
     hipLaunchKernelGGL(generate_kernel, dim3(std::min<uint32_t>((active + 255) / 256, (uint32_t)cus * 8)), dim3(256), 0, s, fp, ctx->sobol_table, q,
                        active);
     NOVA_HIP_CHECK(ctx, hipGetLastError());
@@ -629,7 +637,7 @@ static int render_impl(nova_hip_ctx *ctx, const nova_hip_render_params *p, float
                            traversal_stack_bound(ctx) < (uint32_t)kWaveStack;
     for (int k = 0; k <= p->max_depth; k++) {
       if (timing && (st = stamp())) return st;
-      if (pass == 1 && replay.hit_cache) {
+      if (pass == 1 && replay.hit_cache && (uint32_t)k < replay.cache_bounces) {
         // the replay's rays are pass 0's: their hits are in the cache
       } else if (k == 0 && coherent0) {
         const uint32_t cblocks = std::min<uint32_t>((active + kCoherentBlock - 1) / kCoherentBlock, (uint32_t)cus * NOVA_COHERENT_BLOCKS_PER_CU);
@@ -708,7 +716,7 @@ extern "C" int nova_hip_render_adjoint(nova_hip_ctx *ctx, const nova_hip_render_
                                        float *grad_pool_dev, void *stream) {
   if (!ctx || !p) return NOVA_HIP_INVALID_ARGUMENT;
   if (!d_image_dev || !grad_pool_dev) return set_error(ctx, NOVA_HIP_INVALID_BUFFER_STATE, "null adjoint buffer");
-  const AdjointCtx adj{nullptr, d_image_dev, scale, grad_pool_dev};
+  const AdjointCtx adj{nullptr, d_image_dev, scale, grad_pool_dev, nullptr, 0u, 0u};
   return render_impl(ctx, p, nullptr, nullptr, nullptr, stream, &adj);
 }
 

@@ -10,11 +10,13 @@
 
 namespace nova_hip {
 
-// Traversal record, 48 B, in BVH leaf order: (v0, prim id) (e1, leaf size) (e2, 0); the
-// leaf size (int bits) is set in the first record of each leaf.
+// Traversal record, 48 B, in BVH leaf order: (v0, prim id) (e1, leaf size) (e2, mesh tag);
+// the leaf size (int bits) is set in the first record of each leaf.
 // e1 = v1 - v0, e2 = v2 - v0 are the edges Triangle::hit forms (Triangle.h:158-160); a
 // triangle with repeated indices gets zero edges so that det == 0 rejects it, as
-// Triangle::isDegenerate does.
+// Triangle::isDegenerate does. Mesh tag (int bits) 0: the mesh's transform is the identity
+// and v0, e1, e2 are world (= object) coordinates; k > 0: v0, e1, e2 are object
+// coordinates of mesh k - 1, whose rays are taken to object space first.
 struct alignas(16) TriMT {
   float4 a, b, c;
 };
@@ -61,6 +63,10 @@ struct BNode {
 struct TieTables {
   const BNode *bnodes;
   const int32_t *bparent, *bdepth, *leaf_of_pos;
+  // Rows 0-2 of each mesh's inverse model matrix (three float4 per mesh): the triangle test
+  // of a mesh with a non-identity transform takes the ray to object space, as
+  // Triangle::hit does (Triangle.h:150-153). Read only for such meshes' triangles.
+  const float4 *xf_inv;
 };
 
 // 4-wide node, 64 B, four 16-byte loads per visit. The child boxes are on a per-node grid:
@@ -101,6 +107,7 @@ struct nova_hip_ctx {
   float *prim_aabb = nullptr;                // 6 floats per primitive (min3, max3)
   float *prim_centroid = nullptr;            // 3 floats per primitive
   nova_hip::MeshXf *mesh_xf = nullptr;
+  float4 *xf_inv = nullptr;                  // rows 0-2 of each mesh's inverse model matrix
   nova_hip::MatRec *materials = nullptr;
   nova_hip::TexRec *textures = nullptr;
   uint32_t *tex_u32 = nullptr;

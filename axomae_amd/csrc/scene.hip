@@ -22,29 +22,6 @@ __global__ void flatten_kernel(uint32_t ntri, const TriRef *__restrict__ refs, c
   flatten_triangle(p, refs[p], pos, nrm, tan, bit, uv, xforms, mesh_mat, tri, shade, aabb, centroid);
 }
 
-static void mat3_blocks(const float *M, MeshXf &out) {
-  double a[3][3];
-  for (int c = 0; c < 3; c++)
-    for (int r = 0; r < 3; r++) {
-      a[r][c] = M[c * 4 + r];
-      out.m3[c * 3 + r] = M[c * 4 + r];
-    }
-  double det = a[0][0] * (a[1][1] * a[2][2] - a[1][2] * a[2][1]) - a[0][1] * (a[1][0] * a[2][2] - a[1][2] * a[2][0]) +
-               a[0][2] * (a[1][0] * a[2][1] - a[1][1] * a[2][0]);
-  double inv[3][3];
-  inv[0][0] = (a[1][1] * a[2][2] - a[1][2] * a[2][1]) / det;
-  inv[0][1] = (a[0][2] * a[2][1] - a[0][1] * a[2][2]) / det;
-  inv[0][2] = (a[0][1] * a[1][2] - a[0][2] * a[1][1]) / det;
-  inv[1][0] = (a[1][2] * a[2][0] - a[1][0] * a[2][2]) / det;
-  inv[1][1] = (a[0][0] * a[2][2] - a[0][2] * a[2][0]) / det;
-  inv[1][2] = (a[0][2] * a[1][0] - a[0][0] * a[1][2]) / det;
-  inv[2][0] = (a[1][0] * a[2][1] - a[1][1] * a[2][0]) / det;
-  inv[2][1] = (a[0][1] * a[2][0] - a[0][0] * a[2][1]) / det;
-  inv[2][2] = (a[0][0] * a[1][1] - a[0][1] * a[1][0]) / det;
-  // n = transpose(inverse(m3)): element (row r, col c) = inv[c][r]; stored column-major.
-  for (int c = 0; c < 3; c++)
-    for (int r = 0; r < 3; r++) out.n3[c * 3 + r] = (float)inv[c][r];
-}
 
 template <class T>
 static hipError_t upload(T **dst, const void *src, size_t bytes, hipStream_t s) {
@@ -138,6 +115,7 @@ extern "C" int nova_hip_scene_upload(nova_hip_ctx *ctx, const nova_hip_scene_des
   std::vector<float> pos(nvert * 3), nrm(nvert * 3), tan(nvert * 3), bit(nvert * 3), uv(nvert * 2), xf(sd->mesh_count * 16);
   std::vector<int32_t> mesh_mat(sd->mesh_count);
   std::vector<MeshXf> mxf(sd->mesh_count);
+  std::vector<float4> xinv(sd->mesh_count * 3 + 3);  // never empty
   std::vector<TriRef> refs(ntri);
   uint64_t vo = 0, to = 0;
   for (uint32_t m = 0; m < sd->mesh_count; m++) {
@@ -151,7 +129,8 @@ extern "C" int nova_hip_scene_upload(nova_hip_ctx *ctx, const nova_hip_scene_des
       std::memcpy(&uv[vo * 2], md.uv, md.vertex_count * 2 * sizeof(float));
     }
     std::memcpy(&xf[m * 16], md.transform, 16 * sizeof(float));
-    mat3_blocks(md.transform, mxf[m]);
+    mesh_xf_blocks(md.transform, mxf[m]);
+    mesh_inverse_rows(md.transform, &xinv[m * 3]);
     mesh_mat[m] = md.material;
     for (uint64_t i = 0; i < md.index_count; i += 3)
       refs[to++] = TriRef{(uint32_t)(md.indices[i] + vo), (uint32_t)(md.indices[i + 1] + vo), (uint32_t)(md.indices[i + 2] + vo), m};
@@ -179,7 +158,7 @@ extern "C" int nova_hip_scene_upload(nova_hip_ctx *ctx, const nova_hip_scene_des
 #define UP(dst, vec) \
   if (e == hipSuccess) e = upload(&dst, vec.data(), vec.size() * sizeof(vec[0]), s);
   UP(dpos, pos) UP(dnrm, nrm) UP(dtan, tan) UP(dbit, bit) UP(duv, uv) UP(dxf, xf) UP(dmm, mesh_mat) UP(drefs, refs)
-  UP(ctx->mesh_xf, mxf) UP(ctx->materials, mats) UP(ctx->textures, texrec) UP(ctx->tex_u32, u32pool) UP(ctx->tex_f32, f32pool)
+  UP(ctx->mesh_xf, mxf) UP(ctx->xf_inv, xinv) UP(ctx->materials, mats) UP(ctx->textures, texrec) UP(ctx->tex_u32, u32pool) UP(ctx->tex_f32, f32pool)
   UP(ctx->acos_lut, lut)
 #undef UP
   if (e == hipSuccess) e = hipMalloc(&ctx->tri_world, ntri * sizeof(TriMT));

@@ -9,7 +9,7 @@ them, engine.cpp:380-429).
 
 All arithmetic that defines geometry runs in float64 and is rounded once to
 float32, so the same seed gives the same scene on every machine. Transforms are
-identity: geometry is generated in world space.
+identity (geometry generated in world space) except in `transformed_scene`.
 """
 from dataclasses import dataclass, field
 import math
@@ -342,6 +342,41 @@ def million_triangles(seed=0xA70AE002, width=1920, height=1080, terrain_cells=51
     return sc
 
 
+def _affine(rot_axis, angle, scale, translate):
+    """Column-major 4x4 (glm value_ptr order) of translate * rotate(axis, angle) * scale."""
+    a = np.asarray(rot_axis, dtype=np.float64)
+    a /= np.linalg.norm(a)
+    K = np.array([[0, -a[2], a[1]], [a[2], 0, -a[0]], [-a[1], a[0], 0]])
+    R = np.eye(3) + math.sin(angle) * K + (1 - math.cos(angle)) * (K @ K)
+    M = np.eye(4)
+    M[:3, :3] = R @ np.diag(np.asarray(scale, dtype=np.float64))
+    M[:3, 3] = translate
+    return colmajor(M)
+
+
+def transformed_scene(seed=0xA70AE006, width=256, height=256, terrain_cells=64, spheres=12, sphere_level=3):
+    """Meshes placed by non-identity model matrices (rotation, non-uniform scale,
+    translation), as Object3D transforms place them in the reference: a terrain built around
+    the origin and turned about a tilted axis, unit octaspheres and boxes scaled per axis,
+    rotated about random axes and translated. Diffuse and conductor materials under a sky.
+    The reference intersects such meshes in object space (Triangle.h:146-175)."""
+    rng = np.random.default_rng(seed)
+    sc = Scene(name="transformed")
+    terrain = terrain_mesh(rng, terrain_cells, 1.0, 0.0, 0.15, sc.principled((0.55, 0.5, 0.42), roughness=0.7))
+    terrain.transform = _affine((0.1, 1.0, 0.05), rng.uniform(0, 2 * math.pi), (9.0, 6.0, 9.0), (0.3, -3.0, -0.5))
+    sc.meshes.append(terrain)
+    for i in range(spheres):
+        m = sc.principled(rng.uniform(0.3, 0.95, 3), roughness=rng.uniform(0.05, 0.8), metal=float(i % 2))
+        mesh = octasphere_mesh((0.0, 0.0, 0.0), 1.0, sphere_level, m) if i % 3 else \
+            box_mesh((0.0, 0.0, 0.0), (1.0, 1.0, 1.0), 0.0, m)
+        mesh.transform = _affine(rng.standard_normal(3), rng.uniform(0, 2 * math.pi), rng.uniform(0.4, 1.6, 3),
+                                 (rng.uniform(-6, 6), rng.uniform(-1.0, 3.0), rng.uniform(-6, 4)))
+        sc.meshes.append(mesh)
+    sc.envmap = sc.add_texture(sky_envmap(256, 128, (0.4, 0.8, 0.3), sun_intensity=20.0, sun_size=0.01))
+    set_camera(sc, (0.0, 5.0, 14.0), (0.0, -0.5, 0.0), 55.0, width / height)
+    return sc
+
+
 def scaled_million(scale_cells=64, spheres=8, sphere_level=3, width=128, height=72, seed=0xA70AE002):
     """The million-triangle workload's generator at reduced size, for parity tests."""
     return million_triangles(seed=seed, width=width, height=height, terrain_cells=scale_cells, spheres=spheres,
@@ -357,7 +392,7 @@ def bake_envmaps(seed=0xA70AE005, count=32):
         w, h = sizes[rng.integers(0, len(sizes))]
         d = rng.standard_normal(3)
         inten = float(np.exp(rng.uniform(np.log(1e2), np.log(1e5))))
-        maps.append(sky_envmap(w, h, d, sun_intensity=inten * 1e-3, sun_size=0.002))
+        maps.append(sky_envmap(w, h, d, sun_intensity=inten, sun_size=0.002))
     return maps
 
 
@@ -431,7 +466,7 @@ def bake_envmap_specs(seed=0xA70AE005, count=32):
         w, h = sizes[rng.integers(0, len(sizes))]
         d = rng.standard_normal(3)
         inten = float(np.exp(rng.uniform(np.log(1e2), np.log(1e5))))
-        specs.append((w, h, d, inten * 1e-3, 0.002))
+        specs.append((w, h, d, inten, 0.002))
     return specs
 
 

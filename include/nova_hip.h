@@ -264,8 +264,8 @@ int nova_hip_renderbuffer_read(const nova_hip_renderbuffer *rb, int which, float
 /* One scene of a batched render: a context holding an uploaded scene with its BVH built,
  * the frames to render (as for nova_hip_render_frames; max_depth and rr_start must be the
  * same for every item, sampling_roughness_texture -1) and device RGBA32F buffers of
- * width*height*4 floats: accum is zeroed and receives the sum of the frames, final the
- * image accum / (frame_start + frame_count). */
+ * width*height*4 floats, 16-byte aligned: accum is zeroed and receives the sum of the
+ * frames, final the image accum / (frame_start + frame_count). */
 typedef struct {
   nova_hip_ctx *scene;
   nova_hip_render_params params;
@@ -301,7 +301,7 @@ int nova_hip_render_batch(nova_hip_ctx *executor, const nova_hip_batch_item *ite
                           nova_hip_batch_stats *stats, float *completion_ms, void *stream);
 
 /* Autotuner for nova_hip_render_batch: times the captured wave graph of every candidate
- * (queue capacity 2^22 to 2^27 paths x the trace workgroups per CU that are resident at
+ * (queue capacity 2^22 to 2^26 paths x the trace workgroups per CU that are resident at
  * once x one or two rounds of resident shade workgroups per CU) on the same slots from the
  * middle of the batch, without accumulating, and returns the fastest per path
  * (best_ns_per_path optional). */
@@ -326,8 +326,11 @@ int nova_hip_texture_f32_update(nova_hip_ctx *ctx, int32_t texture, const float 
  * dLoss/d(texel) to grad_pool_dev (laid out like the FP32 texel pool, see
  * nova_hip_texture_f32_info) for the albedo_f32 (channels 0-2) and roughness_f32
  * (channel 0) textures of every material. Each batch of frames is rendered once to get
- * every path's radiance, then replayed from the same sample indices; nothing is stored per
- * path vertex. Sampling is detached: directions, lobe choices and pdfs are constants, the
+ * every path's radiance, then replayed from the same sample indices. The replay takes the
+ * primal pass's hits of its leading bounces from a cache of at most 1 GiB (16 B per path
+ * and bounce, as many bounces as fit) and traces the deeper bounces again; nothing else is
+ * kept per path vertex, so memory stays within the batch's queues plus that cap whatever
+ * the depth (NOVA_HIP_ADJOINT_RETRACE: no cache, every bounce traced). Sampling is detached: directions, lobe choices and pdfs are constants, the
  * gradient flows through the BSDF values and Russian-roulette rescaling. With
  * sampling_roughness_texture >= 0 the replay samples with that texture as the renderer
  * does. No counterpart in the reference. */

@@ -227,6 +227,23 @@ void oracle_render(void *sp, void *bp, int width, int height, int frame_start, i
   render(*static_cast<Scene *>(sp), *static_cast<Bvh *>(bp), p, accum, final_out);
 }
 
+// Rays entering scatter index `bounce` of every path of frame `frame` (width*height*6 floats).
+void oracle_path_rays(void *sp, void *bp, int width, int height, int frame, int bounce, int max_depth, int rr_start, const float *inv_P,
+                      const float *inv_V, int threads, float *rays) {
+  RenderParams p;
+  p.width = width;
+  p.height = height;
+  p.max_depth = max_depth;
+  p.rr_start = rr_start;
+  for (int i = 0; i < 16; i++) {
+    p.inv_P.m[i] = inv_P[i];
+    p.inv_V.m[i] = inv_V[i];
+  }
+  p.threads = threads;
+  p.index_policy = g_index_policy;
+  path_rays(*static_cast<Scene *>(sp), *static_cast<Bvh *>(bp), p, frame, bounce, rays);
+}
+
 // Rows [y0, y1) of frame `frame` into partial (width*height*4 floats), `threads` workers.
 void oracle_render_rows(void *sp, void *bp, int width, int height, int frame, int y0, int y1, int max_depth, int rr_start, const float *inv_P,
                         const float *inv_V, int threads, float *partial) {

@@ -123,6 +123,7 @@ vec4 path_Li(const Scene &scene, const Bvh &bvh, vec3 o, vec3 d, int depth, Sobo
 void render_frame(const Scene &scene, const Bvh &bvh, const RenderParams &p, int frame, float *partial);
 void render(const Scene &scene, const Bvh &bvh, const RenderParams &p, float *accum, float *final_out);
 void render_rows(const Scene &scene, const Bvh &bvh, const RenderParams &p, int frame, int y0, int y1, float *partial);
+void path_rays(const Scene &scene, const Bvh &bvh, const RenderParams &p, int frame, int bounce, float *rays);
 // Render passes (pass 1 normal, 2 depth, 3 emissive) accumulated over p's frames.
 void render_pass(const Scene &scene, const Bvh &bvh, const RenderParams &p, int pass, float z_near, float z_far, float *accum, float *final_out);
 // Reverse-mode gradient of a loss with d loss / d(frame radiance) = scale * d_image (RGBA per

@@ -64,6 +64,7 @@ def lib():
         L.oracle_f32_pool_offset.argtypes = [P, C.c_int]
         L.oracle_f32_pool_offset.restype = C.c_uint64
         L.oracle_render_rows.argtypes = [P, P, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, f32p, f32p, C.c_int, f32p]
+        L.oracle_path_rays.argtypes = [P, P, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, f32p, f32p, C.c_int, f32p]
         L.oracle_bake_irradiance.argtypes = [f32p, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, f32p]
         L.oracle_bake_irradiance_sampler.argtypes = [f32p, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, f32p]
         L.oracle_brdf_lut_value.argtypes = [C.c_float, C.c_float, C.c_int, f32p]
@@ -227,6 +228,16 @@ def render_rows(oscene, width, height, frame, y0, y1, max_depth, rr_start=-1, th
     lib().oracle_render_rows(oscene.h, oscene.bvh, width, height, frame, y0, y1, max_depth, rr_start, oscene.scene.inv_P, oscene.scene.inv_V,
                              threads, out)
     return out.reshape(height, width, 4)
+
+
+def path_rays(oscene, width, height, frame, bounce, max_depth, rr_start=-1, threads=8):
+    """The rays (ox, oy, oz, dx, dy, dz) that enter scatter index `bounce` (0: camera rays) of
+    each pixel's path in one frame of the CPU render, pixel (x, y) at row y*width + x; NaN
+    rows where the path ended earlier."""
+    out = np.zeros(width * height * 6, dtype=np.float32)
+    lib().oracle_path_rays(oscene.h, oscene.bvh, width, height, frame, bounce, max_depth, rr_start, oscene.scene.inv_P, oscene.scene.inv_V,
+                           threads, out)
+    return out.reshape(-1, 6)
 
 
 def bake_irradiance(env, out_w, out_h, samples, threads=8, sobol=False):

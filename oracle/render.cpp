@@ -1,6 +1,7 @@
 // Oracle (test infrastructure only): hit record (Embree-path semantics), texture
 // sampling, Principled BSDF, the recursive PathIntegrator and the frame loop.
 #include "oracle.hpp"
+#include <limits>
 #include "fmath.hpp"
 #include <algorithm>
 #include <atomic>
@@ -753,8 +754,17 @@ static vec4 denan(const vec4 &v) { return vec_nonfinite(v) ? vec4(0.f) : v; }
 // PathIntegrator::Li (Integrator.cpp:59-98), recursive; with the Russian-roulette
 // extension (rr_start >= 0) rolling one extra sampler dimension after each scatter
 // whose index is >= rr_start, with survival q = min(1, max component of throughput).
+// Ray recorder for the closest-hit tests: when set, the ray entering scatter index
+// g_ray_bounce is written to g_ray_out (o, d: six floats).
+static thread_local float *g_ray_out = nullptr;
+static thread_local int g_ray_bounce = -1;
+
 static vec4 Li_rec(const Scene &s, const Bvh &bvh, vec3 o, vec3 d, int depth, SobolSampler &sampler, const RenderParams &p, int scatter_index,
                    Spec throughput) {
+  if (g_ray_out && scatter_index == g_ray_bounce) {
+    const float r[6] = {o.x, o.y, o.z, d.x, d.y, d.z};
+    for (int k = 0; k < 6; k++) g_ray_out[k] = r[k];
+  }
   if (depth >= 0) {
     float t = 0, bu = 0, bv = 0;
     int prim = closest_hit(s, bvh, o, d, 0.000001f, t, bu, bv);
@@ -867,6 +877,43 @@ static void render_tile(const Scene &s, const Bvh &bvh, const RenderParams &p, i
       partial[idx + 2] = rgb.z;
       partial[idx + 3] = rgb.w;
     }
+}
+
+// The rays that enter scatter index `bounce` (0: camera rays) of every path of one frame,
+// pixel (x, y) at rays[(y * width + x) * 6]; NaN where the path ended before that bounce.
+// The paths are the renderer's own (same sample indices, jitter, scatter and roulette);
+// they are cut after that bounce, which changes none of the rays before it.
+void path_rays(const Scene &s, const Bvh &bvh, const RenderParams &p, int frame, int bounce, float *rays) {
+  const float RAND_DX = 0.0005f, RAND_DY = 0.0005f;
+  std::atomic<int> next{0};
+  auto worker = [&]() {
+    SobolSampler sampler;
+    g_ray_bounce = bounce;
+    for (;;) {
+      const int y = next.fetch_add(1);
+      if (y >= p.height) break;
+      for (int x = 0; x < p.width; x++) {
+        float *out = rays + ((size_t)y * p.width + x) * 6;
+        for (int k = 0; k < 6; k++) out[k] = std::numeric_limits<float>::quiet_NaN();
+        g_ray_out = out;
+        sampler.reset(sample_index(p, x, y, frame));
+        float s0 = sampler.sample1D(), s1 = sampler.sample1D();
+        const float dx = s0 * RAND_DX, dy = s1 * RAND_DY;
+        int sy = p.height - y;
+        float ndc_x = (float)(2 * x - p.width) / (float)p.width;
+        float ndc_y = (float)(p.height - 2 * sy) / (float)p.height;
+        vec3 o, d;
+        camera_ray(p, ndc_x + dx, ndc_y + dy, o, d);
+        path_Li(s, bvh, o, d, std::min(p.max_depth, bounce), sampler, p, 0);
+      }
+    }
+    g_ray_out = nullptr;
+    g_ray_bounce = -1;
+  };
+  const int nthreads = std::max(1, p.threads);
+  std::vector<std::thread> pool;
+  for (int i = 0; i < nthreads; i++) pool.emplace_back(worker);
+  for (auto &th : pool) th.join();
 }
 
 void render_frame(const Scene &s, const Bvh &bvh, const RenderParams &p, int frame, float *partial) {

@@ -26,6 +26,7 @@ struct EmuScene {
   std::vector<ShadeTri> shade;
   std::vector<float> aabb, cent;
   std::vector<MeshXf> mxf;
+  std::vector<float4> xf_inv;
 };
 
 void flatten_scene(const nova_hip_scene_desc *sd, EmuScene &es) {
@@ -38,6 +39,7 @@ void flatten_scene(const nova_hip_scene_desc *sd, EmuScene &es) {
   std::vector<int32_t> mesh_mat(sd->mesh_count);
   std::vector<MeshXf> mxf(sd->mesh_count);
   std::vector<TriRef> refs(ntri);
+  es.xf_inv.assign(sd->mesh_count * 3 + 3, make_float4(0.f, 0.f, 0.f, 0.f));
   uint64_t vo = 0, to = 0;
   for (uint32_t m = 0; m < sd->mesh_count; m++) {
     const nova_hip_mesh_desc &md = sd->meshes[m];
@@ -47,11 +49,8 @@ void flatten_scene(const nova_hip_scene_desc *sd, EmuScene &es) {
     std::memcpy(&bit[vo * 3], md.bitangents, md.vertex_count * 12);
     std::memcpy(&uv[vo * 2], md.uv, md.vertex_count * 8);
     std::memcpy(&xf[m * 16], md.transform, 64);
-    for (int c = 0; c < 3; c++)
-      for (int r = 0; r < 3; r++) {
-        mxf[m].m3[c * 3 + r] = md.transform[c * 4 + r];
-        mxf[m].n3[c * 3 + r] = (c == r) ? 1.f : 0.f;  // identity transforms only (the test scenes)
-      }
+    mesh_xf_blocks(md.transform, mxf[m]);
+    mesh_inverse_rows(md.transform, &es.xf_inv[m * 3]);
     mesh_mat[m] = md.material;
     for (uint64_t i = 0; i < md.index_count; i += 3)
       refs[to++] = TriRef{(uint32_t)(md.indices[i] + vo), (uint32_t)(md.indices[i + 1] + vo), (uint32_t)(md.indices[i + 2] + vo), m};
@@ -128,7 +127,7 @@ extern "C" int emu_closest_hit(const nova_hip_scene_desc *sd, const BNode *bnode
   for (uint32_t i = 0; i < n; i++) tri_leaf[i] = es.tri[prim_idx[i]];
   for (int32_t i = 0; i < nbnodes; i++)
     if (bnodes[i].count > 0) tri_leaf[bnodes[i].left].b.w = __builtin_bit_cast(float, bnodes[i].count);
-  const TieTables tie{bnodes, parent.data(), depth.data(), leaf_of_pos.data()};
+  const TieTables tie{bnodes, parent.data(), depth.data(), leaf_of_pos.data(), es.xf_inv.data()};
   TraceCtx tc{wide.data(), tri_leaf.data(), nullptr, 0, &tie};
   HostStack st;
   uint32_t nv = 0, nt = 0;
@@ -202,13 +201,14 @@ static int emu_run(const nova_hip_scene_desc *sd, const nova_hip_render_params *
 
   const uint32_t npix = (uint32_t)p->width * (uint32_t)p->height;
   std::vector<float> partial((size_t)npix * 4);
+  const TieTables xf_tab{nullptr, nullptr, nullptr, nullptr, es.xf_inv.data()};
   auto brute_hit = [&](f3 o, f3 d, float &tb, float &bu, float &bv) {
     tb = 3.402823466e+38f;
     bu = bv = 0.f;
     int best = -1;
     for (uint32_t t = 0; t < ntri; t++) {
       float tt, uu, vv;
-      if (mt_test(tri[t], o, d, 0.000001f, tb, tt, uu, vv)) {
+      if (tri_hit(tri[t], [&]() { return &xf_tab; }, o, d, 0.000001f, tb, tt, uu, vv)) {
         tb = tt;
         bu = uu;
         bv = vv;
@@ -269,7 +269,7 @@ static int emu_run(const nova_hip_scene_desc *sd, const nova_hip_render_params *
     fp.policy = p->index_policy;
     std::memcpy(fp.inv_P, p->inv_P, 64);
     std::memcpy(fp.inv_V, p->inv_V, 64);
-    const AdjointCtx adj{partial.data(), adj_dimg, adj_scale, adj_grad};
+    const AdjointCtx adj{partial.data(), adj_dimg, adj_scale, adj_grad, nullptr, 0u, 0u};
     for (int pass = 0; pass < (adj_grad ? 2 : 1); pass++)
     for (uint32_t slot = 0; slot < npix; slot++) {
       f3 o, d;

@@ -33,6 +33,7 @@ BVH_SCENES = {
     "soup_int_max_fallback": lambda: soup_scene(3000, 3, lo=-3000.0, hi=3000.0, size=40.0),
     "soup_int_max_single_leaf": lambda: soup_scene(2000, 4, lo=100.0, hi=3000.0, size=40.0),
     "single_triangle": lambda: soup_scene(1, 5),
+    "transformed": lambda: scenes.transformed_scene(),
 }
 
 

@@ -1,6 +1,9 @@
 """Path-replay adjoint on the MI355X: against the oracle's CPU gradient, against central
 finite differences of the HIP renderer (common random numbers, float64 loss differences),
-and through the torch.autograd.Function of the inverse-rendering harness."""
+and through the torch.autograd.Function of the inverse-rendering harness. The
+differentiation workload is checked at its own size: 1280x720, 32 spp, depth 4, about
+100k triangles, 512x512 FP32 albedo and roughness textures, L2 loss against a render of
+textures perturbed by Gaussian noise (sigma 0.05)."""
 import numpy as np
 import pytest
 
@@ -140,3 +143,114 @@ def test_gpu_adjoint_replay_reuses_the_primal_hits(engine, rr_start):
     err = np.linalg.norm(g_cache.astype(np.float64) - g_trace) / np.linalg.norm(g_trace.astype(np.float64))
     print(f"[adjoint] cached hits vs retrace rr_start={rr_start}: rel L2 {err:.2e}", flush=True)
     assert np.count_nonzero(g_trace) > 1000 and err <= 1e-5
+
+
+C4 = dict(W=1280, H=720, F=32, D=4)
+
+
+@pytest.fixture(scope="module")
+def c4_engine():
+    from axomae_amd.nova_hip import HipEngine
+    e = HipEngine(0)
+    yield e
+    e.close()
+
+
+@pytest.fixture(scope="module")
+def c4(c4_engine):
+    """The differentiation workload at full size with an unperturbed copy of the roughness
+    texture for detached sampling; target rendered from perturbed textures; the image, the
+    loss gradient per frame and the GPU adjoint's texel gradients."""
+    import torch
+    engine = c4_engine
+    W, H, F, D = C4["W"], C4["H"], C4["F"], C4["D"]
+    sc, alb, rough = scenes.differentiable_scene(width=W, height=H)
+    t = sc.textures[rough]
+    samp = sc.add_texture(scenes.Texture(t.width, t.height, 1, t.data.copy()))
+    engine.upload(sc)
+    engine.build_bvh()
+    rng = np.random.default_rng(0xA70AE004)
+    ta = np.clip(sc.textures[alb].data + rng.normal(0, 0.05, sc.textures[alb].data.shape), 0, 1).astype(np.float32)
+    tr = np.clip(sc.textures[rough].data + rng.normal(0, 0.05, sc.textures[rough].data.shape), 0.01, 1).astype(np.float32)
+    engine.update_texture(alb, torch.from_numpy(ta))
+    engine.update_texture(rough, torch.from_numpy(tr))
+    target = engine.render(W, H, F, D, sampling_roughness_texture=samp)[0].double()
+    engine.update_texture(alb, torch.from_numpy(sc.textures[alb].data))
+    engine.update_texture(rough, torch.from_numpy(sc.textures[rough].data))
+    img = engine.render(W, H, F, D, sampling_roughness_texture=samp)[0]
+    dimg = (2 * (img.double() - target)).float()
+    dimg[..., 3] = 0
+    grad = engine.adjoint(W, H, F, D, dimg, 1.0 / F, sampling_roughness_texture=samp)
+    torch.cuda.synchronize()
+    return dict(sc=sc, alb=alb, rough=rough, samp=samp, target=target, img=img, dimg=dimg, grad=grad)
+
+
+@gpu
+def test_c4_adjoint_matches_oracle_adjoint_full_size(oracle, c4):
+    """Every texel gradient of the full-size workload against the oracle's CPU reverse-mode
+    gradient (float64) of the same 29.5 M paths."""
+    W, H, F, D = C4["W"], C4["H"], C4["F"], C4["D"]
+    osc = oracle.OracleScene(c4["sc"])
+    import os
+    threads = max(1, min(len(os.sched_getaffinity(0)), 16))
+    g_ref = osc.render_adjoint(W, H, F, D, c4["dimg"].cpu().numpy(), 1.0 / F, sampling_roughness_texture=c4["samp"], threads=threads)
+    g = c4["grad"].cpu().numpy().astype(np.float64)
+    err = np.linalg.norm(g - g_ref) / np.linalg.norm(g_ref)
+    print(f"[c4] adjoint vs oracle 1280x720x32 d4: rel L2 {err:.2e}, nonzero texels {np.count_nonzero(g_ref)}", flush=True)
+    assert np.count_nonzero(g_ref) > 100000 and err <= 1e-4
+
+
+@gpu
+def test_c4_adjoint_matches_central_differences_on_random_texels(c4_engine, c4):
+    """Central differences (h = 1e-3, float64 loss differences, common random numbers) on
+    256 uniformly drawn entries of each texture, gated at 1% of the difference.
+
+    Albedo is differenced on the plain renderer (sampling does not read it); roughness with
+    the sampling texture held fixed, as the detached estimator defines its derivative.
+    Noise floor: each image value is a float32 sum of 32 frames divided by 32, so a render
+    rounds pixel p with a standard deviation of about 2.3 ulp(img_p) in the difference of
+    two renders; over the texel's footprint A (pixels whose value the perturbation changed)
+    the loss difference then has sigma = sqrt(sum_A (2.3 ulp(img_p) s_p)^2) / 2h with
+    s = img+ + img- - 2 target. Entries whose |fd| is under 300 sigma (3 sigma above 1% of
+    the difference) are below the floor: they are counted and must agree to 1% plus
+    5 sigma. Entries no path reaches (equal images, zero gradient) are counted apart."""
+    import torch
+    engine = c4_engine
+    W, H, F, D = C4["W"], C4["H"], C4["F"], C4["D"]
+    sc, target = c4["sc"], c4["target"][..., :3]
+    h = 1e-3
+    rng = np.random.default_rng(0xFD04)
+    for tex, samp_tex in ((c4["alb"], -1), (c4["rough"], c4["samp"])):
+        base = torch.from_numpy(sc.textures[tex].data).cuda()
+        g = engine.texture_grad(c4["grad"], tex).reshape(-1).double()
+        picks = rng.choice(base.numel(), 256, replace=False)
+        above = below = unseen = 0
+        worst = 0.0
+        for k in picks.tolist():
+            ims = []
+            for sgn in (1, -1):
+                x = base.clone()
+                x[k] += sgn * h
+                engine.update_texture(tex, x)
+                ims.append(engine.render(W, H, F, D, sampling_roughness_texture=samp_tex)[0][..., :3])
+            engine.update_texture(tex, base)
+            p, m = ims[0].double(), ims[1].double()
+            s = p + m - 2 * target
+            fd = float(((p - m) * s).sum()) / (2 * h)
+            foot = ims[0] != ims[1]
+            ulp = (torch.nextafter(ims[0].abs(), torch.full_like(ims[0], float("inf"))) - ims[0].abs()).double()
+            sigma = float(torch.sqrt(((2.3 * ulp * s) ** 2)[foot].sum())) / (2 * h)
+            gk = float(g[k])
+            if not bool(foot.any()) and gk == 0.0:
+                unseen += 1
+                continue
+            if abs(fd) >= 300 * sigma and abs(fd) > 0:
+                above += 1
+                worst = max(worst, abs(gk - fd) / abs(fd))
+                assert abs(gk - fd) <= 0.01 * abs(fd), (tex, k, gk, fd, sigma)
+            else:
+                below += 1
+                assert abs(gk - fd) <= 0.01 * abs(fd) + 5 * sigma, (tex, k, gk, fd, sigma)
+        print(f"[c4-fd] texture {tex}: 256 entries, {above} above the noise floor (worst relative error {worst:.2e}), "
+              f"{below} below it, {unseen} not reached by any path", flush=True)
+        assert above >= 32

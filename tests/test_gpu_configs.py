@@ -1,7 +1,9 @@
 """Parity at the benchmark configurations' own sizes (MI355X): the box room at 256x256,
-16 spp, depth 5 in full; the 1,048,576-triangle 1080p workload on a band of rows of one
-frame (the CPU renders only that band); one scene of the 64-scene batch; full-size maps of
-the IBL bake batch. Radiance and bakes within relative RMSE 1e-3 of the oracle."""
+16 spp, depth 5 in full; the 1,048,576-triangle 1080p workload's whole 64-frame job and
+single frames, with the closest-hit gate on its camera and first-bounce rays; every image
+of the 64-scene batch against its single-scene render and one scene against the oracle;
+full-size maps of the IBL bake batch. Radiance and bakes within relative RMSE 1e-3 of the
+oracle."""
 import numpy as np
 import pytest
 
@@ -39,28 +41,78 @@ def test_box_room_full_config(oracle, engine):
     assert err <= 1e-3
 
 
-@gpu
-def test_million_triangles_1080p_band(oracle, engine):
-    sc = scenes.million_triangles()
-    engine.upload(sc)
-    engine.build_bvh()
-    W, H, frame = 1920, 1080, 1
-    import torch
-    partial = torch.empty((H, W, 4), dtype=torch.float32, device="cuda")
-    from axomae_amd import nova_hip
+def _render_one_frame(engine, sc, W, H, frame, depth, rr_start):
+    """One frame (1 spp) through nova_hip_render_frames; returns its partial buffer (H, W, 4)."""
     import ctypes as C
-    p = nova_hip.render_params(sc, W, H, 1, 8, frame_start=frame, rr_start=3)
+    import torch
+    from axomae_amd import nova_hip
+    partial = torch.empty((H, W, 4), dtype=torch.float32, device="cuda")
+    p = nova_hip.render_params(sc, W, H, 1, depth, frame_start=frame, rr_start=rr_start)
     acc = torch.zeros((H, W, 4), dtype=torch.float32, device="cuda")
     fin = torch.empty_like(acc)
     engine._check(engine.L.nova_hip_render_frames(engine.ctx, C.byref(p), acc.data_ptr(), fin.data_ptr(), partial.data_ptr(),
                                                   nova_hip._stream(torch)), "render_frames")
-    gpu_img = partial.cpu().numpy()
-    y0, y1 = 500, 524  # rows of the reference's y (row 0 at the bottom of the image buffer order)
-    osc = oracle.OracleScene(sc)
-    ref = oracle.render_rows(osc, W, H, frame, y0, y1, 8, rr_start=3, threads=_threads()).reshape(H, W, 4)
-    err = rel_rmse(gpu_img[y0:y1], ref[y0:y1])
-    print(f"[config] 1M triangles 1080p frame {frame} rows {y0}-{y1 - 1} d8 rr3 rel_rmse={err:.3e}", flush=True)
+    return partial.cpu().numpy()
+
+
+@pytest.fixture(scope="module")
+def million(oracle):
+    """The 1,048,576-triangle 1080p workload, uploaded with its BVH built, and the oracle's copy."""
+    from axomae_amd.nova_hip import HipEngine
+    sc = scenes.million_triangles()
+    e = HipEngine(0)
+    e.upload(sc)
+    e.build_bvh()
+    yield sc, e, oracle.OracleScene(sc)
+    e.close()
+
+
+@gpu
+def test_million_triangles_1080p_full_job(oracle, million):
+    """The whole 1080p job: 64 frames of 1 spp, depth 8, Russian roulette from scatter 3,
+    accumulated by the engine's rule; the CPU renders all 64 frames of the same paths."""
+    sc, engine, osc = million
+    W, H, F, D = 1920, 1080, 64, 8
+    img = engine.render(W, H, F, D, rr_start=3)[0].cpu().numpy()
+    ref, _ = osc.render(W, H, F, D, rr_start=3, threads=_threads())
+    err = rel_rmse(img, ref)
+    print(f"[config] 1M triangles 1920x1080 x {F} spp d{D} rr3: rel_rmse={err:.3e}", flush=True)
     assert err <= 1e-3
+
+
+@gpu
+@pytest.mark.parametrize("frame", [1, 63])
+def test_million_triangles_1080p_single_frames(oracle, million, frame):
+    """One whole frame (1 spp, 2,073,600 paths) at a time: no averaging over frames."""
+    sc, engine, osc = million
+    W, H, D = 1920, 1080, 8
+    img = _render_one_frame(engine, sc, W, H, frame, D, 3)
+    ref = oracle.render_rows(osc, W, H, frame, 0, H, D, rr_start=3, threads=_threads()).reshape(H, W, 4)
+    err = rel_rmse(img, ref)
+    print(f"[config] 1M triangles 1920x1080 frame {frame} d{D} rr3: rel_rmse={err:.3e}", flush=True)
+    assert err <= 1e-3
+
+
+@gpu
+@pytest.mark.parametrize("bounce", [0, 1])
+def test_million_triangles_closest_hit_gate(oracle, million, bounce):
+    """Closest-hit primitive ids of the 1080p workload's camera rays (bounce 0) and
+    first-bounce rays (bounce 1) of frame 1, as the CPU renderer spawns them: equal on at
+    least 99.99% of rays; any mismatch must be a tie (equal t to 1e-6 relative)."""
+    sc, engine, osc = million
+    W, H = 1920, 1080
+    rays = oracle.path_rays(osc, W, H, 1, bounce, 8, rr_start=3, threads=_threads())
+    rays = rays[~np.isnan(rays[:, 0])]
+    prim, tuv = engine.closest_hit(rays)
+    prim, tuv = prim.cpu().numpy(), tuv.cpu().numpy()
+    oprim, otuv = osc.closest_hit(rays, threads=_threads())
+    same = prim == oprim
+    bad = np.flatnonzero(~same)
+    print(f"[config] 1M triangles bounce {bounce}: {len(rays)} rays, id match {same.mean():.7f}, {len(bad)} mismatches, "
+          f"hits {np.mean(oprim >= 0):.3f}", flush=True)
+    assert same.mean() >= 0.9999
+    for k in bad:
+        assert prim[k] >= 0 and oprim[k] >= 0 and abs(tuv[k, 0] - otuv[k, 0]) <= 1e-6 * otuv[k, 0], (int(k), tuv[k], otuv[k])
 
 
 @gpu
@@ -110,3 +162,30 @@ def test_bake_batch_full_size_maps(oracle):
         print(f"[config] bake {env.width}x{env.height} -> {w}x{h} {'irradiance' if r is None else f'prefilter r={r}'} rel_rmse={err:.3e}",
               flush=True)
         assert err <= 1e-3
+
+
+@gpu
+def test_batch_64_scenes_equal_single_scene_renders():
+    """The 64-scene batch (depth 6, Russian roulette off, autotuned launch shape, shortest
+    scene first as the benchmark orders them): every image equals the scene's own
+    nova_hip_render_frames image bit for bit."""
+    import torch
+    from axomae_amd.nova_hip import BatchRenderer, HipEngine
+    specs = scenes.batch_scene_specs()
+    engines, jobs = [], []
+    for seed, tris, res, spp in specs:
+        e = HipEngine(0)
+        e.upload(scenes.batch_scene(seed, tris, res))
+        e.build_bvh()
+        engines.append(e)
+        jobs.append((e, res, res, spp, 6, {}))
+    jobs.sort(key=lambda j: j[1] * j[2] * j[3])
+    br = BatchRenderer(0)
+    cfg, _ = br.autotune(jobs)
+    finals, st, _ = br.render(jobs, cfg)
+    br.close()
+    print(f"[config] batch of {len(jobs)} scenes, {st.paths} paths, {st.waves} waves, config {cfg.as_dict()}", flush=True)
+    for k, (e, w, h, spp, d, _) in enumerate(jobs):
+        ref = e.render(w, h, spp, d)[0]
+        assert torch.equal(finals[k], ref), f"scene {k}: {e.scene.name} {w}x{h} x {spp}"
+        e.close()

@@ -38,6 +38,22 @@ def test_sobol_bit_exact(oracle):
     np.testing.assert_array_equal(out.cpu().numpy(), oracle.sobol_float(I, D))
 
 
+@gpu
+def test_sobol_equals_reference_generator_vectors():
+    """nova_hip_sobol_fill against the integers and floats written by the reference's own
+    scripts/generate_luts_sobol.py (tests/golden/sobol_ref_golden.npz), all 256 dims."""
+    import torch
+    from axomae_amd import nova_hip
+    from test_sampler import reference_vectors
+    g = reference_vectors()
+    I, D = np.meshgrid(g["index"], np.arange(256, dtype=np.uint32), indexing="ij")
+    out, bits = nova_hip.sobol_fill(torch.from_numpy(I.ravel().view(np.int32)), torch.from_numpy(D.ravel().view(np.int32)), return_bits=True)
+    torch.cuda.synchronize()
+    np.testing.assert_array_equal(bits.cpu().numpy().view(np.uint32).reshape(I.shape), g["bits"])
+    keep = g["index"] != 0
+    np.testing.assert_array_equal(out.cpu().numpy().reshape(I.shape)[keep][:, :255], g["value"][keep][:, :255])
+
+
 def _leaf_set(nodes):
     return sorted((int(n["left"]), int(n["count"])) for n in nodes if n["count"] > 0)
 
@@ -61,7 +77,7 @@ def test_bvh_primitive_order_bit_exact(oracle, engine, name):
 
 
 @gpu
-@pytest.mark.parametrize("name", ["box_room", "terrain_spheres", "soup_5000", "soup_duplicates", "soup_int_max_fallback"])
+@pytest.mark.parametrize("name", ["box_room", "terrain_spheres", "soup_5000", "soup_duplicates", "soup_int_max_fallback", "transformed"])
 def test_closest_hit_matches_reference_traversal(oracle, engine, name):
     """Primitive ids, t, u and v identical to the reference walk on every ray (the gate is
     99.99% of ids; the traversal's tie rule and parallel-axis slabs make it exact)."""
@@ -78,6 +94,18 @@ def test_closest_hit_matches_reference_traversal(oracle, engine, name):
     bad = np.where(~same)[0]
     assert len(bad) == 0, [(int(k), int(prim[k]), int(oprim[k]), tuv[k].tolist(), otuv[k].tolist()) for k in bad[:8]]
     np.testing.assert_array_equal(tuv, otuv)
+
+
+@gpu
+def test_render_transformed_meshes_radiance(oracle, engine):
+    sc = scenes.transformed_scene()
+    engine.upload(sc)
+    engine.build_bvh()
+    img = engine.render(256, 256, 16, 5, rr_start=3)[0].cpu().numpy()
+    ref, _ = oracle.OracleScene(sc).render(256, 256, 16, 5, rr_start=3, threads=8)
+    err = rel_rmse(img, ref)
+    print(f"[render] transformed 256x256x16 d5 rr3 rel_rmse={err:.3e}", flush=True)
+    assert err <= 1e-3
 
 
 @gpu

@@ -68,16 +68,28 @@ def test_partition_closed_form_random():
 
 # ------------------------------------------------------------ BVH tree
 
+def _to_world(M, p):
+    """glm mat4 * vec4(p, 1) in float32, (m0 x + m1 y) + (m2 z + m3 w) per row (M column-major)."""
+    M = np.asarray(M, dtype=np.float32).reshape(4, 4)  # M[c, r]
+    one = np.float32(1.0)
+    return np.stack([(M[0, r] * p[..., 0] + M[1, r] * p[..., 1]) + (M[2, r] * p[..., 2] + M[3, r] * one) for r in range(3)], -1)
+
+
 def prim_boxes(sc):
+    """World-space AABB and centroid per primitive as Triangle::computeAABB / centroid form
+    them (Triangle.h:108-125): the corners and the object-space centroid through the model
+    matrix."""
     boxes, cents = [], []
     for m in sc.meshes:
-        v = m.vertices.reshape(-1, 3)[m.indices.reshape(-1, 3)]
-        boxes.append(np.concatenate([v.min(1), v.max(1)], 1))
-        cents.append(((v[:, 0].astype(np.float32) + v[:, 1]) + v[:, 2]) * np.float32(0.333333))
+        v = m.vertices.reshape(-1, 3)[m.indices.reshape(-1, 3)].astype(np.float32)
+        w = _to_world(m.transform, v)
+        boxes.append(np.concatenate([w.min(1), w.max(1)], 1))
+        c = ((v[:, 0] + v[:, 1]) + v[:, 2]) * np.float32(0.333333)
+        cents.append(_to_world(m.transform, c))
     return np.concatenate(boxes).astype(np.float32), np.concatenate(cents).astype(np.float32)
 
 
-@pytest.mark.parametrize("name", ["box_room", "soup_5000", "soup_duplicates", "soup_int_max_fallback", "single_triangle"])
+@pytest.mark.parametrize("name", ["box_room", "soup_5000", "soup_duplicates", "soup_int_max_fallback", "single_triangle", "transformed"])
 def test_oracle_bvh_is_a_valid_tree(oracle, name):
     sc = BVH_SCENES[name]()
     prim, nodes = oracle.OracleScene(sc).bvh_arrays()
@@ -126,7 +138,7 @@ def test_int_max_sentinel_all_positive_gives_one_leaf(oracle):
     assert (prim != np.arange(sc.triangle_count)).any()
 
 
-@pytest.mark.parametrize("name", ["box_room", "soup_5000", "terrain_spheres"])
+@pytest.mark.parametrize("name", ["box_room", "soup_5000", "terrain_spheres", "transformed"])
 def test_oracle_closest_hit_equals_brute_force(oracle, name):
     sc = BVH_SCENES[name]()
     osc = oracle.OracleScene(sc)

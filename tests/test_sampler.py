@@ -1,18 +1,25 @@
 """Owen-scrambled Sobol sampler: the oracle against independent pins.
 
-Pins: scipy's unscrambled Sobol (same Joe-Kuo data; reference dim d is scipy dim d+1),
-the stratification that nested (Owen) scrambling preserves, the reference's own range
-tests (SamplerTest.cpp:7-31, math_random_generatorsTest.cpp:6-12) and the committed
-golden vectors.
+Pins: vectors written by the reference's own Python generator
+(scripts/generate_luts_sobol.py: its direction-number recurrence, gray code, LK hash and
+Owen scramble; golden/make_sobol_ref_golden.py), scipy's unscrambled Sobol (same Joe-Kuo
+data; reference dim d is scipy dim d+1), the stratification that nested (Owen)
+scrambling preserves and the reference's own range tests (SamplerTest.cpp:7-31,
+math_random_generatorsTest.cpp:6-12).
 """
-import json
 import os
 
 import numpy as np
 import pytest
 from scipy.stats import qmc
 
-GOLDEN = os.path.join(os.path.dirname(__file__), "golden", "sobol_golden.json")
+GOLDEN = os.path.join(os.path.dirname(__file__), "golden", "sobol_ref_golden.npz")
+
+
+def reference_vectors():
+    """The fixture written from scripts/generate_luts_sobol.py (loaded without pickle)."""
+    with np.load(GOLDEN) as g:
+        return {k: g[k] for k in g.files}
 
 
 def unscrambled(table, idx, dim):
@@ -78,11 +85,63 @@ def test_reference_range_tests(oracle):
     assert (v >= 0).all() and (v < 1).all()
 
 
-def test_golden_vectors(oracle):
-    g = json.load(open(GOLDEN))
-    idx = np.array(g["index"], dtype=np.uint32)
-    dims = np.array(g["dim"], dtype=np.uint32)
-    np.testing.assert_array_equal(oracle.sobol_bits(idx, dims), np.array(g["bits"], dtype=np.uint32))
+def test_direction_table_equals_reference_generator(oracle):
+    """The 256 x 32 table equals the header generate_luts_sobol.py writes with -d 256 -s 32
+    (math/CMakeLists.txt:71-85), zero row 255 included."""
+    np.testing.assert_array_equal(oracle.direction_table(), reference_vectors()["table"])
+
+
+def test_scrambled_bits_equal_reference_generator(oracle):
+    """Owen-scrambled integers of all 256 dimensions at index 0, the workloads' maximum
+    indices, edge and random u32 indices against the reference script's gray_code, hash,
+    reverse_bits and owen_scramble with seed 0xDEADBEEF (generate_luts_sobol.py:100-125).
+    A changed hash constant, seed, shift or bit-reversal step fails here."""
+    g = reference_vectors()
+    I, D = np.meshgrid(g["index"], np.arange(256, dtype=np.uint32), indexing="ij")
+    np.testing.assert_array_equal(oracle.sobol_bits(I.ravel(), D.ravel()).reshape(I.shape), g["bits"])
+
+
+def test_float_samples_equal_reference_generator(oracle):
+    """float32 of the script's sobol_sample (accum / 2^32 in double, :128-138) equals the
+    oracle's float(accum) * 2^-32 at every nonzero index of dims 0..254."""
+    g = reference_vectors()
+    keep = g["index"] != 0
+    I, D = np.meshgrid(g["index"][keep], np.arange(255, dtype=np.uint32), indexing="ij")
+    got = oracle.sobol_float(I.ravel(), D.ravel()).reshape(I.shape)
+    np.testing.assert_array_equal(got, g["value"][keep][:, :255])
+
+
+def test_reference_vectors_catch_a_wrong_hash_constant():
+    """The fixture separates the reference hash from a one-bit change of any constant or of
+    the seed (a restatement with a typo would not match the vectors)."""
+    def owen(x, seed, c1=0x3d20adea, c2=0x05526c56, c3=0x53a22864):
+        def rev(v):
+            return int(f"{v & 0xFFFFFFFF:032b}"[::-1], 2)
+        x = rev(x)
+        x = (x ^ (x * c1)) & 0xFFFFFFFF
+        x = (x + seed) & 0xFFFFFFFF
+        x = (x * ((seed >> 16) | 1)) & 0xFFFFFFFF
+        x = (x ^ (x * c2)) & 0xFFFFFFFF
+        x = (x ^ (x * c3)) & 0xFFFFFFFF
+        return rev(x)
+    g = reference_vectors()
+    table = g["table"].astype(np.uint64)
+    n = 8
+
+    def bits_for(seed=0xDEADBEEF, **consts):
+        out = np.zeros((n, 256), dtype=np.uint64)
+        for r, i in enumerate(g["index"][:n].tolist()):
+            gc = i ^ (i >> 1)
+            for d in range(256):
+                acc = 0
+                for b in range(32):
+                    if (gc >> b) & 1:
+                        acc ^= int(table[d, b])
+                out[r, d] = owen(acc, seed, **consts)
+        return out
+    np.testing.assert_array_equal(bits_for(), g["bits"][:n])
+    for kw in ({"seed": 0xDEADBEEE}, {"c1": 0x3d20adeb}, {"c2": 0x05526c57}, {"c3": 0x53a22865}):
+        assert (bits_for(**kw) != g["bits"][:n]).any(), kw
 
 
 def test_acos_table_closed_form(oracle):
