@@ -64,8 +64,7 @@ struct BatchQueues {
   float *ray[2];
   int32_t *slot[2];  // wave-relative slot
   int32_t *sid[2];   // scene index
-  int32_t *hit_prim;
-  float *hit_t, *hit_u, *hit_v;
+  float4 *hit;        // closest hit per queue entry: (prim id bits, t, u, v), one 16-byte record
   float *T, *L, *A;
   float *partial;  // wave-relative slot x 4
   uint32_t cap;
@@ -164,10 +163,7 @@ __global__ __launch_bounds__(kTraceBlock) __attribute__((amdgpu_waves_per_eu(6, 
     }
     if (__ballot(have) == 0) break;
     if (have && trace_step<false>(tc, stk, ts, 0.000001f, nv, nt, SceneTie{scenes, sid})) {
-      q.hit_prim[ray] = ts.best;
-      q.hit_t[ray] = ts.tbest;
-      q.hit_u[ray] = ts.bu;
-      q.hit_v[ray] = ts.bv;
+      q.hit[ray] = make_float4(__builtin_bit_cast(float, (int32_t)(ts.best)), ts.tbest, ts.bu, ts.bv);
       have = false;
     }
   }
@@ -197,10 +193,7 @@ __global__ __launch_bounds__(kBatchCoherentBlock) void batch_extend_coherent(con
                valid ? mk3(r[3 * cap + i], r[4 * cap + i], r[5 * cap + i]) : mk3(0.f, 0.f, 1.f));
     coherent_walk<false>(tc, ts, __ballot(valid), 0.000001f, nv, nt);
     if (valid) {
-      q.hit_prim[i] = ts.best;
-      q.hit_t[i] = ts.tbest;
-      q.hit_u[i] = ts.bu;
-      q.hit_v[i] = ts.bv;
+      q.hit[i] = make_float4(__builtin_bit_cast(float, (int32_t)(ts.best)), ts.tbest, ts.bu, ts.bv);
     }
   }
 }
@@ -248,7 +241,8 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4, 4))) voi
       sp.rr = rr;
       sp.dim0 = dim0;
       const uint32_t l = (uint32_t)(wstart + (uint64_t)slot - S.first);
-      cont = shade_path(sp, S.sc, sobn, path_slot(S.fp, l), o, d, q.hit_prim[i], q.hit_t[i], q.hit_u[i], q.hit_v[i], T, L, A, no, nd, escaped);
+      const float4 h = q.hit[i];
+      cont = shade_path(sp, S.sc, sobn, path_slot(S.fp, l), o, d, __builtin_bit_cast(int32_t, h.x), h.y, h.z, h.w, T, L, A, no, nd, escaped);
       q.T[slot] = T.x;
       q.T[cap + slot] = T.y;
       q.T[2 * cap + slot] = T.z;
@@ -398,15 +392,12 @@ static int alloc_buffers(nova_hip_ctx *ex, const BatchPlan &bp, const WavePlan &
   B.q.slot[1] = ip + cap;
   B.q.sid[0] = ip + 2 * (size_t)cap;
   B.q.sid[1] = ip + 3 * (size_t)cap;
-  B.q.hit_prim = ip + 4 * (size_t)cap;
-  float *hf = (float *)(ip + 5 * (size_t)cap);
-  B.q.hit_t = hf;
-  B.q.hit_u = hf + cap;
-  B.q.hit_v = hf + 2 * (size_t)cap;
-  B.q.T = hf + 3 * (size_t)cap;
-  B.q.L = hf + 6 * (size_t)cap;
-  B.q.A = hf + 9 * (size_t)cap;
-  B.q.partial = hf + 10 * (size_t)cap;  // 4 floats per slot, 16-byte aligned (cap is a multiple of 256)
+  B.q.hit = reinterpret_cast<float4 *>(ip + 4 * (size_t)cap);  // 16-byte aligned: cap is a multiple of 256
+  float *hf = (float *)(ip + 8 * (size_t)cap);
+  B.q.T = hf;
+  B.q.L = hf + 3 * (size_t)cap;
+  B.q.A = hf + 6 * (size_t)cap;
+  B.q.partial = hf + 7 * (size_t)cap;  // 4 floats per slot, 16-byte aligned (cap is a multiple of 256)
   B.scenes = (BatchScene *)ensure_scratch(ex, 7, bp.scenes.size() * sizeof(BatchScene), &e);
   if (!B.scenes) return check_hip(ex, e, "batch scene table alloc");
   B.waves = (WaveDesc *)ensure_scratch(ex, 8, std::max<size_t>(1, wp.waves.size()) * sizeof(WaveDesc), &e);

@@ -35,9 +35,8 @@ struct Queues {
   float *ray[2];      // 6 planes of cap floats each: ox oy oz dx dy dz
   int32_t *slot[2];
   uint32_t *count;    // [0], [1]: queue sizes; [2]: segments traced
-  unsigned long long *stats64;  // [0] node visits, [1] triangle tests, [2] escaped segments
-  int32_t *hit_prim;
-  float *hit_t, *hit_u, *hit_v;
+  unsigned long long *stats64;  // [0] node visits, [1] triangle tests, [2] escaped segments (camera-ray launches count into [4], [5])
+  float4 *hit;        // closest hit per queue entry: (prim id bits, t, u, v), one 16-byte record
   float *T, *L, *A;   // path state by slot: T,L 3 planes of cap each; A alpha
   float *partial;     // frames x W x H x 4
   uint32_t cap;
@@ -126,10 +125,7 @@ __global__ __launch_bounds__(kTraceBlock) NOVA_TRACE_OCCUPANCY void extend_kerne
     }
     if (__ballot(have) == 0) break;
     if (have && trace_step<kCount>(tc, stk, ts, 0.000001f, nv, nt)) {
-      q.hit_prim[ray] = ts.best;
-      q.hit_t[ray] = ts.tbest;
-      q.hit_u[ray] = ts.bu;
-      q.hit_v[ray] = ts.bv;
+      q.hit[ray] = make_float4(__builtin_bit_cast(float, (int32_t)(ts.best)), ts.tbest, ts.bu, ts.bv);
       have = false;
     }
   }
@@ -139,10 +135,7 @@ __global__ __launch_bounds__(kTraceBlock) NOVA_TRACE_OCCUPANCY void extend_kerne
     const f3 d = mk3(r[3 * cap + i], r[4 * cap + i], r[5 * cap + i]);
     float t, u, v;
     const int prim = trace_closest<kCount>(tc, stk, o, d, 0.000001f, t, u, v, nv, nt);
-    q.hit_prim[i] = prim;
-    q.hit_t[i] = t;
-    q.hit_u[i] = u;
-    q.hit_v[i] = v;
+    q.hit[i] = make_float4(__builtin_bit_cast(float, (int32_t)(prim)), t, u, v);
   }
 #endif
   if (kCount) {
@@ -181,10 +174,7 @@ __global__ __launch_bounds__(kCoherentBlock) NOVA_COHERENT_OCCUPANCY void extend
                valid ? mk3(r[3 * cap + i], r[4 * cap + i], r[5 * cap + i]) : mk3(0.f, 0.f, 1.f));
     coherent_walk<kCount>(tc, ts, __ballot(valid), 0.000001f, nv, nt);
     if (valid) {
-      q.hit_prim[i] = ts.best;
-      q.hit_t[i] = ts.tbest;
-      q.hit_u[i] = ts.bu;
-      q.hit_v[i] = ts.bv;
+      q.hit[i] = make_float4(__builtin_bit_cast(float, (int32_t)(ts.best)), ts.tbest, ts.bu, ts.bv);
     }
   }
   if (kCount) {
@@ -262,16 +252,18 @@ __global__ __launch_bounds__(kShadeBlock) NOVA_SHADE_OCCUPANCY void shade_kernel
           hu = h.z;
           hv = h.w;
         } else {
-          prim = q.hit_prim[i];
-          ht = q.hit_t[i];
-          hu = q.hit_u[i];
-          hv = q.hit_v[i];
+          const float4 h = q.hit[i];
+          prim = __builtin_bit_cast(int32_t, h.x);
+          ht = h.y;
+          hu = h.z;
+          hv = h.w;
         }
       } else {
-        prim = q.hit_prim[i];
-        ht = q.hit_t[i];
-        hu = q.hit_u[i];
-        hv = q.hit_v[i];
+        const float4 h = q.hit[i];
+        prim = __builtin_bit_cast(int32_t, h.x);
+        ht = h.y;
+        hu = h.z;
+        hv = h.w;
         if (adj.hit_cache && (uint32_t)sp.bounce < adj.cache_bounces)
           adj.hit_cache[(size_t)sp.bounce * adj.cache_paths + pid] = make_float4(__builtin_bit_cast(float, prim), ht, hu, hv);
       }
@@ -531,7 +523,7 @@ static int render_impl(nova_hip_ctx *ctx, const nova_hip_render_params *p, float
   if (!rays) return check_hip(ctx, e, "queue alloc");
   int32_t *slots = (int32_t *)ensure_scratch(ctx, 1, cap * 2 * sizeof(int32_t), &e);
   if (!slots) return check_hip(ctx, e, "queue alloc");
-  uint32_t *counts = (uint32_t *)ensure_scratch(ctx, 2, 64, &e);
+  uint32_t *counts = (uint32_t *)ensure_scratch(ctx, 2, 128, &e);
   if (!counts) return check_hip(ctx, e, "counter alloc");
   unsigned long long *stats64 = (unsigned long long *)(counts + 8);
   float *hits = (float *)ensure_scratch(ctx, 3, cap * 4 * sizeof(float), &e);
@@ -546,14 +538,13 @@ static int render_impl(nova_hip_ctx *ctx, const nova_hip_render_params *p, float
   q.slot[1] = slots + cap;
   q.count = counts;
   q.stats64 = stats64;
-  q.hit_prim = (int32_t *)hits;
-  q.hit_t = hits + cap;
-  q.hit_u = hits + 2 * cap;
-  q.hit_v = hits + 3 * cap;
+  q.hit = reinterpret_cast<float4 *>(hits);
   q.T = state;
   q.L = state + 3 * cap;
   q.A = state + 6 * cap;
   q.partial = partial;
+  Queues q0 = q;  // camera-ray launches count their node visits and triangle tests apart
+  q0.stats64 = stats64 + 4;
 
   const int cus = device_cus(ctx);
   const uint32_t trace_blocks = (uint32_t)std::min<size_t>((cap + kTraceBlock - 1) / kTraceBlock, (size_t)cus * 8);
@@ -586,7 +577,7 @@ static int render_impl(nova_hip_ctx *ctx, const nova_hip_render_params *p, float
   }
   const uint32_t shade_blocks[2] = {(uint32_t)std::min<size_t>((cap + kShadeBlock - 1) / kShadeBlock, (size_t)cus * shade_per_cu[0]),
                                     (uint32_t)std::min<size_t>((cap + kShadeBlock - 1) / kShadeBlock, (size_t)cus * shade_per_cu[1])};
-  NOVA_HIP_CHECK(ctx, hipMemsetAsync(counts, 0, 64, s));
+  NOVA_HIP_CHECK(ctx, hipMemsetAsync(counts, 0, 128, s));
 
   const bool timing = (p->flags & NOVA_HIP_RENDER_TIMING) != 0;
   const bool counting = (p->flags & NOVA_HIP_RENDER_COUNT_TRAVERSAL) != 0;
@@ -602,7 +593,7 @@ static int render_impl(nova_hip_ctx *ctx, const nova_hip_render_params *p, float
   NOVA_HIP_CHECK(ctx, hipEventCreate(&ev0));
   NOVA_HIP_CHECK(ctx, hipEventCreate(&ev1));
   NOVA_HIP_CHECK(ctx, hipEventRecord(ev0, s));
-  uint64_t launches = 0;
+  uint64_t launches = 0, camera_rays = 0;
   for (int f0 = p->frame_start; f0 < p->frame_start + p->frame_count; f0 += fb) {
     const int frames = std::min(fb, p->frame_start + p->frame_count - f0);
     FrameParams fp;
@@ -618,6 +609,7 @@ static int render_impl(nova_hip_ctx *ctx, const nova_hip_render_params *p, float
       fp.inv_V[i] = p->inv_V[i];
     }
     const uint32_t active = (uint32_t)(npix * frames);
+    camera_rays += (uint64_t)active * (adj ? 2 : 1);
     for (int pass = 0; pass < (adj ? 2 : 1); pass++) {  // pass 1: the adjoint replay of pass 0's paths
     fp.frame_minor = pass == 0 ? 1 : 0;
     if (pass == 0 && replay.hit_cache)  // every entry starts as a miss (prim -1)
@@ -642,11 +634,11 @@ static int render_impl(nova_hip_ctx *ctx, const nova_hip_render_params *p, float
       } else if (k == 0 && coherent0) {
         const uint32_t cblocks = std::min<uint32_t>((active + kCoherentBlock - 1) / kCoherentBlock, (uint32_t)cus * NOVA_COHERENT_BLOCKS_PER_CU);
         if (counting)
-          hipLaunchKernelGGL(extend_coherent_kernel<true>, dim3(cblocks), dim3(kCoherentBlock), 0, s, tc, q, which);
+          hipLaunchKernelGGL(extend_coherent_kernel<true>, dim3(cblocks), dim3(kCoherentBlock), 0, s, tc, q0, which);
         else
           hipLaunchKernelGGL(extend_coherent_kernel<false>, dim3(cblocks), dim3(kCoherentBlock), 0, s, tc, q, which);
       } else if (counting) {
-        hipLaunchKernelGGL(extend_kernel<true>, dim3(trace_blocks), dim3(kTraceBlock), 0, s, tc, q, which);
+        hipLaunchKernelGGL(extend_kernel<true>, dim3(trace_blocks), dim3(kTraceBlock), 0, s, tc, k == 0 ? q0 : q, which);
       } else {
         hipLaunchKernelGGL(extend_kernel<false>, dim3(trace_blocks), dim3(kTraceBlock), 0, s, tc, q, which);
       }
@@ -683,16 +675,17 @@ static int render_impl(nova_hip_ctx *ctx, const nova_hip_render_params *p, float
   hipEventDestroy(ev0);
   hipEventDestroy(ev1);
   uint32_t seg = 0;
-  unsigned long long s64[3] = {0, 0, 0};
+  unsigned long long s64[6] = {0, 0, 0, 0, 0, 0};
   NOVA_HIP_CHECK(ctx, hipMemcpy(&seg, counts + 2, sizeof(uint32_t), hipMemcpyDeviceToHost));
   NOVA_HIP_CHECK(ctx, hipMemcpy(s64, stats64, sizeof(s64), hipMemcpyDeviceToHost));
-  float ext_ms = 0.f, shade_ms = 0.f;
+  float ext_ms = 0.f, shade_ms = 0.f, ext0_ms = 0.f;
   for (size_t k = 0; k + 3 < kev.size(); k += 4) {
     float a = 0.f, b = 0.f;
     hipEventElapsedTime(&a, kev[k], kev[k + 1]);
     hipEventElapsedTime(&b, kev[k + 2], kev[k + 3]);
     ext_ms += a;
     shade_ms += b;
+    if ((k / 4) % (size_t)(p->max_depth + 1) == 0) ext0_ms += a;  // bounce 0 of a batch pass
   }
   for (hipEvent_t ev : kev) hipEventDestroy(ev);
   ctx->stats.total_ms = ms;
@@ -701,9 +694,13 @@ static int render_impl(nova_hip_ctx *ctx, const nova_hip_render_params *p, float
   ctx->stats.other_ms = timing ? ms - ext_ms - shade_ms : 0.f;
   ctx->stats.segments = seg;
   ctx->stats.extend_launches = launches;
-  ctx->stats.node_visits = counting ? s64[0] : 0;
-  ctx->stats.triangle_tests = counting ? s64[1] : 0;
+  ctx->stats.node_visits = counting ? s64[0] + s64[4] : 0;
+  ctx->stats.triangle_tests = counting ? s64[1] + s64[5] : 0;
   ctx->stats.escaped = s64[2];
+  ctx->stats.extend_first_ms = timing ? ext0_ms : 0.f;
+  ctx->stats.camera_rays = camera_rays;
+  ctx->stats.node_visits_first = counting ? s64[4] : 0;
+  ctx->stats.triangle_tests_first = counting ? s64[5] : 0;
   return NOVA_HIP_SUCCESS;
 }
 

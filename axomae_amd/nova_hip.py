@@ -108,7 +108,9 @@ ADJOINT_RETRACE = 8  # the adjoint replay traces again instead of reusing the pr
 
 class RenderStats(C.Structure):
     _fields_ = [("total_ms", C.c_float), ("extend_ms", C.c_float), ("shade_ms", C.c_float), ("other_ms", C.c_float), ("segments", C.c_uint64),
-                ("extend_launches", C.c_uint64), ("node_visits", C.c_uint64), ("triangle_tests", C.c_uint64), ("escaped", C.c_uint64)]
+                ("extend_launches", C.c_uint64), ("node_visits", C.c_uint64), ("triangle_tests", C.c_uint64), ("escaped", C.c_uint64),
+                ("extend_first_ms", C.c_float), ("camera_rays", C.c_uint64), ("node_visits_first", C.c_uint64),
+                ("triangle_tests_first", C.c_uint64)]
 
 
 BNODE_DTYPE = np.dtype([("min", np.float32, 3), ("max", np.float32, 3), ("left", np.int32), ("count", np.int32)])

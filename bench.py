@@ -1,23 +1,24 @@
 """Benchmark of the Nova hot path on one MI355X.
 
 Workload: the 1,048,576-triangle scene (512x512-cell fBm terrain + 64 octahedral
-spheres), 1920x1080, depth 8, Russian roulette from the 4th scatter, FP32 sky.
-A step is one frame: one path per pixel (the reference renders 1 spp per frame and
-accumulates). The timed region renders K frames (frames 0..K-1) after W warm-up
-frames; value = width*height*K / t / 1e6 Msamples/s. Scene upload and BVH build are
-outside the timed region and reported separately.
+spheres), 1920x1080, 64 spp, depth 8, Russian roulette from the 4th scatter, FP32 sky.
+A step is the whole job: 64 frames of one path per pixel (the reference renders 1 spp
+per frame and accumulates), 132,710,400 paths. The timed region renders K jobs after W
+warm-up jobs; value = width*height*64*K / t / 1e6 Msamples/s, and the per-job device
+times give the median. Scene upload and BVH build are outside the timed region and
+reported separately.
 
-roofline: the dominant kernel is the extend (closest-hit traversal) step: one walk per wave
-for the camera rays, one walk per ray for the later bounces. Its
-algorithmic bytes per ray are 48 (ray in, hit out) + 64 per 4-wide node visited (the
-whole 64-byte node: grid origin and steps, 8-bit child planes, child references) + 48 per
-triangle tested, with
-visits/tests counted by an instrumented render of the same
-frames; its duration comes from HIP events the library records around every extend
-launch on the render stream during the timed region.
+roofline: the dominant kernel is the per-ray closest-hit walk (extend_kernel, bounces 1-8;
+camera rays take the per-wave walk, extend_coherent_kernel). Its algorithmic bytes per ray
+are 48 (ray in, hit out) + 64 per 4-wide node visited (the whole 64-byte node) + 48 per
+triangle tested, with visits and tests counted by an instrumented render of the same job;
+its duration comes from HIP events the library records around every extend launch on the
+render stream during the timed region. The kernel is bound by VALU issue, not by HBM
+(rocprof counters: the per-launch VALU instructions and busy fraction are reported beside
+the byte rate, from the counter summary in profiles/).
 
-cpu_baseline: the oracle (the CPU restatement of the reference renderer) on the host
-cores, timed on whole frames 1, 2, ... of the same workload.
+cpu_baseline: the oracle (the CPU restatement of the reference renderer) on the host's
+CPUs, timed on whole frames 1, 2, ... of the same workload.
 """
 import argparse
 import json
@@ -37,24 +38,45 @@ def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
-def host_cores():
+def host_info():
+    """The host CPUs this process may use: affinity set, cgroup CPU quota and model."""
+    info = {"nproc": os.cpu_count()}
     try:
-        n = len(os.sched_getaffinity(0))
+        info["affinity"] = len(os.sched_getaffinity(0))
     except AttributeError:
-        n = os.cpu_count() or 1
-    return max(1, min(n, int(os.environ.get("OMP_NUM_THREADS", n)), 16))
+        info["affinity"] = os.cpu_count()
+    try:
+        quota, period = open("/sys/fs/cgroup/cpu.max").read().split()[:2]
+        info["cgroup_cpus"] = None if quota == "max" else round(int(quota) / int(period), 2)
+    except (OSError, ValueError):
+        info["cgroup_cpus"] = None
+    try:
+        info["model"] = next(l.split(":", 1)[1].strip() for l in open("/proc/cpuinfo") if l.startswith("model name"))
+    except (OSError, StopIteration):
+        info["model"] = None
+    return info
+
+
+def host_threads(info):
+    """One worker thread per CPU this process may run on: the affinity set, capped by the
+    cgroup quota when there is one."""
+    n = info["affinity"] or 1
+    if info.get("cgroup_cpus"):
+        n = min(n, max(1, int(info["cgroup_cpus"])))
+    return max(1, n)
 
 
 def cpu_baseline(scene, width, height, depth, rr_start, target_s):
     """The oracle's tile-parallel CPU renderer on whole frames of the same workload (frames
-    1, 2, ... until about target_s seconds of rendering), on up to 16 host threads."""
+    1, 2, ... until about target_s seconds of rendering), one thread per usable host CPU."""
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import pyoracle
-    cores = host_cores()
+    info = host_info()
+    cores = host_threads(info)
     t0 = time.perf_counter()
     osc = pyoracle.OracleScene(scene)
     build_s = time.perf_counter() - t0
-    log(f"[cpu] oracle scene + reference BVH build: {build_s:.1f} s")
+    log(f"[cpu] oracle scene + reference BVH build: {build_s:.1f} s; host {info}")
     frames, dt = 0, 0.0
     while dt < target_s and frames < 64:
         t0 = time.perf_counter()
@@ -63,20 +85,22 @@ def cpu_baseline(scene, width, height, depth, rr_start, target_s):
         frames += 1
         log(f"[cpu] frame {frames}: {dt:.1f} s")
     paths = frames * width * height
-    return {"value": paths / dt / 1e6, "unit": "Msamples/s", "cores": cores, "kind": "port",
+    return {"value": paths / dt / 1e6, "unit": "Msamples/s", "cores": cores, "kind": "port", "host": info,
             "sample": f"frames 1..{frames} in full ({paths} paths, 1 spp each, depth {depth}, RR from scatter {rr_start}) of the same "
-                      f"workload, {dt:.1f} s on {cores} threads; oracle BVH build {build_s:.1f} s excluded"}
+                      f"workload, {dt:.1f} s on {cores} threads (every CPU the process may use); oracle BVH build {build_s:.1f} s excluded"}
 
 
 def adjoint_leg(width=1280, height=720, frames=32, depth=4):
     """Primal and path-replay adjoint throughput on the differentiation workload: ~100k
     triangles, FP32 512x512 albedo and roughness textures, L2 loss against a render of
     perturbed textures (sigma 0.05). Adjoint time covers the whole backward call (its
-    per-batch primal re-render and the replay)."""
+    per-batch primal re-render and the replay), median of three calls. The default replay
+    reuses the primal pass's hits of its leading bounces from a cache of at most 1 GiB; the
+    recompute-only replay (ADJOINT_RETRACE) traces every bounce again and stores nothing."""
     import numpy as np
     import torch
     from axomae_amd import scenes
-    from axomae_amd.nova_hip import HipEngine
+    from axomae_amd.nova_hip import ADJOINT_RETRACE, HipEngine
     sc, alb, rough = scenes.differentiable_scene(width=width, height=height)
     eng = HipEngine(0)
     eng.upload(sc)
@@ -90,28 +114,36 @@ def adjoint_leg(width=1280, height=720, frames=32, depth=4):
     eng.update_texture(alb, torch.from_numpy(sc.textures[alb].data))
     eng.update_texture(rough, torch.from_numpy(sc.textures[rough].data))
     eng.render(width, height, 1, depth)  # warm-up
-    torch.cuda.synchronize()
-    t0 = time.perf_counter()
-    img = eng.render(width, height, frames, depth)[0]
-    torch.cuda.synchronize()
-    t_primal = time.perf_counter() - t0
+
+    def median_time(fn):
+        runs = []
+        for _ in range(3):
+            torch.cuda.synchronize()
+            t0 = time.perf_counter()
+            out = fn()
+            torch.cuda.synchronize()
+            runs.append(time.perf_counter() - t0)
+        return sorted(runs)[1], out
+
+    t_primal, img = median_time(lambda: eng.render(width, height, frames, depth)[0])
     dimg = 2 * (img - target)
     dimg[..., 3] = 0
     loss = float(((img[..., :3].double() - target[..., :3].double()) ** 2).sum())
     eng.adjoint(width, height, 1, depth, dimg, 1.0)  # warm-up
-    torch.cuda.synchronize()
-    t0 = time.perf_counter()
-    grad = eng.adjoint(width, height, frames, depth, dimg, 1.0 / frames)
-    torch.cuda.synchronize()
-    t_adj = time.perf_counter() - t0
+    t_adj, grad = median_time(lambda: eng.adjoint(width, height, frames, depth, dimg, 1.0 / frames))
+    t_re, _ = median_time(lambda: eng.adjoint(width, height, frames, depth, dimg, 1.0 / frames, flags=ADJOINT_RETRACE))
     ga, gr = eng.texture_grad(grad, alb), eng.texture_grad(grad, rough)
     paths = width * height * frames
-    primal, adjoint = paths / t_primal / 1e6, paths / t_adj / 1e6
-    log(f"[bench] adjoint leg: primal {t_primal * 1e3:.1f} ms, adjoint {t_adj * 1e3:.1f} ms, loss {loss:.4e}")
+    primal, adjoint, recompute = paths / t_primal / 1e6, paths / t_adj / 1e6, paths / t_re / 1e6
+    cache_bounces = min(depth + 1, (1 << 30) // (paths * 16))
+    log(f"[bench] adjoint leg: primal {t_primal * 1e3:.1f} ms, adjoint {t_adj * 1e3:.1f} ms (hits of {cache_bounces} bounces cached), "
+        f"recompute-only {t_re * 1e3:.1f} ms, loss {loss:.4e}")
     eng.close()
     return {"workload": "differentiable_100k_720p", "triangles": sc.triangle_count, "width": width, "height": height, "spp": frames,
             "max_depth": depth, "textures": "albedo 512x512x3 + roughness 512x512 FP32", "primal_msamples_s": round(primal, 3),
             "adjoint_msamples_s": round(adjoint, 3), "adjoint_vs_primal": round(adjoint / primal, 4),
+            "hit_cache": {"bytes_max": 1 << 30, "bounces_cached": cache_bounces, "bytes": cache_bounces * paths * 16},
+            "recompute_only_msamples_s": round(recompute, 3), "recompute_only_vs_primal": round(recompute / primal, 4),
             "grad_finite": bool(torch.isfinite(ga).all().item() and torch.isfinite(gr).all().item()),
             "grad_nonzero_texels": int((ga.abs().sum(-1) > 0).sum().item() + (gr.abs().sum(-1) > 0).sum().item())}
 
@@ -151,7 +183,7 @@ def bake_leg(samples=1024, cpu=True):
     if cpu:
         sys.path.insert(0, os.path.join(ROOT, "oracle"))
         import pyoracle as po
-        threads = host_cores()
+        threads = host_threads(host_info())
         w, h, d, si, ss = specs[0]
         env = scenes.sky_envmap(w, h, d, si, ss)
         t0 = time.perf_counter()
@@ -207,16 +239,45 @@ def batch_leg(depth=6):
     log(f"[bench] batch: {paths / 1e6:.0f} M paths, batched {t * 1e3:.0f} ms ({st.waves} waves), one by one {t_seq * 1e3:.0f} ms")
     return {"workload": "batch_64_scenes_depth6", "scenes": len(jobs), "triangles": tris, "paths": paths, "max_depth": depth,
             "value": round(paths / t / 1e6, 3), "unit": "Msamples/s", "device_ms": round(st.total_ms, 2), "waves": int(st.waves),
-            "completion_ms_p50": round(float(np.percentile(comp, 50)), 2), "completion_ms_p99": round(float(np.percentile(comp, 99)), 2),
+            "completion_ms_p50": round(float(np.percentile(comp, 50)), 2), "completion_ms_p90": round(float(np.percentile(comp, 90)), 2),
+            "completion_ms_p99": round(float(np.percentile(comp, 99)), 2), "completion_ms_max": round(float(np.max(comp)), 2),
+            "completion_note": "device time from the batch start to the end of each scene's last wave; with 64 scenes p99 lies "
+                               "between the two latest completions (linear interpolation)",
             "order": "shortest first", "autotuned": cfg.as_dict(), "autotune_ns_per_path": round(ns, 3),
             "one_by_one_msamples_s": round(paths / t_seq / 1e6, 3), "finite": finite}
+
+
+def frame_by_frame_leg(eng, sc, W, H, frames, depth, rr_start):
+    """The drop-in's per-frame call: the reference engine draws one 1-spp frame per
+    traverse call (engine.cpp:461-494), so the library is called frames times with
+    frame_count = 1 through nova_hip_render_to_renderbuffer (render, accumulate, swap on the
+    device). Median of three runs of the whole frame loop."""
+    import torch
+    from axomae_amd.nova_hip import RenderBuffer
+    rb = RenderBuffer(eng, W, H)
+    eng.render_to(rb, 1, depth, frame_start=0, rr_start=rr_start)  # warm-up
+    runs = []
+    for _ in range(3):
+        rb.reset()
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for f in range(frames):
+            eng.render_to(rb, 1, depth, frame_start=f, rr_start=rr_start)
+        torch.cuda.synchronize()
+        runs.append(time.perf_counter() - t0)
+    rb.close()
+    t = sorted(runs)[1]
+    log(f"[bench] frame by frame: {frames} calls of 1 frame, {t * 1e3:.1f} ms (runs {', '.join(f'{r * 1e3:.1f}' for r in runs)})")
+    return {"calls": frames, "frame_count_per_call": 1, "entry_point": "nova_hip_render_to_renderbuffer",
+            "ms": round(t * 1e3, 2), "msamples_s": round(W * H * frames / t / 1e6, 3)}
 
 
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=64, help="frames (1 spp each) in the timed region")
-    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=3, help="whole jobs (64 frames of 1 spp each) in the timed region")
+    ap.add_argument("--warmup", type=int, default=1, help="untimed whole jobs before the timed region")
+    ap.add_argument("--frames", type=int, default=64, help="frames (1 spp each) of one job")
     ap.add_argument("--width", type=int, default=1920)
     ap.add_argument("--height", type=int, default=1080)
     ap.add_argument("--depth", type=int, default=8)
@@ -226,6 +287,7 @@ def main():
     ap.add_argument("--no-adjoint", action="store_true", help="skip the adjoint-vs-primal leg (1280x720, 32 spp, depth 4, ~100k triangles)")
     ap.add_argument("--no-bake", action="store_true", help="skip the IBL bake batch leg (32 maps, irradiance + 6 prefilter mips)")
     ap.add_argument("--no-batch", action="store_true", help="skip the 64-scene batched-render leg")
+    ap.add_argument("--no-frame-loop", action="store_true", help="skip the frame-by-frame (frame_count = 1 per call) leg")
     ap.add_argument("--copy-probe-mb", type=int, default=0,
                     help="before the render, copy a buffer of this size 3 times (a copy of known bytes for counter calibration)")
     args = ap.parse_args()
@@ -234,11 +296,12 @@ def main():
               "the multi-GPU sharding of this path is described in DESIGN.md", file=sys.stderr)
         sys.exit(2)
 
+    import numpy as np
     import torch
     from axomae_amd import scenes
     from axomae_amd.nova_hip import HipEngine, RENDER_COUNT_TRAVERSAL, RENDER_TIMING
 
-    W, H, K = args.width, args.height, args.steps
+    W, H, K, F = args.width, args.height, args.steps, args.frames
     if args.copy_probe_mb > 0:
         n = args.copy_probe_mb * (1 << 20) // 4
         src = torch.ones(n, dtype=torch.float32, device="cuda")
@@ -260,47 +323,78 @@ def main():
     log(f"[bench] upload {upload_s * 1e3:.0f} ms, BVH build {info.build_ms:.1f} ms, binary nodes {info.binary_nodes} depth {info.binary_depth}, "
         f"wide nodes {info.wide_nodes} depth {info.wide_depth}, leaves {info.leaves}")
 
-    # traversal counts of the timed frames (deterministic per ray)
-    eng.render(W, H, K, args.depth, rr_start=args.rr_start, flags=RENDER_COUNT_TRAVERSAL)
+    # traversal counts of one job (deterministic per ray: every timed job traces the same rays)
+    eng.render(W, H, F, args.depth, rr_start=args.rr_start, flags=RENDER_COUNT_TRAVERSAL)
     torch.cuda.synchronize()
     cst = eng.render_stats()
-    log(f"[bench] instrumented: segments {cst.segments}, node visits {cst.node_visits}, triangle tests {cst.triangle_tests}, escaped {cst.escaped}")
+    log(f"[bench] instrumented job: segments {cst.segments} (camera {cst.camera_rays}), node visits {cst.node_visits} "
+        f"(camera {cst.node_visits_first}), triangle tests {cst.triangle_tests} (camera {cst.triangle_tests_first}), escaped {cst.escaped}")
 
-    if args.warmup > 0:
-        eng.render(W, H, args.warmup, args.depth, rr_start=args.rr_start)
+    for _ in range(args.warmup):
+        eng.render(W, H, F, args.depth, rr_start=args.rr_start)
     torch.cuda.synchronize()
+    steps = []
     t0 = time.perf_counter()
-    final, accum = eng.render(W, H, K, args.depth, rr_start=args.rr_start, flags=RENDER_TIMING)
+    for _ in range(K):
+        final, accum = eng.render(W, H, F, args.depth, rr_start=args.rr_start, flags=RENDER_TIMING)
+        steps.append(eng.render_stats())
     torch.cuda.synchronize()
     t = time.perf_counter() - t0
-    st = eng.render_stats()
-    log(f"[bench] timed {K} frames: {t * 1e3:.1f} ms (device {st.total_ms:.1f} ms; extend {st.extend_ms:.1f}, shade {st.shade_ms:.1f}, "
-        f"other {st.other_ms:.1f}), segments {st.segments}, extend launches {st.extend_launches}")
+    dev_ms = [s_.total_ms for s_ in steps]
+    ext_ms = sum(s_.extend_ms for s_ in steps)
+    ext0_ms = sum(s_.extend_first_ms for s_ in steps)
+    shade_ms = sum(s_.shade_ms for s_ in steps)
+    launches = sum(s_.extend_launches for s_ in steps)
+    cam_launches = K  # one camera-ray launch per job (the 64 frames are one batch)
+    log(f"[bench] timed {K} jobs of {F} frames: {t * 1e3:.1f} ms (device per job {', '.join(f'{d:.1f}' for d in dev_ms)} ms; "
+        f"extend {ext_ms / K:.1f} (camera walk {ext0_ms / K:.1f}), shade {shade_ms / K:.1f} ms per job)")
     finite = bool(torch.isfinite(final).all().item())
 
-    paths = W * H * K
-    value = paths / t / 1e6
-    ext_bytes = 48.0 * cst.segments + NODE_BYTES * cst.node_visits + 48.0 * cst.triangle_tests
-    ext_s = st.extend_ms / 1e3
-    achieved = ext_bytes / ext_s / 1e9 if ext_s > 0 else 0.0
-    per_launch = ext_bytes / max(st.extend_launches, 1)
+    paths = W * H * F
+    value = paths * K / t / 1e6
+    med_ms = float(np.median(dev_ms))
+    # the per-ray walk (bounces 1..depth): its rays, visits and tests are the job's minus the camera rays'
+    seg_r = cst.segments - cst.camera_rays
+    vis_r = cst.node_visits - cst.node_visits_first
+    tst_r = cst.triangle_tests - cst.triangle_tests_first
+    launches_r = (launches - cam_launches) // K
+    bytes_r = (48.0 * seg_r + NODE_BYTES * vis_r + 48.0 * tst_r) / launches_r
+    avg_r_ms = (ext_ms - ext0_ms) / (launches - cam_launches)
+    achieved = bytes_r / (avg_r_ms / 1e3) / 1e9
+    bytes_c = 48.0 * cst.camera_rays + NODE_BYTES * cst.node_visits_first + 48.0 * cst.triangle_tests_first
+    avg_c_ms = ext0_ms / cam_launches
     # whole-path algorithmic bytes: 48 + 424 per segment + node/triangle bytes + 20 per escape + 16 per path
     path_bytes = 472.0 * cst.segments + NODE_BYTES * cst.node_visits + 48.0 * cst.triangle_tests + 20.0 * cst.escaped + 16.0 * paths
-    traffic = None
-    tfile = os.path.join(ROOT, "profiles", "extend_hbm_traffic.json")
-    if os.path.exists(tfile):
-        traffic = json.load(open(tfile)).get("bytes_per_launch")
+    counters = None
+    cfile = os.path.join(ROOT, "profiles", "extend_counters.json")
+    if os.path.exists(cfile):
+        counters = json.load(open(cfile)).get("extend_kernel")
+    traffic, valu = None, None
+    if counters:
+        traffic = counters.get("hbm_bytes_per_launch")
+        valu = {"valu_busy": counters.get("valu_busy"),
+                "valu_insts_per_launch": counters.get("sq_insts_valu_per_launch"),
+                "valu_lane_insts_per_node_visit": round(counters["sq_insts_valu_per_launch"] * 64.0 / (vis_r / launches_r), 1)
+                if counters.get("sq_insts_valu_per_launch") else None,
+                "source": "profiles/extend_counters.json (rocprofv3 --pmc on this bench command's workload)"}
 
     out = {
         "metric": METRIC, "value": round(value, 3), "unit": "Msamples/s", "n_gpus": 1, "steps": K, "warmup": args.warmup,
         "ms_per_step": round(t * 1e3 / K, 3), "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f32",
         "data": "synthetic",
-        "config": {"workload": "million_triangles_1080p", "triangles": sc.triangle_count, "width": W, "height": H, "spp": K,
-                   "max_depth": args.depth, "russian_roulette_from_scatter": args.rr_start, "sky": "2048x1024 RGBA32F"},
+        "config": {"workload": "million_triangles_1080p", "triangles": sc.triangle_count, "width": W, "height": H, "spp": F,
+                   "step": f"one whole job: {F} frames of 1 spp", "max_depth": args.depth, "russian_roulette_from_scatter": args.rr_start,
+                   "sky": "2048x1024 RGBA32F"},
+        "median_job_ms": round(med_ms, 3), "median_job_msamples_s": round(paths / med_ms / 1e3, 3),
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic, "kernel": "extend (extend_coherent_kernel for camera rays, extend_kernel for bounces 1-8)",
-                     "algorithmic_bytes_per_launch": round(per_launch), "avg_launch_ms": round(st.extend_ms / max(st.extend_launches, 1), 4)},
-        "path_algorithmic_GBs": round(path_bytes / t / 1e9, 1),
+                     "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
+                     "kernel": "extend_kernel<false> (per-ray closest-hit walk, bounces 1-8)",
+                     "algorithmic_bytes_per_launch": round(bytes_r), "avg_launch_ms": round(avg_r_ms, 4),
+                     "measured_limiter": "VALU issue (not HBM): see valu", "valu": valu,
+                     "camera_walk": {"kernel": "extend_coherent_kernel<false> (bounce 0, one walk per wave)",
+                                     "algorithmic_bytes_per_launch": round(bytes_c), "avg_launch_ms": round(avg_c_ms, 4),
+                                     "achieved_GBs": round(bytes_c / (avg_c_ms / 1e3) / 1e9, 1) if avg_c_ms > 0 else None}},
+        "path_algorithmic_GBs": round(path_bytes * K / t / 1e9, 1),
         "bvh": {"build_ms": round(info.build_ms, 2), "binary_nodes": info.binary_nodes, "binary_depth": info.binary_depth,
                 "wide_nodes": info.wide_nodes, "wide_depth": info.wide_depth, "leaves": info.leaves, "upload_ms": round(upload_s * 1e3, 1)},
         "per_segment": {"node_visits": round(cst.node_visits / max(cst.segments, 1), 2),
@@ -308,6 +402,8 @@ def main():
                         "segments_per_path": round(cst.segments / paths, 3)},
         "image_finite": finite,
     }
+    if not args.no_frame_loop:
+        out["frame_by_frame"] = frame_by_frame_leg(eng, sc, W, H, F, args.depth, args.rr_start)
     eng.close()
     if not args.no_adjoint:
         out["adjoint"] = adjoint_leg()

@@ -202,7 +202,8 @@ int nova_hip_render_frames(nova_hip_ctx *ctx, const nova_hip_render_params *para
 /* Statistics of the last nova_hip_render_frames call: device time of the whole call and,
  * with NOVA_HIP_RENDER_TIMING, summed extend (traversal) and shade kernel times in
  * milliseconds; path segments traced (rays entering an extend launch) and launches; with
- * NOVA_HIP_RENDER_COUNT_TRAVERSAL, 4-wide node visits and triangle tests summed over rays. */
+ * NOVA_HIP_RENDER_COUNT_TRAVERSAL, 4-wide node visits and triangle tests summed over rays.
+ * The *_first fields and camera_rays cover the camera-ray (bounce 0) launches alone. */
 typedef struct {
   float total_ms, extend_ms, shade_ms, other_ms;
   uint64_t segments;
@@ -210,6 +211,9 @@ typedef struct {
   uint64_t node_visits;
   uint64_t triangle_tests;
   uint64_t escaped;          /* segments whose ray left the scene (envmap lookups after a miss) */
+  float extend_first_ms;     /* with NOVA_HIP_RENDER_TIMING: the camera-ray extend launches */
+  uint64_t camera_rays;      /* rays entering the camera-ray launches */
+  uint64_t node_visits_first, triangle_tests_first; /* with NOVA_HIP_RENDER_COUNT_TRAVERSAL */
 } nova_hip_render_stats;
 int nova_hip_render_stats_get(nova_hip_ctx *ctx, nova_hip_render_stats *out);
 
