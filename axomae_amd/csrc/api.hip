@@ -93,6 +93,9 @@ int nova_hip_create(int device, nova_hip_ctx **out) {
   if (e != hipSuccess) return NOVA_HIP_DEVICE_ERROR;
   nova_hip_ctx *ctx = new nova_hip_ctx();
   ctx->device = device;
+  // the no-allocation entry point's scratch address on this device (its status is reported
+  // by that entry point; the others do not use it)
+  noalloc_scratch_setup(device);
   e = hipMalloc(&ctx->sobol_table, sizeof(kSobolTable));
   if (e == hipSuccess) e = hipMemcpy(ctx->sobol_table, kSobolTable, sizeof(kSobolTable), hipMemcpyHostToDevice);
   if (e != hipSuccess) {

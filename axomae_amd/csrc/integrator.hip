@@ -16,6 +16,7 @@
 // L unchanged as soon as c_k has a non-finite component (V_{k+1} is always finite).
 // Alpha follows the same recursion (1 per scattering level plus the terminal alpha).
 #include <algorithm>
+#include <mutex>
 #include <vector>
 
 #include "device_math.h"
@@ -399,6 +400,67 @@ static int ensure_gstack(nova_hip_ctx *ctx, uint32_t threads) {
   return NOVA_HIP_SUCCESS;
 }
 
+// ------------------------------------------------------------ no-allocation scratch
+// Device memory of nova_hip_render_frames_noalloc: queues, hits, path state, radiance and
+// traversal overflow stacks of one batch, carved from a fixed array instead of grown with
+// hipMalloc. Each device the library runs on holds its own copy until the process ends.
+constexpr size_t kNoAllocScratchBytes = (size_t)320 << 20;
+static __device__ __align__(256) unsigned char scratch[kNoAllocScratchBytes];
+
+constexpr int kMaxDevices = 64;
+static std::mutex g_scratch_mutex;
+static void *g_scratch_ptr[kMaxDevices];
+static hipError_t g_scratch_err[kMaxDevices];
+static bool g_scratch_done[kMaxDevices];
+
+// Once per device, with that device current: the device address of `scratch` (host code
+// must not use the array's name as an address).
+int noalloc_scratch_setup(int device) {
+  if (device < 0 || device >= kMaxDevices) return NOVA_HIP_INVALID_ARGUMENT;
+  std::lock_guard<std::mutex> lock(g_scratch_mutex);
+  if (!g_scratch_done[device]) {
+    void *ptr = nullptr;
+    g_scratch_err[device] = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+    g_scratch_ptr[device] = g_scratch_err[device] == hipSuccess ? ptr : nullptr;
+    g_scratch_done[device] = true;
+  }
+  return g_scratch_ptr[device] ? NOVA_HIP_SUCCESS : NOVA_HIP_DEVICE_ERROR;
+}
+
+static void *noalloc_scratch_of(int device) {
+  if (device < 0 || device >= kMaxDevices) return nullptr;
+  std::lock_guard<std::mutex> lock(g_scratch_mutex);
+  return g_scratch_done[device] ? g_scratch_ptr[device] : nullptr;
+}
+
+// Counters start at zero through a kernel, not a memset, so the call can be captured.
+__global__ void zero_words_kernel(uint32_t *p, uint32_t n) {
+  for (uint32_t i = threadIdx.x; i < n; i += blockDim.x) p[i] = 0u;
+}
+
+// Bump allocator over the fixed array (256-byte aligned pieces).
+struct ScratchCarve {
+  unsigned char *base;
+  size_t used = 0, limit = 0;
+  void *take(size_t bytes) {
+    const size_t at = (used + 255) / 256 * 256;
+    if (at + bytes > limit) return nullptr;
+    used = at + bytes;
+    return base + at;
+  }
+};
+
+// Bytes a batch of `cap` paths needs in the fixed array: the queues and path state (116 B
+// per path), the counters and the traversal overflow stacks of `threads` lanes.
+static size_t noalloc_bytes(size_t cap, uint32_t threads, uint32_t stack_bound) {
+  const size_t per_path[] = {cap * 12 * sizeof(float), cap * 2 * sizeof(int32_t), cap * 4 * sizeof(float), cap * 7 * sizeof(float),
+                             cap * 4 * sizeof(float)};
+  size_t total = 256 + 128;  // counters
+  for (size_t b : per_path) total += (b + 255) / 256 * 256;
+  const size_t g = stack_bound > (uint32_t)kLdsStack ? (size_t)(stack_bound - kLdsStack) * threads : 1;
+  return total + (g * sizeof(uint32_t) + 255) / 256 * 256;
+}
+
 }  // namespace nova_hip
 
 using namespace nova_hip;
@@ -479,8 +541,11 @@ static int render_pass(nova_hip_ctx *ctx, const nova_hip_render_params *p, int p
 }
 
 // Renders (adj == nullptr) or renders and replays for the adjoint, batch by batch.
+// With noalloc (the device address of the fixed scratch array) nothing is allocated, no
+// event is recorded and nothing waits: the work is ordered on the stream and the render
+// statistics are left as they were.
 static int render_impl(nova_hip_ctx *ctx, const nova_hip_render_params *p, float *accum_dev, float *final_dev, float *partial_dev, void *stream,
-                       const AdjointCtx *adj) {
+                       const AdjointCtx *adj, unsigned char *noalloc = nullptr) {
   if (!ctx || !p) return NOVA_HIP_INVALID_ARGUMENT;
   if (!ctx->bvh_ready) return set_error(ctx, NOVA_HIP_SCENE_NOT_PROCESSED, "scene/BVH not built");
   if (p->width <= 0 || p->height <= 0 || p->width > 65536 || p->height > 65536)
@@ -514,23 +579,41 @@ static int render_impl(nova_hip_ctx *ctx, const nova_hip_render_params *p, float
   // state each, 15.6 GB at the limit). Large batches matter because the last bounces hold
   // few rays whose launches cost their slowest ray's latency, a fixed cost per batch.
   int fb = (int)std::max<size_t>(1, std::min<size_t>((size_t)p->frame_count, kPathsPerBatch / npix));
+  const int cus = device_cus(ctx);
+  auto trace_threads_of = [&](size_t c) { return (uint32_t)std::min<size_t>((c + kTraceBlock - 1) / kTraceBlock, (size_t)cus * 8) * kTraceBlock; };
+  if (noalloc) {
+    // as many frames per batch as the fixed array holds
+    while (fb > 0 && noalloc_bytes(npix * fb, trace_threads_of(npix * fb), traversal_stack_bound(ctx)) > kNoAllocScratchBytes) fb--;
+    if (fb == 0)
+      return set_error(ctx, NOVA_HIP_INVALID_ARGUMENT,
+                       "image of " + std::to_string(npix) + " pixels needs " +
+                           std::to_string(noalloc_bytes(npix, trace_threads_of(npix), traversal_stack_bound(ctx))) +
+                           " bytes of scratch per frame; the no-allocation entry point holds " + std::to_string(kNoAllocScratchBytes));
+  }
   const size_t cap = npix * fb;
   if (cap >= (1u << 31)) return set_error(ctx, NOVA_HIP_INVALID_ARGUMENT, "image too large");
-  hipError_t e;
+  hipError_t e = hipSuccess;
+  ScratchCarve carve{noalloc, 0, kNoAllocScratchBytes};
+  auto piece = [&](int slot, size_t bytes) -> void * {
+    if (!noalloc) return ensure_scratch(ctx, slot, bytes, &e);
+    void *r = carve.take(bytes);
+    if (!r) e = hipErrorOutOfMemory;
+    return r;
+  };
   Queues q;
   q.cap = (uint32_t)cap;
-  float *rays = (float *)ensure_scratch(ctx, 0, cap * 12 * sizeof(float), &e);
+  float *rays = (float *)piece(0, cap * 12 * sizeof(float));
   if (!rays) return check_hip(ctx, e, "queue alloc");
-  int32_t *slots = (int32_t *)ensure_scratch(ctx, 1, cap * 2 * sizeof(int32_t), &e);
+  int32_t *slots = (int32_t *)piece(1, cap * 2 * sizeof(int32_t));
   if (!slots) return check_hip(ctx, e, "queue alloc");
-  uint32_t *counts = (uint32_t *)ensure_scratch(ctx, 2, 128, &e);
+  uint32_t *counts = (uint32_t *)piece(2, 128);
   if (!counts) return check_hip(ctx, e, "counter alloc");
   unsigned long long *stats64 = (unsigned long long *)(counts + 8);
-  float *hits = (float *)ensure_scratch(ctx, 3, cap * 4 * sizeof(float), &e);
+  float *hits = (float *)piece(3, cap * 4 * sizeof(float));
   if (!hits) return check_hip(ctx, e, "hit alloc");
-  float *state = (float *)ensure_scratch(ctx, 4, cap * 7 * sizeof(float), &e);
+  float *state = (float *)piece(4, cap * 7 * sizeof(float));
   if (!state) return check_hip(ctx, e, "state alloc");
-  float *partial = (float *)ensure_scratch(ctx, 5, cap * 4 * sizeof(float), &e);
+  float *partial = (float *)piece(5, cap * 4 * sizeof(float));
   if (!partial) return check_hip(ctx, e, "partial alloc");
   q.ray[0] = rays;
   q.ray[1] = rays + cap * 6;
@@ -546,12 +629,19 @@ static int render_impl(nova_hip_ctx *ctx, const nova_hip_render_params *p, float
   Queues q0 = q;  // camera-ray launches count their node visits and triangle tests apart
   q0.stats64 = stats64 + 4;
 
-  const int cus = device_cus(ctx);
-  const uint32_t trace_blocks = (uint32_t)std::min<size_t>((cap + kTraceBlock - 1) / kTraceBlock, (size_t)cus * 8);
-  const uint32_t trace_threads = trace_blocks * kTraceBlock;
-  st = ensure_gstack(ctx, trace_threads);
-  if (st) return st;
-  TraceCtx tc{ctx->nodes4, ctx->tri_leaf, ctx->gstack, trace_threads, ctx->tie};
+  const uint32_t trace_threads = trace_threads_of(cap);
+  const uint32_t trace_blocks = trace_threads / kTraceBlock;
+  uint32_t *gstack = nullptr;
+  if (noalloc) {
+    const uint32_t bound = traversal_stack_bound(ctx);
+    gstack = (uint32_t *)carve.take((bound > (uint32_t)kLdsStack ? (size_t)(bound - kLdsStack) * trace_threads : 1) * sizeof(uint32_t));
+    if (!gstack) return set_error(ctx, NOVA_HIP_INVALID_ARGUMENT, "traversal stacks do not fit the no-allocation scratch");
+  } else {
+    st = ensure_gstack(ctx, trace_threads);
+    if (st) return st;
+    gstack = ctx->gstack;
+  }
+  TraceCtx tc{ctx->nodes4, ctx->tri_leaf, gstack, trace_threads, ctx->tie};
   ShadeCtx sc{ctx->textures, ctx->tex_u32, ctx->tex_f32, ctx->materials, ctx->mesh_xf, ctx->shade, ctx->acos_lut, ctx->envmap,
               p->sampling_roughness_texture};
   AdjointCtx replay = adj ? AdjointCtx{partial, adj->dimg, adj->scale, adj->grad, nullptr, 0u, 0u}
@@ -577,7 +667,10 @@ static int render_impl(nova_hip_ctx *ctx, const nova_hip_render_params *p, float
   }
   const uint32_t shade_blocks[2] = {(uint32_t)std::min<size_t>((cap + kShadeBlock - 1) / kShadeBlock, (size_t)cus * shade_per_cu[0]),
                                     (uint32_t)std::min<size_t>((cap + kShadeBlock - 1) / kShadeBlock, (size_t)cus * shade_per_cu[1])};
-  NOVA_HIP_CHECK(ctx, hipMemsetAsync(counts, 0, 128, s));
+  if (noalloc)
+    hipLaunchKernelGGL(zero_words_kernel, dim3(1), dim3(64), 0, s, counts, 32u);
+  else
+    NOVA_HIP_CHECK(ctx, hipMemsetAsync(counts, 0, 128, s));
 
   const bool timing = (p->flags & NOVA_HIP_RENDER_TIMING) != 0;
   const bool counting = (p->flags & NOVA_HIP_RENDER_COUNT_TRAVERSAL) != 0;
@@ -589,10 +682,12 @@ static int render_impl(nova_hip_ctx *ctx, const nova_hip_render_params *p, float
     NOVA_HIP_CHECK(ctx, hipEventRecord(ev, s));
     return NOVA_HIP_SUCCESS;
   };
-  hipEvent_t ev0, ev1;
-  NOVA_HIP_CHECK(ctx, hipEventCreate(&ev0));
-  NOVA_HIP_CHECK(ctx, hipEventCreate(&ev1));
-  NOVA_HIP_CHECK(ctx, hipEventRecord(ev0, s));
+  hipEvent_t ev0 = nullptr, ev1 = nullptr;
+  if (!noalloc) {
+    NOVA_HIP_CHECK(ctx, hipEventCreate(&ev0));
+    NOVA_HIP_CHECK(ctx, hipEventCreate(&ev1));
+    NOVA_HIP_CHECK(ctx, hipEventRecord(ev0, s));
+  }
   uint64_t launches = 0, camera_rays = 0;
   for (int f0 = p->frame_start; f0 < p->frame_start + p->frame_count; f0 += fb) {
     const int frames = std::min(fb, p->frame_start + p->frame_count - f0);
@@ -668,6 +763,8 @@ static int render_impl(nova_hip_ctx *ctx, const nova_hip_render_params *p, float
       NOVA_HIP_CHECK(ctx, hipGetLastError());
     }
   }
+  if (noalloc) return NOVA_HIP_SUCCESS;  // stream-ordered; statistics untouched
+  if (noalloc) return NOVA_HIP_SUCCESS;  // stream-ordered; statistics untouched
   NOVA_HIP_CHECK(ctx, hipEventRecord(ev1, s));
   NOVA_HIP_CHECK(ctx, hipEventSynchronize(ev1));
   float ms = 0.f;
@@ -707,6 +804,20 @@ static int render_impl(nova_hip_ctx *ctx, const nova_hip_render_params *p, float
 extern "C" int nova_hip_render_frames(nova_hip_ctx *ctx, const nova_hip_render_params *p, float *accum_dev, float *final_dev, float *partial_dev,
                                       void *stream) {
   return render_impl(ctx, p, accum_dev, final_dev, partial_dev, stream, nullptr);
+}
+
+extern "C" int nova_hip_render_frames_noalloc(nova_hip_ctx *ctx, const nova_hip_render_params *p, float *accum_dev, float *final_dev,
+                                              float *partial_dev, void *stream) {
+  if (!ctx || !p) return NOVA_HIP_INVALID_ARGUMENT;
+  if (p->flags & (NOVA_HIP_RENDER_TIMING | NOVA_HIP_RENDER_COUNT_TRAVERSAL))
+    return set_error(ctx, NOVA_HIP_INVALID_ARGUMENT, "timing and traversal counting need events and reads: not offered without allocation");
+  int pass = 0;
+  int st = resolve_integrator(ctx, p->integrator_flags, &pass);
+  if (st) return st;
+  if (pass != 0) return set_error(ctx, NOVA_HIP_INVALID_ARGUMENT, "the no-allocation entry point renders the path integrator only");
+  unsigned char *base = static_cast<unsigned char *>(noalloc_scratch_of(ctx->device));
+  if (!base) return set_error(ctx, NOVA_HIP_DEVICE_ERROR, "no-allocation scratch address unavailable on this device");
+  return render_impl(ctx, p, accum_dev, final_dev, partial_dev, stream, nullptr, base);
 }
 
 extern "C" int nova_hip_render_adjoint(nova_hip_ctx *ctx, const nova_hip_render_params *p, const float *d_image_dev, float scale,

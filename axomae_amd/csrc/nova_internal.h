@@ -147,6 +147,9 @@ int check_hip(nova_hip_ctx *ctx, hipError_t e, const char *what);
 void *ensure_scratch(nova_hip_ctx *ctx, int slot, size_t bytes, hipError_t *err);
 uint32_t traversal_stack_bound(const nova_hip_ctx *ctx);
 const uint32_t *device_sobol_table();
+// Device address of the no-allocation entry point's fixed scratch array on `device`, looked
+// up once per device (the device must be current); NOVA_HIP_DEVICE_ERROR when unavailable.
+int noalloc_scratch_setup(int device);
 }  // namespace nova_hip
 
 #define NOVA_HIP_CHECK(ctx, expr)                                   \

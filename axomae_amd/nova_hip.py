@@ -25,7 +25,7 @@ EXPORTS = ["nova_hip_create", "nova_hip_destroy", "nova_hip_last_error", "nova_h
            "nova_hip_bake_batch", "nova_hip_render_batch", "nova_hip_batch_autotune", "nova_hip_renderbuffer_create",
            "nova_hip_renderbuffer_destroy", "nova_hip_renderbuffer_preallocate", "nova_hip_renderbuffer_resize", "nova_hip_renderbuffer_reset",
            "nova_hip_renderbuffer_clean_canvas", "nova_hip_renderbuffer_swap", "nova_hip_renderbuffer_get", "nova_hip_render_to_renderbuffer",
-           "nova_hip_renderbuffer_read"]
+           "nova_hip_renderbuffer_read", "nova_hip_render_frames_noalloc"]
 
 RB_FRONT, RB_BACK, RB_PARTIAL, RB_ACCUMULATOR, RB_DEPTH, RB_NORMAL = range(6)
 RB_CHANNELS = {RB_FRONT: 4, RB_BACK: 4, RB_PARTIAL: 4, RB_ACCUMULATOR: 4, RB_DEPTH: 1, RB_NORMAL: 3}
@@ -136,6 +136,7 @@ def load():
     L.nova_hip_bvh_read.argtypes = [P, P, P]
     L.nova_hip_closest_hit.argtypes = [P, P, C.c_uint64, P, P, P]
     L.nova_hip_render_frames.argtypes = [P, C.POINTER(RenderParams), P, P, P, P]
+    L.nova_hip_render_frames_noalloc.argtypes = [P, C.POINTER(RenderParams), P, P, P, P]
     L.nova_hip_render_stats_get.argtypes = [P, C.POINTER(RenderStats)]
     L.nova_hip_sobol_fill.argtypes = [P, P, C.c_uint64, P, P, P]
     L.nova_hip_render_batch.argtypes = [P, P, C.c_uint32, C.POINTER(BatchConfig), C.POINTER(BatchStats), P, P]

@@ -199,6 +199,21 @@ enum { NOVA_HIP_RENDER_TIMING = 1, NOVA_HIP_RENDER_COUNT_TRAVERSAL = 2, NOVA_HIP
 int nova_hip_render_frames(nova_hip_ctx *ctx, const nova_hip_render_params *params, float *accum_dev, float *final_dev,
                            float *partial_dev, void *stream);
 
+/* nova_hip_render_frames without any allocation, for callers that cannot allocate device
+ * memory (code under stream capture, tight serving loops, hosts that forbid hipMalloc). No
+ * counterpart in the reference. The path integrator only (integrator_flags 0 or
+ * PATH | COMBINED); NOVA_HIP_RENDER_TIMING and NOVA_HIP_RENDER_COUNT_TRAVERSAL are refused.
+ * Queues, path state and traversal stacks come from a fixed 320 MiB device array that the
+ * library holds on each device it runs on, sized for one 1920x1080 frame per batch: frames
+ * are rendered in batches of as many as fit, and an image whose single frame does not fit is
+ * refused with NOVA_HIP_INVALID_ARGUMENT before any work. The call is stream-ordered and
+ * does not wait (it records no event and leaves nova_hip_render_stats as it was). Every call
+ * on a device shares that device's array: calls that overlap on one device (several
+ * streams or threads) race and must be serialized by the caller. The scene and BVH must be
+ * built already (those calls allocate); the image equals nova_hip_render_frames' bit for bit. */
+int nova_hip_render_frames_noalloc(nova_hip_ctx *ctx, const nova_hip_render_params *params, float *accum_dev, float *final_dev,
+                                   float *partial_dev, void *stream);
+
 /* Statistics of the last nova_hip_render_frames call: device time of the whole call and,
  * with NOVA_HIP_RENDER_TIMING, summed extend (traversal) and shade kernel times in
  * milliseconds; path segments traced (rays entering an extend launch) and launches; with
