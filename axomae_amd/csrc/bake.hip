@@ -44,22 +44,24 @@ __device__ __forceinline__ f3 discrete(const BakeMap &m, int x, int y) {
   else if (y < 0) yc = m.h + (y % m.h);
   else yc = y;
   const float *p = m.env + ((size_t)yc * m.w + xc) * m.c;
+  if (m.c == 4) {  // RGBA texels of a 16-byte aligned map: one vector load
+    const float4 t = *reinterpret_cast<const float4 *>(p);
+    return mk3(t.x, t.y, t.z);
+  }
   return mk3(p[0], p[1], p[2]);
 }
 
+// uvSample: the pixel coordinates px, py are whole numbers (an unsigned texel index turned
+// back into a float), so its bilinear weights fu = (px - floor(px)) / 1 and fv are exactly
+// zero and the blend (1-fu)(1-fv) tl + fu tr + ... is the top-left texel (for finite maps;
+// a neighbour holding Inf or NaN would turn the reference's sum into NaN). Only that texel
+// is read: one gather instead of four.
 __device__ __forceinline__ f3 uv_sample(const BakeMap &m, float u, float v) {
   const float ui = floorf(u), vi = floorf(v);
   const float up = (u > 1. || u < 0.) ? u - ui : u;
   const float vp = (v > 1. || v < 0.) ? v - vi : v;
-  const float px = (float)min(uv_to_pixel(up, (unsigned)m.w), (unsigned)m.w), py = (float)min(uv_to_pixel(vp, (unsigned)m.h), (unsigned)m.h);
-  const float tlx = floorf(px), tly = floorf(py);
-  const float fu = (px - tlx) / ((tlx + 1) - tlx), fv = (py - tly) / ((tly + 1) - tly);
-  const f3 tl = discrete(m, (int)tlx, (int)tly), tr = discrete(m, (int)(tlx + 1), (int)tly);
-  const f3 bl = discrete(m, (int)tlx, (int)(tly + 1)), br = discrete(m, (int)(tlx + 1), (int)(tly + 1));
-  const float hd = 1 - fu;
-  const f3 top = hd * tl + fu * tr;
-  const f3 bot = hd * bl + fu * br;
-  return (1 - fv) * top + fv * bot;
+  const unsigned px = min(uv_to_pixel(up, (unsigned)m.w), (unsigned)m.w), py = min(uv_to_pixel(vp, (unsigned)m.h), (unsigned)m.h);
+  return discrete(m, (int)px, (int)py);
 }
 
 __device__ __forceinline__ f3 dir_sample(const BakeMap &m, const float *lut, f3 d) {
@@ -67,6 +69,13 @@ __device__ __forceinline__ f3 dir_sample(const BakeMap &m, const float *lut, f3 
   const double inv_2pi = 1.f / (2.f * 3.14159265358979323846), inv_pi = 1.f / 3.14159265358979323846;
   return uv_sample(m, (float)(phi * inv_2pi), (float)(theta * inv_pi));
 }
+
+// sin and cos of 2*pi*x for x in [0, 1): one float sincospi, no range reduction. The
+// reference rounds 2*pi*x to float and calls the float libm (TextureProcessing.h:270-276,
+// envmap_prefilter.frag); the two differ by a few ulp, far inside the bakes' 1e-3 gate,
+// and the double-precision cos/sin of the render path would make these sample loops
+// FP64-bound.
+__device__ __forceinline__ void sincos_2pi(float x, float &s, float &c) { sincospif(2.f * x, &s, &c); }
 
 // math::importance_sampling::pgc3d (math_importance_sampling.h:7-21)
 __device__ __forceinline__ f3 pgc3d(unsigned x, unsigned y, unsigned z) {
@@ -117,20 +126,36 @@ __device__ __forceinline__ f3 irradiance_texel(const BakeMap &m, const float *__
     const f3 bitangent = cross(nrm, tangent);
     f3 irr = mk3(0.f, 0.f, 0.f);
     const uint32_t base = (j * (uint32_t)m.ow + i) * (uint32_t)samples;
+    // Sobol points of consecutive indices: the gray code of idx differs from idx-1's in bit
+    // ctz(idx), so each sample XORs one direction number into the running values (equal to
+    // the direct gray-code sum bit for bit).
+    uint32_t ax = 0, ay = 0;
+    if (m.kind == NOVA_HIP_BAKE_IRRADIANCE_SOBOL) {
+      const uint32_t g0 = base ^ (base >> 1);
+      ax = sobol_xor(sobol, g0);
+      ay = sobol_xor(sobol + 32, g0);
+    }
     for (int s = 0; s < samples; s++) {
       float rx, ry;
       if (m.kind == NOVA_HIP_BAKE_IRRADIANCE_SOBOL) {
-        const uint32_t idx = base + (uint32_t)s, gray = idx ^ (idx >> 1);
-        rx = u32_to_unit(owen_scramble(sobol_xor(sobol, gray)));
-        ry = u32_to_unit(owen_scramble(sobol_xor(sobol + 32, gray)));
+        if (s > 0) {
+          const uint32_t bit = (uint32_t)__builtin_ctz(base + (uint32_t)s);
+          ax ^= sobol[bit];
+          ay ^= sobol[32 + bit];
+        }
+        rx = u32_to_unit(owen_scramble(ax));
+        ry = u32_to_unit(owen_scramble(ay));
       } else {
         const f3 r = pgc3d(px, py, (unsigned)s);
         rx = r.x;
         ry = r.y;
       }
-      const float sphi = (float)(2 * 3.14159265358979323846 * rx);
-      const float stheta = asin_f(sqrtf(ry));
-      f3 c = sph_to_cart(sphi, stheta);
+      // cosine sample at phi = 2 pi rx, theta = asin(sqrt(ry)) (TextureProcessing.h:270-276):
+      // sin(theta) = sqrt(ry), cos(theta) = sqrt(1 - ry)
+      float sp, cp;
+      sincos_2pi(rx, sp, cp);
+      const float st = sqrtf(ry), ct = sqrtf(1.f - ry);
+      f3 c = mk3(st * cp, st * sp, ct);
       c = c.x * tangent + c.y * bitangent + c.z * nrm;
       irr = irr + dir_sample(m, lut, c);
     }
@@ -163,10 +188,11 @@ __device__ __forceinline__ f3 prefilter_texel(const BakeMap &m, const float *__r
       const f3 b = cross(N, t);
       for (int s = 0; s < samples; s++) {
         const float xi0 = (float)s / (float)samples, xi1 = radical_inverse((unsigned)s);
-        const float phi = 2.0f * 3.14159265f * xi0;
         const float ct = sqrtf((1.0f - xi1) / (1.0f + (a * a - 1.0f) * xi1));
         const float st = sqrtf(1.0f - ct * ct);
-        const f3 H = normalize(t * (cos_f(phi) * st) + b * (sin_f(phi) * st) + N * ct);
+        float sp, cp;
+        sincos_2pi(xi0, sp, cp);
+        const f3 H = normalize(t * (cp * st) + b * (sp * st) + N * ct);
         const f3 L = reflect(-N, H);
         const float ndl = fmaxf(dot(N, L), 0.f);
         if (ndl > 0.f) {
@@ -201,10 +227,11 @@ __device__ __forceinline__ float2 brdf_lut_texel(const BakeMap &m, unsigned i, u
   float A = 0.f, B = 0.f;
   for (int s = 0; s < samples; s++) {
     const float xi0 = (float)s / (float)samples, xi1 = radical_inverse((unsigned)s);
-    const float phi = 2.0f * 3.14159265f * xi0;
     const float ct = sqrtf((1.0f - xi1) / (1.0f + (a * a - 1.0f) * xi1));
     const float st = sqrtf(1.0f - ct * ct);
-    const f3 H = normalize(t * (cos_f(phi) * st) + b * (sin_f(phi) * st) + N * ct);
+    float sp, cp;
+    sincos_2pi(xi0, sp, cp);
+    const f3 H = normalize(t * (cp * st) + b * (sp * st) + N * ct);
     const f3 L = normalize(H * (2.0f * dot(V, H)) - V);
     const float ndl = fmaxf(L.z, 0.0f), ndh = fmaxf(H.z, 0.0f), vdh = fmaxf(dot(V, H), 0.0f);
     if (ndl > 0.0f) {
@@ -220,7 +247,8 @@ __device__ __forceinline__ float2 brdf_lut_texel(const BakeMap &m, unsigned i, u
 
 // Every texel of every job in one grid-stride launch; jobs are laid out back to back, so a
 // wave almost always works on a single map and kind.
-__global__ void bake_kernel(const BakeMap *__restrict__ maps, int count, uint64_t total, const float *__restrict__ lut,
+constexpr unsigned kBakeBlock = 256;
+__global__ __launch_bounds__(kBakeBlock) __attribute__((amdgpu_waves_per_eu(4, 4))) void bake_kernel(const BakeMap *__restrict__ maps, int count, uint64_t total, const float *__restrict__ lut,
                             const uint32_t *__restrict__ sobol, int samples) {
   for (uint64_t g = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; g < total; g += (uint64_t)gridDim.x * blockDim.x) {
     const BakeMap m = maps[find_map(maps, count, g)];
@@ -254,6 +282,7 @@ static int launch_batch(const nova_hip_bake_job *jobs, uint32_t count, int32_t s
     } else {
       if (!jb.env || !jb.out || jb.width < 1 || jb.height < 1 || jb.channels < 3 || jb.channels > 4 || jb.out_width < 1 || jb.out_height < 1)
         return NOVA_HIP_INVALID_ARGUMENT;
+      if (jb.channels == 4 && (reinterpret_cast<uintptr_t>(jb.env) & 15u)) return NOVA_HIP_INVALID_ARGUMENT;  // RGBA maps are read as float4
       if (jb.kind != NOVA_HIP_BAKE_IRRADIANCE && jb.kind != NOVA_HIP_BAKE_IRRADIANCE_SOBOL && jb.kind != NOVA_HIP_BAKE_PREFILTER)
         return NOVA_HIP_INVALID_ARGUMENT;
     }
@@ -283,7 +312,7 @@ static int launch_batch(const nova_hip_bake_job *jobs, uint32_t count, int32_t s
     hipFree(d_maps);
     return NOVA_HIP_DEVICE_ERROR;
   }
-  const unsigned block = 64;
+  const unsigned block = kBakeBlock;
   const unsigned grid = (unsigned)std::min<uint64_t>((total + block - 1) / block, 1u << 20);
   hipLaunchKernelGGL(bake_kernel, dim3(grid), dim3(block), 0, s, d_maps, (int)count, total, d_lut, sobol, samples);
   hipError_t e = hipGetLastError();

@@ -383,7 +383,8 @@ int nova_hip_bake_prefilter(const float *const *env_dev, const int32_t *w, const
                             int32_t samples, void *stream);
 
 /* One job of a batched bake: an FP32 equirect map (device pointer, width x height x channels,
- * channels 3 or 4) and its output (device pointer, out_width x out_height x channels). */
+ * channels 3 or 4; a 4-channel map 16-byte aligned) and its output (device pointer,
+ * out_width x out_height x channels). */
 enum nova_hip_bake_kind {
   NOVA_HIP_BAKE_IRRADIANCE = 0,       /* computeDiffuseIrradiance with its pgc3d hash (the reference's sampler) */
   NOVA_HIP_BAKE_IRRADIANCE_SOBOL = 1, /* the same estimator on Owen-scrambled Sobol points, dims 0-1,
