@@ -318,10 +318,12 @@ __global__ void accumulate_kernel(const float *__restrict__ partial, FrameParams
     const uint32_t p = e >> 2, c = e & 3u;
     const uint32_t i = pixel_offset(fp, p) + c;
     const float *src = partial + (size_t)p * frames * 4 + c;
-    float a = accum[i];
-    for (int f = 0; f < frames; f++) a += src[(size_t)f * 4];  // frame order
-    accum[i] = a;
-    final_out[i] = a / float(fp.frame0 + frames);  // the last frame's average, the one displayed
+    if (accum) {  // without accum/final: the caller accumulates (the drop-in's per-frame traverse)
+      float a = accum[i];
+      for (int f = 0; f < frames; f++) a += src[(size_t)f * 4];  // frame order
+      accum[i] = a;
+      final_out[i] = a / float(fp.frame0 + frames);  // the last frame's average, the one displayed
+    }
     if (partial_out) partial_out[i] = src[(size_t)(frames - 1) * 4];
   }
 }
@@ -553,7 +555,9 @@ static int render_impl(nova_hip_ctx *ctx, const nova_hip_render_params *p, float
   if (p->frame_count <= 0 || p->frame_start < 0 || p->max_depth < 0) return set_error(ctx, NOVA_HIP_INVALID_ARGUMENT, "bad frame range or depth");
   if (p->index_policy != NOVA_HIP_INDEX_CPU && p->index_policy != NOVA_HIP_INDEX_OPTIX)
     return set_error(ctx, NOVA_HIP_INVALID_ARGUMENT, "unknown index_policy");
-  if (!adj && (!accum_dev || !final_dev)) return set_error(ctx, NOVA_HIP_INVALID_BUFFER_STATE, "null render buffer");
+  // accum and final together, or neither with a partial buffer (one frame's radiance only)
+  if (!adj && (!accum_dev != !final_dev || (!accum_dev && !partial_dev)))
+    return set_error(ctx, NOVA_HIP_INVALID_BUFFER_STATE, "null render buffer");
   if ((((uintptr_t)accum_dev | (uintptr_t)final_dev | (uintptr_t)partial_dev) & 3u) != 0)
     return set_error(ctx, NOVA_HIP_INVALID_BUFFER_STATE, "render buffers must be 4-byte aligned float arrays");
   if (p->sampling_roughness_texture >= 0) {
@@ -573,7 +577,10 @@ static int render_impl(nova_hip_ctx *ctx, const nova_hip_render_params *p, float
   if (pass != 0 && adj) return set_error(ctx, NOVA_HIP_INVALID_ARGUMENT, "the adjoint differentiates the path integrator only");
   hipStream_t s = (hipStream_t)stream;
   NOVA_HIP_CHECK(ctx, hipSetDevice(ctx->device));
-  if (pass != 0) return render_pass(ctx, p, pass, accum_dev, final_dev, partial_dev, s);
+  if (pass != 0) {
+    if (!accum_dev) return set_error(ctx, NOVA_HIP_INVALID_BUFFER_STATE, "render passes need accum and final buffers");
+    return render_pass(ctx, p, pass, accum_dev, final_dev, partial_dev, s);
+  }
   const size_t npix = (size_t)p->width * p->height;
   // Frames per batch: up to kPathsPerBatch paths in flight (about 116 B of queue and path
   // state each, 15.6 GB at the limit). Large batches matter because the last bounces hold
@@ -763,7 +770,6 @@ static int render_impl(nova_hip_ctx *ctx, const nova_hip_render_params *p, float
       NOVA_HIP_CHECK(ctx, hipGetLastError());
     }
   }
-  if (noalloc) return NOVA_HIP_SUCCESS;  // stream-ordered; statistics untouched
   if (noalloc) return NOVA_HIP_SUCCESS;  // stream-ordered; statistics untouched
   NOVA_HIP_CHECK(ctx, hipEventRecord(ev1, s));
   NOVA_HIP_CHECK(ctx, hipEventSynchronize(ev1));

@@ -125,6 +125,9 @@ def load():
         return _LIB
     if not os.path.exists(LIB_PATH):
         raise ImportError(f"{LIB_PATH} is missing: build it with `python -m axomae_amd.build`")
+    # torch's HIP runtime and the one this library links share the soname libamdhip64.so.7:
+    # the first loaded serves the process, and torch must have its own, so it loads first
+    import torch  # noqa: F401
     L = C.CDLL(LIB_PATH)
     P = C.c_void_p
     L.nova_hip_create.argtypes = [C.c_int, C.POINTER(P)]

@@ -188,6 +188,8 @@ enum { NOVA_HIP_RENDER_TIMING = 1, NOVA_HIP_RENDER_COUNT_TRAVERSAL = 2, NOVA_HIP
  * final = accum / (frame+1). accum_dev and final_dev are device RGBA32F buffers of
  * width*height*4 floats; accum must hold the previous frames' sum (zero at frame 0).
  * partial_dev (optional) receives the last frame's radiance (HdrBufferStruct::partial_buffer).
+ * With accum_dev and final_dev both NULL and partial_dev set, only the last frame's radiance
+ * is written, as DeviceIntersectorInterface::traverse leaves it for the engine to accumulate.
  * The render passes (integrator_flags NORMAL / DEPTH / EMISSIVE: NormalIntegrator.cpp,
  * DepthIntegrator.cpp, EmissiveIntegrator.cpp) trace one unjittered ray per pixel, give the
  * same value every frame and accumulate it the same way. A flag word without PATH returns
