@@ -247,6 +247,69 @@ __device__ __forceinline__ float2 brdf_lut_texel(const BakeMap &m, unsigned i, u
 
 // Every texel of every job in one grid-stride launch; jobs are laid out back to back, so a
 // wave almost always works on a single map and kind.
+// ----------------------------------------------- GL viewer bakes on equirect maps
+// envmap_bake.frag:10-24 with the capture views of RenderPipeline.cpp:14-26, and
+// irradiance_baker.frag:15-39. Lookup rule: the equirect texture's GL sampler state (the
+// viewer sets its parameters on the cube-map target, so the 2D texture keeps GL's default
+// REPEAT wrap), level 0 with GL_LINEAR filtering: texel centres at (i + 0.5) / w, bilinear
+// weights, indices modulo the map size. atan and acos in double, rounded once.
+__device__ __forceinline__ f3 gl_normalize(f3 v) {
+  const float inv = 1.f / sqrtf((v.x * v.x + v.y * v.y) + v.z * v.z);
+  return mk3(v.x * inv, v.y * inv, v.z * inv);
+}
+
+__device__ __forceinline__ f3 gl_texel(const BakeMap &m, int x, int y) {
+  x %= m.w;
+  if (x < 0) x += m.w;
+  y %= m.h;
+  if (y < 0) y += m.h;
+  const float *p = m.env + ((size_t)y * m.w + x) * m.c;
+  if (m.c == 4) {
+    const float4 t = *reinterpret_cast<const float4 *>(p);
+    return mk3(t.x, t.y, t.z);
+  }
+  return mk3(p[0], p[1], p[2]);
+}
+
+// texture(environment_map, sphericalToUv(d)) for a unit direction d
+__device__ __forceinline__ f3 gl_lookup(const BakeMap &m, f3 d) {
+  const float PI = 3.14159265359f;
+  const float theta = (float)atan2(-(double)d.z, (double)d.x), phi = (float)acos((double)d.y);
+  const float u = theta / (2.0f * PI), v = phi / PI;
+  const float x = u * (float)m.w - 0.5f, y = v * (float)m.h - 0.5f;
+  const float fx = floorf(x), fy = floorf(y);
+  const float a = x - fx, b = y - fy;
+  const int i0 = (int)fx, j0 = (int)fy;
+  const f3 t00 = gl_texel(m, i0, j0), t10 = gl_texel(m, i0 + 1, j0), t01 = gl_texel(m, i0, j0 + 1), t11 = gl_texel(m, i0 + 1, j0 + 1);
+  const float w00 = (1.f - a) * (1.f - b), w10 = a * (1.f - b), w01 = (1.f - a) * b, w11 = a * b;
+  return ((w00 * t00 + w10 * t10) + w01 * t01) + w11 * t11;
+}
+
+// Capture basis of face `face` (glm::lookAt from the origin: f = normalize(center),
+// s = normalize(f x up), u = s x f), faces in capture order: left, right, down, up, back, front.
+__device__ __forceinline__ void capture_basis(int face, f3 &s, f3 &u, f3 &f) {
+  const float cx[6] = {-1, 1, 0, 0, 0, 0}, cy[6] = {0, 0, -1, 1, 0, 0}, cz[6] = {0, 0, 0, 0, 1, -1};
+  const float ux[6] = {0, 0, 0, 0, 0, 0}, uy[6] = {1, 1, 0, 0, 1, 1}, uz[6] = {0, 0, 1, -1, 0, 0};
+  f = gl_normalize(mk3(cx[face], cy[face], cz[face]));
+  s = gl_normalize(cross(f, mk3(ux[face], uy[face], uz[face])));
+  u = cross(s, f);
+}
+
+// Face texel (x, y) of a size x size capture, y = 0 the bottom row: the 90-degree camera's
+// ray through the pixel centre meets the unit cube at ndc_x s + ndc_y u + f.
+__device__ __forceinline__ f3 cube_texel_direction(int face, int x, int y, int size) {
+  f3 s, u, f;
+  capture_basis(face, s, u, f);
+  const float nx = (float)(2 * x + 1) / (float)size - 1.f, ny = (float)(2 * y + 1) / (float)size - 1.f;
+  return gl_normalize((nx * s + ny * u) + f);
+}
+
+// Equirect texel (i, j) -> its centre direction (the inverse of sphericalToUv), in double.
+__device__ __forceinline__ f3 equirect_texel_direction(int i, int j, int w, int h) {
+  const double az = 2.0 * 3.14159265358979323846 * ((double)i + 0.5) / (double)w, po = 3.14159265358979323846 * ((double)j + 0.5) / (double)h;
+  return mk3((float)(cos(az) * sin(po)), (float)cos(po), (float)(-sin(az) * sin(po)));
+}
+
 constexpr unsigned kBakeBlock = 256;
 __global__ __launch_bounds__(kBakeBlock) __attribute__((amdgpu_waves_per_eu(4, 4))) void bake_kernel(const BakeMap *__restrict__ maps, int count, uint64_t total, const float *__restrict__ lut,
                             const uint32_t *__restrict__ sobol, int samples) {
@@ -270,11 +333,79 @@ __global__ __launch_bounds__(kBakeBlock) __attribute__((amdgpu_waves_per_eu(4, 4
   }
 }
 
+// The Riemann irradiance of irradiance_baker.frag, one workgroup per output texel: lane t
+// sums the zenith loop of azimuth steps t, t + 256, ... (the steps' cos and sin, stepped in
+// float as GLSL steps them, come from host tables), then the lanes' sums are added in a
+// fixed tree. The shader keeps one running sum; the sum here is regrouped by azimuth,
+// which moves the result by float rounding only.
+constexpr unsigned kRiemannBlock = 256;
+__global__ __launch_bounds__(kRiemannBlock) void riemann_kernel(const BakeMap *__restrict__ maps, int count, const float2 *__restrict__ phi_tab,
+                                                               int nphi, const float2 *__restrict__ theta_tab, int ntheta) {
+  __shared__ float red[3][kRiemannBlock];
+  const uint64_t g = blockIdx.x;
+  const BakeMap m = maps[find_map(maps, count, g)];
+  const uint64_t local = g - m.first;
+  const int i = (int)(local % (unsigned)m.ow), j = (int)(local / (unsigned)m.ow);
+  const f3 n = equirect_texel_direction(i, j, m.ow, m.oh);
+  const f3 right = gl_normalize(cross(mk3(0.f, 1.f, 0.f), n));
+  const f3 up = gl_normalize(cross(n, right));
+  f3 acc = mk3(0.f, 0.f, 0.f);
+  for (int k = threadIdx.x; k < nphi; k += kRiemannBlock) {
+    const float2 ph = phi_tab[k];
+    for (int t = 0; t < ntheta; t++) {
+      const float2 th = theta_tab[t];  // (cos, sin)
+      const float x = ph.x * th.y, y = th.y * ph.y, z = th.x;
+      const f3 sv = (x * right + y * up) + z * n;
+      acc = acc + (gl_lookup(m, gl_normalize(sv)) * th.x) * th.y;
+    }
+  }
+  red[0][threadIdx.x] = acc.x;
+  red[1][threadIdx.x] = acc.y;
+  red[2][threadIdx.x] = acc.z;
+  __syncthreads();
+  for (unsigned w = kRiemannBlock / 2; w > 0; w >>= 1) {
+    if (threadIdx.x < w)
+      for (int c = 0; c < 3; c++) red[c][threadIdx.x] += red[c][threadIdx.x + w];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) {
+    const float PI = 3.14159265358f;
+    const float inv = 1.f / (float)(nphi * ntheta);
+    float *o = m.out + ((size_t)j * m.ow + i) * m.c;
+    for (int c = 0; c < 3; c++) o[c] = (PI * red[c][0]) * inv;
+    for (int c = 3; c < m.c; c++) o[c] = 0.f;
+  }
+}
+
+// The zenith and azimuth steps of irradiance_baker.frag's loops, stepped in float from 0 by
+// 0.0075 while <= the bound, with their cos and sin (double, rounded once).
+static void riemann_steps(float bound, std::vector<float2> &tab) {
+  tab.clear();
+  for (float a = 0.f; a <= bound; a += 0.0075f) tab.push_back(make_float2((float)std::cos((double)a), (float)std::sin((double)a)));
+}
+
+// Cube-map captures, one texel per lane: six faces of ow x ow, face-major, rows bottom-up.
+// A kernel of its own: its double-precision atan and acos would raise the register budget of
+// the sampled bakes.
+__global__ __launch_bounds__(kBakeBlock) void cubemap_kernel(const BakeMap *__restrict__ maps, int count, uint64_t total) {
+  for (uint64_t g = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; g < total; g += (uint64_t)gridDim.x * blockDim.x) {
+    const BakeMap m = maps[find_map(maps, count, g)];
+    const uint64_t local = g - m.first;
+    const unsigned S = (unsigned)m.ow, face = (unsigned)(local / ((uint64_t)S * S)), r = (unsigned)(local % ((uint64_t)S * S));
+    const f3 col = gl_lookup(m, cube_texel_direction((int)face, (int)(r % S), (int)(r / S), (int)S));
+    float *o = m.out + local * m.c;
+    o[0] = col.x;
+    o[1] = col.y;
+    o[2] = col.z;
+    for (int k = 3; k < m.c; k++) o[k] = 0.f;
+  }
+}
+
 static int launch_batch(const nova_hip_bake_job *jobs, uint32_t count, int32_t samples, hipStream_t s) {
   if (count == 0) return NOVA_HIP_SUCCESS;
   if (!jobs || samples <= 0) return NOVA_HIP_INVALID_ARGUMENT;
-  std::vector<BakeMap> maps(count);
-  uint64_t total = 0;
+  std::vector<BakeMap> maps, cube, riemann;  // sampled bakes; cube captures; Riemann jobs (a workgroup per texel)
+  uint64_t total = 0, cube_total = 0, riemann_total = 0;
   for (uint32_t i = 0; i < count; i++) {
     const nova_hip_bake_job &jb = jobs[i];
     if (jb.kind == NOVA_HIP_BAKE_BRDF_LUT) {  // no environment map; two output channels
@@ -283,15 +414,22 @@ static int launch_batch(const nova_hip_bake_job *jobs, uint32_t count, int32_t s
       if (!jb.env || !jb.out || jb.width < 1 || jb.height < 1 || jb.channels < 3 || jb.channels > 4 || jb.out_width < 1 || jb.out_height < 1)
         return NOVA_HIP_INVALID_ARGUMENT;
       if (jb.channels == 4 && (reinterpret_cast<uintptr_t>(jb.env) & 15u)) return NOVA_HIP_INVALID_ARGUMENT;  // RGBA maps are read as float4
-      if (jb.kind != NOVA_HIP_BAKE_IRRADIANCE && jb.kind != NOVA_HIP_BAKE_IRRADIANCE_SOBOL && jb.kind != NOVA_HIP_BAKE_PREFILTER)
+      if (jb.kind != NOVA_HIP_BAKE_IRRADIANCE && jb.kind != NOVA_HIP_BAKE_IRRADIANCE_SOBOL && jb.kind != NOVA_HIP_BAKE_PREFILTER &&
+          jb.kind != NOVA_HIP_BAKE_CUBEMAP && jb.kind != NOVA_HIP_BAKE_IRRADIANCE_RIEMANN)
         return NOVA_HIP_INVALID_ARGUMENT;
+      if (jb.kind == NOVA_HIP_BAKE_CUBEMAP && jb.out_width != jb.out_height) return NOVA_HIP_INVALID_ARGUMENT;  // square faces
     }
     // Sobol indices texel * samples + s must stay below 2^32
     if (jb.kind == NOVA_HIP_BAKE_IRRADIANCE_SOBOL && (uint64_t)jb.out_width * jb.out_height * (uint64_t)samples > 0xFFFFFFFFull)
       return NOVA_HIP_INVALID_ARGUMENT;
-    maps[i] = BakeMap{jb.env, jb.out, jb.width, jb.height, jb.channels, jb.out_width, jb.out_height, jb.kind, jb.roughness, total};
-    total += (uint64_t)jb.out_width * jb.out_height;
+    const uint64_t texels = (uint64_t)jb.out_width * jb.out_height * (jb.kind == NOVA_HIP_BAKE_CUBEMAP ? 6u : 1u);
+    const bool is_cube = jb.kind == NOVA_HIP_BAKE_CUBEMAP, is_riemann = jb.kind == NOVA_HIP_BAKE_IRRADIANCE_RIEMANN;
+    uint64_t &first = is_riemann ? riemann_total : is_cube ? cube_total : total;
+    (is_riemann ? riemann : is_cube ? cube : maps)
+        .push_back(BakeMap{jb.env, jb.out, jb.width, jb.height, jb.channels, jb.out_width, jb.out_height, jb.kind, jb.roughness, first});
+    first += texels;
   }
+  if (riemann_total > 0x7FFFFFFFull) return NOVA_HIP_INVALID_ARGUMENT;  // one workgroup per texel
   const uint32_t *sobol = device_sobol_table();
   if (!sobol) return NOVA_HIP_OUT_OF_MEMORY;
   // acos table: one device copy per process and device
@@ -306,18 +444,44 @@ static int launch_batch(const nova_hip_bake_job *jobs, uint32_t count, int32_t s
     if (hipMemcpy(d_lut, lut.data(), kAcosLut * sizeof(float), hipMemcpyHostToDevice) != hipSuccess) return NOVA_HIP_DEVICE_ERROR;
     lut_dev = dev;
   }
-  BakeMap *d_maps = nullptr;
-  if (hipMalloc(&d_maps, count * sizeof(BakeMap)) != hipSuccess) return NOVA_HIP_OUT_OF_MEMORY;
-  if (hipMemcpyAsync(d_maps, maps.data(), count * sizeof(BakeMap), hipMemcpyHostToDevice, s) != hipSuccess) {
-    hipFree(d_maps);
-    return NOVA_HIP_DEVICE_ERROR;
+  // job tables and the Riemann step tables in one device buffer
+  std::vector<float2> phi_tab, theta_tab;
+  if (!riemann.empty()) {
+    const float PI = 3.14159265358f, zenith_max = PI / 2, azimuth_max = 2 * PI;
+    riemann_steps(azimuth_max, phi_tab);
+    riemann_steps(zenith_max, theta_tab);
   }
-  const unsigned block = kBakeBlock;
-  const unsigned grid = (unsigned)std::min<uint64_t>((total + block - 1) / block, 1u << 20);
-  hipLaunchKernelGGL(bake_kernel, dim3(grid), dim3(block), 0, s, d_maps, (int)count, total, d_lut, sobol, samples);
-  hipError_t e = hipGetLastError();
+  const size_t maps_bytes = maps.size() * sizeof(BakeMap), cube_bytes = cube.size() * sizeof(BakeMap), riemann_bytes = riemann.size() * sizeof(BakeMap);
+  const size_t tab_bytes = (phi_tab.size() + theta_tab.size()) * sizeof(float2);
+  char *buf = nullptr;
+  if (hipMalloc(&buf, maps_bytes + cube_bytes + riemann_bytes + tab_bytes + 16) != hipSuccess) return NOVA_HIP_OUT_OF_MEMORY;
+  BakeMap *d_maps = reinterpret_cast<BakeMap *>(buf), *d_cube = reinterpret_cast<BakeMap *>(buf + maps_bytes);
+  BakeMap *d_riemann = reinterpret_cast<BakeMap *>(buf + maps_bytes + cube_bytes);
+  float2 *d_phi = reinterpret_cast<float2 *>(buf + maps_bytes + cube_bytes + riemann_bytes), *d_theta = d_phi + phi_tab.size();
+  hipError_t e = hipSuccess;
+  if (maps_bytes) e = hipMemcpyAsync(d_maps, maps.data(), maps_bytes, hipMemcpyHostToDevice, s);
+  if (e == hipSuccess && cube_bytes) e = hipMemcpyAsync(d_cube, cube.data(), cube_bytes, hipMemcpyHostToDevice, s);
+  if (e == hipSuccess && riemann_bytes) e = hipMemcpyAsync(d_riemann, riemann.data(), riemann_bytes, hipMemcpyHostToDevice, s);
+  if (e == hipSuccess && !phi_tab.empty()) e = hipMemcpyAsync(d_phi, phi_tab.data(), phi_tab.size() * sizeof(float2), hipMemcpyHostToDevice, s);
+  if (e == hipSuccess && !theta_tab.empty()) e = hipMemcpyAsync(d_theta, theta_tab.data(), theta_tab.size() * sizeof(float2), hipMemcpyHostToDevice, s);
+  if (e == hipSuccess && total > 0) {
+    const unsigned block = kBakeBlock;
+    const unsigned grid = (unsigned)std::min<uint64_t>((total + block - 1) / block, 1u << 20);
+    hipLaunchKernelGGL(bake_kernel, dim3(grid), dim3(block), 0, s, d_maps, (int)maps.size(), total, d_lut, sobol, samples);
+    e = hipGetLastError();
+  }
+  if (e == hipSuccess && cube_total > 0) {
+    const unsigned grid = (unsigned)std::min<uint64_t>((cube_total + kBakeBlock - 1) / kBakeBlock, 1u << 20);
+    hipLaunchKernelGGL(cubemap_kernel, dim3(grid), dim3(kBakeBlock), 0, s, d_cube, (int)cube.size(), cube_total);
+    e = hipGetLastError();
+  }
+  if (e == hipSuccess && riemann_total > 0) {
+    hipLaunchKernelGGL(riemann_kernel, dim3((unsigned)riemann_total), dim3(kRiemannBlock), 0, s, d_riemann, (int)riemann.size(), d_phi,
+                       (int)phi_tab.size(), d_theta, (int)theta_tab.size());
+    e = hipGetLastError();
+  }
   if (e == hipSuccess) e = hipStreamSynchronize(s);
-  hipFree(d_maps);
+  hipFree(buf);
   return e == hipSuccess ? NOVA_HIP_SUCCESS : NOVA_HIP_DEVICE_ERROR;
 }
 

@@ -87,7 +87,7 @@ class BakeJob(C.Structure):
                 ("out_width", C.c_int32), ("out_height", C.c_int32), ("kind", C.c_int32), ("roughness", C.c_float)]
 
 
-BAKE_IRRADIANCE, BAKE_IRRADIANCE_SOBOL, BAKE_PREFILTER, BAKE_BRDF_LUT = 0, 1, 2, 3
+BAKE_IRRADIANCE, BAKE_IRRADIANCE_SOBOL, BAKE_PREFILTER, BAKE_BRDF_LUT, BAKE_CUBEMAP, BAKE_IRRADIANCE_RIEMANN = 0, 1, 2, 3, 4, 5
 
 class BatchConfig(C.Structure):
     _fields_ = [("queue_capacity", C.c_uint32), ("trace_blocks_per_cu", C.c_uint32), ("shade_blocks_per_cu", C.c_uint32), ("use_graph", C.c_int32)]
@@ -417,7 +417,7 @@ def bake_batch(jobs, samples, keep_on_device=False):
     for k, (env, ow, oh, kind, rough) in enumerate(jobs):
         if kind == BAKE_BRDF_LUT:  # no map: two output channels (A, B)
             out = torch.zeros(oh * ow * 2, dtype=torch.float32, device="cuda")
-            outs.append((out, ow, oh, 2))
+            outs.append((out, ow, oh, 2, 1))
             arr[k] = BakeJob(None, 0, 0, 2, out.data_ptr(), ow, oh, kind, 0.0)
             continue
         if isinstance(env, tuple):
@@ -425,16 +425,18 @@ def bake_batch(jobs, samples, keep_on_device=False):
         else:
             src = torch.as_tensor(np.ascontiguousarray(env.data, dtype=np.float32)).to("cuda")
             w, h, c = env.width, env.height, env.channels
-        out = torch.zeros(oh * ow * c, dtype=torch.float32, device="cuda")
+        faces = 6 if kind == BAKE_CUBEMAP else 1  # a cube map: six ow x ow faces
+        out = torch.zeros(faces * oh * ow * c, dtype=torch.float32, device="cuda")
         keep.append(src)
-        outs.append((out, ow, oh, c))
+        outs.append((out, ow, oh, c, faces))
         arr[k] = BakeJob(src.data_ptr(), w, h, c, out.data_ptr(), ow, oh, kind, float(rough))
     rc = L.nova_hip_bake_batch(arr, len(jobs), samples, _stream(torch))
     if rc:
         raise NovaHipError(rc, "bake_batch")
+    shape = lambda ow, oh, c, faces: (faces, oh, ow, c) if faces > 1 else (oh, ow, c)
     if keep_on_device:
-        return [o.view(oh, ow, c) for o, ow, oh, c in outs]
-    return [o.cpu().numpy().reshape(oh, ow, c) for o, ow, oh, c in outs]
+        return [o.view(*shape(ow, oh, c, f)) for o, ow, oh, c, f in outs]
+    return [o.cpu().numpy().reshape(shape(ow, oh, c, f)) for o, ow, oh, c, f in outs]
 
 
 class BatchItem(C.Structure):

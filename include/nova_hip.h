@@ -392,10 +392,20 @@ enum nova_hip_bake_kind {
   NOVA_HIP_BAKE_IRRADIANCE_SOBOL = 1, /* the same estimator on Owen-scrambled Sobol points, dims 0-1,
                                          index texel * samples + s (extension) */
   NOVA_HIP_BAKE_PREFILTER = 2,        /* envmap_prefilter.frag at `roughness` */
-  NOVA_HIP_BAKE_BRDF_LUT = 3          /* brdf_lookup_table_baker.frag:74-113: no map (env, width, height
+  NOVA_HIP_BAKE_BRDF_LUT = 3,         /* brdf_lookup_table_baker.frag:74-113: no map (env, width, height
                                          unused), channels 2, out_width x out_height x {A, B} with
                                          NdotV = (i + 0.5) / out_width, roughness = (j + 0.5) / out_height;
                                          1024 samples reproduce the shader */
+  NOVA_HIP_BAKE_CUBEMAP = 4,          /* envmap_bake.frag:10-24 under the capture views of
+                                         RenderPipeline.cpp:14-26 (90-degree faces): out_width ==
+                                         out_height == the face size, out holds 6 faces in capture order
+                                         (left, right, down, up, back, front), face-major, rows bottom-up,
+                                         `channels` floats per texel; the map is read bilinearly at level 0
+                                         with REPEAT wrap (the GL state of the equirect texture) */
+  NOVA_HIP_BAKE_IRRADIANCE_RIEMANN = 5 /* irradiance_baker.frag:15-39 (steps of 0.0075 in azimuth and
+                                         zenith) with the map as its environment, sampled by the same rule;
+                                         out is equirect, out_width x out_height x channels, texel (i, j) at
+                                         its centre direction; `samples` is not used */
 };
 typedef struct {
   const float *env;

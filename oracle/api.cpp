@@ -271,6 +271,16 @@ void oracle_bake_irradiance_sampler(const float *env, int w, int h, int c, int o
 
 void oracle_brdf_lut_value(float ndotv, float roughness, int samples, float *ab) { brdf_lut_value(ndotv, roughness, samples, ab[0], ab[1]); }
 void oracle_bake_brdf_lut(int ow, int oh, int samples, int threads, float *out) { bake_brdf_lut(ow, oh, samples, out, threads); }
+void oracle_bake_cubemap(const float *env, int w, int h, int c, int size, int threads, float *out) { bake_cubemap(env, w, h, c, size, out, threads); }
+void oracle_bake_irradiance_riemann(const float *env, int w, int h, int c, int ow, int oh, int threads, float *out) {
+  bake_irradiance_riemann(env, w, h, c, ow, oh, out, threads);
+}
+void oracle_cube_texel_direction(int face, int x, int y, int size, float *out) {
+  const vec3 d = cube_texel_direction(face, x, y, size);
+  out[0] = d.x;
+  out[1] = d.y;
+  out[2] = d.z;
+}
 
 void oracle_bake_prefilter(const float *env, int w, int h, int c, int ow, int oh, float roughness, int samples, int threads, float *out) {
   bake_prefilter(env, w, h, c, ow, oh, roughness, samples, out, threads);

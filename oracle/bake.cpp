@@ -249,4 +249,137 @@ void bake_brdf_lut(int out_w, int out_h, int samples, float *out, int threads) {
   });
 }
 
+// ------------------------------------------------ GL viewer bakes on equirect maps
+// The viewer's cube-map captures (RenderPipeline.cpp:14-26, renderToCubemap :186-206) and
+// its two shaders, restated on the CPU:
+//  * envmap_bake.frag:10-24: a face texel's direction -> sphericalToUv -> the equirect map;
+//  * irradiance_baker.frag:15-39: the Riemann sum over azimuth and zenith steps of 0.0075.
+// Lookup rule (GL sampler2D state of the equirect texture: RenderPipeline's glTexParameteri
+// calls name the cube-map target, so the 2D texture keeps GL's default REPEAT wrap; level 0
+// with GL_LINEAR filtering): texel centres at (i + 0.5) / w, bilinear weights, indices taken
+// modulo the map size. atan, acos, cos and sin are evaluated in double and rounded once.
+
+namespace {
+struct GlEquirect {
+  const float *data;
+  int w, h, c;
+  vec3 texel(int x, int y) const {
+    x %= w;
+    if (x < 0) x += w;
+    y %= h;
+    if (y < 0) y += h;
+    const float *p = data + ((size_t)y * w + x) * c;
+    return {p[0], p[1], p[2]};
+  }
+  // texture(environment_map, vec2(u, v)) with GL_LINEAR and GL_REPEAT
+  vec3 bilinear(float u, float v) const {
+    const float x = u * (float)w - 0.5f, y = v * (float)h - 0.5f;
+    const float fx = std::floor(x), fy = std::floor(y);
+    const float a = x - fx, b = y - fy;
+    const int i0 = (int)fx, j0 = (int)fy;
+    const vec3 t00 = texel(i0, j0), t10 = texel(i0 + 1, j0), t01 = texel(i0, j0 + 1), t11 = texel(i0 + 1, j0 + 1);
+    return ((1.f - a) * (1.f - b)) * t00 + (a * (1.f - b)) * t10 + ((1.f - a) * b) * t01 + (a * b) * t11;
+  }
+  // sphericalToUv (envmap_bake.frag:10-19) of a unit direction, then the lookup
+  vec3 lookup(const vec3 &d) const {
+    const float PI = 3.14159265359f;
+    const float theta = (float)std::atan2((double)-d.z, (double)d.x), phi = (float)std::acos((double)d.y);
+    return bilinear(theta / (2.0f * PI), phi / PI);
+  }
+};
+
+vec3 gl_normalize(const vec3 &v) {  // v * inversesqrt(dot(v, v))
+  const float inv = 1.f / std::sqrt((v.x * v.x + v.y * v.y) + v.z * v.z);
+  return v * inv;
+}
+
+// glm::lookAt basis with the eye at the origin: f = normalize(center), s = normalize(f x up),
+// u = s x f; the face directions of RenderPipeline.cpp:14-26 in capture order
+// (left, right, down, up, back, front).
+void capture_basis(int face, vec3 &s, vec3 &u, vec3 &f) {
+  static const float centers[6][3] = {{-1, 0, 0}, {1, 0, 0}, {0, -1, 0}, {0, 1, 0}, {0, 0, 1}, {0, 0, -1}};
+  static const float ups[6][3] = {{0, 1, 0}, {0, 1, 0}, {0, 0, 1}, {0, 0, -1}, {0, 1, 0}, {0, 1, 0}};
+  f = gl_normalize(vec3(centers[face][0], centers[face][1], centers[face][2]));
+  const vec3 up(ups[face][0], ups[face][1], ups[face][2]);
+  s = gl_normalize(cross(f, up));
+  u = cross(s, f);
+}
+}  // namespace
+
+// Face texel (x, y) of a size x size capture (y = 0 the bottom row, as GL's window
+// coordinates): the 90-degree camera's ray through the pixel centre, ndc = 2 (x + 0.5) /
+// size - 1, meets the unit cube at P = ndc_x s + ndc_y u + f, whose interpolated position the
+// fragment shader normalizes.
+vec3 cube_texel_direction(int face, int x, int y, int size) {
+  vec3 s, u, f;
+  capture_basis(face, s, u, f);
+  const float nx = (float)(2 * x + 1) / (float)size - 1.f, ny = (float)(2 * y + 1) / (float)size - 1.f;
+  return gl_normalize(nx * s + ny * u + f);
+}
+
+// Equirect -> cube map (bakeEnvmapToCubemap, RenderPipeline.cpp:28-62): out holds the six
+// faces in capture order, face-major, rows bottom-up, `env_c` floats per texel (RGB, then
+// zeros).
+void bake_cubemap(const float *env, int env_w, int env_h, int env_c, int size, float *out, int threads) {
+  const GlEquirect m{env, env_w, env_h, env_c};
+  parallel_rows(6 * size, threads, [&](int row) {
+    const int face = row / size, y = row % size;
+    for (int x = 0; x < size; x++) {
+      const vec3 col = m.lookup(cube_texel_direction(face, x, y, size));
+      float *o = out + (((size_t)face * size + y) * size + x) * env_c;
+      o[0] = col.x;
+      o[1] = col.y;
+      o[2] = col.z;
+      for (int k = 3; k < env_c; k++) o[k] = 0.f;
+    }
+  });
+}
+
+// computeIrradiance (irradiance_baker.frag:15-34) at normal n, over the equirect map: the
+// float loops phi = 0, 0.0075, ... <= 2 PI and theta = 0, ... <= PI / 2 exactly as GLSL
+// steps them, each sample weighted by cos(theta) sin(theta), one running sum, then
+// PI * sum / count.
+vec3 riemann_irradiance(const float *env, int env_w, int env_h, int env_c, const vec3 &n) {
+  const GlEquirect m{env, env_w, env_h, env_c};
+  const float PI = 3.14159265358f, zenith_max = PI / 2, azimuth_max = 2 * PI, delta = 0.0075f;
+  const vec3 UP(0.f, 1.f, 0.f);
+  const vec3 right = gl_normalize(cross(UP, n));
+  const vec3 up = gl_normalize(cross(n, right));
+  vec3 irradiance(0.f);
+  int count = 0;
+  for (float phi = 0.f; phi <= azimuth_max; phi += delta) {
+    const float cp = cos_f(phi), sp = sin_f(phi);
+    for (float theta = 0.f; theta <= zenith_max; theta += delta) {
+      const float ct = cos_f(theta), st = sin_f(theta);
+      const float x = cp * st, y = st * sp, z = ct;
+      const vec3 sample_vec = x * right + y * up + z * n;
+      irradiance = irradiance + (m.lookup(gl_normalize(sample_vec)) * ct) * st;
+      count++;
+    }
+  }
+  return (PI * irradiance) * (1.f / (float)count);
+}
+
+// Equirect texel (i, j) -> direction, the inverse of sphericalToUv at the texel centre:
+// azimuth 2 pi (i + 0.5) / w with x = cos, -z = sin; polar pi (j + 0.5) / h with y = cos.
+vec3 equirect_texel_direction(int i, int j, int w, int h) {
+  const double az = 2.0 * M_PI * ((double)i + 0.5) / (double)w, po = M_PI * ((double)j + 0.5) / (double)h;
+  return vec3((float)(std::cos(az) * std::sin(po)), (float)std::cos(po), (float)(-std::sin(az) * std::sin(po)));
+}
+
+// The GL irradiance bake on equirect maps: out_w x out_h x env_c, texel (i, j) at
+// equirect_texel_direction.
+void bake_irradiance_riemann(const float *env, int env_w, int env_h, int env_c, int out_w, int out_h, float *out, int threads) {
+  parallel_rows(out_h, threads, [&](int j) {
+    for (int i = 0; i < out_w; i++) {
+      const vec3 col = riemann_irradiance(env, env_w, env_h, env_c, equirect_texel_direction(i, j, out_w, out_h));
+      float *o = out + ((size_t)j * out_w + i) * env_c;
+      o[0] = col.x;
+      o[1] = col.y;
+      o[2] = col.z;
+      for (int k = 3; k < env_c; k++) o[k] = 0.f;
+    }
+  });
+}
+
 }  // namespace oracle

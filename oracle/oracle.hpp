@@ -137,5 +137,11 @@ void bake_irradiance(const float *env, int env_w, int env_h, int env_c, int out_
 void bake_prefilter(const float *env, int env_w, int env_h, int env_c, int out_w, int out_h, float roughness, int samples, float *out, int threads);
 void brdf_lut_value(float ndotv, float roughness, int samples, float &A, float &B);
 void bake_brdf_lut(int out_w, int out_h, int samples, float *out, int threads);
+// The GL viewer's equirect -> cube map capture and Riemann-sum irradiance, on equirect maps.
+vec3 cube_texel_direction(int face, int x, int y, int size);
+vec3 equirect_texel_direction(int i, int j, int w, int h);
+void bake_cubemap(const float *env, int env_w, int env_h, int env_c, int size, float *out, int threads);
+vec3 riemann_irradiance(const float *env, int env_w, int env_h, int env_c, const vec3 &n);
+void bake_irradiance_riemann(const float *env, int env_w, int env_h, int env_c, int out_w, int out_h, float *out, int threads);
 
 }  // namespace oracle

@@ -69,6 +69,9 @@ def lib():
         L.oracle_bake_irradiance_sampler.argtypes = [f32p, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, f32p]
         L.oracle_brdf_lut_value.argtypes = [C.c_float, C.c_float, C.c_int, f32p]
         L.oracle_bake_brdf_lut.argtypes = [C.c_int, C.c_int, C.c_int, C.c_int, f32p]
+        L.oracle_bake_cubemap.argtypes = [f32p, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, f32p]
+        L.oracle_bake_irradiance_riemann.argtypes = [f32p, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, f32p]
+        L.oracle_cube_texel_direction.argtypes = [C.c_int, C.c_int, C.c_int, C.c_int, f32p]
         L.oracle_bake_prefilter.argtypes = [f32p, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, C.c_float, C.c_int, C.c_int, f32p]
         _LIB = L
     return _LIB
@@ -342,3 +345,26 @@ def bake_brdf_lut(out_w, out_h, samples=1024, threads=8):
     out = np.zeros(out_w * out_h * 2, dtype=np.float32)
     lib().oracle_bake_brdf_lut(out_w, out_h, samples, threads, out)
     return out.reshape(out_h, out_w, 2)
+
+
+def bake_cubemap(env, size, threads=8):
+    """The GL viewer's equirect -> cube map capture: (6, size, size, channels), faces in capture
+    order (left, right, down, up, back, front), rows bottom-up."""
+    data = np.ascontiguousarray(env.data, dtype=np.float32)
+    out = np.zeros(6 * size * size * env.channels, dtype=np.float32)
+    lib().oracle_bake_cubemap(data, env.width, env.height, env.channels, size, threads, out)
+    return out.reshape(6, size, size, env.channels)
+
+
+def bake_irradiance_riemann(env, out_w, out_h, threads=8):
+    """irradiance_baker.frag's Riemann sum on an equirect map: (out_h, out_w, channels)."""
+    data = np.ascontiguousarray(env.data, dtype=np.float32)
+    out = np.zeros(out_w * out_h * env.channels, dtype=np.float32)
+    lib().oracle_bake_irradiance_riemann(data, env.width, env.height, env.channels, out_w, out_h, threads, out)
+    return out.reshape(out_h, out_w, env.channels)
+
+
+def cube_texel_direction(face, x, y, size):
+    out = np.zeros(3, dtype=np.float32)
+    lib().oracle_cube_texel_direction(face, x, y, size, out)
+    return out
