@@ -210,6 +210,7 @@ struct TraceState {
   int best;
   int32_t bpos, node;
   int sp;
+  uint32_t tri_rounds;  // instrumented walks: triangle-loop rounds this lane led for its wave
 };
 
 NOVA_HD void trace_init(TraceState &st, f3 o, f3 d) {
@@ -230,6 +231,7 @@ NOVA_HD void trace_init(TraceState &st, f3 o, f3 d) {
   st.bu = st.bv = 0.f;
   st.node = 0;
   st.sp = 0;
+  st.tri_rounds = 0;
 }
 
 // Where a walk finds its tie tables: the context's pointer by default.
@@ -349,7 +351,13 @@ NOVA_HD bool trace_step(const TraceCtx &tc, Stack &stack, TraceState &st, float 
       for (int32_t i = start; i < start + cnt; i++) {
         const TriMT tr = load_tri(tc.tris + i);
         if (i == start) cnt = __builtin_bit_cast(int32_t, tr.b.w);
-        if (kCount) tri_tests++;
+        if (kCount) {
+          tri_tests++;
+#ifdef __HIP_DEVICE_COMPILE__
+          // the lowest active lane counts the round for its wave
+          if (__lane_id() == (unsigned)(__builtin_ffsll((long long)__ballot(1)) - 1)) st.tri_rounds++;
+#endif
+        }
         float tt, uu, vv;
         if (tri_hit(tr, [&]() { return tie_of(tc); }, st.o, st.d, tnear, st.tbest, tt, uu, vv) &&
             (tt < st.tbest || st.best < 0 || reference_tie_winner_is_new(tie_of(tc), i, st.bpos, st.o, st.d, tt))) {

@@ -36,7 +36,8 @@ struct Queues {
   float *ray[2];      // 6 planes of cap floats each: ox oy oz dx dy dz
   int32_t *slot[2];
   uint32_t *count;    // [0], [1]: queue sizes; [2]: segments traced
-  unsigned long long *stats64;  // [0] node visits, [1] triangle tests, [2] escaped segments (camera-ray launches count into [4], [5])
+  unsigned long long *stats64;  // [0] node visits, [1] triangle tests, [2] escaped segments (camera-ray launches count into [4], [5]);
+                                // per-ray walk: [8] wave loop turns, [9] active lanes summed over turns, [10] wave rounds of the triangle loop
   float4 *hit;        // closest hit per queue entry: (prim id bits, t, u, v), one 16-byte record
   float *T, *L, *A;   // path state by slot: T,L 3 planes of cap each; A alpha
   float *partial;     // frames x W x H x 4
@@ -107,6 +108,7 @@ __global__ __launch_bounds__(kTraceBlock) NOVA_TRACE_OCCUPANCY void extend_kerne
   TraceState ts;
   bool have = false, exhausted = false;
   uint32_t ray = 0;
+  uint32_t wave_turns = 0, lane_turns = 0, tri_rounds = 0;  // instrumented launches only
   for (uint32_t turns = 0; turns < (1u << 30); turns++) {
     const uint64_t idle = __ballot(!have);
     if (!exhausted && __popcll(idle) >= (have_any_fill(turns) ? NOVA_TRACE_REFILL : 1)) {
@@ -124,11 +126,22 @@ __global__ __launch_bounds__(kTraceBlock) NOVA_TRACE_OCCUPANCY void extend_kerne
         }
       }
     }
-    if (__ballot(have) == 0) break;
+    const uint64_t busy = __ballot(have);
+    if (busy == 0) break;
+    if (kCount && lane == 0) {
+      wave_turns++;
+      lane_turns += (uint32_t)__popcll(busy);
+    }
     if (have && trace_step<kCount>(tc, stk, ts, 0.000001f, nv, nt)) {
       q.hit[ray] = make_float4(__builtin_bit_cast(float, (int32_t)(ts.best)), ts.tbest, ts.bu, ts.bv);
       have = false;
+      if (kCount) tri_rounds += ts.tri_rounds;
     }
+  }
+  if (kCount) {
+    atomicAdd(&q.stats64[8], (unsigned long long)wave_turns);
+    atomicAdd(&q.stats64[9], (unsigned long long)lane_turns);
+    atomicAdd(&q.stats64[10], (unsigned long long)tri_rounds);
   }
 #else
   for (uint32_t i = gtid; i < n; i += gridDim.x * blockDim.x) {
@@ -457,7 +470,7 @@ struct ScratchCarve {
 static size_t noalloc_bytes(size_t cap, uint32_t threads, uint32_t stack_bound) {
   const size_t per_path[] = {cap * 12 * sizeof(float), cap * 2 * sizeof(int32_t), cap * 4 * sizeof(float), cap * 7 * sizeof(float),
                              cap * 4 * sizeof(float)};
-  size_t total = 256 + 128;  // counters
+  size_t total = 256 + 256;  // counters
   for (size_t b : per_path) total += (b + 255) / 256 * 256;
   const size_t g = stack_bound > (uint32_t)kLdsStack ? (size_t)(stack_bound - kLdsStack) * threads : 1;
   return total + (g * sizeof(uint32_t) + 255) / 256 * 256;
@@ -613,7 +626,7 @@ static int render_impl(nova_hip_ctx *ctx, const nova_hip_render_params *p, float
   if (!rays) return check_hip(ctx, e, "queue alloc");
   int32_t *slots = (int32_t *)piece(1, cap * 2 * sizeof(int32_t));
   if (!slots) return check_hip(ctx, e, "queue alloc");
-  uint32_t *counts = (uint32_t *)piece(2, 128);
+  uint32_t *counts = (uint32_t *)piece(2, 256);
   if (!counts) return check_hip(ctx, e, "counter alloc");
   unsigned long long *stats64 = (unsigned long long *)(counts + 8);
   float *hits = (float *)piece(3, cap * 4 * sizeof(float));
@@ -675,9 +688,9 @@ static int render_impl(nova_hip_ctx *ctx, const nova_hip_render_params *p, float
   const uint32_t shade_blocks[2] = {(uint32_t)std::min<size_t>((cap + kShadeBlock - 1) / kShadeBlock, (size_t)cus * shade_per_cu[0]),
                                     (uint32_t)std::min<size_t>((cap + kShadeBlock - 1) / kShadeBlock, (size_t)cus * shade_per_cu[1])};
   if (noalloc)
-    hipLaunchKernelGGL(zero_words_kernel, dim3(1), dim3(64), 0, s, counts, 32u);
+    hipLaunchKernelGGL(zero_words_kernel, dim3(1), dim3(64), 0, s, counts, 64u);
   else
-    NOVA_HIP_CHECK(ctx, hipMemsetAsync(counts, 0, 128, s));
+    NOVA_HIP_CHECK(ctx, hipMemsetAsync(counts, 0, 256, s));
 
   const bool timing = (p->flags & NOVA_HIP_RENDER_TIMING) != 0;
   const bool counting = (p->flags & NOVA_HIP_RENDER_COUNT_TRAVERSAL) != 0;
@@ -778,7 +791,7 @@ static int render_impl(nova_hip_ctx *ctx, const nova_hip_render_params *p, float
   hipEventDestroy(ev0);
   hipEventDestroy(ev1);
   uint32_t seg = 0;
-  unsigned long long s64[6] = {0, 0, 0, 0, 0, 0};
+  unsigned long long s64[16] = {};
   NOVA_HIP_CHECK(ctx, hipMemcpy(&seg, counts + 2, sizeof(uint32_t), hipMemcpyDeviceToHost));
   NOVA_HIP_CHECK(ctx, hipMemcpy(s64, stats64, sizeof(s64), hipMemcpyDeviceToHost));
   float ext_ms = 0.f, shade_ms = 0.f, ext0_ms = 0.f;
@@ -804,6 +817,9 @@ static int render_impl(nova_hip_ctx *ctx, const nova_hip_render_params *p, float
   ctx->stats.camera_rays = camera_rays;
   ctx->stats.node_visits_first = counting ? s64[4] : 0;
   ctx->stats.triangle_tests_first = counting ? s64[5] : 0;
+  ctx->stats.walk_wave_turns = counting ? s64[8] + s64[12] : 0;
+  ctx->stats.walk_lane_turns = counting ? s64[9] + s64[13] : 0;
+  ctx->stats.walk_tri_rounds = counting ? s64[10] + s64[14] : 0;
   return NOVA_HIP_SUCCESS;
 }
 

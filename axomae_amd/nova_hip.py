@@ -110,7 +110,8 @@ class RenderStats(C.Structure):
     _fields_ = [("total_ms", C.c_float), ("extend_ms", C.c_float), ("shade_ms", C.c_float), ("other_ms", C.c_float), ("segments", C.c_uint64),
                 ("extend_launches", C.c_uint64), ("node_visits", C.c_uint64), ("triangle_tests", C.c_uint64), ("escaped", C.c_uint64),
                 ("extend_first_ms", C.c_float), ("camera_rays", C.c_uint64), ("node_visits_first", C.c_uint64),
-                ("triangle_tests_first", C.c_uint64)]
+                ("triangle_tests_first", C.c_uint64), ("walk_wave_turns", C.c_uint64), ("walk_lane_turns", C.c_uint64),
+                ("walk_tri_rounds", C.c_uint64)]
 
 
 BNODE_DTYPE = np.dtype([("min", np.float32, 3), ("max", np.float32, 3), ("left", np.int32), ("count", np.int32)])
@@ -409,34 +410,40 @@ def bake(kind, envmaps, out_sizes, samples, roughness=None):
 
 def bake_batch(jobs, samples, keep_on_device=False):
     """One launch for every job: jobs are (envmap Texture or device tensor with (w, h, c), out_w, out_h, kind, roughness).
-    Returns the outputs (numpy, or device tensors with keep_on_device) in job order."""
+    Returns the outputs (numpy, or device tensors with keep_on_device) in job order. The outputs
+    are views of one device allocation (each job's output starts on a 256-byte boundary); the
+    kernels write every element, so it is not cleared."""
     torch = _torch()
     L = load()
     arr = (BakeJob * len(jobs))()
-    keep, outs = [], []
+    keep, layout, offset = [], [], 0
+    for env, ow, oh, kind, rough in jobs:
+        c = 2 if kind == BAKE_BRDF_LUT else (env[1][2] if isinstance(env, tuple) else env.channels)
+        faces = 6 if kind == BAKE_CUBEMAP else 1  # a cube map: six ow x ow faces
+        layout.append((offset, ow, oh, c, faces))
+        offset += -(-faces * oh * ow * c // 64) * 64
+    pool = torch.empty(max(offset, 1), dtype=torch.float32, device="cuda")
+    base = pool.data_ptr()
     for k, (env, ow, oh, kind, rough) in enumerate(jobs):
+        off = layout[k][0]
         if kind == BAKE_BRDF_LUT:  # no map: two output channels (A, B)
-            out = torch.zeros(oh * ow * 2, dtype=torch.float32, device="cuda")
-            outs.append((out, ow, oh, 2, 1))
-            arr[k] = BakeJob(None, 0, 0, 2, out.data_ptr(), ow, oh, kind, 0.0)
+            arr[k] = BakeJob(None, 0, 0, 2, base + 4 * off, ow, oh, kind, 0.0)
             continue
         if isinstance(env, tuple):
             src, (w, h, c) = env
         else:
             src = torch.as_tensor(np.ascontiguousarray(env.data, dtype=np.float32)).to("cuda")
             w, h, c = env.width, env.height, env.channels
-        faces = 6 if kind == BAKE_CUBEMAP else 1  # a cube map: six ow x ow faces
-        out = torch.zeros(faces * oh * ow * c, dtype=torch.float32, device="cuda")
         keep.append(src)
-        outs.append((out, ow, oh, c, faces))
-        arr[k] = BakeJob(src.data_ptr(), w, h, c, out.data_ptr(), ow, oh, kind, float(rough))
+        arr[k] = BakeJob(src.data_ptr(), w, h, c, base + 4 * off, ow, oh, kind, float(rough))
     rc = L.nova_hip_bake_batch(arr, len(jobs), samples, _stream(torch))
     if rc:
         raise NovaHipError(rc, "bake_batch")
     shape = lambda ow, oh, c, faces: (faces, oh, ow, c) if faces > 1 else (oh, ow, c)
+    outs = [pool[off:off + f * oh * ow * c].view(*shape(ow, oh, c, f)) for off, ow, oh, c, f in layout]
     if keep_on_device:
-        return [o.view(*shape(ow, oh, c, f)) for o, ow, oh, c, f in outs]
-    return [o.cpu().numpy().reshape(shape(ow, oh, c, f)) for o, ow, oh, c, f in outs]
+        return outs
+    return [o.cpu().numpy() for o in outs]
 
 
 class BatchItem(C.Structure):

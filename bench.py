@@ -328,7 +328,8 @@ def main():
     torch.cuda.synchronize()
     cst = eng.render_stats()
     log(f"[bench] instrumented job: segments {cst.segments} (camera {cst.camera_rays}), node visits {cst.node_visits} "
-        f"(camera {cst.node_visits_first}), triangle tests {cst.triangle_tests} (camera {cst.triangle_tests_first}), escaped {cst.escaped}")
+        f"(camera {cst.node_visits_first}), triangle tests {cst.triangle_tests} (camera {cst.triangle_tests_first}), escaped {cst.escaped}; "
+        f"per-ray walk: {cst.walk_wave_turns} wave turns, {cst.walk_lane_turns} lane steps, {cst.walk_tri_rounds} triangle rounds")
 
     for _ in range(args.warmup):
         eng.render(W, H, F, args.depth, rr_start=args.rr_start)
@@ -397,6 +398,11 @@ def main():
         "path_algorithmic_GBs": round(path_bytes * K / t / 1e9, 1),
         "bvh": {"build_ms": round(info.build_ms, 2), "binary_nodes": info.binary_nodes, "binary_depth": info.binary_depth,
                 "wide_nodes": info.wide_nodes, "wide_depth": info.wide_depth, "leaves": info.leaves, "upload_ms": round(upload_s * 1e3, 1)},
+        "per_ray_walk_simd": {"lanes_stepping": round(cst.walk_lane_turns / max(64.0 * cst.walk_wave_turns, 1.0), 4),
+                              "lanes_testing_per_triangle_round": round(tst_r / max(64.0 * cst.walk_tri_rounds, 1.0), 4),
+                              "triangle_rounds_per_turn": round(cst.walk_tri_rounds / max(cst.walk_wave_turns, 1), 4),
+                              "note": "instrumented job: fraction of a wave's lanes taking a node step per loop turn, and of lanes "
+                                      "testing a triangle per round of the triangle loop"},
         "per_segment": {"node_visits": round(cst.node_visits / max(cst.segments, 1), 2),
                         "triangle_tests": round(cst.triangle_tests / max(cst.segments, 1), 2),
                         "segments_per_path": round(cst.segments / paths, 3)},

@@ -231,6 +231,11 @@ typedef struct {
   float extend_first_ms;     /* with NOVA_HIP_RENDER_TIMING: the camera-ray extend launches */
   uint64_t camera_rays;      /* rays entering the camera-ray launches */
   uint64_t node_visits_first, triangle_tests_first; /* with NOVA_HIP_RENDER_COUNT_TRAVERSAL */
+  /* With NOVA_HIP_RENDER_COUNT_TRAVERSAL, the per-ray walk's SIMD use: loop turns of its
+   * waves, active lanes summed over those turns (lane_turns / (64 * wave_turns) is the
+   * fraction of lanes doing a node step), and wave rounds of the triangle loop (tests /
+   * (64 * rounds) is the fraction of lanes testing a triangle in a round). */
+  uint64_t walk_wave_turns, walk_lane_turns, walk_tri_rounds;
 } nova_hip_render_stats;
 int nova_hip_render_stats_get(nova_hip_ctx *ctx, nova_hip_render_stats *out);
 
