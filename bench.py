@@ -292,8 +292,8 @@ def main():
                     help="before the render, copy a buffer of this size 3 times (a copy of known bytes for counter calibration)")
     args = ap.parse_args()
     if args.gpus != 1:
-        print(f"bench.py measures one GPU: every box here has one MI355X (asked for --gpus {args.gpus}); "
-              "the multi-GPU sharding of this path is described in DESIGN.md", file=sys.stderr)
+        print(f"bench.py measures one GPU (asked for --gpus {args.gpus}): the multi-GPU split of frames and scenes "
+              "is a design, not built here", file=sys.stderr)
         sys.exit(2)
 
     import numpy as np
