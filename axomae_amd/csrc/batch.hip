@@ -135,30 +135,34 @@ __global__ __launch_bounds__(kTraceBlock) __attribute__((amdgpu_waves_per_eu(6, 
   const uint32_t gtid = blockIdx.x * blockDim.x + threadIdx.x;
   if (gtid == 0) st->segments += n;
   LdsStack stk{lds, gstack, gstride, gtid, threadIdx.x};
+  // first rays by thread index, refills claimed past them (as extend_kernel)
   const uint32_t lane = threadIdx.x & 63;
+  const uint32_t nthreads = gridDim.x * blockDim.x;
   TraceState ts;
-  bool have = false, exhausted = false;
+  bool have = false, exhausted = nthreads >= n;
   uint32_t ray = 0;
   int32_t sid = 0;
   TraceCtx tc{nullptr, nullptr, nullptr, 0, nullptr};  // the lane's scene, loaded when it takes a ray (not at every step)
   uint32_t nv = 0, nt = 0;
   for (uint32_t turns = 0; turns < (1u << 30); turns++) {
     const uint64_t idle = __ballot(!have);
-    if (!exhausted && (uint32_t)__popcll(idle) >= (turns > 0 ? kBatchRefill : 1u)) {
+    uint32_t base = gtid - lane;  // first turn: lane l takes ray gtid
+    bool fill = turns == 0;
+    if (!fill && !exhausted && (uint32_t)__popcll(idle) >= kBatchRefill) {
       const int leader = __builtin_ffsll((long long)idle) - 1;
-      uint32_t base = 0;
-      if ((int)lane == leader) base = atomicAdd(&st->work, (uint32_t)__popcll(idle));
+      if ((int)lane == leader) base = nthreads + atomicAdd(&st->work, (uint32_t)__popcll(idle));
       base = __shfl(base, leader, 64);
       if (base >= n) exhausted = true;
-      if (!have) {
-        const uint32_t i = base + __builtin_amdgcn_mbcnt_hi((uint32_t)(idle >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)idle, 0));
-        if (i < n) {
-          trace_init(ts, mk3(r[i], r[cap + i], r[2 * cap + i]), mk3(r[3 * cap + i], r[4 * cap + i], r[5 * cap + i]));
-          ray = i;
-          sid = q.sid[which][i];
-          tc = scenes[sid].tc;
-          have = true;
-        }
+      fill = true;
+    }
+    if (fill && !have) {
+      const uint32_t i = base + __builtin_amdgcn_mbcnt_hi((uint32_t)(idle >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)idle, 0));
+      if (i < n) {
+        trace_init(ts, mk3(r[i], r[cap + i], r[2 * cap + i]), mk3(r[3 * cap + i], r[4 * cap + i], r[5 * cap + i]));
+        ray = i;
+        sid = q.sid[which][i];
+        tc = scenes[sid].tc;
+        have = true;
       }
     }
     if (__ballot(have) == 0) break;

@@ -78,9 +78,6 @@ __global__ __launch_bounds__(256) void generate_kernel(FrameParams fp, const uin
 #ifndef NOVA_TRACE_REFILL
 #define NOVA_TRACE_REFILL 56  // idle lanes that trigger a refill in the extend kernel (0: grid-stride, one ray per lane at a time)
 #endif
-// the first turn fills every lane; later turns wait for NOVA_TRACE_REFILL idle lanes
-__device__ __forceinline__ bool have_any_fill(uint32_t turn) { return turn > 0; }
-
 #ifndef NOVA_TRACE_WAVES
 #define NOVA_TRACE_WAVES 6  // waves per SIMD the extend kernel is compiled for (0: compiler's choice)
 #endif
@@ -104,26 +101,32 @@ __global__ __launch_bounds__(kTraceBlock) NOVA_TRACE_OCCUPANCY void extend_kerne
   // One traversal step per loop turn; a lane whose ray has ended takes the next unclaimed
   // ray as soon as NOVA_TRACE_REFILL lanes of its wave are idle (one atomic per refill for
   // the wave), so lanes do not wait for the wave's longest ray.
+  // The first rays go one per lane by thread index, without an atomic: a launch over a
+  // nearly empty queue (the last bounces) then makes no claims at all, where one claim per
+  // wave on the shared counter cost about 90 us per launch. Refills claim rays past those.
   const uint32_t lane = threadIdx.x & 63;
+  const uint32_t nthreads = gridDim.x * blockDim.x;
   TraceState ts;
-  bool have = false, exhausted = false;
+  bool have = false, exhausted = nthreads >= n;
   uint32_t ray = 0;
   uint32_t wave_turns = 0, lane_turns = 0, tri_rounds = 0;  // instrumented launches only
   for (uint32_t turns = 0; turns < (1u << 30); turns++) {
     const uint64_t idle = __ballot(!have);
-    if (!exhausted && __popcll(idle) >= (have_any_fill(turns) ? NOVA_TRACE_REFILL : 1)) {
+    uint32_t base = gtid - lane;  // first turn: every lane idle, lane l takes ray gtid
+    bool fill = turns == 0;
+    if (!fill && !exhausted && __popcll(idle) >= NOVA_TRACE_REFILL) {
       const int leader = __builtin_ffsll((long long)idle) - 1;
-      uint32_t base = 0;
-      if ((int)lane == leader) base = atomicAdd(&q.count[kWorkCounter], (uint32_t)__popcll(idle));
+      if ((int)lane == leader) base = nthreads + atomicAdd(&q.count[kWorkCounter], (uint32_t)__popcll(idle));
       base = __shfl(base, leader, 64);
       if (base >= n) exhausted = true;
-      if (!have) {
-        const uint32_t i = base + __builtin_amdgcn_mbcnt_hi((uint32_t)(idle >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)idle, 0));
-        if (i < n) {
-          trace_init(ts, mk3(r[i], r[cap + i], r[2 * cap + i]), mk3(r[3 * cap + i], r[4 * cap + i], r[5 * cap + i]));
-          ray = i;
-          have = true;
-        }
+      fill = true;
+    }
+    if (fill && !have) {
+      const uint32_t i = base + __builtin_amdgcn_mbcnt_hi((uint32_t)(idle >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)idle, 0));
+      if (i < n) {
+        trace_init(ts, mk3(r[i], r[cap + i], r[2 * cap + i]), mk3(r[3 * cap + i], r[4 * cap + i], r[5 * cap + i]));
+        ray = i;
+        have = true;
       }
     }
     const uint64_t busy = __ballot(have);
