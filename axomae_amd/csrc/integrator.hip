@@ -16,6 +16,7 @@
 // L unchanged as soon as c_k has a non-finite component (V_{k+1} is always finite).
 // Alpha follows the same recursion (1 per scattering level plus the terminal alpha).
 #include <algorithm>
+#include <cstdlib>
 #include <mutex>
 #include <vector>
 
@@ -31,13 +32,15 @@ namespace nova_hip {
 // ------------------------------------------------------------------ kernels
 
 constexpr int kWorkCounter = 4;  // count[] slot of the extend work counter, zeroed before each extend
+constexpr int kTailSlot = 5;     // count[] slot: 0, or 1 + the bounce from which the batch's paths finish in tail_kernel
 
 struct Queues {
   float *ray[2];      // 6 planes of cap floats each: ox oy oz dx dy dz
   int32_t *slot[2];
   uint32_t *count;    // [0], [1]: queue sizes; [2]: segments traced
   unsigned long long *stats64;  // [0] node visits, [1] triangle tests, [2] escaped segments (camera-ray launches count into [4], [5]);
-                                // per-ray walk: [8] wave loop turns, [9] active lanes summed over turns, [10] wave rounds of the triangle loop
+                                // per-ray walk: [8] wave loop turns, [9] active lanes summed over turns, [10] wave rounds of the triangle loop;
+                                // tail_kernel: [16] segments, [17] node visits, [18] triangle tests
   float4 *hit;        // closest hit per queue entry: (prim id bits, t, u, v), one 16-byte record
   float *T, *L, *A;   // path state by slot: T,L 3 planes of cap each; A alpha
   float *partial;     // frames x W x H x 4
@@ -72,6 +75,7 @@ __global__ __launch_bounds__(256) void generate_kernel(FrameParams fp, const uin
     q.count[0] = active;
     q.count[1] = 0;
     q.count[kWorkCounter] = 0;
+    q.count[kTailSlot] = 0;
   }
 }
 
@@ -88,13 +92,20 @@ __global__ __launch_bounds__(256) void generate_kernel(FrameParams fp, const uin
 #endif
 
 template <bool kCount>
-__global__ __launch_bounds__(kTraceBlock) NOVA_TRACE_OCCUPANCY void extend_kernel(TraceCtx tc, Queues q, int which) {
+__global__ __launch_bounds__(kTraceBlock) NOVA_TRACE_OCCUPANCY void extend_kernel(TraceCtx tc, Queues q, int which, int bounce, uint32_t tail_rays) {
   __shared__ uint32_t lds[kLdsStack * kTraceBlock];
   uint32_t nv = 0, nt = 0;
   const uint32_t n = q.count[which];
   const float *r = q.ray[which];
   const size_t cap = q.cap;
   const uint32_t gtid = blockIdx.x * blockDim.x + threadIdx.x;
+  if (tail_rays) {  // this bounce may hand the batch's remaining paths to tail_kernel
+    if (q.count[kTailSlot] != 0) return;  // an earlier bounce did
+    if (n <= tail_rays) {
+      if (gtid == 0) q.count[kTailSlot] = (uint32_t)bounce + 1u;
+      return;
+    }
+  }
   if (gtid == 0) q.count[2] += n;
   LdsStack stk{lds, tc.gstack, tc.gstride, gtid, threadIdx.x};
 #if NOVA_TRACE_REFILL > 0
@@ -226,6 +237,7 @@ constexpr int kShadeBlock = 256;  // 512 and 1024 measured slower (barrier waits
 template <bool kAdj>
 __global__ __launch_bounds__(kShadeBlock) NOVA_SHADE_OCCUPANCY void shade_kernel(ShadeParams sp, ShadeCtx sc, const uint32_t *__restrict__ sobol, Queues q, int which,
                                                     AdjointCtx adj) {
+  if (q.count[kTailSlot] != 0) return;  // tail_kernel finishes this batch's paths
   // nibble tables of this scatter's sampler dimensions (dim0 .. dim0+4)
   __shared__ uint32_t sob_tab[5 * 128];
   sobol_nibble_fill(sobol, sp.dim0, 5, sob_tab, threadIdx.x, blockDim.x);
@@ -315,6 +327,68 @@ __global__ __launch_bounds__(kShadeBlock) NOVA_SHADE_OCCUPANCY void shade_kernel
     }
   }
   if (escapes != 0 && (threadIdx.x & 63) == 0) atomicAdd(&q.stats64[2], (unsigned long long)escapes);
+}
+
+// The last bounces of a batch, once few paths remain (the extend launch of bounce `sp.bounce`
+// found at most tail_rays rays and recorded it in count[kTailSlot]): one path per lane from
+// that bounce to the end, the walk (trace_closest) and shade_path in turn, the radiance
+// written when the path ends. Per-bounce launches would each wait for their slowest ray;
+// here a path's bounces follow each other at once. Every path takes the same steps with the
+// same arithmetic as in the per-bounce launches, so the image is the same. Dynamic LDS: the
+// traversal stacks, then the nibble tables of the sampler dimensions from sp.dim0 on.
+template <bool kCount>
+__global__ __launch_bounds__(kTraceBlock) void tail_kernel(TraceCtx tc, ShadeParams sp, int max_depth, int rr_start, ShadeCtx sc,
+                                                          const uint32_t *__restrict__ sobol, Queues q, int which, int ndims) {
+  extern __shared__ uint32_t tail_lds[];
+  if (q.count[kTailSlot] != (uint32_t)sp.bounce + 1u) return;
+  uint32_t *sob_tab = tail_lds + kLdsStack * kTraceBlock;
+  sobol_nibble_fill(sobol, sp.dim0, ndims, sob_tab, threadIdx.x, blockDim.x);
+  __syncthreads();
+  const SobolNibbles sobn{sob_tab, sp.dim0};
+  const uint32_t n = q.count[which];
+  const float *rin = q.ray[which];
+  const size_t cap = q.cap;
+  const uint32_t gtid = blockIdx.x * blockDim.x + threadIdx.x;
+  LdsStack stk{tail_lds, tc.gstack, tc.gstride, gtid, threadIdx.x};
+  uint32_t nv = 0, nt = 0, segs = 0, escapes = 0;
+  for (uint32_t i = gtid; i < n; i += gridDim.x * blockDim.x) {
+    const int32_t pid = q.slot[which][i];
+    f3 o = mk3(rin[i], rin[cap + i], rin[2 * cap + i]);
+    f3 d = mk3(rin[3 * cap + i], rin[4 * cap + i], rin[5 * cap + i]);
+    f3 T = mk3(q.T[pid], q.T[cap + pid], q.T[2 * cap + pid]);
+    f3 L = mk3(q.L[pid], q.L[cap + pid], q.L[2 * cap + pid]);
+    float A = q.A[pid];
+    const uint32_t slot = path_slot(sp.fp, (uint32_t)pid);
+    ShadeParams s = sp;
+    for (int k = sp.bounce; k <= max_depth; k++) {
+      s.bounce = k;
+      s.last = k == max_depth ? 1 : 0;
+      s.rr = (rr_start >= 0 && k >= rr_start) ? 1 : 0;
+      float t, u, v;
+      const int prim = trace_closest<kCount>(tc, stk, o, d, 0.000001f, t, u, v, nv, nt);
+      segs++;
+      f3 no = mk3(0, 0, 0), nd = mk3(0, 0, 0);
+      bool escaped = false;
+      const bool cont = shade_path<false>(s, sc, sobn, slot, o, d, prim, t, u, v, T, L, A, no, nd, escaped);
+      escapes += escaped ? 1u : 0u;
+      if (!cont) break;
+      s.dim0 += 4u + (s.rr ? 1u : 0u);
+      o = no;
+      d = nd;
+    }
+    write_partial(q, (uint32_t)pid, L.x, L.y, L.z, A);
+  }
+  if (segs) {
+    atomicAdd(&q.count[2], segs);
+    atomicAdd(&q.stats64[16], (unsigned long long)segs);
+  }
+  if (escapes) atomicAdd(&q.stats64[2], (unsigned long long)escapes);
+  if (kCount && (nv | nt)) {
+    atomicAdd(&q.stats64[0], (unsigned long long)nv);
+    atomicAdd(&q.stats64[1], (unsigned long long)nt);
+    atomicAdd(&q.stats64[17], (unsigned long long)nv);
+    atomicAdd(&q.stats64[18], (unsigned long long)nt);
+  }
 }
 
 // Between an extend and a shade launch: the shade's output queue and the next extend's work
@@ -449,6 +523,19 @@ static void *noalloc_scratch_of(int device) {
   if (device < 0 || device >= kMaxDevices) return nullptr;
   std::lock_guard<std::mutex> lock(g_scratch_mutex);
   return g_scratch_done[device] ? g_scratch_ptr[device] : nullptr;
+}
+
+#ifndef NOVA_TAIL_RAYS
+#define NOVA_TAIL_RAYS 262144  // a bounce with at most this many rays hands the batch's paths to tail_kernel (0: never)
+#endif
+// NOVA_TAIL_RAYS, or the NOVA_HIP_TAIL_RAYS environment variable when set (read once per
+// process; a tuning knob).
+static uint32_t tail_rays_setting() {
+  static const uint32_t v = [] {
+    const char *e = std::getenv("NOVA_HIP_TAIL_RAYS");
+    return e ? (uint32_t)std::strtoul(e, nullptr, 10) : (uint32_t)NOVA_TAIL_RAYS;
+  }();
+  return v;
 }
 
 // Counters start at zero through a kernel, not a memset, so the call can be captured.
@@ -697,7 +784,8 @@ static int render_impl(nova_hip_ctx *ctx, const nova_hip_render_params *p, float
 
   const bool timing = (p->flags & NOVA_HIP_RENDER_TIMING) != 0;
   const bool counting = (p->flags & NOVA_HIP_RENDER_COUNT_TRAVERSAL) != 0;
-  std::vector<hipEvent_t> kev;  // pairs around extend, then shade launches
+  std::vector<hipEvent_t> kev;  // pairs of events around launches; kkind[i]: what pair i timed
+  std::vector<int> kkind;       // 0 extend (bounce 0), 1 extend, 2 shade, 3 tail
   auto stamp = [&](void) -> int {
     hipEvent_t ev;
     NOVA_HIP_CHECK(ctx, hipEventCreate(&ev));
@@ -705,6 +793,9 @@ static int render_impl(nova_hip_ctx *ctx, const nova_hip_render_params *p, float
     NOVA_HIP_CHECK(ctx, hipEventRecord(ev, s));
     return NOVA_HIP_SUCCESS;
   };
+  // The last bounces go to tail_kernel once at most tail_rays rays remain (the primal
+  // integrator only; the adjoint replay keeps per-bounce launches).
+  const uint32_t tail_rays = (adj || (p->flags & NOVA_HIP_RENDER_NO_TAIL)) ? 0u : tail_rays_setting();
   hipEvent_t ev0 = nullptr, ev1 = nullptr;
   if (!noalloc) {
     NOVA_HIP_CHECK(ctx, hipEventCreate(&ev0));
@@ -746,7 +837,14 @@ static int render_impl(nova_hip_ctx *ctx, const nova_hip_render_params *p, float
     const bool coherent0 = fp.frame_minor && frames >= NOVA_COHERENT_MIN_FRAMES && !(p->flags & NOVA_HIP_RENDER_PER_RAY_TRAVERSAL) &&
                            traversal_stack_bound(ctx) < (uint32_t)kWaveStack;
     for (int k = 0; k <= p->max_depth; k++) {
+      // this bounce may start the tail: the sampler dimensions from its scatter to the last
+      // (4 per scatter, 5 where it rolls RR) as nibble tables beside the traversal stacks
+      int ndims = 0;
+      for (int j = k; j <= p->max_depth; j++) ndims += 4 + ((p->rr_start >= 0 && j >= p->rr_start) ? 1 : 0);
+      const size_t tail_lds = ((size_t)kLdsStack * kTraceBlock + (size_t)ndims * 128) * sizeof(uint32_t);
+      const bool tail_here = tail_rays > 0 && k >= 1 && k < p->max_depth && tail_lds <= 65536;
       if (timing && (st = stamp())) return st;
+      if (timing) kkind.push_back(k == 0 ? 0 : 1);
       if (pass == 1 && replay.hit_cache && (uint32_t)k < replay.cache_bounces) {
         // the replay's rays are pass 0's: their hits are in the cache
       } else if (k == 0 && coherent0) {
@@ -756,9 +854,9 @@ static int render_impl(nova_hip_ctx *ctx, const nova_hip_render_params *p, float
         else
           hipLaunchKernelGGL(extend_coherent_kernel<false>, dim3(cblocks), dim3(kCoherentBlock), 0, s, tc, q, which);
       } else if (counting) {
-        hipLaunchKernelGGL(extend_kernel<true>, dim3(trace_blocks), dim3(kTraceBlock), 0, s, tc, k == 0 ? q0 : q, which);
+        hipLaunchKernelGGL(extend_kernel<true>, dim3(trace_blocks), dim3(kTraceBlock), 0, s, tc, k == 0 ? q0 : q, which, k, tail_here ? tail_rays : 0u);
       } else {
-        hipLaunchKernelGGL(extend_kernel<false>, dim3(trace_blocks), dim3(kTraceBlock), 0, s, tc, q, which);
+        hipLaunchKernelGGL(extend_kernel<false>, dim3(trace_blocks), dim3(kTraceBlock), 0, s, tc, q, which, k, tail_here ? tail_rays : 0u);
       }
       if (timing && (st = stamp())) return st;
       hipLaunchKernelGGL(reset_count_kernel, dim3(1), dim3(1), 0, s, counts, which ^ 1);
@@ -769,12 +867,28 @@ static int render_impl(nova_hip_ctx *ctx, const nova_hip_render_params *p, float
       spp.rr = (p->rr_start >= 0 && k >= p->rr_start) ? 1 : 0;
       spp.dim0 = dim;
       if (timing && (st = stamp())) return st;
+      if (timing) kkind.push_back(2);
       if (pass == 0)
         hipLaunchKernelGGL(shade_kernel<false>, dim3(shade_blocks[0]), dim3(kShadeBlock), 0, s, spp, sc, ctx->sobol_table, q, which, replay);
       else
         hipLaunchKernelGGL(shade_kernel<true>, dim3(shade_blocks[1]), dim3(kShadeBlock), 0, s, spp, sc, ctx->sobol_table, q, which, replay);
       if (timing && (st = stamp())) return st;
       NOVA_HIP_CHECK(ctx, hipGetLastError());
+      if (tail_here) {
+        // at most trace_threads lanes: the overflow stacks are sized for the extend grid
+        const uint32_t tblocks = std::min<uint32_t>({(uint32_t)((std::min<size_t>(tail_rays, cap) + kTraceBlock - 1) / kTraceBlock), trace_blocks,
+                                                     (uint32_t)cus * 4});
+        if (timing && (st = stamp())) return st;
+        if (timing) kkind.push_back(3);
+        if (counting)
+          hipLaunchKernelGGL(tail_kernel<true>, dim3(tblocks), dim3(kTraceBlock), tail_lds, s, tc, spp, p->max_depth, p->rr_start, sc, ctx->sobol_table, q,
+                             which, ndims);
+        else
+          hipLaunchKernelGGL(tail_kernel<false>, dim3(tblocks), dim3(kTraceBlock), tail_lds, s, tc, spp, p->max_depth, p->rr_start, sc, ctx->sobol_table, q,
+                             which, ndims);
+        if (timing && (st = stamp())) return st;
+        NOVA_HIP_CHECK(ctx, hipGetLastError());
+      }
       dim += 4 + (spp.rr ? 1 : 0);
       which ^= 1;
       launches++;
@@ -794,23 +908,27 @@ static int render_impl(nova_hip_ctx *ctx, const nova_hip_render_params *p, float
   hipEventDestroy(ev0);
   hipEventDestroy(ev1);
   uint32_t seg = 0;
-  unsigned long long s64[16] = {};
+  unsigned long long s64[20] = {};
   NOVA_HIP_CHECK(ctx, hipMemcpy(&seg, counts + 2, sizeof(uint32_t), hipMemcpyDeviceToHost));
   NOVA_HIP_CHECK(ctx, hipMemcpy(s64, stats64, sizeof(s64), hipMemcpyDeviceToHost));
-  float ext_ms = 0.f, shade_ms = 0.f, ext0_ms = 0.f;
-  for (size_t k = 0; k + 3 < kev.size(); k += 4) {
-    float a = 0.f, b = 0.f;
-    hipEventElapsedTime(&a, kev[k], kev[k + 1]);
-    hipEventElapsedTime(&b, kev[k + 2], kev[k + 3]);
-    ext_ms += a;
-    shade_ms += b;
-    if ((k / 4) % (size_t)(p->max_depth + 1) == 0) ext0_ms += a;  // bounce 0 of a batch pass
+  float ext_ms = 0.f, shade_ms = 0.f, ext0_ms = 0.f, tail_ms = 0.f;
+  for (size_t k = 0; k < kkind.size() && 2 * k + 1 < kev.size(); k++) {
+    float a = 0.f;
+    hipEventElapsedTime(&a, kev[2 * k], kev[2 * k + 1]);
+    if (kkind[k] <= 1) ext_ms += a;
+    if (kkind[k] == 0) ext0_ms += a;  // bounce 0 of a batch pass
+    if (kkind[k] == 2) shade_ms += a;
+    if (kkind[k] == 3) tail_ms += a;
   }
   for (hipEvent_t ev : kev) hipEventDestroy(ev);
   ctx->stats.total_ms = ms;
   ctx->stats.extend_ms = ext_ms;
   ctx->stats.shade_ms = shade_ms;
-  ctx->stats.other_ms = timing ? ms - ext_ms - shade_ms : 0.f;
+  ctx->stats.other_ms = timing ? ms - ext_ms - shade_ms - tail_ms : 0.f;
+  ctx->stats.tail_ms = timing ? tail_ms : 0.f;
+  ctx->stats.tail_segments = s64[16];
+  ctx->stats.tail_node_visits = counting ? s64[17] : 0;
+  ctx->stats.tail_triangle_tests = counting ? s64[18] : 0;
   ctx->stats.segments = seg;
   ctx->stats.extend_launches = launches;
   ctx->stats.node_visits = counting ? s64[0] + s64[4] : 0;

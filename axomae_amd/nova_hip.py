@@ -104,6 +104,7 @@ RENDER_TIMING = 1
 RENDER_COUNT_TRAVERSAL = 2
 RENDER_PER_RAY_TRAVERSAL = 4  # camera rays one walk per ray (testing and tuning)
 ADJOINT_RETRACE = 8  # the adjoint replay traces again instead of reusing the primal's hits (testing)
+RENDER_NO_TAIL = 16  # one extend and one shade launch per bounce to the end (testing and tuning)
 
 
 class RenderStats(C.Structure):
@@ -111,7 +112,8 @@ class RenderStats(C.Structure):
                 ("extend_launches", C.c_uint64), ("node_visits", C.c_uint64), ("triangle_tests", C.c_uint64), ("escaped", C.c_uint64),
                 ("extend_first_ms", C.c_float), ("camera_rays", C.c_uint64), ("node_visits_first", C.c_uint64),
                 ("triangle_tests_first", C.c_uint64), ("walk_wave_turns", C.c_uint64), ("walk_lane_turns", C.c_uint64),
-                ("walk_tri_rounds", C.c_uint64)]
+                ("walk_tri_rounds", C.c_uint64), ("tail_segments", C.c_uint64), ("tail_node_visits", C.c_uint64),
+                ("tail_triangle_tests", C.c_uint64), ("tail_ms", C.c_float)]
 
 
 BNODE_DTYPE = np.dtype([("min", np.float32, 3), ("max", np.float32, 3), ("left", np.int32), ("count", np.int32)])

@@ -329,7 +329,8 @@ def main():
     cst = eng.render_stats()
     log(f"[bench] instrumented job: segments {cst.segments} (camera {cst.camera_rays}), node visits {cst.node_visits} "
         f"(camera {cst.node_visits_first}), triangle tests {cst.triangle_tests} (camera {cst.triangle_tests_first}), escaped {cst.escaped}; "
-        f"per-ray walk: {cst.walk_wave_turns} wave turns, {cst.walk_lane_turns} lane steps, {cst.walk_tri_rounds} triangle rounds")
+        f"per-ray walk: {cst.walk_wave_turns} wave turns, {cst.walk_lane_turns} lane steps, {cst.walk_tri_rounds} triangle rounds; "
+        f"last bounces in one launch: {cst.tail_segments} segments, {cst.tail_node_visits} node visits")
 
     for _ in range(args.warmup):
         eng.render(W, H, F, args.depth, rr_start=args.rr_start)
@@ -345,19 +346,21 @@ def main():
     ext_ms = sum(s_.extend_ms for s_ in steps)
     ext0_ms = sum(s_.extend_first_ms for s_ in steps)
     shade_ms = sum(s_.shade_ms for s_ in steps)
+    tail_ms = sum(s_.tail_ms for s_ in steps)
     launches = sum(s_.extend_launches for s_ in steps)
     cam_launches = K  # one camera-ray launch per job (the 64 frames are one batch)
     log(f"[bench] timed {K} jobs of {F} frames: {t * 1e3:.1f} ms (device per job {', '.join(f'{d:.1f}' for d in dev_ms)} ms; "
-        f"extend {ext_ms / K:.1f} (camera walk {ext0_ms / K:.1f}), shade {shade_ms / K:.1f} ms per job)")
+        f"extend {ext_ms / K:.1f} (camera walk {ext0_ms / K:.1f}), shade {shade_ms / K:.1f}, last bounces {tail_ms / K:.2f} ms per job)")
     finite = bool(torch.isfinite(final).all().item())
 
     paths = W * H * F
     value = paths * K / t / 1e6
     med_ms = float(np.median(dev_ms))
-    # the per-ray walk (bounces 1..depth): its rays, visits and tests are the job's minus the camera rays'
-    seg_r = cst.segments - cst.camera_rays
-    vis_r = cst.node_visits - cst.node_visits_first
-    tst_r = cst.triangle_tests - cst.triangle_tests_first
+    # the per-ray walk (bounces 1..depth in extend launches): its rays, visits and tests are the
+    # job's minus the camera rays' and minus the last bounces' that tail_kernel finished
+    seg_r = cst.segments - cst.camera_rays - cst.tail_segments
+    vis_r = cst.node_visits - cst.node_visits_first - cst.tail_node_visits
+    tst_r = cst.triangle_tests - cst.triangle_tests_first - cst.tail_triangle_tests
     launches_r = (launches - cam_launches) // K
     bytes_r = (48.0 * seg_r + NODE_BYTES * vis_r + 48.0 * tst_r) / launches_r
     avg_r_ms = (ext_ms - ext0_ms) / (launches - cam_launches)

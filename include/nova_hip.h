@@ -179,7 +179,16 @@ enum {
  * would walk the tree once for its rays (the result is the same; for tests and tuning). */
 /* NOVA_HIP_ADJOINT_RETRACE: the adjoint's replay traces its rays again instead of reusing
  * the primal pass's hits (the same rays, so the same hits; for tests). */
-enum { NOVA_HIP_RENDER_TIMING = 1, NOVA_HIP_RENDER_COUNT_TRAVERSAL = 2, NOVA_HIP_RENDER_PER_RAY_TRAVERSAL = 4, NOVA_HIP_ADJOINT_RETRACE = 8 };
+/* NOVA_HIP_RENDER_NO_TAIL: keep one extend and one shade launch per bounce to the end, where
+ * the renderer otherwise finishes the last bounces of a batch in one launch, one path per
+ * lane, once few paths remain (the result is the same; for tests and tuning). */
+enum {
+  NOVA_HIP_RENDER_TIMING = 1,
+  NOVA_HIP_RENDER_COUNT_TRAVERSAL = 2,
+  NOVA_HIP_RENDER_PER_RAY_TRAVERSAL = 4,
+  NOVA_HIP_ADJOINT_RETRACE = 8,
+  NOVA_HIP_RENDER_NO_TAIL = 16
+};
 
 /* Renders frames [frame_start, frame_start+frame_count) of the path integrator
  * (PathIntegrator::Li, corelib/nova/integrator/Integrator.cpp:59-98, sample indices by
@@ -236,6 +245,12 @@ typedef struct {
    * fraction of lanes doing a node step), and wave rounds of the triangle loop (tests /
    * (64 * rounds) is the fraction of lanes testing a triangle in a round). */
   uint64_t walk_wave_turns, walk_lane_turns, walk_tri_rounds;
+  /* The last bounces finished one path per lane (see NOVA_HIP_RENDER_NO_TAIL): segments,
+   * and with NOVA_HIP_RENDER_COUNT_TRAVERSAL node visits and triangle tests, included in the
+   * totals above; with NOVA_HIP_RENDER_TIMING the time of those launches (in neither
+   * extend_ms nor shade_ms). */
+  uint64_t tail_segments, tail_node_visits, tail_triangle_tests;
+  float tail_ms;
 } nova_hip_render_stats;
 int nova_hip_render_stats_get(nova_hip_ctx *ctx, nova_hip_render_stats *out);
 

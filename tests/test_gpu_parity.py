@@ -280,21 +280,45 @@ def test_bvh_primitive_order_bit_exact_large(oracle, engine, name):
 @gpu
 @pytest.mark.parametrize("frames", [8, 16, 64])
 def test_wave_coherent_camera_walk_matches_the_per_ray_walk(engine, frames):
-    """Camera rays of pixel-major batches (>= 8 frames) are traced by one walk per wave; the
-    hits, and so the images and accumulators, equal the per-ray walk's bit for bit."""
+    """Camera rays of pixel-major batches (>= 8 frames) are traced by one walk per wave of 64
+    consecutive rays (the last packet partly empty at 50x37); the hits, and so the images and
+    accumulators, equal the per-ray walk's bit for bit."""
     from axomae_amd.nova_hip import RENDER_COUNT_TRAVERSAL, RENDER_PER_RAY_TRAVERSAL
 
     for sc in (scenes.box_room(), scenes.scaled_million(scale_cells=48, spheres=8, sphere_level=3, width=96, height=54)):
         engine.upload(sc)
         engine.build_bvh()
-        print(f"[coherent] {sc.name} frames {frames}", flush=True)
-        a, acc_a = engine.render(48, 40, frames=frames, max_depth=4, rr_start=2)
-        b, acc_b = engine.render(48, 40, frames=frames, max_depth=4, rr_start=2, flags=RENDER_PER_RAY_TRAVERSAL)
+        for w, h in ((48, 40), (50, 37)):
+            print(f"[coherent] {sc.name} {w}x{h} frames {frames}", flush=True)
+            a, acc_a = engine.render(w, h, frames=frames, max_depth=4, rr_start=2)
+            b, acc_b = engine.render(w, h, frames=frames, max_depth=4, rr_start=2, flags=RENDER_PER_RAY_TRAVERSAL)
+            np.testing.assert_array_equal(a.cpu().numpy(), b.cpu().numpy())
+            np.testing.assert_array_equal(acc_a.cpu().numpy(), acc_b.cpu().numpy())
+            # the instrumented render takes the same walks
+            c, _ = engine.render(w, h, frames=frames, max_depth=4, rr_start=2, flags=RENDER_COUNT_TRAVERSAL)
+            np.testing.assert_array_equal(a.cpu().numpy(), c.cpu().numpy())
+
+
+@gpu
+@pytest.mark.parametrize("frames,depth,rr", [(1, 8, 3), (8, 5, -1), (16, 6, 2)])
+def test_last_bounces_in_one_launch_match_per_bounce_launches(engine, frames, depth, rr):
+    """Once few paths remain, the last bounces of a batch run in one launch, one path per lane
+    (tail_kernel); images, accumulators and traced segments equal those of one extend and
+    one shade launch per bounce (NOVA_HIP_RENDER_NO_TAIL) bit for bit."""
+    from axomae_amd.nova_hip import RENDER_COUNT_TRAVERSAL, RENDER_NO_TAIL
+
+    for sc in (scenes.box_room(), scenes.transformed_scene(), scenes.scaled_million(scale_cells=48, spheres=8, sphere_level=3, width=96, height=54)):
+        engine.upload(sc)
+        engine.build_bvh()
+        a, acc_a = engine.render(64, 48, frames=frames, max_depth=depth, rr_start=rr, flags=RENDER_COUNT_TRAVERSAL)
+        sa = engine.render_stats()
+        b, acc_b = engine.render(64, 48, frames=frames, max_depth=depth, rr_start=rr, flags=RENDER_COUNT_TRAVERSAL | RENDER_NO_TAIL)
+        sb = engine.render_stats()
+        print(f"[tail] {sc.name} frames {frames} depth {depth} rr {rr}: segments {sa.segments} (tail {sa.tail_segments}) vs {sb.segments}", flush=True)
+        assert sa.tail_segments > 0 and sb.tail_segments == 0
         np.testing.assert_array_equal(a.cpu().numpy(), b.cpu().numpy())
         np.testing.assert_array_equal(acc_a.cpu().numpy(), acc_b.cpu().numpy())
-        # the instrumented render takes the same walks
-        c, _ = engine.render(48, 40, frames=frames, max_depth=4, rr_start=2, flags=RENDER_COUNT_TRAVERSAL)
-        np.testing.assert_array_equal(a.cpu().numpy(), c.cpu().numpy())
+        assert (sa.segments, sa.node_visits, sa.triangle_tests, sa.escaped) == (sb.segments, sb.node_visits, sb.triangle_tests, sb.escaped)
 
 
 @gpu
