@@ -213,7 +213,10 @@ __global__ __launch_bounds__(kCoherentBlock) NOVA_COHERENT_OCCUPANCY void extend
 
 // The adjoint replay reuses the primal pass's hits of its first bounces, in a cache of at
 // most this size whatever the job (16 B per path and cached bounce).
-constexpr size_t kHitCacheBytes = (size_t)1 << 30;
+#ifndef NOVA_HIT_CACHE_MB
+#define NOVA_HIT_CACHE_MB 2048
+#endif
+constexpr size_t kHitCacheBytes = (size_t)NOVA_HIT_CACHE_MB << 20;
 
 // A finished path's radiance and alpha, by path id.
 __device__ __forceinline__ void write_partial(const Queues &q, uint32_t pid, float lx, float ly, float lz, float a) {

@@ -31,6 +31,7 @@ sys.path.insert(0, ROOT)
 
 METRIC = "Msamples/s (paths) at 1080p, depth 8; adjoint Msamples/s vs primal, 1 GPU"
 HBM_PEAK_GBS = 8000.0
+HIT_CACHE_BYTES = 2 << 30  # the library's adjoint hit-cache budget (include/nova_hip.h)
 NODE_BYTES = 64.0  # bytes of a 4-wide node one visit reads (child boxes and references)
 
 
@@ -95,7 +96,7 @@ def adjoint_leg(width=1280, height=720, frames=32, depth=4):
     triangles, FP32 512x512 albedo and roughness textures, L2 loss against a render of
     perturbed textures (sigma 0.05). Adjoint time covers the whole backward call (its
     per-batch primal re-render and the replay), median of three calls. The default replay
-    reuses the primal pass's hits of its leading bounces from a cache of at most 1 GiB; the
+    reuses the primal pass's hits of its leading bounces from a cache of at most 2 GiB; the
     recompute-only replay (ADJOINT_RETRACE) traces every bounce again and stores nothing."""
     import numpy as np
     import torch
@@ -135,14 +136,14 @@ def adjoint_leg(width=1280, height=720, frames=32, depth=4):
     ga, gr = eng.texture_grad(grad, alb), eng.texture_grad(grad, rough)
     paths = width * height * frames
     primal, adjoint, recompute = paths / t_primal / 1e6, paths / t_adj / 1e6, paths / t_re / 1e6
-    cache_bounces = min(depth + 1, (1 << 30) // (paths * 16))
+    cache_bounces = min(depth + 1, HIT_CACHE_BYTES // (paths * 16))
     log(f"[bench] adjoint leg: primal {t_primal * 1e3:.1f} ms, adjoint {t_adj * 1e3:.1f} ms (hits of {cache_bounces} bounces cached), "
         f"recompute-only {t_re * 1e3:.1f} ms, loss {loss:.4e}")
     eng.close()
     return {"workload": "differentiable_100k_720p", "triangles": sc.triangle_count, "width": width, "height": height, "spp": frames,
             "max_depth": depth, "textures": "albedo 512x512x3 + roughness 512x512 FP32", "primal_msamples_s": round(primal, 3),
             "adjoint_msamples_s": round(adjoint, 3), "adjoint_vs_primal": round(adjoint / primal, 4),
-            "hit_cache": {"bytes_max": 1 << 30, "bounces_cached": cache_bounces, "bytes": cache_bounces * paths * 16},
+            "hit_cache": {"bytes_max": HIT_CACHE_BYTES, "bounces_cached": cache_bounces, "bytes": cache_bounces * paths * 16},
             "recompute_only_msamples_s": round(recompute, 3), "recompute_only_vs_primal": round(recompute / primal, 4),
             "grad_finite": bool(torch.isfinite(ga).all().item() and torch.isfinite(gr).all().item()),
             "grad_nonzero_texels": int((ga.abs().sum(-1) > 0).sum().item() + (gr.abs().sum(-1) > 0).sum().item())}

@@ -368,7 +368,7 @@ int nova_hip_texture_f32_update(nova_hip_ctx *ctx, int32_t texture, const float 
  * nova_hip_texture_f32_info) for the albedo_f32 (channels 0-2) and roughness_f32
  * (channel 0) textures of every material. Each batch of frames is rendered once to get
  * every path's radiance, then replayed from the same sample indices. The replay takes the
- * primal pass's hits of its leading bounces from a cache of at most 1 GiB (16 B per path
+ * primal pass's hits of its leading bounces from a cache of at most 2 GiB (16 B per path
  * and bounce, as many bounces as fit) and traces the deeper bounces again; nothing else is
  * kept per path vertex, so memory stays within the batch's queues plus that cap whatever
  * the depth (NOVA_HIP_ADJOINT_RETRACE: no cache, every bounce traced). Sampling is detached: directions, lobe choices and pdfs are constants, the
