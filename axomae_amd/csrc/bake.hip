@@ -311,9 +311,15 @@ __device__ __forceinline__ f3 equirect_texel_direction(int i, int j, int w, int 
 }
 
 constexpr unsigned kBakeBlock = 256;
+// Workgroups go to the MI355X's 8 XCDs (each with its own L2) round robin. Workgroup b works
+// on texel block (b % 8) * (grid / 8) + b / 8 instead of b, so each XCD's L2 sees a
+// contiguous run of texels -- one or two maps at a time -- rather than every eighth block
+// of all of them. The grid is a multiple of 8, so this is a permutation of the blocks.
+constexpr unsigned kXcds = 8;
 __global__ __launch_bounds__(kBakeBlock) __attribute__((amdgpu_waves_per_eu(4, 4))) void bake_kernel(const BakeMap *__restrict__ maps, int count, uint64_t total, const float *__restrict__ lut,
                             const uint32_t *__restrict__ sobol, int samples) {
-  for (uint64_t g = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; g < total; g += (uint64_t)gridDim.x * blockDim.x) {
+  const uint64_t block = (uint64_t)(blockIdx.x % kXcds) * (gridDim.x / kXcds) + blockIdx.x / kXcds;
+  for (uint64_t g = block * blockDim.x + threadIdx.x; g < total; g += (uint64_t)gridDim.x * blockDim.x) {
     const BakeMap m = maps[find_map(maps, count, g)];
     const uint64_t local = g - m.first;
     const unsigned i = (unsigned)(local % (unsigned)m.ow), j = (unsigned)(local / (unsigned)m.ow);
@@ -466,7 +472,7 @@ static int launch_batch(const nova_hip_bake_job *jobs, uint32_t count, int32_t s
   if (e == hipSuccess && !theta_tab.empty()) e = hipMemcpyAsync(d_theta, theta_tab.data(), theta_tab.size() * sizeof(float2), hipMemcpyHostToDevice, s);
   if (e == hipSuccess && total > 0) {
     const unsigned block = kBakeBlock;
-    const unsigned grid = (unsigned)std::min<uint64_t>((total + block - 1) / block, 1u << 20);
+    const unsigned grid = (unsigned)std::min<uint64_t>(((total + block - 1) / block + kXcds - 1) / kXcds * kXcds, 1u << 20);  // a multiple of kXcds
     hipLaunchKernelGGL(bake_kernel, dim3(grid), dim3(block), 0, s, d_maps, (int)maps.size(), total, d_lut, sobol, samples);
     e = hipGetLastError();
   }
