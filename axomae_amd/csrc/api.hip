@@ -110,7 +110,7 @@ static void free_scene(nova_hip_ctx *ctx) {
   void **ptrs[] = {(void **)&ctx->tri_world, (void **)&ctx->shade,     (void **)&ctx->prim_aabb, (void **)&ctx->prim_centroid,
                    (void **)&ctx->mesh_xf,   (void **)&ctx->materials, (void **)&ctx->textures,  (void **)&ctx->tex_u32,
                    (void **)&ctx->tex_f32,   (void **)&ctx->acos_lut,  (void **)&ctx->prim_idx,  (void **)&ctx->bnodes,
-                   (void **)&ctx->nodes4,    (void **)&ctx->tri_leaf,  (void **)&ctx->bparent, (void **)&ctx->bdepth,
+                   (void **)&ctx->nodes4,    (void **)&ctx->nodes4f,   (void **)&ctx->tri_leaf,  (void **)&ctx->bparent, (void **)&ctx->bdepth,
                    (void **)&ctx->leaf_of_pos, (void **)&ctx->tie,   (void **)&ctx->xf_inv};
   for (void **p : ptrs)
     if (*p) {

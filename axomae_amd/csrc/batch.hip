@@ -515,7 +515,7 @@ static int make_plan(nova_hip_ctx *ex, const nova_hip_batch_item *items, uint32_
     const uint64_t slots = npix * (uint64_t)p->frame_count;
     if (slots >= (1ull << 31)) return set_error(ex, NOVA_HIP_INVALID_ARGUMENT, "batch item too large");
     BatchScene &S = bp.scenes[i];
-    S.tc = TraceCtx{c->nodes4, c->tri_leaf, nullptr, 0, nullptr};
+    S.tc = TraceCtx{c->nodes4, c->tri_leaf, nullptr, 0, nullptr, c->nodes4f};
     S.tie = TieTables{c->bnodes, c->bparent, c->bdepth, c->leaf_of_pos, c->xf_inv};
     S.sc = ShadeCtx{c->textures, c->tex_u32, c->tex_f32, c->materials, c->mesh_xf, c->shade, c->acos_lut, c->envmap, -1};
     S.fp.width = p->width;

@@ -639,6 +639,26 @@ __global__ void collapse_kernel(const int32_t *__restrict__ frontier_bin, const 
   out[slot] = nd;
 }
 
+// Node4 -> Node4F: every plane decoded with grid_plane, the value the walks compute.
+__global__ void decode_nodes_kernel(const Node4 *__restrict__ in, Node4F *__restrict__ out, uint32_t n) {
+  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const Node4 nd = in[i];
+  const float org[3] = {nd.orgx, nd.orgy, nd.orgz}, step[3] = {nd.stepx, nd.stepy, nd.stepz};
+  const uint32_t qmn[3] = {nd.qminx, nd.qminy, nd.qminz}, qmx[3] = {nd.qmaxx, nd.qmaxy, nd.qmaxz};
+  Node4F f;
+  for (int a = 0; a < 3; a++)
+    for (int k = 0; k < 4; k++) {
+      f.plane[a][0][k] = grid_plane((qmn[a] >> (8 * k)) & 255u, step[a], org[a]);
+      f.plane[a][1][k] = grid_plane((qmx[a] >> (8 * k)) & 255u, step[a], org[a]);
+    }
+  for (int k = 0; k < 4; k++) {
+    f.child[k] = nd.child[k];
+    f.pad[k] = 0;
+  }
+  out[i] = f;
+}
+
 __global__ void gather_leaf_tris(const int32_t *__restrict__ prim_idx, const TriMT *__restrict__ tri, TriMT *__restrict__ out, uint32_t n) {
   uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i < n) out[i] = tri[prim_idx[i]];
@@ -659,8 +679,8 @@ extern "C" int nova_hip_bvh_build(nova_hip_ctx *ctx, int build_candidates, nova_
   NOVA_HIP_CHECK(ctx, hipSetDevice(ctx->device));
   const uint32_t n = ctx->prim_count;
   const size_t maxnodes = 2 * (size_t)n - 1;
-  for (void **p : {(void **)&ctx->prim_idx, (void **)&ctx->bnodes, (void **)&ctx->nodes4, (void **)&ctx->tri_leaf, (void **)&ctx->bparent,
-                   (void **)&ctx->bdepth, (void **)&ctx->leaf_of_pos, (void **)&ctx->tie})
+  for (void **p : {(void **)&ctx->prim_idx, (void **)&ctx->bnodes, (void **)&ctx->nodes4, (void **)&ctx->nodes4f, (void **)&ctx->tri_leaf,
+                   (void **)&ctx->bparent, (void **)&ctx->bdepth, (void **)&ctx->leaf_of_pos, (void **)&ctx->tie})
     if (*p) {
       hipFree(*p);
       *p = nullptr;
@@ -791,6 +811,8 @@ extern "C" int nova_hip_bvh_build(nova_hip_ctx *ctx, int build_candidates, nova_
   }
   int32_t wide = 0;
   NOVA_HIP_CHECK(ctx, hipMemcpy(&wide, counters + 2, sizeof(int32_t), hipMemcpyDeviceToHost));
+  NOVA_HIP_CHECK(ctx, hipMalloc(&ctx->nodes4f, std::max<size_t>(1, (size_t)wide) * sizeof(Node4F)));
+  hipLaunchKernelGGL(decode_nodes_kernel, dim3((wide + 255) / 256), dim3(256), 0, s, ctx->nodes4, ctx->nodes4f, (uint32_t)wide);
   NOVA_HIP_CHECK(ctx, hipMalloc(&ctx->tri_leaf, (size_t)n * sizeof(TriMT)));
   hipLaunchKernelGGL(gather_leaf_tris, dim3((n + 255) / 256), dim3(256), 0, s, ctx->prim_idx, ctx->tri_world, ctx->tri_leaf, n);
   hipLaunchKernelGGL(leaf_of_position_kernel, dim3(std::min<uint32_t>(4096, (used + 255) / 256)), dim3(256), 0, s, ctx->bnodes, (uint32_t)used,

@@ -30,6 +30,7 @@ struct TraceCtx {
   uint32_t *gstack;  // overflow entries: [level - lds levels][global thread]
   uint32_t gstride;  // number of threads in the grid
   const TieTables *tie;  // binary tree, only read to settle exact equal-t ties
+  const Node4F *nodesf = nullptr;  // decoded-plane copy of the nodes (the wave-coherent walk reads these)
 };
 
 // BoundingBox::halfArea (BoundingBox.cpp:162-165)
@@ -405,10 +406,12 @@ NOVA_HD int trace_closest(const TraceCtx &tc, Stack &stack, f3 o, f3 d, float tn
 // not depend on the order. The stack is the wave's: (node, lane mask) entries, entry e held
 // in lane e of three registers, so a push is a compare-and-select and a pop a lane read.
 #ifdef __HIP_DEVICE_COMPILE__
-typedef const __attribute__((address_space(4))) Node4 ConstNode4;  // scalar-loadable: uniform address, never written by the walk
+typedef const __attribute__((address_space(4))) Node4F ConstNode4F;  // scalar-loadable: uniform address, never written by the walk
+typedef const __attribute__((address_space(4))) float4 ConstFloat4;
 typedef const __attribute__((address_space(4))) TriMT ConstTriMT;
 #else
-typedef const Node4 ConstNode4;
+typedef const Node4F ConstNode4F;
+typedef const float4 ConstFloat4;
 typedef const TriMT ConstTriMT;
 #endif
 
@@ -418,7 +421,7 @@ template <bool kCount>
 __device__ void coherent_walk(const TraceCtx &tc, TraceState &st, uint64_t lanes, float tnear, uint32_t &nodes_visited, uint32_t &tri_tests) {
   const float kUp = 1.000001f;
   const float kInf = __builtin_inff();
-  ConstNode4 *nodes = (ConstNode4 *)tc.nodes;
+  ConstNode4F *nodes = (ConstNode4F *)tc.nodesf;
   ConstTriMT *tris = (ConstTriMT *)tc.tris;
   const uint32_t lane = __lane_id();
   int32_t node = 0;
@@ -426,20 +429,44 @@ __device__ void coherent_walk(const TraceCtx &tc, TraceState &st, uint64_t lanes
   int sp = 0;
   // the wave's stack: entry e is lane e of three registers (node, mask low, mask high)
   int32_t stk_node = 0, stk_lo = 0, stk_hi = 0;
+  // When the wave's rays agree on the sign of each direction component (one pixel's rays
+  // nearly always do) and none is axis-parallel, the near and far plane rows of every axis
+  // are the same for all lanes: wave-uniform row indices, no per-lane select.
+  const uint64_t negx = __ballot(st.inv.x < 0.f) & lanes, negy = __ballot(st.inv.y < 0.f) & lanes, negz = __ballot(st.inv.z < 0.f) & lanes;
+  const bool uniform = __ballot(st.parallel) == 0 && (negx == 0 || negx == lanes) && (negy == 0 || negy == lanes) && (negz == 0 || negz == lanes);
+  // float4 rows of a node: 2 * axis + side (side 0 min, 1 max), then the children at 6
+  const uint32_t rnx = negx ? 1u : 0u, rny = negy ? 3u : 2u, rnz = negz ? 5u : 4u;  // near rows
+  const uint32_t rfx = rnx ^ 1u, rfy = rny ^ 1u, rfz = rnz ^ 1u;                     // far rows
   for (uint32_t guard = 0; guard < (1u << 24) && mask; guard++) {
     node = __builtin_amdgcn_readfirstlane(node);
-    const Node4 nd = nodes[node];
+    ConstNode4F &nd = nodes[node];
     const bool in = (mask >> lane) & 1u;
     float key[4] = {kInf, kInf, kInf, kInf};
     bool hit[4] = {false, false, false, false};
     if (in) {
       if (kCount) nodes_visited++;
-      if (st.parallel) {
+      if (uniform) {
+        ConstFloat4 *rows = (ConstFloat4 *)&nodes[node];
+        const float4 qnx = rows[rnx], qfx = rows[rfx], qny = rows[rny], qfy = rows[rfy], qnz = rows[rnz], qfz = rows[rfz];
+        const float nxs[4] = {qnx.x, qnx.y, qnx.z, qnx.w}, fxs[4] = {qfx.x, qfx.y, qfx.z, qfx.w};
+        const float nys[4] = {qny.x, qny.y, qny.z, qny.w}, fys[4] = {qfy.x, qfy.y, qfy.z, qfy.w};
+        const float nzs[4] = {qnz.x, qnz.y, qnz.z, qnz.w}, fzs[4] = {qfz.x, qfz.y, qfz.z, qfz.w};
 #pragma unroll
         for (int k = 0; k < 4; k++) {
-          const float bmnx = grid_plane((nd.qminx >> (8 * k)) & 255u, nd.stepx, nd.orgx), bmxx = grid_plane((nd.qmaxx >> (8 * k)) & 255u, nd.stepx, nd.orgx);
-          const float bmny = grid_plane((nd.qminy >> (8 * k)) & 255u, nd.stepy, nd.orgy), bmxy = grid_plane((nd.qmaxy >> (8 * k)) & 255u, nd.stepy, nd.orgy);
-          const float bmnz = grid_plane((nd.qminz >> (8 * k)) & 255u, nd.stepz, nd.orgz), bmxz = grid_plane((nd.qmaxz >> (8 * k)) & 255u, nd.stepz, nd.orgz);
+          const float tnx = (nxs[k] - st.o.x) * st.inv.x, tfx = (fxs[k] - st.o.x) * st.inv.x;
+          const float tny = (nys[k] - st.o.y) * st.inv.y, tfy = (fys[k] - st.o.y) * st.inv.y;
+          const float tnz = (nzs[k] - st.o.z) * st.inv.z, tfz = (fzs[k] - st.o.z) * st.inv.z;
+          const float t0 = fmaxf(fmaxf(tnx, tny), tnz);
+          const float t1 = fminf(fminf(tfx, tfy), tfz) * kUp;
+          hit[k] = nd.child[k] != -1 && t0 <= t1 && t1 >= 0.f && t0 <= st.tbest * kUp;
+          key[k] = hit[k] ? t0 : kInf;
+        }
+      } else {  // mixed signs or axis-parallel rays: both planes of each axis, then min/max (the same values)
+#pragma unroll
+        for (int k = 0; k < 4; k++) {
+          const float bmnx = nd.plane[0][0][k], bmxx = nd.plane[0][1][k];
+          const float bmny = nd.plane[1][0][k], bmxy = nd.plane[1][1][k];
+          const float bmnz = nd.plane[2][0][k], bmxz = nd.plane[2][1][k];
           float tx1 = (bmnx - st.o.x) * st.inv.x, tx2 = (bmxx - st.o.x) * st.inv.x;
           float ty1 = (bmny - st.o.y) * st.inv.y, ty2 = (bmxy - st.o.y) * st.inv.y;
           float tz1 = (bmnz - st.o.z) * st.inv.z, tz2 = (bmxz - st.o.z) * st.inv.z;
@@ -460,23 +487,6 @@ __device__ void coherent_walk(const TraceCtx &tc, TraceState &st, uint64_t lanes
           }
           const float t0 = fmaxf(fmaxf(fminf(tx1, tx2), fminf(ty1, ty2)), fminf(tz1, tz2));
           const float t1 = fminf(fminf(fmaxf(tx1, tx2), fmaxf(ty1, ty2)), fmaxf(tz1, tz2)) * kUp;
-          hit[k] = nd.child[k] != -1 && t0 <= t1 && t1 >= 0.f && t0 <= st.tbest * kUp;
-          key[k] = hit[k] ? t0 : kInf;
-        }
-      } else {
-        const uint32_t nqx = st.inv.x < 0.f ? nd.qmaxx : nd.qminx, fqx = st.inv.x < 0.f ? nd.qminx : nd.qmaxx;
-        const uint32_t nqy = st.inv.y < 0.f ? nd.qmaxy : nd.qminy, fqy = st.inv.y < 0.f ? nd.qminy : nd.qmaxy;
-        const uint32_t nqz = st.inv.z < 0.f ? nd.qmaxz : nd.qminz, fqz = st.inv.z < 0.f ? nd.qminz : nd.qmaxz;
-#pragma unroll
-        for (int k = 0; k < 4; k++) {
-          const float tnx = (grid_plane((nqx >> (8 * k)) & 255u, nd.stepx, nd.orgx) - st.o.x) * st.inv.x;
-          const float tfx = (grid_plane((fqx >> (8 * k)) & 255u, nd.stepx, nd.orgx) - st.o.x) * st.inv.x;
-          const float tny = (grid_plane((nqy >> (8 * k)) & 255u, nd.stepy, nd.orgy) - st.o.y) * st.inv.y;
-          const float tfy = (grid_plane((fqy >> (8 * k)) & 255u, nd.stepy, nd.orgy) - st.o.y) * st.inv.y;
-          const float tnz = (grid_plane((nqz >> (8 * k)) & 255u, nd.stepz, nd.orgz) - st.o.z) * st.inv.z;
-          const float tfz = (grid_plane((fqz >> (8 * k)) & 255u, nd.stepz, nd.orgz) - st.o.z) * st.inv.z;
-          const float t0 = fmaxf(fmaxf(tnx, tny), tnz);
-          const float t1 = fminf(fminf(tfx, tfy), tfz) * kUp;
           hit[k] = nd.child[k] != -1 && t0 <= t1 && t1 >= 0.f && t0 <= st.tbest * kUp;
           key[k] = hit[k] ? t0 : kInf;
         }

@@ -754,7 +754,7 @@ static int render_impl(nova_hip_ctx *ctx, const nova_hip_render_params *p, float
     if (st) return st;
     gstack = ctx->gstack;
   }
-  TraceCtx tc{ctx->nodes4, ctx->tri_leaf, gstack, trace_threads, ctx->tie};
+  TraceCtx tc{ctx->nodes4, ctx->tri_leaf, gstack, trace_threads, ctx->tie, ctx->nodes4f};
   ShadeCtx sc{ctx->textures, ctx->tex_u32, ctx->tex_f32, ctx->materials, ctx->mesh_xf, ctx->shade, ctx->acos_lut, ctx->envmap,
               p->sampling_roughness_texture};
   AdjointCtx replay = adj ? AdjointCtx{partial, adj->dimg, adj->scale, adj->grad, nullptr, 0u, 0u}

@@ -88,6 +88,18 @@ __host__ __device__ inline int32_t leaf_ref(int32_t first) { return -(first + 2)
 __host__ __device__ inline int32_t leaf_first(int32_t ref) { return -(ref + 2); }
 static_assert(sizeof(Node4) == 64, "Node4 layout");
 
+// The same node with its planes decoded (each exactly the float fmaf(q, step, org) of the
+// Node4), 128 B, for the wave-coherent walk, which reads nodes through the scalar unit:
+// plane[axis][0][k] / plane[axis][1][k] are child k's min / max planes on that axis, so the
+// near and far planes of an axis are a choice of row, made once per walk when the wave's
+// rays agree on the direction's sign.
+struct alignas(16) Node4F {
+  float plane[3][2][4];
+  int32_t child[4];
+  int32_t pad[4];
+};
+static_assert(sizeof(Node4F) == 128, "Node4F layout");
+
 struct DeviceBuffer {
   void *ptr = nullptr;
   size_t bytes = 0;
@@ -124,6 +136,7 @@ struct nova_hip_ctx {
   nova_hip::BNode *bnodes = nullptr;
   uint32_t bnode_count = 0;
   nova_hip::Node4 *nodes4 = nullptr;
+  nova_hip::Node4F *nodes4f = nullptr;        // the nodes with decoded planes (wave-coherent walk)
   nova_hip::TriMT *tri_leaf = nullptr;       // MT records in leaf order (prim id in a.w)
   int32_t *bparent = nullptr;                // binary tree parent (-1 at the root)
   int32_t *bdepth = nullptr;                 // binary tree depth
