@@ -842,6 +842,7 @@ extern "C" int nova_hip_bvh_build(nova_hip_ctx *ctx, int build_candidates, nova_
   ctx->bvh_info.leaves = leaves;
   ctx->bvh_info.build_ms = ms;
   if (info) *info = ctx->bvh_info;
+  ctx->tail_hint = 0;
   ctx->bvh_ready = true;
   return NOVA_HIP_SUCCESS;
 }

@@ -92,7 +92,8 @@ __global__ __launch_bounds__(256) void generate_kernel(FrameParams fp, const uin
 #endif
 
 template <bool kCount>
-__global__ __launch_bounds__(kTraceBlock) NOVA_TRACE_OCCUPANCY void extend_kernel(TraceCtx tc, Queues q, int which, int bounce, uint32_t tail_rays) {
+__global__ __launch_bounds__(kTraceBlock) NOVA_TRACE_OCCUPANCY void extend_kernel(TraceCtx tc, Queues q, int which, int bounce, uint32_t tail_rays,
+                                                                                  int tail_force) {
   __shared__ uint32_t lds[kLdsStack * kTraceBlock];
   uint32_t nv = 0, nt = 0;
   const uint32_t n = q.count[which];
@@ -101,8 +102,8 @@ __global__ __launch_bounds__(kTraceBlock) NOVA_TRACE_OCCUPANCY void extend_kerne
   const uint32_t gtid = blockIdx.x * blockDim.x + threadIdx.x;
   if (tail_rays) {  // this bounce may hand the batch's remaining paths to tail_kernel
     if (q.count[kTailSlot] != 0) return;  // an earlier bounce did
-    if (n <= tail_rays) {
-      if (gtid == 0) q.count[kTailSlot] = (uint32_t)bounce + 1u;
+    if (n <= tail_rays || tail_force) {   // forced: the host launches no later bounce (bit 31: more than tail_rays rays)
+      if (gtid == 0) q.count[kTailSlot] = ((uint32_t)bounce + 1u) | (n > tail_rays ? 0x80000000u : 0u);
       return;
     }
   }
@@ -343,7 +344,7 @@ template <bool kCount>
 __global__ __launch_bounds__(kTraceBlock) void tail_kernel(TraceCtx tc, ShadeParams sp, int max_depth, int rr_start, ShadeCtx sc,
                                                           const uint32_t *__restrict__ sobol, Queues q, int which, int ndims) {
   extern __shared__ uint32_t tail_lds[];
-  if (q.count[kTailSlot] != (uint32_t)sp.bounce + 1u) return;
+  if ((q.count[kTailSlot] & 0x7FFFFFFFu) != (uint32_t)sp.bounce + 1u) return;
   uint32_t *sob_tab = tail_lds + kLdsStack * kTraceBlock;
   sobol_nibble_fill(sobol, sp.dim0, ndims, sob_tab, threadIdx.x, blockDim.x);
   __syncthreads();
@@ -799,6 +800,10 @@ static int render_impl(nova_hip_ctx *ctx, const nova_hip_render_params *p, float
   // The last bounces go to tail_kernel once at most tail_rays rays remain (the primal
   // integrator only; the adjoint replay keeps per-bounce launches).
   const uint32_t tail_rays = (adj || (p->flags & NOVA_HIP_RENDER_NO_TAIL)) ? 0u : tail_rays_setting();
+  // a hint from the previous call: same batch size, depth and RR (and scene: a new upload or
+  // BVH clears it); not for the no-allocation entry point, which reads nothing back
+  const bool use_hint = tail_rays > 0 && !noalloc && ctx->tail_hint > 0 && ctx->tail_hint_key[0] == (uint64_t)cap &&
+                        ctx->tail_hint_key[1] == (uint64_t)p->max_depth && ctx->tail_hint_key[2] == (uint64_t)(int64_t)p->rr_start;
   hipEvent_t ev0 = nullptr, ev1 = nullptr;
   if (!noalloc) {
     NOVA_HIP_CHECK(ctx, hipEventCreate(&ev0));
@@ -846,6 +851,9 @@ static int render_impl(nova_hip_ctx *ctx, const nova_hip_render_params *p, float
       for (int j = k; j <= p->max_depth; j++) ndims += 4 + ((p->rr_start >= 0 && j >= p->rr_start) ? 1 : 0);
       const size_t tail_lds = ((size_t)kLdsStack * kTraceBlock + (size_t)ndims * 128) * sizeof(uint32_t);
       const bool tail_here = tail_rays > 0 && k >= 1 && k < p->max_depth && tail_lds <= 65536;
+      // The previous call of the same shape started its tail at bounce tail_hint: start it
+      // there at the latest (the extend launch forces it) and launch nothing after it.
+      const bool tail_last = tail_here && use_hint && k == ctx->tail_hint;
       if (timing && (st = stamp())) return st;
       if (timing) kkind.push_back(k == 0 ? 0 : 1);
       if (pass == 1 && replay.hit_cache && (uint32_t)k < replay.cache_bounces) {
@@ -857,9 +865,11 @@ static int render_impl(nova_hip_ctx *ctx, const nova_hip_render_params *p, float
         else
           hipLaunchKernelGGL(extend_coherent_kernel<false>, dim3(cblocks), dim3(kCoherentBlock), 0, s, tc, q, which);
       } else if (counting) {
-        hipLaunchKernelGGL(extend_kernel<true>, dim3(trace_blocks), dim3(kTraceBlock), 0, s, tc, k == 0 ? q0 : q, which, k, tail_here ? tail_rays : 0u);
+        hipLaunchKernelGGL(extend_kernel<true>, dim3(trace_blocks), dim3(kTraceBlock), 0, s, tc, k == 0 ? q0 : q, which, k, tail_here ? tail_rays : 0u,
+                           tail_last ? 1 : 0);
       } else {
-        hipLaunchKernelGGL(extend_kernel<false>, dim3(trace_blocks), dim3(kTraceBlock), 0, s, tc, q, which, k, tail_here ? tail_rays : 0u);
+        hipLaunchKernelGGL(extend_kernel<false>, dim3(trace_blocks), dim3(kTraceBlock), 0, s, tc, q, which, k, tail_here ? tail_rays : 0u,
+                           tail_last ? 1 : 0);
       }
       if (timing && (st = stamp())) return st;
       hipLaunchKernelGGL(reset_count_kernel, dim3(1), dim3(1), 0, s, counts, which ^ 1);
@@ -879,8 +889,8 @@ static int render_impl(nova_hip_ctx *ctx, const nova_hip_render_params *p, float
       NOVA_HIP_CHECK(ctx, hipGetLastError());
       if (tail_here) {
         // at most trace_threads lanes: the overflow stacks are sized for the extend grid
-        const uint32_t tblocks = std::min<uint32_t>({(uint32_t)((std::min<size_t>(tail_rays, cap) + kTraceBlock - 1) / kTraceBlock), trace_blocks,
-                                                     (uint32_t)cus * 4});
+        const uint32_t tblocks = std::min<uint32_t>({(uint32_t)((std::min<size_t>(tail_last ? cap : tail_rays, cap) + kTraceBlock - 1) / kTraceBlock),
+                                                     trace_blocks, (uint32_t)cus * 4});
         if (timing && (st = stamp())) return st;
         if (timing) kkind.push_back(3);
         if (counting)
@@ -895,6 +905,7 @@ static int render_impl(nova_hip_ctx *ctx, const nova_hip_render_params *p, float
       dim += 4 + (spp.rr ? 1 : 0);
       which ^= 1;
       launches++;
+      if (tail_last) break;  // tail_kernel has finished every path
     }
     }
     if (!adj) {
@@ -914,6 +925,14 @@ static int render_impl(nova_hip_ctx *ctx, const nova_hip_render_params *p, float
   unsigned long long s64[20] = {};
   NOVA_HIP_CHECK(ctx, hipMemcpy(&seg, counts + 2, sizeof(uint32_t), hipMemcpyDeviceToHost));
   NOVA_HIP_CHECK(ctx, hipMemcpy(s64, stats64, sizeof(s64), hipMemcpyDeviceToHost));
+  if (tail_rays > 0) {  // where the last batch's tail started: the next call's hint
+    uint32_t slot = 0;
+    NOVA_HIP_CHECK(ctx, hipMemcpy(&slot, counts + kTailSlot, sizeof(uint32_t), hipMemcpyDeviceToHost));
+    ctx->tail_hint = (slot == 0 || (slot & 0x80000000u)) ? 0 : (int)slot - 1;  // a forced start with many rays: learn again
+    ctx->tail_hint_key[0] = (uint64_t)cap;
+    ctx->tail_hint_key[1] = (uint64_t)p->max_depth;
+    ctx->tail_hint_key[2] = (uint64_t)(int64_t)p->rr_start;
+  }
   float ext_ms = 0.f, shade_ms = 0.f, ext0_ms = 0.f, tail_ms = 0.f;
   for (size_t k = 0; k < kkind.size() && 2 * k + 1 < kev.size(); k++) {
     float a = 0.f;

@@ -144,6 +144,10 @@ struct nova_hip_ctx {
   nova_hip::TieTables *tie = nullptr;        // device record of bnodes, bparent, bdepth, leaf_of_pos
   nova_hip_bvh_info bvh_info{};
   bool bvh_ready = false;
+  // bounce at which the last render's tail_kernel started (0: none), for batch size,
+  // max_depth and rr_start tail_hint_key; cleared by a new scene or BVH
+  int tail_hint = 0;
+  uint64_t tail_hint_key[3] = {0, 0, 0};
 
   // integrator scratch (grown on demand)
   nova_hip::DeviceBuffer scratch[16];

@@ -183,6 +183,7 @@ extern "C" int nova_hip_scene_upload(nova_hip_ctx *ctx, const nova_hip_scene_des
   ctx->envmap = sd->envmap_texture;
   ctx->host_textures = texrec;
   ctx->tex_f32_floats = nf32;
+  ctx->tail_hint = 0;
   ctx->scene_ready = true;
   ctx->bvh_ready = false;
   return NOVA_HIP_SUCCESS;
