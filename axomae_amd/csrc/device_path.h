@@ -188,7 +188,35 @@ struct AdjointCtx {
   float4 *hit_cache;
   uint32_t cache_paths;    // paths per bounce in the cache
   uint32_t cache_bounces;  // bounces 0 .. cache_bounces-1 are cached
+  // The replay kernel's LDS table (kGradSlots keys, 3 values each) that sums its workgroup's
+  // texel gradients before they go to device memory; null: straight to device memory.
+  uint32_t *table_keys = nullptr;
+  float *table_vals = nullptr;
 };
+
+// Texel-gradient table of a replay workgroup: neighbouring pixels' paths splat mostly onto
+// the same texels, so each distinct texel gets one device atomic per channel and workgroup
+// iteration instead of one per lane. Open addressing on the texel's index in the gradient
+// pool; kGradEmpty marks a free slot.
+constexpr int kGradSlots = 2048;
+constexpr uint32_t kGradEmpty = 0xFFFFFFFFu;
+#ifdef __HIP_DEVICE_COMPILE__
+// Adds v[0..n) at `key` to the table; false when no slot was found within the probe limit
+// (the caller then adds to device memory).
+__device__ __forceinline__ bool grad_table_add(uint32_t *keys, float *vals, uint32_t key, const float *v, int n) {
+  uint32_t h = (key * 2654435761u) >> 21;  // 11 bits: kGradSlots
+  for (int probe = 0; probe < 32; probe++) {
+    const uint32_t old = atomicCAS(&keys[h], kGradEmpty, key);
+    if (old == kGradEmpty || old == key) {
+      for (int c = 0; c < n; c++)
+        if (v[c] != 0.f) atomicAdd(&vals[h * 3 + c], v[c]);
+      return true;
+    }
+    h = (h + 1) & (kGradSlots - 1);
+  }
+  return false;
+}
+#endif
 
 NOVA_HD void add_grad(float *p, float v) {
 #ifdef __HIP_DEVICE_COMPILE__
@@ -199,16 +227,23 @@ NOVA_HD void add_grad(float *p, float v) {
 }
 
 // Adds g[ch] * (bilinear weight) to the gradient of channel ch of the four texels an FP32
-// texture lookup blended.
-NOVA_HD void splat_texture_grad(const ShadeCtx &sc, float *grad, int tex, float u, float v, const float *g, int nch) {
+// texture lookup blended (through the workgroup's table when the replay gives one).
+NOVA_HD void splat_texture_grad(const ShadeCtx &sc, const AdjointCtx *adj, int tex, float u, float v, const float *g, int nch) {
   unsigned idx[4];
   float w[4];
   bilinear_taps(sc, tex, u, v, idx, w);
   const TexRec t = sc.tex[tex];
-  float *base = grad + t.offset;
-  for (int k = 0; k < 4; k++)
-    for (int ch = 0; ch < nch && ch < t.channels; ch++)
-      if (g[ch] != 0.f) add_grad(base + (size_t)idx[k] * t.channels + ch, w[k] * g[ch]);
+  const int nc = nch < t.channels ? nch : t.channels;  // at most 3
+  for (int k = 0; k < 4; k++) {
+    const uint64_t key = t.offset + (uint64_t)idx[k] * (uint32_t)t.channels;  // the texel's channel 0 in the pool
+    float val[3] = {0.f, 0.f, 0.f};
+    for (int ch = 0; ch < nc; ch++) val[ch] = w[k] * g[ch];
+#ifdef __HIP_DEVICE_COMPILE__
+    if (adj->table_keys && key < kGradEmpty && grad_table_add(adj->table_keys, adj->table_vals, (uint32_t)key, val, nc)) continue;
+#endif
+    for (int ch = 0; ch < nc; ch++)
+      if (g[ch] != 0.f) add_grad(adj->grad + key + ch, val[ch]);
+  }
 }
 
 // The surface half of PrincipledMaterial::scatter (NovaMaterials.h:89-137) at a hit: the
@@ -388,8 +423,8 @@ NOVA_HD bool shade_path(const ShadeParams &sp, const ShadeCtx &sc, Sobol sobol, 
       ga[rr_channel] -= (G[0] + G[1] + G[2]) * dm[rr_channel];
       gr[0] = 0.f;
     }
-    if (mat.albedo_f32 >= 0) splat_texture_grad(sc, adj->grad, mat.albedo_f32, h.u, h.v, ga, 3);
-    if (mat.roughness_f32 >= 0) splat_texture_grad(sc, adj->grad, mat.roughness_f32, h.u, h.v, gr, 1);
+    if (mat.albedo_f32 >= 0) splat_texture_grad(sc, adj, mat.albedo_f32, h.u, h.v, ga, 3);
+    if (mat.roughness_f32 >= 0) splat_texture_grad(sc, adj, mat.roughness_f32, h.u, h.v, gr, 1);
   }
   T = T * c;
   // Ray::spawn (Ray.h:22-27)

@@ -245,7 +245,31 @@ __global__ __launch_bounds__(kShadeBlock) NOVA_SHADE_OCCUPANCY void shade_kernel
   // nibble tables of this scatter's sampler dimensions (dim0 .. dim0+4)
   __shared__ uint32_t sob_tab[5 * 128];
   sobol_nibble_fill(sobol, sp.dim0, 5, sob_tab, threadIdx.x, blockDim.x);
+  // the replay's texel-gradient table (an array of one element in the primal kernel)
+  __shared__ uint32_t gkeys[kAdj ? kGradSlots : 1];
+  __shared__ float gvals[kAdj ? 3 * kGradSlots : 1];
+  if (kAdj) {
+    for (int e = threadIdx.x; e < kGradSlots; e += blockDim.x) {
+      gkeys[e] = kGradEmpty;
+      gvals[3 * e] = gvals[3 * e + 1] = gvals[3 * e + 2] = 0.f;
+    }
+    adj.table_keys = gkeys;
+    adj.table_vals = gvals;
+  }
   __syncthreads();
+  // the table's sums go to device memory (one atomic per texel channel), the table is cleared
+  auto flush_table = [&]() {
+    for (int e = threadIdx.x; e < kGradSlots; e += blockDim.x) {
+      const uint32_t key = gkeys[e];
+      if (key == kGradEmpty) continue;
+      for (int c = 0; c < 3; c++) {
+        const float v = gvals[3 * e + c];
+        if (v != 0.f) add_grad(adj.grad + key + c, v);
+        gvals[3 * e + c] = 0.f;
+      }
+      gkeys[e] = kGradEmpty;
+    }
+  };
   const SobolNibbles sobn{sob_tab, sp.dim0};
   const uint32_t n = q.count[which];
   const size_t cap = q.cap;
@@ -319,6 +343,7 @@ __global__ __launch_bounds__(kShadeBlock) NOVA_SHADE_OCCUPANCY void shade_kernel
     const uint64_t m = __ballot(cont);
     const uint32_t lane_rank = __builtin_amdgcn_mbcnt_hi((uint32_t)(m >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)m, 0));
     const uint32_t base = block_reserve<kShadeBlock>(m, &q.count[which ^ 1], it, wave_cnt, blk_base);
+    if (kAdj) flush_table();  // every lane has splatted this iteration (block_reserve waited)
     if (cont) {
       const uint32_t j = base + lane_rank;
       rout[j] = no.x;
@@ -329,6 +354,7 @@ __global__ __launch_bounds__(kShadeBlock) NOVA_SHADE_OCCUPANCY void shade_kernel
       rout[5 * cap + j] = nd.z;
       sout[j] = pid;
     }
+    if (kAdj) __syncthreads();  // the table is clear before the next iteration's splats
   }
   if (escapes != 0 && (threadIdx.x & 63) == 0) atomicAdd(&q.stats64[2], (unsigned long long)escapes);
 }
