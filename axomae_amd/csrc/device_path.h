@@ -33,8 +33,8 @@ NOVA_HD uint32_t pixel_offset(const FrameParams &fp, uint32_t p) {
 // consecutive frames (they differ only by the jitter and walk the same nodes), and the path
 // state and the per-path radiance (partial) are indexed by path id. The sampler's slot
 // keeps the frame-major numbering, slot = frame * npix + pixel. The adjoint's replay numbers
-// paths frame-major (path id = slot): its texel-gradient atomics would otherwise collide,
-// all 64 lanes of a wave adding to the same pixel's texels.
+// its paths the same way; its workgroups sum the texel gradients of a pixel's frames in LDS
+// before the device atomics. (frame_minor = 0, path id = slot, is the frame-major numbering.)
 NOVA_HD uint32_t path_slot(const FrameParams &fp, uint32_t pid) {
   if (!fp.frame_minor) return pid;
   const uint32_t npix = (uint32_t)fp.width * (uint32_t)fp.height;

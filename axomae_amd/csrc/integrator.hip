@@ -297,9 +297,9 @@ __global__ __launch_bounds__(kShadeBlock) NOVA_SHADE_OCCUPANCY void shade_kernel
       int prim;
       float ht, hu, hv;
       AdjointCtx padj = adj;
-      if (kAdj) {  // this path's primal radiance (and hits), left by pass 0 under its frame-minor path id
+      if (kAdj) {  // this path's primal radiance (and hits), left by pass 0 under its pixel-major path id
         const uint32_t slot = path_slot(sp.fp, (uint32_t)pid), npix = (uint32_t)sp.fp.width * (uint32_t)sp.fp.height;
-        const size_t pid0 = (size_t)(slot % npix) * (uint32_t)sp.fp.frames + slot / npix;
+        const size_t pid0 = (size_t)(slot % npix) * (uint32_t)sp.fp.frames + slot / npix;  // = pid
         padj.ltot = adj.ltot + pid0 * 4;
         if (adj.hit_cache && (uint32_t)sp.bounce < adj.cache_bounces) {
           // entries a primal path did not reach hold prim -1 (a miss): always in bounds
@@ -854,7 +854,7 @@ static int render_impl(nova_hip_ctx *ctx, const nova_hip_render_params *p, float
     const uint32_t active = (uint32_t)(npix * frames);
     camera_rays += (uint64_t)active * (adj ? 2 : 1);
     for (int pass = 0; pass < (adj ? 2 : 1); pass++) {  // pass 1: the adjoint replay of pass 0's paths
-    fp.frame_minor = pass == 0 ? 1 : 0;
+    fp.frame_minor = 1;  // the replay numbers paths as pass 0 does
     if (pass == 0 && replay.hit_cache)  // every entry starts as a miss (prim -1)
       NOVA_HIP_CHECK(ctx, hipMemsetAsync(replay.hit_cache, 0xFF, (size_t)replay.cache_bounces * cap * sizeof(float4), s));
     hipLaunchKernelGGL(generate_kernel, dim3(std::min<uint32_t>((active + 255) / 256, (uint32_t)cus * 8)), dim3(256), 0, s, fp, ctx->sobol_table, q,
@@ -864,7 +864,7 @@ static int render_impl(nova_hip_ctx *ctx, const nova_hip_render_params *p, float
     int which = 0;
     // camera rays of a pixel-major batch with at least NOVA_COHERENT_MIN_FRAMES frames: a
     // wave holds 64 / frames pixels' rays, close enough to walk the tree together (a wave of
-    // 64 pixels of one row is not: the frame-major adjoint replay keeps the per-ray walk)
+    // 64 pixels of one row is not)
 #ifndef NOVA_COHERENT_MIN_FRAMES
 #define NOVA_COHERENT_MIN_FRAMES 8
 #endif
