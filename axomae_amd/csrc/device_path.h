@@ -192,6 +192,8 @@ struct AdjointCtx {
   // texel gradients before they go to device memory; null: straight to device memory.
   uint32_t *table_keys = nullptr;
   float *table_vals = nullptr;
+  uint16_t *table_list = nullptr;   // slots taken since the last flush, in claim order
+  uint32_t *table_count = nullptr;  // entries of table_list
 };
 
 // Texel-gradient table of a replay workgroup: neighbouring pixels' paths splat mostly onto
@@ -201,12 +203,15 @@ struct AdjointCtx {
 constexpr int kGradSlots = 2048;
 constexpr uint32_t kGradEmpty = 0xFFFFFFFFu;
 #ifdef __HIP_DEVICE_COMPILE__
-// Adds v[0..n) at `key` to the table; false when no slot was found within the probe limit
-// (the caller then adds to device memory).
-__device__ __forceinline__ bool grad_table_add(uint32_t *keys, float *vals, uint32_t key, const float *v, int n) {
+// Adds v[0..n) at `key` to the table, listing a newly taken slot; false when no slot was
+// found within the probe limit (the caller then adds to device memory).
+__device__ __forceinline__ bool grad_table_add(const AdjointCtx *adj, uint32_t key, const float *v, int n) {
+  uint32_t *keys = adj->table_keys;
+  float *vals = adj->table_vals;
   uint32_t h = (key * 2654435761u) >> 21;  // 11 bits: kGradSlots
   for (int probe = 0; probe < 32; probe++) {
     const uint32_t old = atomicCAS(&keys[h], kGradEmpty, key);
+    if (old == kGradEmpty) adj->table_list[atomicAdd(adj->table_count, 1u)] = (uint16_t)h;
     if (old == kGradEmpty || old == key) {
       for (int c = 0; c < n; c++)
         if (v[c] != 0.f) atomicAdd(&vals[h * 3 + c], v[c]);
@@ -239,7 +244,7 @@ NOVA_HD void splat_texture_grad(const ShadeCtx &sc, const AdjointCtx *adj, int t
     float val[3] = {0.f, 0.f, 0.f};
     for (int ch = 0; ch < nc; ch++) val[ch] = w[k] * g[ch];
 #ifdef __HIP_DEVICE_COMPILE__
-    if (adj->table_keys && key < kGradEmpty && grad_table_add(adj->table_keys, adj->table_vals, (uint32_t)key, val, nc)) continue;
+    if (adj->table_keys && key < kGradEmpty && grad_table_add(adj, (uint32_t)key, val, nc)) continue;
 #endif
     for (int ch = 0; ch < nc; ch++)
       if (g[ch] != 0.f) add_grad(adj->grad + key + ch, val[ch]);

@@ -248,27 +248,35 @@ __global__ __launch_bounds__(kShadeBlock) NOVA_SHADE_OCCUPANCY void shade_kernel
   // the replay's texel-gradient table (an array of one element in the primal kernel)
   __shared__ uint32_t gkeys[kAdj ? kGradSlots : 1];
   __shared__ float gvals[kAdj ? 3 * kGradSlots : 1];
+  __shared__ uint16_t glist[kAdj ? kGradSlots : 1];
+  __shared__ uint32_t gcount[2];  // list lengths of even and odd iterations
   if (kAdj) {
     for (int e = threadIdx.x; e < kGradSlots; e += blockDim.x) {
       gkeys[e] = kGradEmpty;
       gvals[3 * e] = gvals[3 * e + 1] = gvals[3 * e + 2] = 0.f;
     }
+    if (threadIdx.x < 2) gcount[threadIdx.x] = 0;
     adj.table_keys = gkeys;
     adj.table_vals = gvals;
+    adj.table_list = glist;
   }
   __syncthreads();
-  // the table's sums go to device memory (one atomic per texel channel), the table is cleared
-  auto flush_table = [&]() {
-    for (int e = threadIdx.x; e < kGradSlots; e += blockDim.x) {
-      const uint32_t key = gkeys[e];
-      if (key == kGradEmpty) continue;
+  // Iteration it's taken slots go to device memory (one atomic per texel channel) and are
+  // cleared; the next iteration's count starts at zero (it was read in iteration it - 1,
+  // before that iteration's closing barrier).
+  auto flush_table = [&](uint32_t it) {
+    const uint32_t used = gcount[it & 1u];
+    for (uint32_t e = threadIdx.x; e < used; e += blockDim.x) {
+      const uint32_t h = glist[e];
+      const uint32_t key = gkeys[h];
       for (int c = 0; c < 3; c++) {
-        const float v = gvals[3 * e + c];
+        const float v = gvals[3 * h + c];
         if (v != 0.f) add_grad(adj.grad + key + c, v);
-        gvals[3 * e + c] = 0.f;
+        gvals[3 * h + c] = 0.f;
       }
-      gkeys[e] = kGradEmpty;
+      gkeys[h] = kGradEmpty;
     }
+    if (threadIdx.x == 0) gcount[(it + 1u) & 1u] = 0;
   };
   const SobolNibbles sobn{sob_tab, sp.dim0};
   const uint32_t n = q.count[which];
@@ -297,6 +305,7 @@ __global__ __launch_bounds__(kShadeBlock) NOVA_SHADE_OCCUPANCY void shade_kernel
       int prim;
       float ht, hu, hv;
       AdjointCtx padj = adj;
+      padj.table_count = &gcount[it & 1u];
       if (kAdj) {  // this path's primal radiance (and hits), left by pass 0 under its pixel-major path id
         const uint32_t slot = path_slot(sp.fp, (uint32_t)pid), npix = (uint32_t)sp.fp.width * (uint32_t)sp.fp.height;
         const size_t pid0 = (size_t)(slot % npix) * (uint32_t)sp.fp.frames + slot / npix;  // = pid
@@ -343,7 +352,7 @@ __global__ __launch_bounds__(kShadeBlock) NOVA_SHADE_OCCUPANCY void shade_kernel
     const uint64_t m = __ballot(cont);
     const uint32_t lane_rank = __builtin_amdgcn_mbcnt_hi((uint32_t)(m >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)m, 0));
     const uint32_t base = block_reserve<kShadeBlock>(m, &q.count[which ^ 1], it, wave_cnt, blk_base);
-    if (kAdj) flush_table();  // every lane has splatted this iteration (block_reserve waited)
+    if (kAdj) flush_table(it);  // every lane has splatted this iteration (block_reserve waited)
     if (cont) {
       const uint32_t j = base + lane_rank;
       rout[j] = no.x;
