@@ -651,10 +651,15 @@ extern "C" int nova_hip_batch_autotune(nova_hip_ctx *ex, const nova_hip_batch_it
     occ_s = 4;
   const uint32_t trace_occ[] = {(uint32_t)occ_t};
   const uint32_t shade_occ[] = {(uint32_t)occ_s, 2u * (uint32_t)occ_s};
-  // every candidate runs the same slots: one wave from the middle of the batch. A wave must
-  // start on a multiple of 64 paths (a packet of the per-wave camera walk never straddles
-  // two scenes), so the start is rounded down to a multiple of 256, as capacities are.
+  // every candidate runs the same slots: the largest capacity's worth (or the whole batch)
+  // from the middle of the batch, as many waves of its own capacity as that takes, so a
+  // candidate's time per path includes its waves' fixed costs over the same scenes. A wave
+  // must start on a multiple of 64 paths (a packet of the per-wave camera walk never
+  // straddles two scenes), so the start is rounded down to a multiple of 256, as capacities are.
+  const uint64_t span = std::min<uint64_t>(bp.total, caps[sizeof(caps) / sizeof(caps[0]) - 1]);
   const uint64_t mid = bp.total / 2 / 256 * 256;
+  const uint64_t start = bp.total - mid >= span ? mid : (bp.total - span) / 256 * 256;
+  const uint64_t end = std::min<uint64_t>(bp.total, start + span);
   float best_t = -1.f;
   hipEvent_t e0, e1;
   NOVA_HIP_CHECK(ex, hipEventCreate(&e0));
@@ -662,9 +667,8 @@ extern "C" int nova_hip_batch_autotune(nova_hip_ctx *ex, const nova_hip_batch_it
   for (uint32_t cap0 : caps)
     for (uint32_t to : trace_occ)
       for (uint32_t so : shade_occ) {
-        const uint32_t cap = (uint32_t)std::min<uint64_t>(cap0, (bp.total + 255) / 256 * 256);
-        const uint64_t start = bp.total - mid >= cap ? mid : (bp.total > cap ? (bp.total - cap) / 256 * 256 : 0);
-        const WavePlan wp = plan_waves(bp, start, std::min<uint64_t>(bp.total, start + cap), cap);
+        const uint32_t cap = (uint32_t)std::min<uint64_t>(cap0, (span + 255) / 256 * 256);
+        const WavePlan wp = plan_waves(bp, start, end, cap);
         if (!wp.ok) return set_error(ex, NOVA_HIP_INVALID_ARGUMENT, "batch wave start not on a packet boundary");
         nova_hip_batch_config cfg{cap, to, so, 1};
         const uint32_t trace_threads = std::min<uint32_t>((cap + kTraceBlock - 1) / kTraceBlock, (uint32_t)cus * to) * kTraceBlock;
@@ -676,14 +680,14 @@ extern "C" int nova_hip_batch_autotune(nova_hip_ctx *ex, const nova_hip_batch_it
         for (int rep = 0; rep < 3; rep++) {  // rep 0 warms up
           hipLaunchKernelGGL(batch_set, dim3(1), dim3(1), 0, s, B.state, 0u, 0u);
           NOVA_HIP_CHECK(ex, hipEventRecord(e0, s));
-          NOVA_HIP_CHECK(ex, hipGraphLaunch(g.exec, s));
+          for (size_t w = 0; w < wp.waves.size(); w++) NOVA_HIP_CHECK(ex, hipGraphLaunch(g.exec, s));
           NOVA_HIP_CHECK(ex, hipEventRecord(e1, s));
           NOVA_HIP_CHECK(ex, hipEventSynchronize(e1));
           float ms = 0.f;
           hipEventElapsedTime(&ms, e0, e1);
           if (rep > 0) t += ms;
         }
-        const float ns = t * 1e6f / (2.f * (float)wp.waves[0].count);
+        const float ns = t * 1e6f / (2.f * (float)(end - start));
         if (best_t < 0.f || ns < best_t) {
           best_t = ns;
           *best = cfg;

@@ -343,8 +343,9 @@ int nova_hip_render_batch(nova_hip_ctx *executor, const nova_hip_batch_item *ite
 
 /* Autotuner for nova_hip_render_batch: times the captured wave graph of every candidate
  * (queue capacity 2^22 to 2^26 paths x the trace workgroups per CU that are resident at
- * once x one or two rounds of resident shade workgroups per CU) on the same slots from the
- * middle of the batch, without accumulating, and returns the fastest per path
+ * once x one or two rounds of resident shade workgroups per CU) on the same slots, 2^26
+ * paths from the middle of the batch (the whole batch when smaller) run as as many waves as
+ * the candidate's capacity takes, without accumulating, and returns the fastest per path
  * (best_ns_per_path optional). */
 int nova_hip_batch_autotune(nova_hip_ctx *executor, const nova_hip_batch_item *items, uint32_t count, nova_hip_batch_config *best,
                             float *best_ns_per_path, void *stream);

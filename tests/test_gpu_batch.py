@@ -21,16 +21,17 @@ SPECS64 = [(11, 3000, 40, 8), (13, 12000, 56, 2), (14, 1500, 32, 4), (16, 2000, 
 def batch_jobs(request):
     from axomae_amd.nova_hip import HipEngine
     specs = SPECS if request.param == "mixed" else SPECS64
-    engines, jobs, refs = [], [], []
+    engines, jobs, refs, segments = [], [], [], 0
     for seed, tris, res, spp in specs:
         e = HipEngine(0)
         e.upload(scenes.batch_scene(seed, tris, res))
         e.build_bvh()
         engines.append(e)
         ref, _ = e.render(res, res, spp, DEPTH, rr_start=3)
+        segments += e.render_stats().segments
         refs.append(ref.cpu().numpy())
         jobs.append((e, res, res, spp, DEPTH, {"rr_start": 3}))
-    yield jobs, refs
+    yield jobs, refs, segments
     for e in engines:
         e.close()
 
@@ -40,11 +41,12 @@ def batch_jobs(request):
 @pytest.mark.parametrize("cap", [2048, 1 << 20])
 def test_batch_equals_single_scene_renders(batch_jobs, use_graph, cap):
     from axomae_amd.nova_hip import BatchConfig, BatchRenderer
-    jobs, refs = batch_jobs
+    jobs, refs, segments = batch_jobs
     br = BatchRenderer(0)
     finals, st, comp = br.render(jobs, BatchConfig(cap, 8, 8, use_graph))
-    print(f"[batch] cap {cap} graph {use_graph}: {st.waves} waves, {st.paths} paths, {st.total_ms:.2f} ms", flush=True)
+    print(f"[batch] cap {cap} graph {use_graph}: {st.waves} waves, {st.paths} paths, {st.segments} segments, {st.total_ms:.2f} ms", flush=True)
     assert st.paths == sum(w * h * s for _, w, h, s, _, _ in jobs)
+    assert st.segments == segments  # rays traced, as the single-scene renders count them
     for k, (f, ref) in enumerate(zip(finals, refs)):
         np.testing.assert_array_equal(f.cpu().numpy(), ref, err_msg=f"scene {k}")
     assert np.all(np.diff(comp) >= 0) and comp[-1] <= st.total_ms + 1e-3
@@ -54,7 +56,7 @@ def test_batch_equals_single_scene_renders(batch_jobs, use_graph, cap):
 @gpu
 def test_batch_autotune_picks_a_candidate(batch_jobs):
     from axomae_amd.nova_hip import BatchRenderer
-    jobs, refs = batch_jobs
+    jobs, refs, _ = batch_jobs
     br = BatchRenderer(0)
     cfg, ns = br.autotune(jobs)
     print(f"[batch] autotune: {cfg.as_dict()} {ns:.3f} ns/path", flush=True)
@@ -71,9 +73,9 @@ def test_batch_autotune_picks_a_candidate(batch_jobs):
 def test_batch_autotune_wave_starts_on_a_packet_boundary():
     """Two 1000x1000 scenes at 5 and 4 spp: 9,000,000 paths, 64 modulo 128, every scene's
     path count a multiple of 64 (camera rays go in packets of 64). The autotuner's trial
-    wave starts in the middle of the batch; it must start on a packet boundary so that no
-    packet mixes two scenes. The tuned configuration is accepted by the renderer and every
-    image equals its single-scene render."""
+    waves (here over the whole batch, at every candidate capacity) must each start on a
+    packet boundary so that no packet mixes two scenes. The tuned configuration is accepted
+    by the renderer and every image equals its single-scene render."""
     from axomae_amd.nova_hip import BatchRenderer, HipEngine
     engines, jobs, refs = [], [], []
     for seed, spp in ((21, 5), (22, 4)):
