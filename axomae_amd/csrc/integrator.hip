@@ -17,6 +17,7 @@
 // Alpha follows the same recursion (1 per scattering level plus the terminal alpha).
 #include <algorithm>
 #include <cstdlib>
+#include <cstring>
 #include <mutex>
 #include <vector>
 
@@ -375,8 +376,16 @@ __global__ __launch_bounds__(kShadeBlock) NOVA_SHADE_OCCUPANCY void shade_kernel
 // here a path's bounces follow each other at once. Every path takes the same steps with the
 // same arithmetic as in the per-bounce launches, so the image is the same. Dynamic LDS: the
 // traversal stacks, then the nibble tables of the sampler dimensions from sp.dim0 on.
+#ifndef NOVA_TAIL_WAVES
+#define NOVA_TAIL_WAVES 4  // waves per SIMD tail_kernel is compiled for (0: compiler's choice, 3 at its 130 registers)
+#endif
+#if NOVA_TAIL_WAVES > 0
+#define NOVA_TAIL_OCCUPANCY __attribute__((amdgpu_waves_per_eu(NOVA_TAIL_WAVES, NOVA_TAIL_WAVES)))
+#else
+#define NOVA_TAIL_OCCUPANCY
+#endif
 template <bool kCount>
-__global__ __launch_bounds__(kTraceBlock) void tail_kernel(TraceCtx tc, ShadeParams sp, int max_depth, int rr_start, ShadeCtx sc,
+__global__ __launch_bounds__(kTraceBlock) NOVA_TAIL_OCCUPANCY void tail_kernel(TraceCtx tc, ShadeParams sp, int max_depth, int rr_start, ShadeCtx sc,
                                                           const uint32_t *__restrict__ sobol, Queues q, int which, int ndims) {
   extern __shared__ uint32_t tail_lds[];
   if ((q.count[kTailSlot] & 0x7FFFFFFFu) != (uint32_t)sp.bounce + 1u) return;
@@ -956,13 +965,16 @@ static int render_impl(nova_hip_ctx *ctx, const nova_hip_render_params *p, float
   hipEventElapsedTime(&ms, ev0, ev1);
   hipEventDestroy(ev0);
   hipEventDestroy(ev1);
-  uint32_t seg = 0;
-  unsigned long long s64[20] = {};
-  NOVA_HIP_CHECK(ctx, hipMemcpy(&seg, counts + 2, sizeof(uint32_t), hipMemcpyDeviceToHost));
-  NOVA_HIP_CHECK(ctx, hipMemcpy(s64, stats64, sizeof(s64), hipMemcpyDeviceToHost));
+  // the counters and the 64-bit statistics after them in one copy (the per-frame engine
+  // loop pays for every synchronous copy)
+  uint32_t words[8 + 2 * 20] = {};
+  static_assert(sizeof(words) <= 256, "counters piece");
+  NOVA_HIP_CHECK(ctx, hipMemcpy(words, counts, sizeof(words), hipMemcpyDeviceToHost));
+  const uint32_t seg = words[2];
+  unsigned long long s64[20];
+  std::memcpy(s64, words + 8, sizeof(s64));
   if (tail_rays > 0) {  // where the last batch's tail started: the next call's hint
-    uint32_t slot = 0;
-    NOVA_HIP_CHECK(ctx, hipMemcpy(&slot, counts + kTailSlot, sizeof(uint32_t), hipMemcpyDeviceToHost));
+    const uint32_t slot = words[kTailSlot];
     ctx->tail_hint = (slot == 0 || (slot & 0x80000000u)) ? 0 : (int)slot - 1;  // a forced start with many rays: learn again
     ctx->tail_hint_key[0] = (uint64_t)cap;
     ctx->tail_hint_key[1] = (uint64_t)p->max_depth;
