@@ -522,6 +522,7 @@ static int make_plan(nova_hip_ctx *ex, const nova_hip_batch_item *items, uint32_
     S.fp.height = p->height;
     S.fp.frame0 = p->frame_start;
     S.fp.frame_minor = 1;
+    S.fp.tiled = 0;  // batch_accumulate takes a wave's pixels as one run per scene
     S.fp.frames = p->frame_count;
     S.fp.offset = p->max_samples_offset;
     S.fp.vinvert = p->vertical_invert;

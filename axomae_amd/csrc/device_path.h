@@ -13,6 +13,7 @@ struct FrameParams {
   int32_t vinvert;
   int32_t policy;  // NOVA_HIP_INDEX_CPU or NOVA_HIP_INDEX_OPTIX
   int32_t frame_minor;  // integrator path numbering: 1 pixel-major / frame-minor, 0 frame-major (= slot)
+  int32_t tiled;        // with frame_minor: pixels taken in 8x8 tiles (pixel_of_rank), 0 row by row
   float inv_P[16], inv_V[16];
 };
 
@@ -28,17 +29,33 @@ NOVA_HD uint32_t pixel_offset(const FrameParams &fp, uint32_t p) {
   return fp.vinvert ? (((uint32_t)fp.height - 1 - y) * (uint32_t)fp.width + x) * 4u : (y * (uint32_t)fp.width + x) * 4u;
 }
 
-// Renders number paths pixel-major, frame-minor: path id = pixel * frames + frame, which is
-// also the path's entry in the first queue. So a wave traces one pixel's camera rays of
-// consecutive frames (they differ only by the jitter and walk the same nodes), and the path
-// state and the per-path radiance (partial) are indexed by path id. The sampler's slot
-// keeps the frame-major numbering, slot = frame * npix + pixel. The adjoint's replay numbers
-// its paths the same way; its workgroups sum the texel gradients of a pixel's frames in LDS
-// before the device atomics. (frame_minor = 0, path id = slot, is the frame-major numbering.)
+// Pixel of rank r in the order the paths take them. Row by row, or with fp.tiled: each band
+// of 8 full rows in 8x8 tiles from left to right (row by row inside a tile), then the band's
+// last width % 8 columns row by row, and the rows below the last full band row by row. With
+// few frames per pixel a wave's camera rays then come from a compact block of pixels
+// instead of a strip of one row. A bijection on [0, width * height).
+NOVA_HD uint32_t pixel_of_rank(const FrameParams &fp, uint32_t r) {
+  if (!fp.tiled) return r;
+  const uint32_t W = (uint32_t)fp.width, H8 = (uint32_t)fp.height & ~7u, W8 = W & ~7u;
+  if (r >= H8 * W) return r;
+  const uint32_t b = r / (8u * W), j = r - b * (8u * W);
+  if (j < 8u * W8) return (8u * b + ((j & 63u) >> 3)) * W + 8u * (j >> 6) + (j & 7u);
+  const uint32_t rem = W - W8, k = j - 8u * W8;
+  return (8u * b + k / rem) * W + W8 + k % rem;
+}
+
+// Renders number paths pixel-major, frame-minor: path id = rank * frames + frame (rank: the
+// pixel's place in pixel_of_rank's order), which is also the path's entry in the first
+// queue. So a wave traces one pixel's camera rays of consecutive frames (they differ only by
+// the jitter and walk the same nodes), and the path state and the per-path radiance
+// (partial) are indexed by path id. The sampler's slot keeps the frame-major numbering,
+// slot = frame * npix + pixel. The adjoint's replay numbers its paths the same way; its
+// workgroups sum the texel gradients of a pixel's frames in LDS before the device atomics.
+// (frame_minor = 0, path id = slot, is the frame-major numbering.)
 NOVA_HD uint32_t path_slot(const FrameParams &fp, uint32_t pid) {
   if (!fp.frame_minor) return pid;
   const uint32_t npix = (uint32_t)fp.width * (uint32_t)fp.height;
-  return (pid % (uint32_t)fp.frames) * npix + pid / (uint32_t)fp.frames;
+  return (pid % (uint32_t)fp.frames) * npix + pixel_of_rank(fp, pid / (uint32_t)fp.frames);
 }
 
 // Sampler index of a path slot (frame-major slots), u32 wraparound. CPU policy:

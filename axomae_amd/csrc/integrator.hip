@@ -307,9 +307,8 @@ __global__ __launch_bounds__(kShadeBlock) NOVA_SHADE_OCCUPANCY void shade_kernel
       float ht, hu, hv;
       AdjointCtx padj = adj;
       padj.table_count = &gcount[it & 1u];
-      if (kAdj) {  // this path's primal radiance (and hits), left by pass 0 under its pixel-major path id
-        const uint32_t slot = path_slot(sp.fp, (uint32_t)pid), npix = (uint32_t)sp.fp.width * (uint32_t)sp.fp.height;
-        const size_t pid0 = (size_t)(slot % npix) * (uint32_t)sp.fp.frames + slot / npix;  // = pid
+      if (kAdj) {  // this path's primal radiance (and hits), left by pass 0 under the same path id
+        const size_t pid0 = (uint32_t)pid;
         padj.ltot = adj.ltot + pid0 * 4;
         if (adj.hit_cache && (uint32_t)sp.bounce < adj.cache_bounces) {
           // entries a primal path did not reach hold prim -1 (a miss): always in bounds
@@ -453,8 +452,8 @@ __global__ void accumulate_kernel(const float *__restrict__ partial, FrameParams
   const uint32_t npix4 = (uint32_t)fp.width * (uint32_t)fp.height * 4u;
   const int frames = fp.frames;
   for (uint32_t e = blockIdx.x * blockDim.x + threadIdx.x; e < npix4; e += gridDim.x * blockDim.x) {
-    const uint32_t p = e >> 2, c = e & 3u;
-    const uint32_t i = pixel_offset(fp, p) + c;
+    const uint32_t p = e >> 2, c = e & 3u;  // p: the pixel's rank in the path numbering
+    const uint32_t i = pixel_offset(fp, pixel_of_rank(fp, p)) + c;
     const float *src = partial + (size_t)p * frames * 4 + c;
     if (accum) {  // without accum/final: the caller accumulates (the drop-in's per-frame traverse)
       float a = accum[i];
@@ -586,6 +585,19 @@ static uint32_t tail_rays_setting() {
   return v;
 }
 
+#ifndef NOVA_TILE_FRAMES
+#define NOVA_TILE_FRAMES 64  // batches of fewer frames take their pixels in 8x8 tiles (pixel_of_rank)
+#endif
+// NOVA_TILE_FRAMES, or the NOVA_HIP_TILE_FRAMES environment variable when set (read once per
+// process; a tuning knob).
+static int tile_frames_setting() {
+  static const int v = [] {
+    const char *e = std::getenv("NOVA_HIP_TILE_FRAMES");
+    return e ? (int)std::strtol(e, nullptr, 10) : (int)NOVA_TILE_FRAMES;
+  }();
+  return v;
+}
+
 // Counters start at zero through a kernel, not a memset, so the call can be captured.
 __global__ void zero_words_kernel(uint32_t *p, uint32_t n) {
   for (uint32_t i = threadIdx.x; i < n; i += blockDim.x) p[i] = 0u;
@@ -666,6 +678,7 @@ static int render_pass(nova_hip_ctx *ctx, const nova_hip_render_params *p, int p
   fp.frame0 = p->frame_start;
   fp.frames = p->frame_count;
   fp.frame_minor = 0;
+  fp.tiled = 0;
   fp.offset = 0;
   fp.vinvert = p->vertical_invert;
   fp.policy = NOVA_HIP_INDEX_CPU;
@@ -873,6 +886,7 @@ static int render_impl(nova_hip_ctx *ctx, const nova_hip_render_params *p, float
     camera_rays += (uint64_t)active * (adj ? 2 : 1);
     for (int pass = 0; pass < (adj ? 2 : 1); pass++) {  // pass 1: the adjoint replay of pass 0's paths
     fp.frame_minor = 1;  // the replay numbers paths as pass 0 does
+    fp.tiled = frames < tile_frames_setting() ? 1 : 0;
     if (pass == 0 && replay.hit_cache)  // every entry starts as a miss (prim -1)
       NOVA_HIP_CHECK(ctx, hipMemsetAsync(replay.hit_cache, 0xFF, (size_t)replay.cache_bounces * cap * sizeof(float4), s));
     hipLaunchKernelGGL(generate_kernel, dim3(std::min<uint32_t>((active + 255) / 256, (uint32_t)cus * 8)), dim3(256), 0, s, fp, ctx->sobol_table, q,

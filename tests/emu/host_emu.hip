@@ -230,6 +230,7 @@ static int emu_run(const nova_hip_scene_desc *sd, const nova_hip_render_params *
     fp.height = p->height;
     fp.frame0 = p->frame_start;
     fp.frame_minor = 0;
+    fp.tiled = 0;
     fp.frames = p->frame_count;
     fp.offset = 0;
     fp.vinvert = p->vertical_invert;
@@ -263,6 +264,7 @@ static int emu_run(const nova_hip_scene_desc *sd, const nova_hip_render_params *
     fp.height = p->height;
     fp.frame0 = f;
     fp.frame_minor = 0;
+    fp.tiled = 0;
     fp.frames = 1;
     fp.offset = p->max_samples_offset;
     fp.vinvert = p->vertical_invert;
@@ -320,6 +322,20 @@ extern "C" int emu_render(const nova_hip_scene_desc *sd, const nova_hip_render_p
 // Texel gradients (FP32 pool layout) from the device path's replay code run on the host.
 extern "C" int emu_adjoint(const nova_hip_scene_desc *sd, const nova_hip_render_params *p, const float *d_image, float scale, float *grad) {
   return emu_run(sd, p, nullptr, nullptr, d_image, scale, grad);
+}
+
+// Sampler slot of every path id of a width x height x frames batch in the renderers' path
+// numbering (pixel-major, frame-minor; pixels in 8x8 tiles when tiled), out: npix * frames.
+extern "C" int emu_path_slots(int32_t width, int32_t height, int32_t frames, int32_t tiled, uint32_t *out) {
+  FrameParams fp{};
+  fp.width = width;
+  fp.height = height;
+  fp.frames = frames;
+  fp.frame_minor = 1;
+  fp.tiled = tiled;
+  const uint32_t n = (uint32_t)width * (uint32_t)height * (uint32_t)frames;
+  for (uint32_t pid = 0; pid < n; pid++) out[pid] = path_slot(fp, pid);
+  return 0;
 }
 
 // Sobol samples of dims [dim0, dim0+5) for the given indices, from the direction table and

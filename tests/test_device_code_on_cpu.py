@@ -161,6 +161,28 @@ def test_sobol_nibble_tables_match_direction_table(emu, oracle):
         np.testing.assert_array_equal(b[:, :len(dims)], ref)
 
 
+@pytest.mark.parametrize("w,h,frames", [(1, 1, 1), (7, 5, 3), (20, 20, 3), (37, 29, 1), (64, 16, 2), (1920, 1080, 1), (50, 37, 8)])
+def test_tiled_path_numbering_is_a_permutation(emu, w, h, frames):
+    """Path id -> sampler slot with pixels in 8x8 tiles (renders of fewer than 64 frames):
+    every slot frame * npix + pixel is reached exactly once, a path keeps its frame
+    (id % frames), the first 64 pixel ranks of a band of full tiles cover one 8x8 block,
+    and the row-by-row numbering is pixel * frames + frame."""
+    emu.emu_path_slots.argtypes = [C.c_int32, C.c_int32, C.c_int32, C.c_int32, C.c_void_p]
+    npix, n = w * h, w * h * frames
+    tiled = np.empty(n, np.uint32)
+    plain = np.empty(n, np.uint32)
+    assert emu.emu_path_slots(w, h, frames, 1, tiled.ctypes.data) == 0
+    assert emu.emu_path_slots(w, h, frames, 0, plain.ctypes.data) == 0
+    np.testing.assert_array_equal(np.sort(tiled), np.arange(n, dtype=np.uint32))
+    pid = np.arange(n, dtype=np.uint64)
+    np.testing.assert_array_equal(tiled // npix, pid % frames)
+    np.testing.assert_array_equal(plain, (pid % frames) * npix + pid // frames)
+    if w >= 8 and h >= 8:
+        px = tiled[: 64 * frames: frames] % npix
+        xs, ys = px % w, px // w
+        assert sorted(zip(ys.tolist(), xs.tolist())) == [(y, x) for y in range(8) for x in range(8)]
+
+
 @pytest.mark.parametrize("pass_", [1, 2, 3])
 def test_render_passes_device_code_match_oracle(oracle, emu, pass_):
     """Normal / depth / emissive passes (NormalIntegrator, DepthIntegrator, EmissiveIntegrator):
