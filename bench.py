@@ -377,7 +377,7 @@ def main():
     traffic, valu = None, None
     if counters:
         traffic = counters.get("hbm_bytes_per_launch")
-        valu = {"valu_busy": counters.get("valu_busy"),
+        valu = {"valu_busy": counters.get("valu_busy"), "texture_data_busy": counters.get("td_busy"),
                 "valu_insts_per_launch": counters.get("sq_insts_valu_per_launch"),
                 "valu_lane_insts_per_node_visit": round(counters["sq_insts_valu_per_launch"] * 64.0 / (vis_r / launches_r), 1)
                 if counters.get("sq_insts_valu_per_launch") else None,
@@ -395,7 +395,7 @@ def main():
                      "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
                      "kernel": "extend_kernel<false> (per-ray closest-hit walk, bounces 1-8)",
                      "algorithmic_bytes_per_launch": round(bytes_r), "avg_launch_ms": round(avg_r_ms, 4),
-                     "measured_limiter": "VALU issue (not HBM): see valu", "valu": valu,
+                     "measured_limiter": "VALU issue and texture data return (not HBM): see valu", "valu": valu,
                      "camera_walk": {"kernel": "extend_coherent_kernel<false> (bounce 0, one walk per wave)",
                                      "algorithmic_bytes_per_launch": round(bytes_c), "avg_launch_ms": round(avg_c_ms, 4),
                                      "achieved_GBs": round(bytes_c / (avg_c_ms / 1e3) / 1e9, 1) if avg_c_ms > 0 else None}},
