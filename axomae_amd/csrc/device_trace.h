@@ -117,15 +117,9 @@ NOVA_HD void set_node_boxes(Node4 &nd, const BNode *bn, const int32_t ch[4], int
       lo[a] = fminf(lo[a], bn[ch[k]].mn[a]);
       hi[a] = fmaxf(hi[a], bn[ch[k]].mx[a]);
     }
-    // The origin is a multiple of the power-of-two step at or below lo (at most one step
-    // below it), so every plane org + q*step = (k + q)*step is exactly representable: the
-    // walk's one-FMA slab distances then carry no rounding of the plane itself.
+    org[a] = lo[a];
     step[a] = grid_step(lo[a], hi[a]);
-    org[a] = floorf(lo[a] / step[a]) * step[a];
-    while (grid_plane(255u, step[a], org[a]) < hi[a]) {  // not taken for finite boxes; a guard
-      step[a] *= 2.f;
-      org[a] = floorf(lo[a] / step[a]) * step[a];
-    }
+    while (grid_plane(255u, step[a], org[a]) < hi[a]) step[a] *= 2.f;  // not taken for finite boxes; a guard
   }
   for (int k = 0; k < m; k++)
     for (int a = 0; a < 3; a++) {
@@ -231,10 +225,7 @@ NOVA_HD void trace_init(TraceState &st, f3 o, f3 d) {
   st.px = !(fabsf(st.inv.x) < kInf);
   st.py = !(fabsf(st.inv.y) < kInf);
   st.pz = !(fabsf(st.inv.z) < kInf);
-  // Rays with a reciprocal above 2^64 also take the walk's per-plane form (trace_step): its
-  // one-FMA form could overflow for them.
-  const float kHuge = 1.8446744e19f;
-  st.parallel = st.px || st.py || st.pz || !(fabsf(st.inv.x) <= kHuge && fabsf(st.inv.y) <= kHuge && fabsf(st.inv.z) <= kHuge);
+  st.parallel = st.px || st.py || st.pz;
   st.tbest = 3.402823466e+38f;
   st.best = -1;
   st.bpos = -1;
@@ -272,10 +263,10 @@ NOVA_HD bool trace_step(const TraceCtx &tc, Stack &stack, TraceState &st, float 
     // per child: entry distance (+inf when culled) and child reference (-1 when culled)
     float key[4];
     int32_t cref[4];
-    // Two copies of the slab test: rays with an axis-parallel direction (or a reciprocal
-    // above 2^64) take the containment form with each plane decoded; the common copy keeps
-    // every min/max operand a product of the same block, which spares the NaN-quieting
-    // copies the compiler adds to values merged from branches.
+    // Two copies of the slab test: rays with an axis-parallel direction take the
+    // containment form; the common copy keeps every min/max operand a product of the same
+    // block, which spares the NaN-quieting copies the compiler adds to values merged from
+    // branches.
     if (st.parallel) {
 #pragma unroll
       for (int k = 0; k < 4; k++) {
@@ -314,32 +305,17 @@ NOVA_HD bool trace_step(const TraceCtx &tc, Stack &stack, TraceState &st, float 
       const uint32_t nqx = st.inv.x < 0.f ? qmxx : qmnx, fqx = st.inv.x < 0.f ? qmnx : qmxx;
       const uint32_t nqy = st.inv.y < 0.f ? qmxy : qmny, fqy = st.inv.y < 0.f ? qmny : qmxy;
       const uint32_t nqz = st.inv.z < 0.f ? qmxz : qmnz, fqz = st.inv.z < 0.f ? qmnz : qmxz;
-      // The slab distance of grid plane q, (org + q*step - o) * inv, as one FMA per plane:
-      // q*A + B with A = step*inv (exact: the step is a power of two) and B = (org - o)*inv,
-      // made once per node and axis. The plane org + q*step is exact (set_node_boxes), so
-      // this value and the per-plane form fl(fl(plane - o) * inv) both lie within
-      // 2^-24 * (765|A| + 5|B|) of the exact product; e = 2^-21 * (160|A| + |B|) bounds that
-      // with room for its own rounding. Near planes take B - e and far planes B + e, so each
-      // slab interval contains the per-plane form's: every box that form accepts is accepted
-      // here, and the tests are written so that a NaN keeps the box.
-      const float Ax = sx * st.inv.x, Ay = sy * st.inv.y, Az = sz * st.inv.z;
-      const float Bx = (orgx - st.o.x) * st.inv.x, By = (orgy - st.o.y) * st.inv.y, Bz = (orgz - st.o.z) * st.inv.z;
-      const float ex = 4.76837158203125e-7f * fmaf(160.f, fabsf(Ax), fabsf(Bx));
-      const float ey = 4.76837158203125e-7f * fmaf(160.f, fabsf(Ay), fabsf(By));
-      const float ez = 4.76837158203125e-7f * fmaf(160.f, fabsf(Az), fabsf(Bz));
-      const float Bnx = Bx - ex, Bfx = Bx + ex, Bny = By - ey, Bfy = By + ey, Bnz = Bz - ez, Bfz = Bz + ez;
-      const float tlim = st.tbest * kUp;
 #pragma unroll
       for (int k = 0; k < 4; k++) {
-        const float tnx = fmaf((float)((nqx >> (8 * k)) & 255u), Ax, Bnx);
-        const float tfx = fmaf((float)((fqx >> (8 * k)) & 255u), Ax, Bfx);
-        const float tny = fmaf((float)((nqy >> (8 * k)) & 255u), Ay, Bny);
-        const float tfy = fmaf((float)((fqy >> (8 * k)) & 255u), Ay, Bfy);
-        const float tnz = fmaf((float)((nqz >> (8 * k)) & 255u), Az, Bnz);
-        const float tfz = fmaf((float)((fqz >> (8 * k)) & 255u), Az, Bfz);
+        const float tnx = (grid_plane((nqx >> (8 * k)) & 255u, sx, orgx) - st.o.x) * st.inv.x;
+        const float tfx = (grid_plane((fqx >> (8 * k)) & 255u, sx, orgx) - st.o.x) * st.inv.x;
+        const float tny = (grid_plane((nqy >> (8 * k)) & 255u, sy, orgy) - st.o.y) * st.inv.y;
+        const float tfy = (grid_plane((fqy >> (8 * k)) & 255u, sy, orgy) - st.o.y) * st.inv.y;
+        const float tnz = (grid_plane((nqz >> (8 * k)) & 255u, sz, orgz) - st.o.z) * st.inv.z;
+        const float tfz = (grid_plane((fqz >> (8 * k)) & 255u, sz, orgz) - st.o.z) * st.inv.z;
         const float t0 = fmaxf(fmaxf(tnx, tny), tnz);
         const float t1 = fminf(fminf(tfx, tfy), tfz) * kUp;
-        const bool hit = chv[k] != -1 && !(t0 > t1) && !(t1 < 0.f) && !(t0 > tlim);
+        const bool hit = chv[k] != -1 && t0 <= t1 && t1 >= 0.f && t0 <= st.tbest * kUp;
         key[k] = hit ? t0 : kInf;
         cref[k] = hit ? chv[k] : -1;
       }

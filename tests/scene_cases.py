@@ -75,7 +75,29 @@ def axis_rays(n, seed, box):
     return np.concatenate([o, d], 1).astype(np.float32)
 
 
+def near_axis_rays(n, seed, box):
+    """axis_rays with each zero direction component replaced by a tiny one, +-10^-k for k in
+    3 .. 38: large finite reciprocals, below and above 2^64 (the walk takes its per-plane slab
+    form above that), down to the smallest normal floats."""
+    r = axis_rays(n, seed, box).astype(np.float64)
+    rng = np.random.default_rng(seed + 1)
+    d = r[:, 3:]
+    zero = d == 0.0
+    tiny = rng.choice([1e-3, 1e-6, 1e-9, 1e-12, 1e-17, 1e-19, 1e-21, 1e-25, 1e-30, 1e-37], size=d.shape) * rng.choice([-1.0, 1.0], size=d.shape)
+    d[zero] = tiny[zero]
+    r[:, 3:] = d
+    out = r.astype(np.float32)
+    assert np.all(out[:, 3:] != 0.0)
+    return out
+
+
 def hit_test_rays(sc, name, n_random):
     """Pixel-centre camera rays, random rays and axis-parallel rays for one test scene."""
     scale = 1.0 if "int_max" not in name else 3000.0
     return np.concatenate([camera_rays(sc, 96, 96), random_rays(n_random, 7, 1.5 * scale), axis_rays(n_random // 4, 5, 1.2 * scale)])
+
+
+def near_axis_test_rays(name, n):
+    """Nearly axis-parallel rays for one test scene, origins scaled like hit_test_rays'."""
+    scale = 1.0 if "int_max" not in name else 3000.0
+    return near_axis_rays(n, 11, 1.2 * scale)

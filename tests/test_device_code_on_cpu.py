@@ -10,7 +10,7 @@ import pytest
 
 from axomae_amd import nova_hip, scenes
 from conftest import rel_rmse
-from scene_cases import BVH_SCENES, hit_test_rays
+from scene_cases import BVH_SCENES, hit_test_rays, near_axis_test_rays
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 EMU_DIR = os.path.join(HERE, "emu")
@@ -131,6 +131,33 @@ def test_device_traversal_matches_reference_traversal(oracle, emu, name):
     print(f"[emu-hit] {name}: {len(rays)} rays, {len(bad)} mismatches, nodes/ray {counts[0] / len(rays):.1f}", flush=True)
     assert len(bad) == 0, [(int(k), int(prim[k]), int(oprim[k]), tuv[k].tolist(), otuv[k].tolist()) for k in bad[:8]]
     np.testing.assert_array_equal(tuv, otuv)
+
+
+def check_never_misses(name, prim, tuv, oprim, otuv, max_rate):
+    """The walk's widened boxes are supersets of the reference's: it never misses a hit the
+    reference walk finds and never returns a farther one. Where it differs, it found a closer
+    hit in a box the reference's division-form slab test rounds away (or a hit where the
+    reference found none); the rate of those is bounded."""
+    missed = np.where((oprim >= 0) & ((prim < 0) | (tuv[:, 0] > otuv[:, 0])))[0]
+    assert len(missed) == 0, [(int(k), int(prim[k]), int(oprim[k]), tuv[k].tolist(), otuv[k].tolist()) for k in missed[:8]]
+    same = prim == oprim
+    np.testing.assert_array_equal(tuv[same], otuv[same])
+    rate = 1.0 - same.mean()
+    print(f"[near-axis] {name}: {len(prim)} rays, {int((~same).sum())} closer hits than the reference walk ({rate:.2e})", flush=True)
+    assert rate <= max_rate
+
+
+@pytest.mark.parametrize("name", ["box_room", "terrain_spheres", "soup_5000", "soup_duplicates", "soup_int_max_fallback", "transformed"])
+def test_near_axis_rays_never_miss_a_reference_hit(oracle, emu, name):
+    """Rays whose direction has components of 1e-3 .. 1e-37 (reciprocals up to 1e37, on both
+    sides of the 2^64 switch to the per-plane slab form), half of them from origins on box
+    face planes."""
+    sc = BVH_SCENES[name]()
+    rays = near_axis_test_rays(name, 8000)
+    os_ = oracle.OracleScene(sc)
+    oprim, otuv = os_.closest_hit(rays)
+    prim, tuv, _ = emu_closest_hit(emu, sc, os_, rays)
+    check_never_misses(name, prim, tuv, oprim, otuv, 1e-3)
 
 
 def test_transformed_meshes_render_matches_oracle(oracle, emu):

@@ -11,7 +11,7 @@ import pytest
 
 from axomae_amd import scenes
 from conftest import rel_rmse
-from scene_cases import BVH_SCENES, hit_test_rays, soup_scene
+from scene_cases import BVH_SCENES, hit_test_rays, near_axis_test_rays, soup_scene
 
 gpu = pytest.mark.gpu
 
@@ -94,6 +94,27 @@ def test_closest_hit_matches_reference_traversal(oracle, engine, name):
     bad = np.where(~same)[0]
     assert len(bad) == 0, [(int(k), int(prim[k]), int(oprim[k]), tuv[k].tolist(), otuv[k].tolist()) for k in bad[:8]]
     np.testing.assert_array_equal(tuv, otuv)
+
+
+@gpu
+@pytest.mark.parametrize("name", ["box_room", "terrain_spheres", "soup_5000", "transformed"])
+def test_near_axis_rays_never_miss_a_reference_hit(oracle, engine, name):
+    """Nearly axis-parallel rays (direction components of 1e-3 .. 1e-37): the device walk never
+    misses a hit the reference walk finds nor returns a farther one; closer hits, in boxes the
+    reference's division-form slab test rounds away, stay under 1 in 1000 rays."""
+    sc = BVH_SCENES[name]()
+    rays = near_axis_test_rays(name, 16000)
+    engine.upload(sc)
+    engine.build_bvh()
+    prim, tuv = engine.closest_hit(rays)
+    prim, tuv = prim.cpu().numpy(), tuv.cpu().numpy()
+    oprim, otuv = oracle.OracleScene(sc).closest_hit(rays)
+    missed = np.where((oprim >= 0) & ((prim < 0) | (tuv[:, 0] > otuv[:, 0])))[0]
+    assert len(missed) == 0, [(int(k), int(prim[k]), int(oprim[k]), tuv[k].tolist(), otuv[k].tolist()) for k in missed[:8]]
+    same = prim == oprim
+    np.testing.assert_array_equal(tuv[same], otuv[same])
+    print(f"[near-axis] {name}: {len(rays)} rays, {int((~same).sum())} closer hits than the reference walk", flush=True)
+    assert 1.0 - same.mean() <= 1e-3
 
 
 @gpu
