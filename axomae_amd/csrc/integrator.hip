@@ -376,7 +376,7 @@ __global__ __launch_bounds__(kShadeBlock) NOVA_SHADE_OCCUPANCY void shade_kernel
 // same arithmetic as in the per-bounce launches, so the image is the same. Dynamic LDS: the
 // traversal stacks, then the nibble tables of the sampler dimensions from sp.dim0 on.
 #ifndef NOVA_TAIL_WAVES
-#define NOVA_TAIL_WAVES 0  // waves per SIMD tail_kernel is compiled for (0: the compiler's choice, 3 at its 130 registers)
+#define NOVA_TAIL_WAVES 0  // waves per SIMD tail_kernel is compiled for (0: the compiler's choice, 3 at its 132 registers)
 #endif
 #if NOVA_TAIL_WAVES > 0
 #define NOVA_TAIL_OCCUPANCY __attribute__((amdgpu_waves_per_eu(NOVA_TAIL_WAVES, NOVA_TAIL_WAVES)))

@@ -76,9 +76,9 @@ def axis_rays(n, seed, box):
 
 
 def near_axis_rays(n, seed, box):
-    """axis_rays with each zero direction component replaced by a tiny one, +-10^-k for k in
-    3 .. 38: large finite reciprocals, below and above 2^64 (the walk takes its per-plane slab
-    form above that), down to the smallest normal floats."""
+    """axis_rays with each zero direction component replaced by a tiny one, +-10^-k for k from
+    3 to 37: large finite reciprocals, up to near the largest float, instead of the infinite
+    ones that make an axis a containment test."""
     r = axis_rays(n, seed, box).astype(np.float64)
     rng = np.random.default_rng(seed + 1)
     d = r[:, 3:]

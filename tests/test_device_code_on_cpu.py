@@ -149,9 +149,8 @@ def check_never_misses(name, prim, tuv, oprim, otuv, max_rate):
 
 @pytest.mark.parametrize("name", ["box_room", "terrain_spheres", "soup_5000", "soup_duplicates", "soup_int_max_fallback", "transformed"])
 def test_near_axis_rays_never_miss_a_reference_hit(oracle, emu, name):
-    """Rays whose direction has components of 1e-3 .. 1e-37 (reciprocals up to 1e37, on both
-    sides of the 2^64 switch to the per-plane slab form), half of them from origins on box
-    face planes."""
+    """Rays whose direction has components of 1e-3 .. 1e-37 (finite reciprocals up to 1e37),
+    half of them from origins on box face planes."""
     sc = BVH_SCENES[name]()
     rays = near_axis_test_rays(name, 8000)
     os_ = oracle.OracleScene(sc)
