@@ -119,6 +119,7 @@ static void free_scene(nova_hip_ctx *ctx) {
     }
   ctx->scene_ready = false;
   ctx->bvh_ready = false;
+  ctx->scene_gen++;
 }
 
 void nova_hip_free_scene_internal(nova_hip_ctx *ctx) { free_scene(ctx); }

@@ -12,7 +12,8 @@
 // Each path computes exactly what the single-scene renderer computes for it (same scene
 // data, sample index and arithmetic) and the accumulate stage adds a wave's frames to the
 // per-scene accumulators in frame order, so a scene's image equals its nova_hip_render_frames
-// image bit for bit.
+// image bit for bit where both trace the camera rays with the same walk (per ray or per wave;
+// the two differ, rarely, on a ray grazing a shared triangle edge).
 #include <algorithm>
 #include <vector>
 

@@ -844,6 +844,7 @@ extern "C" int nova_hip_bvh_build(nova_hip_ctx *ctx, int build_candidates, nova_
   if (info) *info = ctx->bvh_info;
   ctx->tail_hint = 0;
   ctx->bvh_ready = true;
+  ctx->scene_gen++;
   return NOVA_HIP_SUCCESS;
 }
 

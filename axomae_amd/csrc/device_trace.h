@@ -402,8 +402,12 @@ NOVA_HD int trace_closest(const TraceCtx &tc, Stack &stack, f3 o, f3 d, float tn
 // vector gathers); each lane runs the slab test of its own ray, and the wave visits a child
 // when any lane enters it, carrying the mask of those lanes. A lane thus tests exactly the
 // boxes and triangles its own walk would reach -- only the order is the wave's (near to far
-// for the first lane of the mask) -- and the closest hit with the reference's tie rule does
-// not depend on the order. The stack is the wave's: (node, lane mask) entries, entry e held
+// for the first lane of the mask). The closest hit with the reference's tie rule does not
+// depend on the order, with one exception: when Moller-Trumbore's t for a triangle lies
+// before its box's entry (a ray grazing the edge two triangles share, where t carries the
+// rounding of a small determinant), a walk that first finds the neighbouring triangle culls
+// that box and one that reaches the box first keeps the closer hit. The reference's own
+// walk is order-dependent there too; the per-ray walk follows each ray's near-to-far order. The stack is the wave's: (node, lane mask) entries, entry e held
 // in lane e of three registers, so a push is a compare-and-select and a pop a lane read.
 #ifdef __HIP_DEVICE_COMPILE__
 typedef const __attribute__((address_space(4))) Node4F ConstNode4F;  // scalar-loadable: uniform address, never written by the walk

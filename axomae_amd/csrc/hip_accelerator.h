@@ -226,6 +226,10 @@ class HipAccelerator final : public DeviceAcceleratorInterface {
   void setIndexPolicy(int policy) { index_policy_ = policy; }
   void setRussianRoulette(int rr_start) { rr_start_ = rr_start; }
   void setVerticalInvert(bool v) { vertical_invert_ = v; }
+  // Frames traverse() renders ahead while the camera and settings stay the same
+  // (nova_hip_render_frame_lookahead): 1 (the default) renders each frame alone. Every
+  // frame's radiance is the same either way.
+  void setFrameLookahead(int frames) { lookahead_ = frames; }
   void build(primitive_aggregate_data_s scene) override;
   void cleanup() override;
   unsigned getMaxRecursiveDepth() const override { return 8; } /* MAX_REC_DEPTH (optix/host.cpp:25) */
@@ -242,6 +246,7 @@ class HipAccelerator final : public DeviceAcceleratorInterface {
   int index_policy_ = NOVA_HIP_INDEX_CPU;
   int rr_start_ = -1;
   bool vertical_invert_ = false;
+  int lookahead_ = 1;
   mutable device_traversal_param_s params_{};
 };
 

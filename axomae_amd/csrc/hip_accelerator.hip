@@ -1,5 +1,7 @@
 // HipAccelerator: the C++ device-accelerator facade over the C ABI, and flatten(), which
 // turns the engine's scene holders into the C-ABI descriptor.
+#include <hip/hip_runtime.h>
+
 #include <cmath>
 #include <cstring>
 
@@ -162,8 +164,8 @@ void HipAccelerator::copyParamsToDevice(const device_traversal_param_s &params) 
 namespace {
 class HipIntersector final : public DeviceIntersectorInterface {
  public:
-  HipIntersector(nova_hip_ctx *ctx, int policy, int rr_start, bool vertical_invert)
-      : ctx_(ctx), policy_(policy), rr_start_(rr_start), vertical_invert_(vertical_invert) {}
+  HipIntersector(nova_hip_ctx *ctx, int policy, int rr_start, bool vertical_invert, int lookahead)
+      : ctx_(ctx), policy_(policy), rr_start_(rr_start), vertical_invert_(vertical_invert), lookahead_(lookahead) {}
   // One frame (1 spp) of the path integrator into render_buffers.partial_buffer, stream
   // ordered on the null stream and finished on return, as the reference's synchronous
   // traverse is (aggregate/api/optix/device.cpp:18-21).
@@ -183,18 +185,26 @@ class HipIntersector final : public DeviceIntersectorInterface {
     rp.z_far = p.camera.far;
     std::memcpy(rp.inv_P, p.camera.inv_P.m, sizeof(rp.inv_P));
     std::memcpy(rp.inv_V, p.camera.inv_V.m, sizeof(rp.inv_V));
-    throw_if(ctx_, nova_hip_render_frames(ctx_, &rp, nullptr, nullptr, p.render_buffers.partial_buffer, nullptr));
+    if (lookahead_ <= 1) {
+      throw_if(ctx_, nova_hip_render_frames(ctx_, &rp, nullptr, nullptr, p.render_buffers.partial_buffer, nullptr));
+      return;
+    }
+    throw_if(ctx_, nova_hip_render_frame_lookahead(ctx_, &rp, lookahead_, nullptr, nullptr, p.render_buffers.partial_buffer, nullptr));
+    // a kept frame is copied by a stream-ordered kernel: wait for it, so the frame is in
+    // partial_buffer on return as after the reference's synchronous traverse
+    if (hipStreamSynchronize(nullptr) != hipSuccess) throw HipDeviceError(NOVA_HIP_DEVICE_ERROR, "traverse: stream synchronization failed");
   }
 
  private:
   nova_hip_ctx *ctx_;
   int policy_, rr_start_;
   bool vertical_invert_;
+  int lookahead_;
 };
 }  // namespace
 
 std::unique_ptr<DeviceIntersectorInterface> HipAccelerator::getIntersectorObject() const {
-  return std::make_unique<HipIntersector>(ctx_, index_policy_, rr_start_, vertical_invert_);
+  return std::make_unique<HipIntersector>(ctx_, index_policy_, rr_start_, vertical_invert_, lookahead_);
 }
 
 std::unique_ptr<DeviceAcceleratorInterface> DeviceAcceleratorInterface::make(int device) { return std::make_unique<HipAccelerator>(device); }

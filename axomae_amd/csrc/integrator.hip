@@ -447,8 +447,9 @@ __global__ void reset_count_kernel(uint32_t *count, int which) {
 
 // Engine::writeFramebuffer per frame, in frame order (engine.cpp:351-363); one thread per
 // pixel channel, the pixel's frames contiguous in the path-id-ordered partial buffer.
+// frames_out (optional) receives every frame's radiance image, frame f at f * npix * 4 floats.
 __global__ void accumulate_kernel(const float *__restrict__ partial, FrameParams fp, float *__restrict__ accum, float *__restrict__ final_out,
-                                  float *__restrict__ partial_out) {
+                                  float *__restrict__ partial_out, float *__restrict__ frames_out) {
   const uint32_t npix4 = (uint32_t)fp.width * (uint32_t)fp.height * 4u;
   const int frames = fp.frames;
   for (uint32_t e = blockIdx.x * blockDim.x + threadIdx.x; e < npix4; e += gridDim.x * blockDim.x) {
@@ -462,6 +463,24 @@ __global__ void accumulate_kernel(const float *__restrict__ partial, FrameParams
       final_out[i] = a / float(fp.frame0 + frames);  // the last frame's average, the one displayed
     }
     if (partial_out) partial_out[i] = src[(size_t)(frames - 1) * 4];
+    if (frames_out)
+      for (int f = 0; f < frames; f++) frames_out[(size_t)f * npix4 + i] = src[(size_t)f * 4];
+  }
+}
+
+// One kept frame image into the caller's buffers, as accumulate_kernel adds a one-frame
+// batch: accum += image, final = accum / (frame + 1), partial = image.
+__global__ void frame_add_kernel(const float *__restrict__ image, uint32_t n4, float *__restrict__ accum, float *__restrict__ final_out,
+                                 float *__restrict__ partial_out, float frames_so_far) {
+  for (uint32_t e = blockIdx.x * blockDim.x + threadIdx.x; e < n4; e += gridDim.x * blockDim.x) {
+    const float v = image[e];
+    if (accum) {
+      float a = accum[e];
+      a += v;
+      accum[e] = a;
+      final_out[e] = a / frames_so_far;
+    }
+    if (partial_out) partial_out[e] = v;
   }
 }
 
@@ -711,7 +730,7 @@ static int render_pass(nova_hip_ctx *ctx, const nova_hip_render_params *p, int p
 // event is recorded and nothing waits: the work is ordered on the stream and the render
 // statistics are left as they were.
 static int render_impl(nova_hip_ctx *ctx, const nova_hip_render_params *p, float *accum_dev, float *final_dev, float *partial_dev, void *stream,
-                       const AdjointCtx *adj, unsigned char *noalloc = nullptr) {
+                       const AdjointCtx *adj, unsigned char *noalloc = nullptr, float *frames_out = nullptr) {
   if (!ctx || !p) return NOVA_HIP_INVALID_ARGUMENT;
   if (!ctx->bvh_ready) return set_error(ctx, NOVA_HIP_SCENE_NOT_PROCESSED, "scene/BVH not built");
   if (p->width <= 0 || p->height <= 0 || p->width > 65536 || p->height > 65536)
@@ -720,7 +739,8 @@ static int render_impl(nova_hip_ctx *ctx, const nova_hip_render_params *p, float
   if (p->index_policy != NOVA_HIP_INDEX_CPU && p->index_policy != NOVA_HIP_INDEX_OPTIX)
     return set_error(ctx, NOVA_HIP_INVALID_ARGUMENT, "unknown index_policy");
   // accum and final together, or neither with a partial buffer (one frame's radiance only)
-  if (!adj && (!accum_dev != !final_dev || (!accum_dev && !partial_dev)))
+  // or the per-frame images
+  if (!adj && (!accum_dev != !final_dev || (!accum_dev && !partial_dev && !frames_out)))
     return set_error(ctx, NOVA_HIP_INVALID_BUFFER_STATE, "null render buffer");
   if ((((uintptr_t)accum_dev | (uintptr_t)final_dev | (uintptr_t)partial_dev) & 3u) != 0)
     return set_error(ctx, NOVA_HIP_INVALID_BUFFER_STATE, "render buffers must be 4-byte aligned float arrays");
@@ -968,7 +988,8 @@ static int render_impl(nova_hip_ctx *ctx, const nova_hip_render_params *p, float
     }
     if (!adj) {
       hipLaunchKernelGGL(accumulate_kernel, dim3(std::min<size_t>((npix * 4 + 255) / 256, (size_t)cus * 16)), dim3(256), 0, s, partial, fp,
-                         accum_dev, final_dev, (f0 + frames == p->frame_start + p->frame_count) ? partial_dev : (float *)nullptr);
+                         accum_dev, final_dev, (f0 + frames == p->frame_start + p->frame_count) ? partial_dev : (float *)nullptr,
+                         frames_out ? frames_out + (size_t)(f0 - p->frame_start) * npix * 4 : (float *)nullptr);
       NOVA_HIP_CHECK(ctx, hipGetLastError());
     }
   }
@@ -1030,6 +1051,54 @@ static int render_impl(nova_hip_ctx *ctx, const nova_hip_render_params *p, float
 extern "C" int nova_hip_render_frames(nova_hip_ctx *ctx, const nova_hip_render_params *p, float *accum_dev, float *final_dev, float *partial_dev,
                                       void *stream) {
   return render_impl(ctx, p, accum_dev, final_dev, partial_dev, stream, nullptr);
+}
+
+extern "C" int nova_hip_render_frame_lookahead(nova_hip_ctx *ctx, const nova_hip_render_params *p, int32_t lookahead, float *accum_dev,
+                                               float *final_dev, float *partial_dev, void *stream) {
+  if (!ctx || !p) return NOVA_HIP_INVALID_ARGUMENT;
+  if (p->frame_count != 1 || p->frame_start < 0) return set_error(ctx, NOVA_HIP_INVALID_ARGUMENT, "one frame per call (frame_count 1)");
+  if (lookahead < 1 || lookahead > 1024) return set_error(ctx, NOVA_HIP_INVALID_ARGUMENT, "lookahead must be in [1, 1024]");
+  if (!accum_dev != !final_dev || (!accum_dev && !partial_dev)) return set_error(ctx, NOVA_HIP_INVALID_BUFFER_STATE, "null render buffer");
+  if ((((uintptr_t)accum_dev | (uintptr_t)final_dev | (uintptr_t)partial_dev) & 3u) != 0)
+    return set_error(ctx, NOVA_HIP_INVALID_BUFFER_STATE, "render buffers must be 4-byte aligned float arrays");
+  int pass = 0;
+  int st = resolve_integrator(ctx, p->integrator_flags, &pass);
+  if (st) return st;
+  if (pass != 0 || lookahead == 1) return render_impl(ctx, p, accum_dev, final_dev, partial_dev, stream, nullptr);
+  if (p->width <= 0 || p->height <= 0 || p->width > 65536 || p->height > 65536)
+    return set_error(ctx, NOVA_HIP_INVALID_RENDERBUFFER_DIM, "Wrong buffer width / height dimension.");
+  const size_t n4 = (size_t)p->width * p->height * 4;
+  if (n4 >= (1ull << 32)) return set_error(ctx, NOVA_HIP_INVALID_ARGUMENT, "image too large to keep frames of");
+  nova_hip_render_params key = *p;
+  key.frame_start = 0;
+  const bool hit = ctx->ahead_valid && ctx->ahead_gen == ctx->scene_gen && std::memcmp(&key, &ctx->ahead_key, sizeof(key)) == 0 &&
+                   p->frame_start >= ctx->ahead_first && p->frame_start < ctx->ahead_first + ctx->ahead_count;
+  hipStream_t s = (hipStream_t)stream;
+  NOVA_HIP_CHECK(ctx, hipSetDevice(ctx->device));
+  if (!hit) {
+    ctx->ahead_valid = false;
+    hipError_t e = hipSuccess;
+    float *store = (float *)ensure_scratch(ctx, 12, (size_t)lookahead * n4 * sizeof(float), &e);
+    if (!store) return check_hip(ctx, e, "lookahead frame store alloc");
+    nova_hip_render_params ahead = *p;
+    ahead.frame_count = lookahead;
+    // camera rays by the per-ray walk, as a one-frame render traces them: the per-wave walk
+    // of many frames can settle a grazing ray's near-tie differently (its lanes visit leaves
+    // in the first lane's order)
+    ahead.flags |= NOVA_HIP_RENDER_PER_RAY_TRAVERSAL;
+    st = render_impl(ctx, &ahead, nullptr, nullptr, nullptr, stream, nullptr, nullptr, store);
+    if (st) return st;
+    ctx->ahead_key = key;
+    ctx->ahead_first = p->frame_start;
+    ctx->ahead_count = lookahead;
+    ctx->ahead_gen = ctx->scene_gen;
+    ctx->ahead_valid = true;
+  }
+  const float *image = (const float *)ctx->scratch[12].ptr + (size_t)(p->frame_start - ctx->ahead_first) * n4;
+  hipLaunchKernelGGL(frame_add_kernel, dim3((uint32_t)std::min<size_t>((n4 + 255) / 256, (size_t)device_cus(ctx) * 16)), dim3(256), 0, s, image,
+                     (uint32_t)n4, accum_dev, final_dev, partial_dev, float(p->frame_start + 1));
+  NOVA_HIP_CHECK(ctx, hipGetLastError());
+  return NOVA_HIP_SUCCESS;
 }
 
 extern "C" int nova_hip_render_frames_noalloc(nova_hip_ctx *ctx, const nova_hip_render_params *p, float *accum_dev, float *final_dev,

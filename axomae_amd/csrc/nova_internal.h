@@ -148,6 +148,15 @@ struct nova_hip_ctx {
   // max_depth and rr_start tail_hint_key; cleared by a new scene or BVH
   int tail_hint = 0;
   uint64_t tail_hint_key[3] = {0, 0, 0};
+  // bumped by every change of the scene, its BVH or its textures (keys the frames below)
+  uint64_t scene_gen = 0;
+  // frames rendered ahead by nova_hip_render_frame_lookahead (images in scratch slot 12):
+  // frames ahead_first .. ahead_first + ahead_count - 1 of the parameters ahead_key
+  // (frame_start 0) on scene generation ahead_gen
+  nova_hip_render_params ahead_key{};
+  int32_t ahead_first = 0, ahead_count = 0;
+  uint64_t ahead_gen = 0;
+  bool ahead_valid = false;
 
   // integrator scratch (grown on demand)
   nova_hip::DeviceBuffer scratch[16];

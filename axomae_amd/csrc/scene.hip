@@ -186,6 +186,7 @@ extern "C" int nova_hip_scene_upload(nova_hip_ctx *ctx, const nova_hip_scene_des
   ctx->tail_hint = 0;
   ctx->scene_ready = true;
   ctx->bvh_ready = false;
+  ctx->scene_gen++;
   return NOVA_HIP_SUCCESS;
 }
 
@@ -209,5 +210,6 @@ extern "C" int nova_hip_texture_f32_update(nova_hip_ctx *ctx, int32_t texture, c
   const size_t n = (size_t)t.width * t.height * t.channels;
   NOVA_HIP_CHECK(ctx, hipSetDevice(ctx->device));
   NOVA_HIP_CHECK(ctx, hipMemcpyAsync(ctx->tex_f32 + off, texels_dev, n * sizeof(float), hipMemcpyDeviceToDevice, (hipStream_t)stream));
+  ctx->scene_gen++;
   return NOVA_HIP_SUCCESS;
 }

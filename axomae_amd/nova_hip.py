@@ -25,7 +25,7 @@ EXPORTS = ["nova_hip_create", "nova_hip_destroy", "nova_hip_last_error", "nova_h
            "nova_hip_bake_batch", "nova_hip_render_batch", "nova_hip_batch_autotune", "nova_hip_renderbuffer_create",
            "nova_hip_renderbuffer_destroy", "nova_hip_renderbuffer_preallocate", "nova_hip_renderbuffer_resize", "nova_hip_renderbuffer_reset",
            "nova_hip_renderbuffer_clean_canvas", "nova_hip_renderbuffer_swap", "nova_hip_renderbuffer_get", "nova_hip_render_to_renderbuffer",
-           "nova_hip_renderbuffer_read", "nova_hip_render_frames_noalloc"]
+           "nova_hip_renderbuffer_read", "nova_hip_render_frames_noalloc", "nova_hip_render_frame_lookahead"]
 
 RB_FRONT, RB_BACK, RB_PARTIAL, RB_ACCUMULATOR, RB_DEPTH, RB_NORMAL = range(6)
 RB_CHANNELS = {RB_FRONT: 4, RB_BACK: 4, RB_PARTIAL: 4, RB_ACCUMULATOR: 4, RB_DEPTH: 1, RB_NORMAL: 3}
@@ -143,6 +143,7 @@ def load():
     L.nova_hip_closest_hit.argtypes = [P, P, C.c_uint64, P, P, P]
     L.nova_hip_render_frames.argtypes = [P, C.POINTER(RenderParams), P, P, P, P]
     L.nova_hip_render_frames_noalloc.argtypes = [P, C.POINTER(RenderParams), P, P, P, P]
+    L.nova_hip_render_frame_lookahead.argtypes = [P, C.POINTER(RenderParams), C.c_int32, P, P, P, P]
     L.nova_hip_render_stats_get.argtypes = [P, C.POINTER(RenderStats)]
     L.nova_hip_sobol_fill.argtypes = [P, P, C.c_uint64, P, P, P]
     L.nova_hip_render_batch.argtypes = [P, P, C.c_uint32, C.POINTER(BatchConfig), C.POINTER(BatchStats), P, P]
@@ -328,6 +329,21 @@ class HipEngine:
         final = torch.empty((height, width, 4), dtype=torch.float32, device="cuda")
         self._check(self.L.nova_hip_render_frames(self.ctx, C.byref(p), accum.data_ptr(), final.data_ptr(), None, _stream(torch)), "render_frames")
         return final, accum
+
+    def render_frame_lookahead(self, width, height, frame, max_depth, lookahead, accum, final=None, partial=None, rr_start=-1, inv_P=None,
+                               inv_V=None, vertical_invert=False, flags=0):
+        """One frame as :meth:`render` renders it (accum += frame, final = accum / (frame + 1), partial = the
+        frame), rendering `lookahead` frames ahead on a miss and serving later frames of the same
+        parameters from them (nova_hip_render_frame_lookahead). accum, final and partial are device
+        tensors (H, W, 4); final and partial may be None."""
+        torch = self.torch
+        p = render_params(self.scene, width, height, 1, max_depth, frame, rr_start, 0, vertical_invert, inv_P, inv_V, flags)
+        ptr = lambda t: None if t is None else t.data_ptr()
+        if final is None and accum is not None:
+            final = torch.empty_like(accum)
+        self._check(self.L.nova_hip_render_frame_lookahead(self.ctx, C.byref(p), lookahead, ptr(accum), ptr(final), ptr(partial), _stream(torch)),
+                    "render_frame_lookahead")
+        return final
 
     def render_to(self, rb, frames, max_depth, frame_start=0, rr_start=-1, max_samples_offset=0, vertical_invert=False, inv_P=None,
                   inv_V=None, integrator_flags=0, index_policy=INDEX_CPU):

@@ -253,6 +253,7 @@ def frame_by_frame_leg(eng, sc, W, H, frames, depth, rr_start):
     traverse call (engine.cpp:461-494), so the library is called frames times with
     frame_count = 1 through nova_hip_render_to_renderbuffer (render, accumulate, swap on the
     device). Median of three runs of the whole frame loop."""
+    import numpy as np
     import torch
     from axomae_amd.nova_hip import RenderBuffer
     rb = RenderBuffer(eng, W, H)
@@ -266,11 +267,35 @@ def frame_by_frame_leg(eng, sc, W, H, frames, depth, rr_start):
             eng.render_to(rb, 1, depth, frame_start=f, rr_start=rr_start)
         torch.cuda.synchronize()
         runs.append(time.perf_counter() - t0)
+    from axomae_amd.nova_hip import RB_ACCUMULATOR
+    acc_plain = rb.read(RB_ACCUMULATOR)
     rb.close()
     t = sorted(runs)[1]
     log(f"[bench] frame by frame: {frames} calls of 1 frame, {t * 1e3:.1f} ms (runs {', '.join(f'{r * 1e3:.1f}' for r in runs)})")
-    return {"calls": frames, "frame_count_per_call": 1, "entry_point": "nova_hip_render_to_renderbuffer",
-            "ms": round(t * 1e3, 2), "msamples_s": round(W * H * frames / t / 1e6, 3)}
+    out = {"calls": frames, "frame_count_per_call": 1, "entry_point": "nova_hip_render_to_renderbuffer",
+           "ms": round(t * 1e3, 2), "msamples_s": round(W * H * frames / t / 1e6, 3)}
+    # the same calls with frames rendered ahead (nova_hip_render_frame_lookahead): a call that
+    # misses renders `ahead` frames at once, the following ones copy theirs
+    ahead = 8
+    acc = torch.zeros((H, W, 4), dtype=torch.float32, device="cuda")
+    fin, part = torch.empty_like(acc), torch.empty_like(acc)
+    eng.render_frame_lookahead(W, H, 0, depth, 1, torch.zeros_like(acc), fin, part, rr_start=rr_start)  # warm-up
+    runs = []
+    for _ in range(3):
+        acc.zero_()
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for f in range(frames):
+            eng.render_frame_lookahead(W, H, f, depth, ahead, acc, fin, part, rr_start=rr_start)
+        torch.cuda.synchronize()
+        runs.append(time.perf_counter() - t0)
+    same = bool(np.array_equal(acc.cpu().numpy().reshape(-1), np.asarray(acc_plain).reshape(-1)))
+    t = sorted(runs)[1]
+    log(f"[bench] frame by frame, {ahead} frames rendered ahead: {t * 1e3:.1f} ms (runs {', '.join(f'{r * 1e3:.1f}' for r in runs)}); "
+        f"accumulator equal to the one-frame calls' {same}")
+    out["frames_ahead"] = {"lookahead": ahead, "entry_point": "nova_hip_render_frame_lookahead", "ms": round(t * 1e3, 2),
+                           "msamples_s": round(W * H * frames / t / 1e6, 3), "accumulator_equal": same}
+    return out
 
 
 def main():

@@ -210,6 +210,22 @@ enum {
 int nova_hip_render_frames(nova_hip_ctx *ctx, const nova_hip_render_params *params, float *accum_dev, float *final_dev,
                            float *partial_dev, void *stream);
 
+/* One frame (params->frame_count must be 1) as nova_hip_render_frames renders it, served from
+ * frames rendered ahead, for an engine loop that renders one 1-spp frame per call with the
+ * same camera and settings (Engine::renderGpu -> DeviceIntersectorInterface::traverse,
+ * engine.cpp:461-494). When an earlier call of this function on the context kept frame
+ * frame_start for otherwise identical parameters on the same scene (no scene upload, BVH
+ * build or texture update since), that frame's radiance is added to accum_dev (final_dev =
+ * accum_dev / (frame_start + 1)) and copied to partial_dev, as a one-frame render would; else
+ * frames frame_start .. frame_start + lookahead - 1 are rendered in one call, their images kept
+ * in device memory (lookahead * width * height * 16 bytes, grown on demand, freed with the
+ * context), and the first is returned. Each frame equals its nova_hip_render_frames frame bit
+ * for bit: a path's samples and arithmetic do not depend on the batch it runs in. lookahead 1
+ * (or a render pass) renders the frame alone. nova_hip_render_stats describes the last call
+ * that rendered. Buffers as for nova_hip_render_frames. No counterpart in the reference. */
+int nova_hip_render_frame_lookahead(nova_hip_ctx *ctx, const nova_hip_render_params *params, int32_t lookahead, float *accum_dev,
+                                    float *final_dev, float *partial_dev, void *stream);
+
 /* nova_hip_render_frames without any allocation, for callers that cannot allocate device
  * memory (code under stream capture, tight serving loops, hosts that forbid hipMalloc). No
  * counterpart in the reference. The path integrator only (integrator_flags 0 or
@@ -330,11 +346,15 @@ typedef struct {
 /* Renders every item in shared launches: all scenes' paths are numbered back to back and
  * processed in waves of queue_capacity paths, each wave one generate, (extend, shade) per
  * depth level and one accumulate launch over its paths whatever their scene. Each image
- * equals the item's nova_hip_render_frames image bit for bit. completion_ms (optional,
+ * equals the item's nova_hip_render_frames image bit for bit when both trace its camera rays
+ * the same way (see below). completion_ms (optional,
  * count floats) receives the device time at which each item's last wave ended. Items are
  * processed in the order given (sorting them by path count gives shortest-first completion).
  * When every item's path count (width * height * frame_count) is a multiple of 64, camera
- * rays are traced 64 at a time by one walk per wave, which is faster; the image is the same.
+ * rays are traced 64 at a time by one walk per wave, which is faster; nova_hip_render_frames
+ * does so from 8 frames per batch. The two walks give the same hits except, rarely, on a ray
+ * grazing the edge two triangles share (measured: 1 camera ray in 16.6 M), so an item that
+ * one of them traces per ray and the other per wave can differ in such a pixel.
  * `executor` is a context that owns the queues and the wave graph (it needs no scene).
  * Replaces a loop of Engine renders, one per scene (engine.cpp:498-551); no counterpart in
  * the reference's API. */

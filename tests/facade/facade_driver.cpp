@@ -140,7 +140,7 @@ extern "C" int facade_engine_loop(int mesh_count, float *const *vertices, float 
                                   const float *transforms, const int32_t *mesh_material, int material_count, const int32_t *mat_tex,
                                   const float *mat_f, int texture_count, const int32_t *tex_info, const void *const *tex_data, int envmap,
                                   int omit_bitangents, const float *inv_P, const float *inv_V, int width, int height, int frames, int depth,
-                                  int rr_start, float *image_out, float *accum_out, char *err, int err_len) {
+                                  int rr_start, int lookahead, float *image_out, float *accum_out, char *err, int err_len) {
   float *partial = nullptr;
   try {
     Holders h;
@@ -151,6 +151,7 @@ extern "C" int facade_engine_loop(int mesh_count, float *const *vertices, float 
     hip->setShadingResources(shading_resources_s{span<const material::principled_material_s>{h.materials.data(), h.materials.size()},
                                                  span<const texturing::TextureRawData>{h.textures.data(), h.textures.size()}, envmap});
     hip->setRussianRoulette(rr_start);
+    hip->setFrameLookahead(lookahead);
     accel->build(aggregate_of(h));
     const size_t n = (size_t)width * height * 4;
     if (hipMalloc(&partial, n * sizeof(float)) != hipSuccess) throw std::runtime_error("partial buffer alloc");

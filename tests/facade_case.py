@@ -18,7 +18,7 @@ def load():
     P = C.c_void_p
     common = [C.c_int, P, P, P, P, P, P, P, P, P, P, C.c_int, P, P, C.c_int, P, P, C.c_int, C.c_int]
     L.facade_flatten.argtypes = common + [P, P, P, P, C.c_char_p, C.c_int]
-    L.facade_engine_loop.argtypes = common + [P, P, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, P, P, C.c_char_p, C.c_int]
+    L.facade_engine_loop.argtypes = common + [P, P, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, P, P, C.c_char_p, C.c_int]
     return L
 
 

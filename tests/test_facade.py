@@ -87,9 +87,34 @@ def test_engine_frame_loop_through_the_facade_matches_the_oracle(oracle, driver,
     err = C.create_string_buffer(256)
     inv_P = np.ascontiguousarray(sc.inv_P, dtype=np.float32)
     inv_V = np.ascontiguousarray(sc.inv_V, dtype=np.float32)
-    rc = driver.facade_engine_loop(*pk.args(), inv_P.ctypes.data, inv_V.ctypes.data, W, H, F, D, -1, img.ctypes.data, acc.ctypes.data, err, 256)
+    rc = driver.facade_engine_loop(*pk.args(), inv_P.ctypes.data, inv_V.ctypes.data, W, H, F, D, -1, 1, img.ctypes.data, acc.ctypes.data, err,
+                                   256)
     assert rc == 0, err.value
     ref, racc = oracle.OracleScene(sc).render(W, H, F, D, threads=8)
     e1, e2 = rel_rmse(img.reshape(H, W, 4), ref), rel_rmse(acc.reshape(H, W, 4), racc.reshape(H, W, 4))
     print(f"[facade] {name} bitangents {'completed' if omit else 'given'}: image rel_rmse {e1:.3e}, accumulator {e2:.3e}", flush=True)
     assert e1 <= 1e-3 and e2 <= 1e-3
+
+
+@gpu
+def test_engine_frame_loop_with_frames_rendered_ahead_is_unchanged(driver):
+    """setFrameLookahead(4): traverse renders four frames in one call and serves the next three
+    from them; every frame's radiance, so the engine's accumulator and image, equal the
+    frame-by-frame loop's bit for bit (10 frames: two full lookahead batches and a cut one)."""
+    sc = scenes.box_room()
+    W, H, F, D = 96, 80, 10, 5
+    pk = Packed(sc)
+    inv_P = np.ascontiguousarray(sc.inv_P, dtype=np.float32)
+    inv_V = np.ascontiguousarray(sc.inv_V, dtype=np.float32)
+    out = {}
+    for la in (1, 4):
+        img = np.zeros(W * H * 4, dtype=np.float32)
+        acc = np.zeros(W * H * 4, dtype=np.float32)
+        err = C.create_string_buffer(256)
+        rc = driver.facade_engine_loop(*pk.args(), inv_P.ctypes.data, inv_V.ctypes.data, W, H, F, D, 3, la, img.ctypes.data, acc.ctypes.data,
+                                       err, 256)
+        assert rc == 0, err.value
+        out[la] = (img, acc)
+    print("[facade] lookahead 4 vs 1: 10 frames", flush=True)
+    np.testing.assert_array_equal(out[4][1], out[1][1])
+    np.testing.assert_array_equal(out[4][0], out[1][0])

@@ -1,0 +1,79 @@
+"""Frames rendered ahead (nova_hip_render_frame_lookahead) on the MI355X: one frame per call as
+an engine loop renders it, every frame's accumulator, image and radiance equal to one-frame
+renders bit for bit, and frames kept from an earlier call are not served after the camera or
+a texture changed."""
+import numpy as np
+import pytest
+import torch
+
+from axomae_amd import scenes
+
+gpu = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def diff_scene():
+    from axomae_amd.nova_hip import HipEngine
+    sc, alb, rough = scenes.differentiable_scene(width=64, height=48, tex=32, terrain_cells=12, spheres=2, sphere_level=1, sky=(64, 32))
+    e = HipEngine(0)
+    e.upload(sc)
+    e.build_bvh()
+    yield e, sc, alb
+    e.close()
+
+
+def _moved_camera(sc):
+    v = np.array(sc.inv_V, dtype=np.float32).reshape(4, 4).copy()
+    v[3, 0] += 0.05  # column-major: translation in the last column
+    return v.reshape(-1)
+
+
+@gpu
+@pytest.mark.parametrize("lookahead", [3, 8])
+def test_lookahead_frames_equal_one_frame_renders(diff_scene, lookahead):
+    """13 frames, one per call; from frame 6 the camera moves and at frame 10 a texture changes.
+    After every call the accumulator, image and frame radiance equal those of one-frame
+    renders with the same sequence of camera and texture."""
+    e, sc, alb = diff_scene
+    W, H, D = 64, 48, 4
+    base_tex = torch.from_numpy(sc.textures[alb].data.copy())
+    other_tex = torch.from_numpy(np.ascontiguousarray(sc.textures[alb].data[::-1]).copy())
+    runs = {}
+    for mode in ("plain", "ahead"):
+        e.update_texture(alb, base_tex)
+        acc = torch.zeros((H, W, 4), dtype=torch.float32, device="cuda")
+        fin = torch.empty_like(acc)
+        part = torch.empty_like(acc)
+        seq = []
+        for f in range(13):
+            inv_V = _moved_camera(sc) if f >= 6 else None
+            if f == 10:
+                e.update_texture(alb, other_tex)
+            if mode == "plain":
+                fin, acc = e.render(W, H, 1, D, frame_start=f, accum=acc, rr_start=2, inv_V=inv_V)
+                seq.append((acc.cpu().numpy().copy(), fin.cpu().numpy().copy()))
+            else:
+                e.render_frame_lookahead(W, H, f, D, lookahead, acc, fin, part, rr_start=2, inv_V=inv_V)
+                seq.append((acc.cpu().numpy().copy(), fin.cpu().numpy().copy(), part.cpu().numpy().copy()))
+        runs[mode] = seq
+    e.update_texture(alb, base_tex)
+    print(f"[lookahead] {lookahead}: 13 frames, camera moved at 6, texture changed at 10", flush=True)
+    for f in range(13):
+        np.testing.assert_array_equal(runs["ahead"][f][0], runs["plain"][f][0], err_msg=f"accumulator after frame {f}")
+        np.testing.assert_array_equal(runs["ahead"][f][1], runs["plain"][f][1], err_msg=f"image after frame {f}")
+        prev = runs["plain"][f - 1][0] if f else np.zeros_like(runs["plain"][0][0])
+        np.testing.assert_array_equal(prev + runs["ahead"][f][2], runs["plain"][f][0], err_msg=f"radiance of frame {f}")
+
+
+@gpu
+def test_lookahead_arguments(diff_scene):
+    """More than one frame per call and a lookahead outside [1, 1024] are refused."""
+    from axomae_amd.nova_hip import render_params
+    import ctypes as C
+    e, sc, _ = diff_scene
+    acc = torch.zeros((48, 64, 4), dtype=torch.float32, device="cuda")
+    fin = torch.empty_like(acc)
+    for frames, la in ((2, 4), (1, 0), (1, 2000)):
+        p = render_params(sc, 64, 48, frames, 4)
+        rc = e.L.nova_hip_render_frame_lookahead(e.ctx, C.byref(p), la, acc.data_ptr(), fin.data_ptr(), None, None)
+        assert rc == 15, (frames, la, rc)  # INVALID_ARGUMENT
