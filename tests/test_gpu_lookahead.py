@@ -77,3 +77,27 @@ def test_lookahead_arguments(diff_scene):
         p = render_params(sc, 64, 48, frames, 4)
         rc = e.L.nova_hip_render_frame_lookahead(e.ctx, C.byref(p), la, acc.data_ptr(), fin.data_ptr(), None, None)
         assert rc == 15, (frames, la, rc)  # INVALID_ARGUMENT
+
+
+@gpu
+def test_lookahead_longer_than_one_batch():
+    """4096 x 4096 with 9 frames ahead: the renderer splits them into batches of at most 2^27
+    paths (8 frames, then 1), and each batch's frames land at their own place among the kept
+    images. Frames 7 (end of the first batch) and 8 (the second) equal one-frame renders."""
+    from axomae_amd.nova_hip import HipEngine
+    W = H = 4096
+    sc = scenes.box_room(width=W, height=H)
+    e = HipEngine(0)
+    e.upload(sc)
+    e.build_bvh()
+    acc = torch.zeros((H, W, 4), dtype=torch.float32, device="cuda")
+    fin, part = torch.empty_like(acc), torch.empty_like(acc)
+    acc_p = None
+    for f in range(9):
+        e.render_frame_lookahead(W, H, f, 2, 9, acc, fin, part)
+        fin_p, acc_p = e.render(W, H, 1, 2, frame_start=f, accum=acc_p)
+        if f in (7, 8):
+            print(f"[lookahead] 4096^2, 9 ahead: frame {f}", flush=True)
+            assert torch.equal(acc, acc_p), f
+            assert torch.equal(fin, fin_p), f
+    e.close()
