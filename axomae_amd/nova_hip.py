@@ -331,13 +331,14 @@ class HipEngine:
         return final, accum
 
     def render_frame_lookahead(self, width, height, frame, max_depth, lookahead, accum, final=None, partial=None, rr_start=-1, inv_P=None,
-                               inv_V=None, vertical_invert=False, flags=0):
+                               inv_V=None, vertical_invert=False, flags=0, index_policy=INDEX_CPU, max_samples_offset=0):
         """One frame as :meth:`render` renders it (accum += frame, final = accum / (frame + 1), partial = the
         frame), rendering `lookahead` frames ahead on a miss and serving later frames of the same
         parameters from them (nova_hip_render_frame_lookahead). accum, final and partial are device
         tensors (H, W, 4); final and partial may be None."""
         torch = self.torch
-        p = render_params(self.scene, width, height, 1, max_depth, frame, rr_start, 0, vertical_invert, inv_P, inv_V, flags)
+        p = render_params(self.scene, width, height, 1, max_depth, frame, rr_start, max_samples_offset, vertical_invert, inv_P, inv_V, flags,
+                          index_policy=index_policy)
         ptr = lambda t: None if t is None else t.data_ptr()
         if final is None and accum is not None:
             final = torch.empty_like(accum)

@@ -66,6 +66,24 @@ def test_lookahead_frames_equal_one_frame_renders(diff_scene, lookahead):
 
 
 @gpu
+def test_lookahead_with_the_gpu_index_policy_and_vertical_invert(diff_scene):
+    """The reference GPU raygen's sample index ((x*H + y)*sample_max + frame) and flipped rows:
+    9 frames, 4 ahead, equal to one-frame renders after every call."""
+    from axomae_amd.nova_hip import INDEX_OPTIX
+    e, sc, _ = diff_scene
+    W, H, D = 48, 40, 3
+    acc = torch.zeros((H, W, 4), dtype=torch.float32, device="cuda")
+    fin, part = torch.empty_like(acc), torch.empty_like(acc)
+    acc_p = None
+    for f in range(9):
+        e.render_frame_lookahead(W, H, f, D, 4, acc, fin, part, vertical_invert=True, index_policy=INDEX_OPTIX, max_samples_offset=9)
+        fin_p, acc_p = e.render(W, H, 1, D, frame_start=f, accum=acc_p, vertical_invert=True, index_policy=INDEX_OPTIX, max_samples_offset=9)
+        assert torch.equal(acc, acc_p), f
+        assert torch.equal(fin, fin_p), f
+    print("[lookahead] OPTIX policy, vertical invert: 9 frames, 4 ahead", flush=True)
+
+
+@gpu
 def test_lookahead_arguments(diff_scene):
     """More than one frame per call and a lookahead outside [1, 1024] are refused."""
     from axomae_amd.nova_hip import render_params
