@@ -140,6 +140,21 @@ extern "C" int nova_hip_render_to_renderbuffer(nova_hip_ctx *ctx, const nova_hip
   return NOVA_HIP_SUCCESS;
 }
 
+extern "C" int nova_hip_render_to_renderbuffer_lookahead(nova_hip_ctx *ctx, const nova_hip_render_params *params, int32_t lookahead,
+                                                         nova_hip_renderbuffer *rb, void *stream) {
+  if (!ctx || !params || !rb) return NOVA_HIP_INVALID_ARGUMENT;
+  if (rb->ctx->device != ctx->device) return nova_hip::set_error(ctx, NOVA_HIP_INVALID_ARGUMENT, "render buffer belongs to another device");
+  if (!rb->capacity) return nova_hip::set_error(ctx, NOVA_HIP_INVALID_BUFFER_STATE, "render buffer not allocated");
+  if ((unsigned)params->width != rb->width || (unsigned)params->height != rb->height)
+    return nova_hip::set_error(ctx, NOVA_HIP_INVALID_RENDERBUFFER_DIM,
+                               "render size " + std::to_string(params->width) + "x" + std::to_string(params->height) + " differs from the render buffer's " +
+                                   std::to_string(rb->width) + "x" + std::to_string(rb->height));
+  const int rc = nova_hip_render_frame_lookahead(ctx, params, lookahead, rb->accumulator, rb->back, rb->partial, stream);
+  if (rc != NOVA_HIP_SUCCESS) return rc;
+  std::swap(rb->front, rb->back);
+  return NOVA_HIP_SUCCESS;
+}
+
 extern "C" int nova_hip_renderbuffer_read(const nova_hip_renderbuffer *rb, int which, float *host, void *stream) {
   if (!rb || !host) return NOVA_HIP_INVALID_ARGUMENT;
   nova_hip_ctx *ctx = rb->ctx;

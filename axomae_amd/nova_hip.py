@@ -25,7 +25,8 @@ EXPORTS = ["nova_hip_create", "nova_hip_destroy", "nova_hip_last_error", "nova_h
            "nova_hip_bake_batch", "nova_hip_render_batch", "nova_hip_batch_autotune", "nova_hip_renderbuffer_create",
            "nova_hip_renderbuffer_destroy", "nova_hip_renderbuffer_preallocate", "nova_hip_renderbuffer_resize", "nova_hip_renderbuffer_reset",
            "nova_hip_renderbuffer_clean_canvas", "nova_hip_renderbuffer_swap", "nova_hip_renderbuffer_get", "nova_hip_render_to_renderbuffer",
-           "nova_hip_renderbuffer_read", "nova_hip_render_frames_noalloc", "nova_hip_render_frame_lookahead"]
+           "nova_hip_renderbuffer_read", "nova_hip_render_frames_noalloc", "nova_hip_render_frame_lookahead",
+           "nova_hip_render_to_renderbuffer_lookahead"]
 
 RB_FRONT, RB_BACK, RB_PARTIAL, RB_ACCUMULATOR, RB_DEPTH, RB_NORMAL = range(6)
 RB_CHANNELS = {RB_FRONT: 4, RB_BACK: 4, RB_PARTIAL: 4, RB_ACCUMULATOR: 4, RB_DEPTH: 1, RB_NORMAL: 3}
@@ -163,6 +164,7 @@ def load():
     L.nova_hip_renderbuffer_swap.argtypes = [P]
     L.nova_hip_renderbuffer_get.argtypes = [P, C.POINTER(RenderBufferView)]
     L.nova_hip_render_to_renderbuffer.argtypes = [P, C.POINTER(RenderParams), P, P]
+    L.nova_hip_render_to_renderbuffer_lookahead.argtypes = [P, C.POINTER(RenderParams), C.c_int32, P, P]
     L.nova_hip_renderbuffer_read.argtypes = [P, C.c_int, P, P]
     _LIB = L
     return L
@@ -352,6 +354,12 @@ class HipEngine:
         p = render_params(self.scene, rb.width, rb.height, frames, max_depth, frame_start, rr_start, max_samples_offset, vertical_invert, inv_P,
                           inv_V, 0, -1, index_policy, integrator_flags, 0.1, 1000.0)
         self._check(self.L.nova_hip_render_to_renderbuffer(self.ctx, C.byref(p), rb.rb, _stream(self.torch)), "render_to_renderbuffer")
+
+    def render_to_lookahead(self, rb, frame, max_depth, lookahead, rr_start=-1):
+        """One engine-loop frame into a :class:`RenderBuffer`, served from frames rendered `lookahead` at a time."""
+        p = render_params(self.scene, rb.width, rb.height, 1, max_depth, frame, rr_start)
+        self._check(self.L.nova_hip_render_to_renderbuffer_lookahead(self.ctx, C.byref(p), lookahead, rb.rb, _stream(self.torch)),
+                    "render_to_renderbuffer_lookahead")
 
     # ------------------------------------------------------ differentiation
     def texture_info(self, tex):

@@ -274,26 +274,26 @@ def frame_by_frame_leg(eng, sc, W, H, frames, depth, rr_start):
     log(f"[bench] frame by frame: {frames} calls of 1 frame, {t * 1e3:.1f} ms (runs {', '.join(f'{r * 1e3:.1f}' for r in runs)})")
     out = {"calls": frames, "frame_count_per_call": 1, "entry_point": "nova_hip_render_to_renderbuffer",
            "ms": round(t * 1e3, 2), "msamples_s": round(W * H * frames / t / 1e6, 3)}
-    # the same calls with frames rendered ahead (nova_hip_render_frame_lookahead): a call that
-    # misses renders `ahead` frames at once, the following ones copy theirs
+    # the same engine loop with frames rendered ahead: a call that misses renders `ahead`
+    # frames at once, the following ones copy theirs (nova_hip_render_to_renderbuffer_lookahead)
     ahead = 8
-    acc = torch.zeros((H, W, 4), dtype=torch.float32, device="cuda")
-    fin, part = torch.empty_like(acc), torch.empty_like(acc)
-    eng.render_frame_lookahead(W, H, 0, depth, 1, torch.zeros_like(acc), fin, part, rr_start=rr_start)  # warm-up
+    rb = RenderBuffer(eng, W, H)
+    eng.render_to_lookahead(rb, 0, depth, 1, rr_start=rr_start)  # warm-up
     runs = []
     for _ in range(3):
-        acc.zero_()
+        rb.reset()
         torch.cuda.synchronize()
         t0 = time.perf_counter()
         for f in range(frames):
-            eng.render_frame_lookahead(W, H, f, depth, ahead, acc, fin, part, rr_start=rr_start)
+            eng.render_to_lookahead(rb, f, depth, ahead, rr_start=rr_start)
         torch.cuda.synchronize()
         runs.append(time.perf_counter() - t0)
-    same = bool(np.array_equal(acc.cpu().numpy().reshape(-1), np.asarray(acc_plain).reshape(-1)))
+    same = bool(np.array_equal(rb.read(RB_ACCUMULATOR), acc_plain))
+    rb.close()
     t = sorted(runs)[1]
     log(f"[bench] frame by frame, {ahead} frames rendered ahead: {t * 1e3:.1f} ms (runs {', '.join(f'{r * 1e3:.1f}' for r in runs)}); "
         f"accumulator equal to the one-frame calls' {same}")
-    out["frames_ahead"] = {"lookahead": ahead, "entry_point": "nova_hip_render_frame_lookahead", "ms": round(t * 1e3, 2),
+    out["frames_ahead"] = {"lookahead": ahead, "entry_point": "nova_hip_render_to_renderbuffer_lookahead", "ms": round(t * 1e3, 2),
                            "msamples_s": round(W * H * frames / t / 1e6, 3), "accumulator_equal": same}
     return out
 

@@ -312,6 +312,12 @@ int nova_hip_renderbuffer_get(const nova_hip_renderbuffer *rb, nova_hip_renderbu
  * buffers are swapped once, so the front buffer holds the last frame's image and the
  * accumulator all frames' sum. */
 int nova_hip_render_to_renderbuffer(nova_hip_ctx *ctx, const nova_hip_render_params *params, nova_hip_renderbuffer *rb, void *stream);
+/* One iteration of the engine's loop (params->frame_count 1) as nova_hip_render_to_renderbuffer
+ * does it, the frame served by nova_hip_render_frame_lookahead (rendered `lookahead` frames at a
+ * time while the parameters stay the same; the same frames bit for bit). No counterpart in the
+ * reference. */
+int nova_hip_render_to_renderbuffer_lookahead(nova_hip_ctx *ctx, const nova_hip_render_params *params, int32_t lookahead, nova_hip_renderbuffer *rb,
+                                              void *stream);
 /* Copies one buffer (NOVA_HIP_RB_*; width * height pixels) to host memory, for display:
  * getFramebuffer / sampleColor (renderbuffer.cpp:175-220) without the per-frame copy. */
 int nova_hip_renderbuffer_read(const nova_hip_renderbuffer *rb, int which, float *host, void *stream);

@@ -119,3 +119,24 @@ def test_lookahead_longer_than_one_batch():
             assert torch.equal(acc, acc_p), f
             assert torch.equal(fin, fin_p), f
     e.close()
+
+
+@gpu
+def test_renderbuffer_loop_with_frames_rendered_ahead(diff_scene):
+    """The device engine loop (render, accumulate, swap) served from frames rendered 4 ahead:
+    after each of 10 frames the accumulator and the displayed (front) image equal the plain
+    loop's."""
+    from axomae_amd.nova_hip import RB_ACCUMULATOR, RB_FRONT, RenderBuffer
+    e, sc, _ = diff_scene
+    W, H, D = 64, 48, 4
+    plain, ahead = RenderBuffer(e, W, H), RenderBuffer(e, W, H)
+    plain.reset()
+    ahead.reset()
+    for f in range(10):
+        e.render_to(plain, 1, D, frame_start=f, rr_start=2)
+        e.render_to_lookahead(ahead, f, D, 4, rr_start=2)
+        np.testing.assert_array_equal(ahead.read(RB_ACCUMULATOR), plain.read(RB_ACCUMULATOR), err_msg=f"accumulator after frame {f}")
+        np.testing.assert_array_equal(ahead.read(RB_FRONT), plain.read(RB_FRONT), err_msg=f"front after frame {f}")
+    print("[lookahead] render buffer loop: 10 frames, 4 ahead", flush=True)
+    plain.close()
+    ahead.close()
