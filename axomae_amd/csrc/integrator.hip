@@ -735,7 +735,8 @@ static int render_impl(nova_hip_ctx *ctx, const nova_hip_render_params *p, float
   if (!ctx->bvh_ready) return set_error(ctx, NOVA_HIP_SCENE_NOT_PROCESSED, "scene/BVH not built");
   if (p->width <= 0 || p->height <= 0 || p->width > 65536 || p->height > 65536)
     return set_error(ctx, NOVA_HIP_INVALID_RENDERBUFFER_DIM, "Wrong buffer width / height dimension.");
-  if (p->frame_count <= 0 || p->frame_start < 0 || p->max_depth < 0) return set_error(ctx, NOVA_HIP_INVALID_ARGUMENT, "bad frame range or depth");
+  if (p->frame_count <= 0 || p->frame_start < 0 || p->max_depth < 0 || (int64_t)p->frame_start + p->frame_count > INT32_MAX)
+    return set_error(ctx, NOVA_HIP_INVALID_ARGUMENT, "bad frame range or depth");
   if (p->index_policy != NOVA_HIP_INDEX_CPU && p->index_policy != NOVA_HIP_INDEX_OPTIX)
     return set_error(ctx, NOVA_HIP_INVALID_ARGUMENT, "unknown index_policy");
   // accum and final together, or neither with a partial buffer (one frame's radiance only)
@@ -1064,7 +1065,9 @@ extern "C" int nova_hip_render_frame_lookahead(nova_hip_ctx *ctx, const nova_hip
   int pass = 0;
   int st = resolve_integrator(ctx, p->integrator_flags, &pass);
   if (st) return st;
-  if (pass != 0 || lookahead == 1) return render_impl(ctx, p, accum_dev, final_dev, partial_dev, stream, nullptr);
+  // frames past INT32_MAX do not exist: at most the frames left below it
+  lookahead = (int32_t)std::min<int64_t>(lookahead, (int64_t)INT32_MAX - p->frame_start);
+  if (pass != 0 || lookahead <= 1) return render_impl(ctx, p, accum_dev, final_dev, partial_dev, stream, nullptr);
   if (p->width <= 0 || p->height <= 0 || p->width > 65536 || p->height > 65536)
     return set_error(ctx, NOVA_HIP_INVALID_RENDERBUFFER_DIM, "Wrong buffer width / height dimension.");
   const size_t n4 = (size_t)p->width * p->height * 4;
