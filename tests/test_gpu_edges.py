@@ -81,7 +81,8 @@ def test_render_argument_errors(engine):
     for kw, code in ((dict(width=0, height=8, frames=1, max_depth=2), INVALID_RENDERBUFFER_DIM),
                      (dict(width=70000, height=8, frames=1, max_depth=2), INVALID_RENDERBUFFER_DIM),
                      (dict(width=8, height=8, frames=0, max_depth=2), INVALID_ARGUMENT),
-                     (dict(width=8, height=8, frames=1, max_depth=-1), INVALID_ARGUMENT)):
+                     (dict(width=8, height=8, frames=1, max_depth=-1), INVALID_ARGUMENT),
+                     (dict(width=8, height=8, frames=2, max_depth=2, frame_start=2**31 - 2), INVALID_ARGUMENT)):
         with pytest.raises(NovaHipError) as e:
             engine.render(**kw)
         assert e.value.code == code, kw
