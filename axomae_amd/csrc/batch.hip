@@ -496,7 +496,8 @@ static int make_plan(nova_hip_ctx *ex, const nova_hip_batch_item *items, uint32_
     if (c->device != ex->device) return set_error(ex, NOVA_HIP_INVALID_ARGUMENT, "batch items must live on the executor's device");
     if (p->width <= 0 || p->height <= 0 || p->width > 65536 || p->height > 65536)
       return set_error(ex, NOVA_HIP_INVALID_RENDERBUFFER_DIM, "Wrong buffer width / height dimension.");
-    if (p->frame_count <= 0 || p->frame_start < 0 || p->max_depth < 0) return set_error(ex, NOVA_HIP_INVALID_ARGUMENT, "bad frame range or depth");
+    if (p->frame_count <= 0 || p->frame_start < 0 || p->max_depth < 0 || (int64_t)p->frame_start + p->frame_count > INT32_MAX)
+      return set_error(ex, NOVA_HIP_INVALID_ARGUMENT, "bad frame range or depth");
     if (p->index_policy != NOVA_HIP_INDEX_CPU && p->index_policy != NOVA_HIP_INDEX_OPTIX)
       return set_error(ex, NOVA_HIP_INVALID_ARGUMENT, "unknown index_policy");
     const int32_t passes = NOVA_HIP_INTEGRATOR_NORMAL | NOVA_HIP_INTEGRATOR_DEPTH | NOVA_HIP_INTEGRATOR_EMISSIVE;

@@ -889,12 +889,14 @@ static int render_impl(nova_hip_ctx *ctx, const nova_hip_render_params *p, float
     NOVA_HIP_CHECK(ctx, hipEventRecord(ev0, s));
   }
   uint64_t launches = 0, camera_rays = 0;
-  for (int f0 = p->frame_start; f0 < p->frame_start + p->frame_count; f0 += fb) {
-    const int frames = std::min(fb, p->frame_start + p->frame_count - f0);
+  // 64-bit frame counter: the step past a range that ends at INT32_MAX does not wrap
+  const int64_t frame_end = (int64_t)p->frame_start + p->frame_count;
+  for (int64_t f0 = p->frame_start; f0 < frame_end; f0 += fb) {
+    const int frames = (int)std::min<int64_t>(fb, frame_end - f0);
     FrameParams fp;
     fp.width = p->width;
     fp.height = p->height;
-    fp.frame0 = f0;
+    fp.frame0 = (int32_t)f0;
     fp.frames = frames;
     fp.offset = p->max_samples_offset;
     fp.vinvert = p->vertical_invert;
@@ -989,7 +991,7 @@ static int render_impl(nova_hip_ctx *ctx, const nova_hip_render_params *p, float
     }
     if (!adj) {
       hipLaunchKernelGGL(accumulate_kernel, dim3(std::min<size_t>((npix * 4 + 255) / 256, (size_t)cus * 16)), dim3(256), 0, s, partial, fp,
-                         accum_dev, final_dev, (f0 + frames == p->frame_start + p->frame_count) ? partial_dev : (float *)nullptr,
+                         accum_dev, final_dev, (f0 + frames == frame_end) ? partial_dev : (float *)nullptr,
                          frames_out ? frames_out + (size_t)(f0 - p->frame_start) * npix * 4 : (float *)nullptr);
       NOVA_HIP_CHECK(ctx, hipGetLastError());
     }

@@ -95,3 +95,16 @@ def test_batch_autotune_wave_starts_on_a_packet_boundary():
     br.close()
     for e in engines:
         e.close()
+
+
+@gpu
+def test_batch_refuses_a_frame_range_past_int32(batch_jobs):
+    """A job whose frames would run past INT32_MAX is refused with INVALID_ARGUMENT before any work."""
+    from axomae_amd.nova_hip import BatchConfig, BatchRenderer, NovaHipError
+    jobs, _, _ = batch_jobs
+    e, w, h, _, d, _ = jobs[0]
+    br = BatchRenderer(0)
+    with pytest.raises(NovaHipError) as err:
+        br.render([(e, w, h, 2, d, {"frame_start": 2**31 - 2})], BatchConfig(1 << 20, 8, 8, 1))
+    assert err.value.code == 15  # INVALID_ARGUMENT
+    br.close()

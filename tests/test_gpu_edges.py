@@ -50,19 +50,21 @@ def test_view_of_the_sky_only(oracle, engine):
 
 
 @gpu
-def test_render_larger_than_one_batch_equals_frame_by_frame(engine):
+@pytest.mark.parametrize("first", [0, 2**31 - 10])
+def test_render_larger_than_one_batch_equals_frame_by_frame(engine, first):
     """4096 x 4096 x 9 frames (151 M paths) needs two batches of at most 2^27 paths; its
     accumulator and image equal those of nine one-frame calls accumulating in turn, bit for
-    bit (frames are accumulated in frame order either way)."""
+    bit (frames are accumulated in frame order either way). The second case's range ends at
+    frame INT32_MAX - 1, where the step past the second batch would leave 32-bit range."""
     W = H = 4096
     sc = scenes.box_room(width=W, height=H)
     engine.upload(sc)
     engine.build_bvh()
-    img, acc = engine.render(W, H, 9, 2)
+    img, acc = engine.render(W, H, 9, 2, frame_start=first)
     acc_seq = None
     for f in range(9):
-        img_seq, acc_seq = engine.render(W, H, 1, 2, frame_start=f, accum=acc_seq)
-    print("[edge] 4096^2 x 9: one call vs nine", flush=True)
+        img_seq, acc_seq = engine.render(W, H, 1, 2, frame_start=first + f, accum=acc_seq)
+    print(f"[edge] 4096^2 x 9 from frame {first}: one call vs nine", flush=True)
     np.testing.assert_array_equal(acc.cpu().numpy(), acc_seq.cpu().numpy())
     np.testing.assert_array_equal(img.cpu().numpy(), img_seq.cpu().numpy())
 

@@ -98,6 +98,23 @@ def test_lookahead_arguments(diff_scene):
 
 
 @gpu
+def test_lookahead_at_the_end_of_the_frame_range(diff_scene):
+    """Frames INT32_MAX - 4 .. INT32_MAX - 1 with 8 ahead: a call renders only the frames left
+    below INT32_MAX, and every call equals a one-frame render."""
+    e, sc, _ = diff_scene
+    W, H, D = 32, 24, 3
+    acc = torch.zeros((H, W, 4), dtype=torch.float32, device="cuda")
+    fin, part = torch.empty_like(acc), torch.empty_like(acc)
+    acc_p = None
+    for f in range(2**31 - 5, 2**31 - 1):
+        e.render_frame_lookahead(W, H, f, D, 8, acc, fin, part)
+        fin_p, acc_p = e.render(W, H, 1, D, frame_start=f, accum=acc_p)
+        assert torch.equal(acc, acc_p), f
+        assert torch.equal(fin, fin_p), f
+    print("[lookahead] frames INT32_MAX-4 .. INT32_MAX-1, 8 ahead", flush=True)
+
+
+@gpu
 def test_lookahead_longer_than_one_batch():
     """4096 x 4096 with 9 frames ahead: the renderer splits them into batches of at most 2^27
     paths (8 frames, then 1), and each batch's frames land at their own place among the kept
