@@ -196,6 +196,8 @@ enum {
  * Engine::writeFramebuffer does (engine.cpp:351-363): accum += partial,
  * final = accum / (frame+1). accum_dev and final_dev are device RGBA32F buffers of
  * width*height*4 floats; accum must hold the previous frames' sum (zero at frame 0).
+ * frame_start >= 0, frame_count >= 1 and frame_start + frame_count <= INT32_MAX, else
+ * NOVA_HIP_INVALID_ARGUMENT (frame indices are the engine's int sample index).
  * partial_dev (optional) receives the last frame's radiance (HdrBufferStruct::partial_buffer).
  * With accum_dev and final_dev both NULL and partial_dev set, only the last frame's radiance
  * is written, as DeviceIntersectorInterface::traverse leaves it for the engine to accumulate.
@@ -219,7 +221,8 @@ int nova_hip_render_frames(nova_hip_ctx *ctx, const nova_hip_render_params *para
  * accum_dev / (frame_start + 1)) and copied to partial_dev, as a one-frame render would; else
  * frames frame_start .. frame_start + lookahead - 1 are rendered in one call, their images kept
  * in device memory (lookahead * width * height * 16 bytes, grown on demand, freed with the
- * context), and the first is returned. Each frame equals its nova_hip_render_frames frame bit
+ * context), and the first is returned. lookahead is in [1, 1024]; frames past INT32_MAX - 1
+ * are not rendered ahead. Each frame equals its nova_hip_render_frames frame bit
  * for bit: a path's samples and arithmetic do not depend on the batch it runs in. lookahead 1
  * (or a render pass) renders the frame alone. nova_hip_render_stats describes the last call
  * that rendered. Buffers as for nova_hip_render_frames. No counterpart in the reference. */
