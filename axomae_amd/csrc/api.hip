@@ -26,6 +26,8 @@ void *ensure_scratch(nova_hip_ctx *ctx, int slot, size_t bytes, hipError_t *err)
   *err = hipSuccess;
   if (b.bytes >= bytes && b.ptr) return b.ptr;
   if (b.ptr) {
+    // work queued by a call that did not wait (NOVA_HIP_RENDER_ASYNC) may still use it
+    (void)hipDeviceSynchronize();
     hipFree(b.ptr);
     b.ptr = nullptr;
     b.bytes = 0;

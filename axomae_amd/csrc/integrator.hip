@@ -550,7 +550,10 @@ static int ensure_gstack(nova_hip_ctx *ctx, uint32_t threads) {
   const uint32_t bound = traversal_stack_bound(ctx);
   const size_t need = bound > (uint32_t)kLdsStack ? (size_t)(bound - kLdsStack) * threads : 1;
   if (ctx->gstack && ctx->gstack_entries >= need) return NOVA_HIP_SUCCESS;
-  if (ctx->gstack) hipFree(ctx->gstack);
+  if (ctx->gstack) {
+    (void)hipDeviceSynchronize();  // queued work of a call that did not wait may still use it
+    hipFree(ctx->gstack);
+  }
   ctx->gstack = nullptr;
   ctx->gstack_entries = 0;
   NOVA_HIP_CHECK(ctx, hipMalloc(&ctx->gstack, need * sizeof(uint32_t)));
@@ -683,7 +686,7 @@ static int resolve_integrator(nova_hip_ctx *ctx, int32_t flags, int *pass) {
 }
 
 static int render_pass(nova_hip_ctx *ctx, const nova_hip_render_params *p, int pass, float *accum_dev, float *final_dev, float *partial_dev,
-                       hipStream_t s) {
+                       hipStream_t s, bool async) {
   const uint32_t npix = (uint32_t)p->width * (uint32_t)p->height;
   const uint32_t blocks = std::min<uint32_t>((npix + kTraceBlock - 1) / kTraceBlock, (uint32_t)device_cus(ctx) * 8);
   const uint32_t threads = blocks * kTraceBlock;
@@ -704,6 +707,12 @@ static int render_pass(nova_hip_ctx *ctx, const nova_hip_render_params *p, int p
   for (int i = 0; i < 16; i++) {
     fp.inv_P[i] = p->inv_P[i];
     fp.inv_V[i] = p->inv_V[i];
+  }
+  if (async) {  // queued; statistics untouched
+    hipLaunchKernelGGL(pass_kernel, dim3(blocks), dim3(kTraceBlock), 0, s, tc, sc, ctx->sobol_table, fp, pass, p->z_near, p->z_far, accum_dev,
+                       final_dev, partial_dev);
+    NOVA_HIP_CHECK(ctx, hipGetLastError());
+    return NOVA_HIP_SUCCESS;
   }
   hipEvent_t ev0, ev1;
   NOVA_HIP_CHECK(ctx, hipEventCreate(&ev0));
@@ -739,6 +748,10 @@ static int render_impl(nova_hip_ctx *ctx, const nova_hip_render_params *p, float
     return set_error(ctx, NOVA_HIP_INVALID_ARGUMENT, "bad frame range or depth");
   if (p->index_policy != NOVA_HIP_INDEX_CPU && p->index_policy != NOVA_HIP_INDEX_OPTIX)
     return set_error(ctx, NOVA_HIP_INVALID_ARGUMENT, "unknown index_policy");
+  // queued without a wait: nothing is read back, so neither timing nor counting
+  const bool async = (p->flags & NOVA_HIP_RENDER_ASYNC) != 0;
+  if (async && (adj || (p->flags & (NOVA_HIP_RENDER_TIMING | NOVA_HIP_RENDER_COUNT_TRAVERSAL))))
+    return set_error(ctx, NOVA_HIP_INVALID_ARGUMENT, "ASYNC reads nothing back: not with TIMING, COUNT_TRAVERSAL or the adjoint");
   // accum and final together, or neither with a partial buffer (one frame's radiance only)
   // or the per-frame images
   if (!adj && (!accum_dev != !final_dev || (!accum_dev && !partial_dev && !frames_out)))
@@ -764,7 +777,7 @@ static int render_impl(nova_hip_ctx *ctx, const nova_hip_render_params *p, float
   NOVA_HIP_CHECK(ctx, hipSetDevice(ctx->device));
   if (pass != 0) {
     if (!accum_dev) return set_error(ctx, NOVA_HIP_INVALID_BUFFER_STATE, "render passes need accum and final buffers");
-    return render_pass(ctx, p, pass, accum_dev, final_dev, partial_dev, s);
+    return render_pass(ctx, p, pass, accum_dev, final_dev, partial_dev, s, async);
   }
   const size_t npix = (size_t)p->width * p->height;
   // Frames per batch: up to kPathsPerBatch paths in flight (about 116 B of queue and path
@@ -883,7 +896,7 @@ static int render_impl(nova_hip_ctx *ctx, const nova_hip_render_params *p, float
   const bool use_hint = tail_rays > 0 && !noalloc && ctx->tail_hint > 0 && ctx->tail_hint_key[0] == (uint64_t)cap &&
                         ctx->tail_hint_key[1] == (uint64_t)p->max_depth && ctx->tail_hint_key[2] == (uint64_t)(int64_t)p->rr_start;
   hipEvent_t ev0 = nullptr, ev1 = nullptr;
-  if (!noalloc) {
+  if (!noalloc && !async) {
     NOVA_HIP_CHECK(ctx, hipEventCreate(&ev0));
     NOVA_HIP_CHECK(ctx, hipEventCreate(&ev1));
     NOVA_HIP_CHECK(ctx, hipEventRecord(ev0, s));
@@ -996,7 +1009,7 @@ static int render_impl(nova_hip_ctx *ctx, const nova_hip_render_params *p, float
       NOVA_HIP_CHECK(ctx, hipGetLastError());
     }
   }
-  if (noalloc) return NOVA_HIP_SUCCESS;  // stream-ordered; statistics untouched
+  if (noalloc || async) return NOVA_HIP_SUCCESS;  // stream-ordered; statistics untouched
   NOVA_HIP_CHECK(ctx, hipEventRecord(ev1, s));
   NOVA_HIP_CHECK(ctx, hipEventSynchronize(ev1));
   float ms = 0.f;

@@ -106,6 +106,7 @@ RENDER_COUNT_TRAVERSAL = 2
 RENDER_PER_RAY_TRAVERSAL = 4  # camera rays one walk per ray (testing and tuning)
 ADJOINT_RETRACE = 8  # the adjoint replay traces again instead of reusing the primal's hits (testing)
 RENDER_NO_TAIL = 16  # one extend and one shade launch per bounce to the end (testing and tuning)
+RENDER_ASYNC = 32  # return once the work is queued; statistics are not read back
 
 
 class RenderStats(C.Structure):
@@ -349,10 +350,10 @@ class HipEngine:
         return final
 
     def render_to(self, rb, frames, max_depth, frame_start=0, rr_start=-1, max_samples_offset=0, vertical_invert=False, inv_P=None,
-                  inv_V=None, integrator_flags=0, index_policy=INDEX_CPU):
+                  inv_V=None, integrator_flags=0, index_policy=INDEX_CPU, flags=0):
         """The engine's frame loop into a :class:`RenderBuffer` (render, accumulate, swap) at its dimensions."""
         p = render_params(self.scene, rb.width, rb.height, frames, max_depth, frame_start, rr_start, max_samples_offset, vertical_invert, inv_P,
-                          inv_V, 0, -1, index_policy, integrator_flags, 0.1, 1000.0)
+                          inv_V, flags, -1, index_policy, integrator_flags, 0.1, 1000.0)
         self._check(self.L.nova_hip_render_to_renderbuffer(self.ctx, C.byref(p), rb.rb, _stream(self.torch)), "render_to_renderbuffer")
 
     def render_to_lookahead(self, rb, frame, max_depth, lookahead, rr_start=-1):

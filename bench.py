@@ -267,13 +267,30 @@ def frame_by_frame_leg(eng, sc, W, H, frames, depth, rr_start):
             eng.render_to(rb, 1, depth, frame_start=f, rr_start=rr_start)
         torch.cuda.synchronize()
         runs.append(time.perf_counter() - t0)
-    from axomae_amd.nova_hip import RB_ACCUMULATOR
+    from axomae_amd.nova_hip import RB_ACCUMULATOR, RENDER_ASYNC
     acc_plain = rb.read(RB_ACCUMULATOR)
-    rb.close()
     t = sorted(runs)[1]
     log(f"[bench] frame by frame: {frames} calls of 1 frame, {t * 1e3:.1f} ms (runs {', '.join(f'{r * 1e3:.1f}' for r in runs)})")
     out = {"calls": frames, "frame_count_per_call": 1, "entry_point": "nova_hip_render_to_renderbuffer",
            "ms": round(t * 1e3, 2), "msamples_s": round(W * H * frames / t / 1e6, 3)}
+    # the same loop with calls that return once their frame is queued (NOVA_HIP_RENDER_ASYNC):
+    # the host queues the next frame while the device renders
+    runs = []
+    for _ in range(3):
+        rb.reset()
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for f in range(frames):
+            eng.render_to(rb, 1, depth, frame_start=f, rr_start=rr_start, flags=RENDER_ASYNC)
+        torch.cuda.synchronize()
+        runs.append(time.perf_counter() - t0)
+    same = bool(np.array_equal(rb.read(RB_ACCUMULATOR), acc_plain))
+    rb.close()
+    t = sorted(runs)[1]
+    log(f"[bench] frame by frame, calls that do not wait: {t * 1e3:.1f} ms (runs {', '.join(f'{r * 1e3:.1f}' for r in runs)}); "
+        f"accumulator equal to the waiting calls' {same}")
+    out["queued"] = {"flags": "NOVA_HIP_RENDER_ASYNC", "ms": round(t * 1e3, 2), "msamples_s": round(W * H * frames / t / 1e6, 3),
+                     "accumulator_equal": same}
     # the same engine loop with frames rendered ahead: a call that misses renders `ahead`
     # frames at once, the following ones copy theirs (nova_hip_render_to_renderbuffer_lookahead)
     ahead = 8

@@ -182,12 +182,22 @@ enum {
 /* NOVA_HIP_RENDER_NO_TAIL: keep one extend and one shade launch per bounce to the end, where
  * the renderer otherwise finishes the last bounces of a batch in one launch, one path per
  * lane, once few paths remain (the result is the same; for tests and tuning). */
+/* NOVA_HIP_RENDER_ASYNC: a render call returns once its work is queued on the stream instead
+ * of waiting for it to finish (an engine loop can queue its next frame meanwhile). The call
+ * reads nothing back: nova_hip_render_stats keeps describing the last call that waited. The
+ * context's buffers stay in use until the stream has done the work, so until then the caller
+ * makes every other call on the context on the same stream (render calls with the flag or
+ * without), and synchronizes the stream before reading results on the host or calling the
+ * context's scene, BVH, texture or render-buffer functions. Not with TIMING or
+ * COUNT_TRAVERSAL, which read back (NOVA_HIP_INVALID_ARGUMENT), nor for the adjoint. No
+ * counterpart in the reference, whose traverse waits for its frame. */
 enum {
   NOVA_HIP_RENDER_TIMING = 1,
   NOVA_HIP_RENDER_COUNT_TRAVERSAL = 2,
   NOVA_HIP_RENDER_PER_RAY_TRAVERSAL = 4,
   NOVA_HIP_ADJOINT_RETRACE = 8,
-  NOVA_HIP_RENDER_NO_TAIL = 16
+  NOVA_HIP_RENDER_NO_TAIL = 16,
+  NOVA_HIP_RENDER_ASYNC = 32
 };
 
 /* Renders frames [frame_start, frame_start+frame_count) of the path integrator

@@ -75,3 +75,57 @@ def test_renderbuffer_dimensions(engine):
     rb.reset()
     assert not rb.read(RB_ACCUMULATOR).any()
     rb.close()
+
+
+@gpu
+def test_engine_loop_with_calls_that_do_not_wait(engine):
+    """The engine loop with NOVA_HIP_RENDER_ASYNC (each call returns once its frame is queued on
+    the stream): 12 frames at depth 6 with Russian roulette, the accumulator and displayed image
+    after the loop equal the waiting loop's bit for bit; the render statistics still describe
+    the last call that waited; an AOV pass with the flag equals its waiting render."""
+    import torch
+    from axomae_amd.nova_hip import RB_ACCUMULATOR, RB_FRONT, RENDER_ASYNC, RenderBuffer
+    sc = scenes.scaled_million(scale_cells=24, spheres=4, sphere_level=2, width=96, height=64)
+    engine.upload(sc)
+    engine.build_bvh()
+    W, H, D = 96, 64, 6
+    waiting, queued = RenderBuffer(engine, W, H), RenderBuffer(engine, W, H)
+    waiting.reset()
+    queued.reset()
+    for f in range(12):
+        engine.render_to(waiting, 1, D, frame_start=f, rr_start=3)
+    st = engine.render_stats()
+    for f in range(12):
+        engine.render_to(queued, 1, D, frame_start=f, rr_start=3, flags=RENDER_ASYNC)
+    torch.cuda.synchronize()
+    st2 = engine.render_stats()
+    print(f"[renderbuffer] 12 frames queued without waits: {st.segments} segments in the last waiting call", flush=True)
+    np.testing.assert_array_equal(queued.read(RB_ACCUMULATOR), waiting.read(RB_ACCUMULATOR))
+    np.testing.assert_array_equal(queued.read(RB_FRONT), waiting.read(RB_FRONT))
+    assert (st2.segments, st2.total_ms) == (st.segments, st.total_ms)
+    from axomae_amd.nova_hip import INTEGRATOR_NORMAL, INTEGRATOR_PATH
+    img, _ = engine.render(W, H, 2, D, integrator_flags=INTEGRATOR_PATH | INTEGRATOR_NORMAL)
+    img_q, _ = engine.render(W, H, 2, D, integrator_flags=INTEGRATOR_PATH | INTEGRATOR_NORMAL, flags=RENDER_ASYNC)
+    torch.cuda.synchronize()
+    assert torch.equal(img, img_q)
+    waiting.close()
+    queued.close()
+
+
+@gpu
+def test_calls_that_do_not_wait_refuse_readbacks(engine):
+    """NOVA_HIP_RENDER_ASYNC with TIMING or COUNT_TRAVERSAL, or on the adjoint, is refused with
+    INVALID_ARGUMENT: those read results back."""
+    import torch
+    from axomae_amd.nova_hip import RENDER_ASYNC, RENDER_COUNT_TRAVERSAL, RENDER_TIMING, NovaHipError
+    sc = scenes.box_room(width=16, height=16)
+    engine.upload(sc)
+    engine.build_bvh()
+    for fl in (RENDER_ASYNC | RENDER_TIMING, RENDER_ASYNC | RENDER_COUNT_TRAVERSAL):
+        with pytest.raises(NovaHipError) as e:
+            engine.render(16, 16, 1, 3, flags=fl)
+        assert e.value.code == 15, fl
+    d_img = torch.zeros((16, 16, 4), dtype=torch.float32, device="cuda")
+    with pytest.raises(NovaHipError) as e:
+        engine.adjoint(16, 16, 1, 3, d_img, 1.0, flags=RENDER_ASYNC)
+    assert e.value.code == 15
