@@ -109,6 +109,8 @@ int nova_hip_create(int device, nova_hip_ctx **out) {
 }
 
 static void free_scene(nova_hip_ctx *ctx) {
+  // render work queued without a wait (NOVA_HIP_RENDER_ASYNC) may still read these
+  (void)hipDeviceSynchronize();
   void **ptrs[] = {(void **)&ctx->tri_world, (void **)&ctx->shade,     (void **)&ctx->prim_aabb, (void **)&ctx->prim_centroid,
                    (void **)&ctx->mesh_xf,   (void **)&ctx->materials, (void **)&ctx->textures,  (void **)&ctx->tex_u32,
                    (void **)&ctx->tex_f32,   (void **)&ctx->acos_lut,  (void **)&ctx->prim_idx,  (void **)&ctx->bnodes,

@@ -679,6 +679,7 @@ extern "C" int nova_hip_bvh_build(nova_hip_ctx *ctx, int build_candidates, nova_
   NOVA_HIP_CHECK(ctx, hipSetDevice(ctx->device));
   const uint32_t n = ctx->prim_count;
   const size_t maxnodes = 2 * (size_t)n - 1;
+  (void)hipDeviceSynchronize();  // render work queued without a wait may still read the old tree
   for (void **p : {(void **)&ctx->prim_idx, (void **)&ctx->bnodes, (void **)&ctx->nodes4, (void **)&ctx->nodes4f, (void **)&ctx->tri_leaf,
                    (void **)&ctx->bparent, (void **)&ctx->bdepth, (void **)&ctx->leaf_of_pos, (void **)&ctx->tie})
     if (*p) {

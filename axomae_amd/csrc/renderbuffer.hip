@@ -23,6 +23,7 @@ namespace {
 constexpr unsigned kColor = 4, kDepth = 1, kNormal = 3;  // RenderBufferPack::CHANNELS_*
 
 void release(nova_hip_renderbuffer *rb) {
+  if (rb->capacity) (void)hipDeviceSynchronize();  // queued render work may still write them
   for (float **p : {&rb->front, &rb->back, &rb->partial, &rb->accumulator, &rb->depth, &rb->normal}) {
     if (*p) (void)hipFree(*p);
     *p = nullptr;

@@ -187,8 +187,9 @@ enum {
  * reads nothing back: nova_hip_render_stats keeps describing the last call that waited. The
  * context's buffers stay in use until the stream has done the work, so until then the caller
  * makes every other call on the context on the same stream (render calls with the flag or
- * without), and synchronizes the stream before reading results on the host or calling the
- * context's scene, BVH, texture or render-buffer functions. Not with TIMING or
+ * without, texture updates) and synchronizes the stream before reading results on the host.
+ * Calls that free or replace device buffers (scene upload, BVH build, render-buffer resize
+ * or destroy, a render that needs more scratch, nova_hip_destroy) wait for the device first. Not with TIMING or
  * COUNT_TRAVERSAL, which read back (NOVA_HIP_INVALID_ARGUMENT), nor for the adjoint. No
  * counterpart in the reference, whose traverse waits for its frame. */
 enum {
