@@ -129,3 +129,50 @@ def test_calls_that_do_not_wait_refuse_readbacks(engine):
     with pytest.raises(NovaHipError) as e:
         engine.adjoint(16, 16, 1, 3, d_img, 1.0, flags=RENDER_ASYNC)
     assert e.value.code == 15
+
+
+@gpu
+def test_buffers_replaced_right_after_queued_frames(engine):
+    """Frames queued with NOVA_HIP_RENDER_ASYNC and, without a synchronization, a render-buffer
+    resize, a scene upload and a BVH build: each of those waits for the queued work before it
+    frees buffers, and renders afterwards equal those of a context that never queued any."""
+    import torch
+    from axomae_amd.nova_hip import RB_ACCUMULATOR, RENDER_ASYNC, HipEngine, RenderBuffer
+    sc = scenes.scaled_million(scale_cells=24, spheres=4, sphere_level=2, width=64, height=48)
+    other = scenes.box_room(width=64, height=48)
+    engine.upload(sc)
+    engine.build_bvh()
+    rb = RenderBuffer(engine, 64, 48)
+    rb.reset()
+    for f in range(6):
+        engine.render_to(rb, 1, 5, frame_start=f, rr_start=3, flags=RENDER_ASYNC)
+    rb.resize(80, 40)  # frees and reallocates the buffers the queued frames write
+    rb.resize(64, 48)
+    for f in range(4):
+        engine.render_to(rb, 1, 5, frame_start=f, rr_start=3, flags=RENDER_ASYNC)
+    engine.upload(other)  # frees the scene the queued frames read
+    engine.build_bvh()
+    img, _ = engine.render(64, 48, 3, 5, rr_start=3, flags=RENDER_ASYNC)
+    engine.build_bvh()  # frees the tree the queued render reads
+    torch.cuda.synchronize()
+    fresh = HipEngine(0)
+    fresh.upload(other)
+    fresh.build_bvh()
+    ref, _ = fresh.render(64, 48, 3, 5, rr_start=3)
+    print("[renderbuffer] queued frames, then resize / upload / BVH build", flush=True)
+    assert torch.equal(img, ref)
+    rb.reset()
+    engine.upload(sc)
+    engine.build_bvh()
+    for f in range(3):
+        engine.render_to(rb, 1, 5, frame_start=f, rr_start=3)
+    fresh.upload(sc)
+    fresh.build_bvh()
+    rb2 = RenderBuffer(fresh, 64, 48)
+    rb2.reset()
+    for f in range(3):
+        fresh.render_to(rb2, 1, 5, frame_start=f, rr_start=3)
+    np.testing.assert_array_equal(rb.read(RB_ACCUMULATOR), rb2.read(RB_ACCUMULATOR))
+    rb.close()
+    rb2.close()
+    fresh.close()
