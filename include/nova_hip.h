@@ -340,7 +340,8 @@ int nova_hip_renderbuffer_read(const nova_hip_renderbuffer *rb, int which, float
 
 /* One scene of a batched render: a context holding an uploaded scene with its BVH built,
  * the frames to render (as for nova_hip_render_frames; max_depth and rr_start must be the
- * same for every item, sampling_roughness_texture -1) and device RGBA32F buffers of
+ * same for every item, sampling_roughness_texture -1; the render flags are not used: a
+ * batch always waits and reports nova_hip_batch_stats) and device RGBA32F buffers of
  * width*height*4 floats, 16-byte aligned: accum is zeroed and receives the sum of the
  * frames, final the image accum / (frame_start + frame_count). */
 typedef struct {
