@@ -86,6 +86,14 @@ __global__ __launch_bounds__(256) void generate_kernel(FrameParams fp, const uin
 #ifndef NOVA_TRACE_WAVES
 #define NOVA_TRACE_WAVES 6  // waves per SIMD the extend kernel is compiled for (0: compiler's choice)
 #endif
+// Extend workgroups per CU: as many as are resident at once (a 256-thread workgroup is one
+// wave on each of the CU's 4 SIMDs). A lane's first ray is fixed by its thread index, so a
+// workgroup that waits for a free slot would start its rays only after the queue has drained
+// and add a whole walk's latency at the end of the launch.
+#ifndef NOVA_TRACE_BLOCKS_PER_CU
+#define NOVA_TRACE_BLOCKS_PER_CU (NOVA_TRACE_WAVES > 0 ? NOVA_TRACE_WAVES : 6)
+#endif
+static_assert(kTraceBlock == 256, "the workgroups-per-CU count assumes one wave per SIMD per workgroup");
 #if NOVA_TRACE_WAVES > 0
 #define NOVA_TRACE_OCCUPANCY __attribute__((amdgpu_waves_per_eu(NOVA_TRACE_WAVES, NOVA_TRACE_WAVES)))
 #else
@@ -785,7 +793,9 @@ static int render_impl(nova_hip_ctx *ctx, const nova_hip_render_params *p, float
   // few rays whose launches cost their slowest ray's latency, a fixed cost per batch.
   int fb = (int)std::max<size_t>(1, std::min<size_t>((size_t)p->frame_count, kPathsPerBatch / npix));
   const int cus = device_cus(ctx);
-  auto trace_threads_of = [&](size_t c) { return (uint32_t)std::min<size_t>((c + kTraceBlock - 1) / kTraceBlock, (size_t)cus * 8) * kTraceBlock; };
+  auto trace_threads_of = [&](size_t c) {
+    return (uint32_t)std::min<size_t>((c + kTraceBlock - 1) / kTraceBlock, (size_t)cus * NOVA_TRACE_BLOCKS_PER_CU) * kTraceBlock;
+  };
   if (noalloc) {
     // as many frames per batch as the fixed array holds
     while (fb > 0 && noalloc_bytes(npix * fb, trace_threads_of(npix * fb), traversal_stack_bound(ctx)) > kNoAllocScratchBytes) fb--;
