@@ -115,6 +115,28 @@ def test_lookahead_at_the_end_of_the_frame_range(diff_scene):
 
 
 @gpu
+def test_lookahead_calls_that_do_not_wait(diff_scene):
+    """Frames rendered 4 ahead by calls that return once queued (NOVA_HIP_RENDER_ASYNC): a call
+    that misses queues the batch and the first frame's accumulation without waiting, later
+    calls queue their copies; after 10 calls and one synchronization the accumulator and image
+    equal those of waiting one-frame renders."""
+    from axomae_amd.nova_hip import RENDER_ASYNC
+    e, sc, _ = diff_scene
+    W, H, D = 64, 48, 4
+    acc = torch.zeros((H, W, 4), dtype=torch.float32, device="cuda")
+    fin, part = torch.empty_like(acc), torch.empty_like(acc)
+    for f in range(10):
+        e.render_frame_lookahead(W, H, f, D, 4, acc, fin, part, rr_start=2, flags=RENDER_ASYNC)
+    torch.cuda.synchronize()
+    acc_p = None
+    for f in range(10):
+        fin_p, acc_p = e.render(W, H, 1, D, frame_start=f, accum=acc_p, rr_start=2)
+    print("[lookahead] 10 frames, 4 ahead, calls that do not wait", flush=True)
+    assert torch.equal(acc, acc_p)
+    assert torch.equal(fin, fin_p)
+
+
+@gpu
 def test_lookahead_longer_than_one_batch():
     """4096 x 4096 with 9 frames ahead: the renderer splits them into batches of at most 2^27
     paths (8 frames, then 1), and each batch's frames land at their own place among the kept
