@@ -476,6 +476,60 @@ __global__ void accumulate_kernel(const float *__restrict__ partial, FrameParams
   }
 }
 
+// accumulate_kernel for batches of many frames: a workgroup takes 64 pixels and brings their
+// frames into LDS 16 at a time with contiguous 16-byte loads (a pixel's frames are adjacent
+// in the partial buffer), where one thread per pixel channel reading its own frames strides
+// over 1 KiB per pixel. Each channel still adds its frames one by one in frame order, so the
+// sums are those of accumulate_kernel bit for bit.
+#ifndef NOVA_ACC_TILED_FRAMES
+#define NOVA_ACC_TILED_FRAMES 8  // batches of at least this many frames accumulate through LDS tiles
+#endif
+constexpr int kAccPix = 64, kAccChunk = 16, kAccRow = kAccChunk * 4 + 4;  // row padded against bank conflicts
+static_assert(kAccChunk == 16 && kAccRow % 4 == 0, "chunk index split by shifts; rows of whole float4s");
+__global__ __launch_bounds__(256) void accumulate_tiled_kernel(const float *__restrict__ partial, FrameParams fp, float *__restrict__ accum,
+                                                               float *__restrict__ final_out, float *__restrict__ partial_out,
+                                                               float *__restrict__ frames_out) {
+  __shared__ __align__(16) float tile[kAccPix * kAccRow];
+  const uint32_t npix = (uint32_t)fp.width * (uint32_t)fp.height, npix4 = npix * 4u;
+  const int frames = fp.frames;
+  const float4 *src4 = reinterpret_cast<const float4 *>(partial);
+  const uint32_t q = threadIdx.x >> 2, c = threadIdx.x & 3u;
+  for (uint32_t p0 = blockIdx.x * kAccPix; p0 < npix; p0 += gridDim.x * kAccPix) {  // uniform per workgroup
+    const uint32_t p = p0 + q;  // this thread's pixel rank
+    const bool valid = p < npix;
+    const uint32_t i = valid ? pixel_offset(fp, pixel_of_rank(fp, p)) + c : 0u;
+    float a = (valid && accum) ? accum[i] : 0.f, last = 0.f;
+    for (int f0 = 0; f0 < frames; f0 += kAccChunk) {
+      const int nf = min(kAccChunk, frames - f0);
+      __syncthreads();  // the previous chunk has been read
+      for (int j = threadIdx.x; j < kAccPix * nf; j += blockDim.x) {
+        // a full chunk (the common case) splits the index with shifts
+        const uint32_t qq = nf == kAccChunk ? (uint32_t)j >> 4 : (uint32_t)j / (uint32_t)nf;
+        const uint32_t fl = nf == kAccChunk ? (uint32_t)j & 15u : (uint32_t)j % (uint32_t)nf;
+        if (p0 + qq < npix)
+          *reinterpret_cast<float4 *>(tile + qq * kAccRow + fl * 4) = src4[(size_t)(p0 + qq) * frames + f0 + fl];
+      }
+      __syncthreads();
+      if (valid) {
+        const float *row = tile + q * kAccRow + c;
+        for (int fl = 0; fl < nf; fl++) {
+          const float v = row[fl * 4];
+          a += v;  // frame order
+          if (frames_out) frames_out[(size_t)(f0 + fl) * npix4 + i] = v;
+        }
+        last = row[(nf - 1) * 4];
+      }
+    }
+    if (valid) {
+      if (accum) {
+        accum[i] = a;
+        final_out[i] = a / float(fp.frame0 + frames);
+      }
+      if (partial_out) partial_out[i] = last;
+    }
+  }
+}
+
 // One kept frame image into the caller's buffers, as accumulate_kernel adds a one-frame
 // batch: accum += image, final = accum / (frame + 1), partial = image.
 __global__ void frame_add_kernel(const float *__restrict__ image, uint32_t n4, float *__restrict__ accum, float *__restrict__ final_out,
@@ -1013,9 +1067,14 @@ static int render_impl(nova_hip_ctx *ctx, const nova_hip_render_params *p, float
     }
     }
     if (!adj) {
-      hipLaunchKernelGGL(accumulate_kernel, dim3(std::min<size_t>((npix * 4 + 255) / 256, (size_t)cus * 16)), dim3(256), 0, s, partial, fp,
-                         accum_dev, final_dev, (f0 + frames == frame_end) ? partial_dev : (float *)nullptr,
-                         frames_out ? frames_out + (size_t)(f0 - p->frame_start) * npix * 4 : (float *)nullptr);
+      float *const fr_out = frames_out ? frames_out + (size_t)(f0 - p->frame_start) * npix * 4 : (float *)nullptr;
+      float *const last_out = (f0 + frames == frame_end) ? partial_dev : (float *)nullptr;
+      if (frames >= NOVA_ACC_TILED_FRAMES)
+        hipLaunchKernelGGL(accumulate_tiled_kernel, dim3(std::min<size_t>((npix + kAccPix - 1) / kAccPix, (size_t)cus * 16)), dim3(256), 0, s, partial,
+                           fp, accum_dev, final_dev, last_out, fr_out);
+      else
+        hipLaunchKernelGGL(accumulate_kernel, dim3(std::min<size_t>((npix * 4 + 255) / 256, (size_t)cus * 16)), dim3(256), 0, s, partial, fp,
+                           accum_dev, final_dev, last_out, fr_out);
       NOVA_HIP_CHECK(ctx, hipGetLastError());
     }
   }
