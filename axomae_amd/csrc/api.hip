@@ -137,6 +137,9 @@ int nova_hip_destroy(nova_hip_ctx *ctx) {
     if (b.ptr) hipFree(b.ptr);
   if (ctx->gstack) hipFree(ctx->gstack);
   if (ctx->batch_stream) hipStreamDestroy(ctx->batch_stream);
+  if (ctx->ev_begin) hipEventDestroy(ctx->ev_begin);
+  if (ctx->ev_end) hipEventDestroy(ctx->ev_end);
+  for (hipEvent_t ev : ctx->ev_pool) hipEventDestroy(ev);
   delete ctx;
   return NOVA_HIP_SUCCESS;
 }

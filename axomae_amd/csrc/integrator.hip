@@ -747,6 +747,15 @@ static int resolve_integrator(nova_hip_ctx *ctx, int32_t flags, int *pass) {
   return NOVA_HIP_SUCCESS;
 }
 
+// The context's start and end events, made on first use.
+static int call_events(nova_hip_ctx *ctx, hipEvent_t *ev0, hipEvent_t *ev1) {
+  if (!ctx->ev_begin) NOVA_HIP_CHECK(ctx, hipEventCreate(&ctx->ev_begin));
+  if (!ctx->ev_end) NOVA_HIP_CHECK(ctx, hipEventCreate(&ctx->ev_end));
+  *ev0 = ctx->ev_begin;
+  *ev1 = ctx->ev_end;
+  return NOVA_HIP_SUCCESS;
+}
+
 static int render_pass(nova_hip_ctx *ctx, const nova_hip_render_params *p, int pass, float *accum_dev, float *final_dev, float *partial_dev,
                        hipStream_t s, bool async) {
   const uint32_t npix = (uint32_t)p->width * (uint32_t)p->height;
@@ -777,8 +786,8 @@ static int render_pass(nova_hip_ctx *ctx, const nova_hip_render_params *p, int p
     return NOVA_HIP_SUCCESS;
   }
   hipEvent_t ev0, ev1;
-  NOVA_HIP_CHECK(ctx, hipEventCreate(&ev0));
-  NOVA_HIP_CHECK(ctx, hipEventCreate(&ev1));
+  int est = call_events(ctx, &ev0, &ev1);
+  if (est) return est;
   NOVA_HIP_CHECK(ctx, hipEventRecord(ev0, s));
   hipLaunchKernelGGL(pass_kernel, dim3(blocks), dim3(kTraceBlock), 0, s, tc, sc, ctx->sobol_table, fp, pass, p->z_near, p->z_far, accum_dev, final_dev,
                      partial_dev);
@@ -787,8 +796,6 @@ static int render_pass(nova_hip_ctx *ctx, const nova_hip_render_params *p, int p
   NOVA_HIP_CHECK(ctx, hipEventSynchronize(ev1));
   float ms = 0.f;
   NOVA_HIP_CHECK(ctx, hipEventElapsedTime(&ms, ev0, ev1));
-  NOVA_HIP_CHECK(ctx, hipEventDestroy(ev0));
-  NOVA_HIP_CHECK(ctx, hipEventDestroy(ev1));
   ctx->stats = nova_hip_render_stats{};
   ctx->stats.total_ms = ms;
   ctx->stats.segments = npix;
@@ -946,8 +953,12 @@ static int render_impl(nova_hip_ctx *ctx, const nova_hip_render_params *p, float
   std::vector<hipEvent_t> kev;  // pairs of events around launches; kkind[i]: what pair i timed
   std::vector<int> kkind;       // 0 extend (bounce 0), 1 extend, 2 shade, 3 tail
   auto stamp = [&](void) -> int {
-    hipEvent_t ev;
-    NOVA_HIP_CHECK(ctx, hipEventCreate(&ev));
+    if (kev.size() == ctx->ev_pool.size()) {  // the pool grows to the most launches one call timed
+      hipEvent_t ev;
+      NOVA_HIP_CHECK(ctx, hipEventCreate(&ev));
+      ctx->ev_pool.push_back(ev);
+    }
+    const hipEvent_t ev = ctx->ev_pool[kev.size()];
     kev.push_back(ev);
     NOVA_HIP_CHECK(ctx, hipEventRecord(ev, s));
     return NOVA_HIP_SUCCESS;
@@ -961,8 +972,7 @@ static int render_impl(nova_hip_ctx *ctx, const nova_hip_render_params *p, float
                         ctx->tail_hint_key[1] == (uint64_t)p->max_depth && ctx->tail_hint_key[2] == (uint64_t)(int64_t)p->rr_start;
   hipEvent_t ev0 = nullptr, ev1 = nullptr;
   if (!noalloc && !async) {
-    NOVA_HIP_CHECK(ctx, hipEventCreate(&ev0));
-    NOVA_HIP_CHECK(ctx, hipEventCreate(&ev1));
+    if ((st = call_events(ctx, &ev0, &ev1))) return st;
     NOVA_HIP_CHECK(ctx, hipEventRecord(ev0, s));
   }
   uint64_t launches = 0, camera_rays = 0;
@@ -1083,8 +1093,6 @@ static int render_impl(nova_hip_ctx *ctx, const nova_hip_render_params *p, float
   NOVA_HIP_CHECK(ctx, hipEventSynchronize(ev1));
   float ms = 0.f;
   hipEventElapsedTime(&ms, ev0, ev1);
-  hipEventDestroy(ev0);
-  hipEventDestroy(ev1);
   // the counters and the 64-bit statistics after them in one copy (the per-frame engine
   // loop pays for every synchronous copy)
   uint32_t words[8 + 2 * 20] = {};
@@ -1109,7 +1117,6 @@ static int render_impl(nova_hip_ctx *ctx, const nova_hip_render_params *p, float
     if (kkind[k] == 2) shade_ms += a;
     if (kkind[k] == 3) tail_ms += a;
   }
-  for (hipEvent_t ev : kev) hipEventDestroy(ev);
   ctx->stats.total_ms = ms;
   ctx->stats.extend_ms = ext_ms;
   ctx->stats.shade_ms = shade_ms;

@@ -157,6 +157,10 @@ struct nova_hip_ctx {
   int32_t ahead_first = 0, ahead_count = 0;
   uint64_t ahead_gen = 0;
   bool ahead_valid = false;
+  // timing events reused by every render call (created on first use, destroyed with the
+  // context): the call's start and end, then two per timed launch
+  hipEvent_t ev_begin = nullptr, ev_end = nullptr;
+  std::vector<hipEvent_t> ev_pool;
 
   // integrator scratch (grown on demand)
   nova_hip::DeviceBuffer scratch[16];
